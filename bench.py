@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Stitch benchmark: 2 x 5312x2988 -> 8500x3300, Laplacian blend with 6 levels.
+
+One step = one CudaStitchPano::process of one frame pair (inputs already in
+HBM), as the reference's `test_cuda_blend --perf` loop measures
+(tests/testCudaBlend.cpp:224-241).  Prints ONE JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--levels L] [--no-cpu]
+
+Single GPU only: --gpus N with N > 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("hm-cupano_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+WORKLOAD = "2 x 5312x2988 uint8 BGRA -> 8500x3300 canvas, batch 1"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(rig, levels, pano_levels):
+    """Minimum HBM bytes per launch of each kernel (what it must read + write).
+
+    Maps are read as one packed 32-bit word per pixel; a source pixel is read
+    once per valid map entry; uint8 BGRA pixels are 4 B, float pyramid pixels
+    16 B, the 2-image mask 4 B.
+    """
+    cw, ch = rig.canvas_w, rig.canvas_h
+    map_px = [m.size for m in rig.map_x]
+    valid_px = [int((m != 65535).sum()) for m in rig.map_x]
+    out = {}
+    if levels == 0:
+        # each canvas pixel: seam byte, the selected image's map word + source pixel, canvas write
+        sel = 0
+        sel_valid = 0
+        seam = rig.seam
+        for i, lab in ((0, 1), (1, 0)):
+            h, w = rig.map_x[i].shape
+            sub = seam[rig.pos_y[i]: rig.pos_y[i] + h, rig.pos_x[i]: rig.pos_x[i] + w] == lab
+            sel += int(sub.sum())
+            sel_valid += int((sub & (rig.map_x[i] != 65535)).sum())
+        out["hard_seam"] = cw * ch * 1 + sel * 4 + sel_valid * 4 + cw * ch * 4
+        return out
+    x2 = rig.pos_x[1]
+    w1 = rig.map_x[0].shape[1]
+    bw = w1 - x2 + 256
+    bh = ch
+    a = [(bw, bh)]
+    for _ in range(1, pano_levels):
+        a.append(((a[-1][0] + 1) // 2, (a[-1][1] + 1) // 2))
+    A = [w * h for w, h in a]
+    out["remap_l0"] = sum(map_px) * 4 + sum(valid_px) * 4 + (cw - bw) * ch * 4 + 2 * A[0] * 4
+    out["down_l0"] = 2 * A[0] * 4 + 2 * A[1] * 16
+    out["up_l0"] = 2 * A[0] * 4 + 2 * A[1] * 16 + A[0] * 4 + A[1] * 16 + A[0] * 4
+    return out
+
+
+def cpu_baseline(rig, f1, f2, levels, budget_s=12.0):
+    """The CPU oracle (single thread) on the same workload, bounded in time."""
+    import hm_oracle
+
+    hm_oracle.lib()
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        hm_oracle.pano2(rig, f1, f2, levels)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    mpix = rig.canvas_w * rig.canvas_h * n / el / 1e6
+    return {"value": round(mpix, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": f"{n} frame(s) of the same 8500x3300 L={levels} stitch on the oracle in {el:.1f} s",
+            "fps": round(n / el, 4)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--latency-frames", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the stitch path shards by rig/frame with no collectives, "
+              "and multi-GPU runs are not part of this build. Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import hm_stitch
+    import hm_synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    hm_stitch.lib()
+    log("generating the 8500x3300 rig and frames")
+    rig = hm_synth.config_rig("cfg1")
+    f1 = hm_synth.make_frame(rig.src_h, rig.src_w, 101)[None]
+    f2 = hm_synth.make_frame(rig.src_h, rig.src_w, 202)[None]
+    d1 = torch.from_numpy(f1).cuda()
+    d2 = torch.from_numpy(f2).cuda()
+    stream = torch.cuda.current_stream()
+    results = {}
+    for levels in ([args.levels, 0] if args.levels != 0 else [0]):
+        pano = hm_stitch.StitchPano(1, levels, rig)
+        assert pano.status == 0, pano.status
+        canvas = pano.alloc_canvas()
+        cw, ch = pano.canvas_size()
+        for _ in range(args.warmup):
+            hm_stitch.check(pano.process(d1, d2, canvas, stream), "process")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            hm_stitch.check(pano.process(d1, d2, canvas, stream), "process")
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ms = el * 1e3 / args.steps
+        log(f"levels={levels}: {ms:.4f} ms/frame, {1e3 / ms:.1f} fps")
+        # per-frame latency, one frame in flight (process + synchronize)
+        lat = []
+        for _ in range(args.latency_frames):
+            t = time.perf_counter()
+            pano.process(d1, d2, canvas, stream)
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - t) * 1e3)
+        # per-kernel durations with HIP events on the launch stream
+        pano.set_timing(True)
+        pano.timing_report(reset=True)
+        kt = max(20, args.steps // 3)
+        for _ in range(kt):
+            pano.process(d1, d2, canvas, stream)
+        rep = pano.timing_report(reset=True)
+        pano.set_timing(False)
+        kernels = {k: {"avg_ms": v[0] / v[1], "per_frame_ms": v[0] / kt, "launches": v[1]} for k, v in rep.items()}
+        alg = algorithmic_bytes(rig, levels, len(pano.levels()) if levels else 0)
+        for k, b in alg.items():
+            if k in kernels:
+                kernels[k]["alg_bytes"] = b
+                kernels[k]["achieved_GBps"] = b / (kernels[k]["avg_ms"] * 1e-3) / 1e9
+        results[levels] = dict(ms=ms, lat=np.array(lat), kernels=kernels, canvas=(cw, ch))
+        del pano, canvas
+        torch.cuda.synchronize()
+
+    main_l = args.levels
+    r = results[main_l]
+    cw, ch = r["canvas"]
+    mpix = cw * ch / (r["ms"] * 1e-3) / 1e6
+    # dominant kernel = largest time per frame among those with an algorithmic byte count
+    cands = {k: v for k, v in r["kernels"].items() if "alg_bytes" in v}
+    dom = max(cands, key=lambda k: cands[k]["per_frame_ms"])
+    dk = cands[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(dk["achieved_GBps"], 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(dk["achieved_GBps"] / HBM_PEAK_GBPS, 4), "traffic": None,
+                "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": round(dk["avg_ms"], 5)}
+    line = {
+        "metric": "pano Mpix/s + fps @8500x3300 L=6; p50/p99 frame latency, served vs plain",
+        "value": round(mpix, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded frames, cylindrical uint16 maps, wavy seam)",
+        "config": {"workload": WORKLOAD, "levels": main_l, "canvas": f"{cw}x{ch}", "batch": 1,
+                   "blend_roi": "minimized (overlap + 2x128)"},
+        "fps": round(1e3 / r["ms"], 1),
+        "latency_ms_p50": round(float(np.percentile(r["lat"], 50)), 4),
+        "latency_ms_p99": round(float(np.percentile(r["lat"], 99)), 4),
+        "roofline": roofline,
+        "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in r["kernels"].items()},
+    }
+    if 0 in results and main_l != 0:
+        r0 = results[0]
+        line["levels0"] = {"ms_per_step": round(r0["ms"], 5), "fps": round(1e3 / r0["ms"], 1),
+                           "value_Mpix_s": round(cw * ch / (r0["ms"] * 1e-3) / 1e6, 1),
+                           "latency_ms_p50": round(float(np.percentile(r0["lat"], 50)), 4),
+                           "latency_ms_p99": round(float(np.percentile(r0["lat"], 99)), 4),
+                           "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                                       for k, v in r0["kernels"].items()}}
+    if not args.no_cpu:
+        log("timing the CPU oracle on the same workload")
+        line["cpu_baseline"] = cpu_baseline(rig, f1, f2, main_l)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+// hm_device.hpp -- device-side building blocks for the gfx950 stitch kernels.
+//
+// Pixel arithmetic follows the hm-cupano CUDA sources exactly (order of
+// operations, casts, alpha rules); the whole library is compiled with
+// -ffp-contract=off so no multiply-add is fused.  Index arithmetic is 64-bit
+// throughout (the reference's 32-bit products overflow on batched canvases).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hmdev {
+
+constexpr int kUnmapped = 65535;  // src/cuda/cudaRemap.cu:13
+
+// Pitched, batch-major image: base + (b*h + y)*pitch + x*sizeof(px)
+// (src/cuda/cudaTypes.h:195-230, with 64-bit offsets).
+struct Surf {
+  uint8_t* ptr;
+  uint32_t w, h, pitch;
+};
+
+__device__ __forceinline__ uint8_t* row_ptr(const Surf& s, int b, int y) {
+  return s.ptr + ((size_t)b * s.h + (size_t)y) * (size_t)s.pitch;
+}
+
+// ---- pixel load/store (C = 3 or 4), values carried in a float4 ----------
+template <int C>
+__device__ __forceinline__ float4 ld_px(const float* base, size_t pix) {
+  if constexpr (C == 4) {
+    return *reinterpret_cast<const float4*>(base + pix * 4);
+  } else {
+    const float* p = base + pix * 3;
+    return make_float4(p[0], p[1], p[2], 0.0f);
+  }
+}
+template <int C>
+__device__ __forceinline__ float4 ld_px(const uint8_t* base, size_t pix) {
+  if constexpr (C == 4) {
+    const uchar4 u = *reinterpret_cast<const uchar4*>(base + pix * 4);
+    return make_float4((float)u.x, (float)u.y, (float)u.z, (float)u.w);
+  } else {
+    const uint8_t* p = base + pix * 3;
+    return make_float4((float)p[0], (float)p[1], (float)p[2], 0.0f);
+  }
+}
+template <int C>
+__device__ __forceinline__ void st_px(float* base, size_t pix, float4 v) {
+  if constexpr (C == 4) {
+    *reinterpret_cast<float4*>(base + pix * 4) = v;
+  } else {
+    float* p = base + pix * 3;
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+  }
+}
+
+// round_to_uchar: clamp to [0,255] then truncate (src/cuda/cudaUtils.cuh:18-28).
+__device__ __forceinline__ uint8_t to_u8(float x) {
+  if (x <= 0.0f) return 0;
+  if (x >= 255.0f) return 255;
+  return (uint8_t)x;
+}
+
+// float -> uint8 pixel store with perform_cast (cudaUtils.cuh:118-127, 190-193).
+template <int C>
+__device__ __forceinline__ void st_px_u8(uint8_t* p, float4 v) {
+  if constexpr (C == 4) {
+    *reinterpret_cast<uchar4*>(p) = make_uchar4(to_u8(v.x), to_u8(v.y), to_u8(v.z), to_u8(v.w));
+  } else {
+    p[0] = to_u8(v.x);
+    p[1] = to_u8(v.y);
+    p[2] = to_u8(v.z);
+  }
+}
+
+// ---- the top-left aligned upsample taps (src/cuda/cudaBlend.cu:383-390,
+// :613-623).  For pyramid levels built with (w+1)/2 the reconstruct kernel's
+// clamp of the base tap never changes it, so one form serves both.
+struct Taps {
+  int x0, x1, y0, y1;
+  float dx, dy;
+};
+__device__ __forceinline__ Taps make_taps(int x, int y, int lw, int lh) {
+  Taps t;
+  const float gx = (float)x / 2.0f, gy = (float)y / 2.0f;
+  int gxi = (int)floorf(gx), gyi = (int)floorf(gy);
+  gxi = max(0, min(gxi, lw - 1));
+  gyi = max(0, min(gyi, lh - 1));
+  t.x0 = gxi;
+  t.y0 = gyi;
+  t.x1 = min(gxi + 1, lw - 1);
+  t.y1 = min(gyi + 1, lh - 1);
+  t.dx = gx - (float)gxi;
+  t.dy = gy - (float)gyi;
+  return t;
+}
+
+// Alpha-aware bilinear (cudaBlend.cu:416-444, :649-666): taps 00,10,01,11;
+// with C == 4 a tap whose alpha is 0 is skipped; value = sumV/sumW or 0.
+template <int C>
+__device__ __forceinline__ float4 up_alpha(const float* low, size_t row0, size_t row1, const Taps& t) {
+  const float4 p00 = ld_px<C>(low, row0 + t.x0);
+  const float4 p10 = ld_px<C>(low, row0 + t.x1);
+  const float4 p01 = ld_px<C>(low, row1 + t.x0);
+  const float4 p11 = ld_px<C>(low, row1 + t.x1);
+  const float w00 = (1.0f - t.dx) * (1.0f - t.dy);
+  const float w10 = t.dx * (1.0f - t.dy);
+  const float w01 = (1.0f - t.dx) * t.dy;
+  const float w11 = t.dx * t.dy;
+  float sw = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  if (C != 4 || p00.w != 0.0f) { sw += w00; sx += p00.x * w00; sy += p00.y * w00; sz += p00.z * w00; }
+  if (C != 4 || p10.w != 0.0f) { sw += w10; sx += p10.x * w10; sy += p10.y * w10; sz += p10.z * w10; }
+  if (C != 4 || p01.w != 0.0f) { sw += w01; sx += p01.x * w01; sy += p01.y * w01; sz += p01.z * w01; }
+  if (C != 4 || p11.w != 0.0f) { sw += w11; sx += p11.x * w11; sy += p11.y * w11; sz += p11.z * w11; }
+  if (sw > 0.0f) return make_float4(sx / sw, sy / sw, sz / sw, 0.0f);
+  return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Plain two-stage bilinear of the 3-channel Laplacian (cudaBlend.cu:446-449).
+__device__ __forceinline__ float lerp2(float v00, float v10, float v01, float v11, float dx, float dy) {
+  const float v0 = v00 * (1.0f - dx) + v10 * dx;
+  const float v1 = v01 * (1.0f - dx) + v11 * dx;
+  return v0 * (1.0f - dy) + v1 * dy;
+}
+__device__ __forceinline__ float4 up_plain3(const float* low, size_t row0, size_t row1, const Taps& t) {
+  const float4 p00 = ld_px<3>(low, row0 + t.x0);
+  const float4 p10 = ld_px<3>(low, row0 + t.x1);
+  const float4 p01 = ld_px<3>(low, row1 + t.x0);
+  const float4 p11 = ld_px<3>(low, row1 + t.x1);
+  return make_float4(lerp2(p00.x, p10.x, p01.x, p11.x, t.dx, t.dy), lerp2(p00.y, p10.y, p01.y, p11.y, t.dx, t.dy),
+                     lerp2(p00.z, p10.z, p01.z, p11.z, t.dx, t.dy), 0.0f);
+}
+
+// Upsample used by the Laplacian: alpha-aware for RGBA, plain for RGB.
+template <int C>
+__device__ __forceinline__ float4 up_lap(const float* low, size_t row0, size_t row1, const Taps& t) {
+  if constexpr (C == 4)
+    return up_alpha<4>(low, row0, row1, t);
+  else
+    return up_plain3(low, row0, row1, t);
+}
+
+}  // namespace hmdev

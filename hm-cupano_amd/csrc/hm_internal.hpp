@@ -1,0 +1,138 @@
+// hm_internal.hpp -- host-side pieces shared by the C-ABI entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hm_pyramid_kernels.hpp"
+
+#define HM_RETURN_IF_ERROR(expr)            \
+  do {                                      \
+    hipError_t _e = (expr);                 \
+    if (_e != hipSuccess) return _e;        \
+  } while (0)
+
+namespace hm {
+
+// Device allocation owned by a host object.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  hipError_t alloc(size_t n) {
+    reset();
+    bytes = n;
+    return hipMalloc(&p, n ? n : 1);
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Optional per-launch event timing on the launch stream.  When enabled every
+// recorded launch is bracketed by two events; collect() sums the elapsed
+// times per kernel name.
+class LaunchTimer {
+ public:
+  ~LaunchTimer();
+  void enable(bool on) { enabled_ = on; }
+  bool enabled() const { return enabled_; }
+  void begin(hipStream_t s);
+  void end(hipStream_t s, const char* name);
+  // Synchronizes the recorded events and accumulates into totals.
+  hipError_t collect();
+  void reset_totals() { totals_.clear(); }
+  std::string report() const;
+
+ private:
+  struct Pending {
+    hipEvent_t a, b;
+    std::string name;
+  };
+  bool enabled_ = false;
+  hipEvent_t cur_begin_ = nullptr;
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  std::map<std::string, std::pair<double, long>> totals_;
+  hipEvent_t get_event();
+};
+
+// Dispatchers over the kernel templates (hm_pyramid.hip).
+hipError_t launch_down(int c, int n, bool in_u8, const void* const* in, float* const* out, int in_w, int in_h,
+                       int out_w, int out_h, int batch, hipStream_t s);
+hipError_t launch_mask_down(const float* in, int in_w, int in_h, float* out, int out_w, int out_h, int m,
+                            hipStream_t s);
+
+struct UpLaunch {
+  const void* g_hi[hmdev::kMaxImages] = {};
+  const float* g_lo[hmdev::kMaxImages] = {};
+  const float* mask = nullptr;
+  const float* r_lo = nullptr;
+  float* lap_out[hmdev::kMaxImages] = {};
+  float* blend_out = nullptr;
+  float* r_out = nullptr;
+  uint8_t* u8_out = nullptr;
+  size_t u8_pitch = 0;
+  int u8_w = 0, u8_h = 0, ox = 0, oy = 0;
+  int hi_w = 0, hi_h = 0, lo_w = 0, lo_h = 0;
+  int win_x = 0, win_y = 0, win_w = 0, win_h = 0;
+};
+hipError_t launch_up(int c, int n, int kind, bool hi_u8, const UpLaunch& a, int batch, hipStream_t s);
+
+// Where the level-0 reconstruction goes.
+struct Level0Out {
+  float* f32 = nullptr;   // tight float output (blend-context API)
+  uint8_t* u8 = nullptr;  // pitched uint8 canvas (pano classes)
+  size_t u8_pitch = 0;
+  int u8_w = 0, u8_h = 0, ox = 0, oy = 0;
+  int win_x = 0, win_y = 0, win_w = -1, win_h = -1;  // level-0 pixels evaluated (-1: all)
+};
+
+// A Laplacian-blend pyramid for N images of one geometry.  Levels >= 1 are
+// owned; level 0 (images and mask) is supplied by the caller.
+class Pyramid {
+ public:
+  // kind: hmdev::kBlend2 (1-channel mask, level count unclamped),
+  //       hmdev::kBlend3 or hmdev::kBlendN (N-channel mask, stop at a side < 2).
+  hipError_t init(int kind, int n, int channels, int width, int height, int levels, int batch, bool debug);
+  hipError_t build_mask_pyramid(const float* mask0, hipStream_t s);
+  // g0: n level-0 images (uint8 if g0_u8 else float), tight, batch-major.
+  hipError_t run(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s, LaunchTimer* timer);
+
+  int kind = 0, n = 0, c = 0, levels = 0, batch = 0;
+  bool debug = false;
+  int W[hmdev::kMaxLevels] = {}, H[hmdev::kMaxLevels] = {};
+  const float* mask0 = nullptr;
+  DevBuf g[hmdev::kMaxImages][hmdev::kMaxLevels];  // level >= 1
+  DevBuf mask[hmdev::kMaxLevels];                   // level >= 1
+  DevBuf r[hmdev::kMaxLevels];                      // level >= 1
+  DevBuf lap[hmdev::kMaxImages][hmdev::kMaxLevels]; // debug
+  DevBuf blend[hmdev::kMaxLevels];                  // debug
+  DevBuf r0_debug;                                  // debug copy of level-0 reconstruction
+  size_t allocation_size = 0;
+  int mask_channels() const { return kind == hmdev::kBlend2 ? 1 : n; }
+};
+
+// Level sizes: the 2-image rule (src/cuda/cudaBlend.cu:860-866) or the
+// N/3-image rule that stops when a side < 2 (src/cuda/cudaBlendN.cu:675-691).
+int pyramid_sizes(int w, int h, int levels, bool clamp_small_sides, int* widths, int* heights);
+
+inline dim3 grid_for(int w, int h, int batch, dim3 block) {
+  return dim3((unsigned)((w + (int)block.x - 1) / (int)block.x), (unsigned)((h + (int)block.y - 1) / (int)block.y),
+              (unsigned)batch);
+}
+
+}  // namespace hm

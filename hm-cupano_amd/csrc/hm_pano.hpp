@@ -1,0 +1,47 @@
+// hm_pano.hpp -- state shared by the 2-, 3- and N-image pano objects.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "hm_internal.hpp"
+#include "hm_stitch.h"
+
+struct hmStitchPano {
+  virtual ~hmStitchPano() = default;
+
+  int status = hipSuccess;  // sticky, like CudaStitchPano::status_
+  int n_images = 0;
+  int batch = 1;
+  int num_levels = 0;
+  int channels = 4;
+  int canvas_w = 0, canvas_h = 0;
+  bool hard_seam = false;
+  bool debug = false;
+  bool initialized_pyramid = false;
+
+  hm::Pyramid pyr;                       // soft-seam pyramid (levels >= 1 owned)
+  hm::DevBuf g0[hmdev::kMaxImages];      // level-0 images, uint8 tight, resident
+  hm::DevBuf mask0;                      // level-0 mask, float, resident
+  hm::LaunchTimer timer;
+
+  hm::LaunchTimer* timer_or_null() { return timer.enabled() ? &timer : nullptr; }
+
+  // Allocates the pyramid and the resident mask pyramid for a bw x bh blend
+  // region; mask_host is bw x bh x M float.
+  hipError_t init_pyramid(int kind, int bw, int bh, const std::vector<float>& mask_host);
+};
+
+namespace hm {
+
+// Packs a CV_16U map pair into one 32-bit word per pixel (x | y << 16) so the
+// remap reads one dword per lane; the values are unchanged.
+hipError_t upload_packed_map(const uint16_t* map_x, const uint16_t* map_y, int w, int h, DevBuf& dst);
+
+// cv::copyMakeBorder(BORDER_REPLICATE) on the right/bottom edges, as
+// CanvasManager::convertMaskMat (src/pano/canvasManager.cpp:63-101).
+bool pad_replicate(const uint8_t* src, int sw, int sh, int cw, int ch, std::vector<uint8_t>& out);
+
+bool surface_ok(const hmSurface& s, int channels, bool need_vec_align);
+
+}  // namespace hm

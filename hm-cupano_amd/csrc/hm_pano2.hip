@@ -1,0 +1,397 @@
+// hm_pano2.hip -- CudaStitchPano (2-image) on gfx950.
+//
+// Reference: src/pano/cudaPano.inl (ctor :19-94, process_impl :167-418),
+// src/pano/canvasManager.cpp (:16-101).  The reference runs, per frame,
+// memset(canvas, full1, full2) -> remap1 -> make_full1 -> remap2 -> make_full2
+// -> 34-launch Laplacian blend -> copy-back (about 44 stream operations).
+// Here one kernel produces, from the two remaps, both the canvas outside the
+// blend region and the level-0 blend inputs (kept as exact uint8), the
+// pyramid runs one down launch and one fused up launch per level, and the
+// level-0 up launch writes the truncated result straight into the canvas.
+// The hard seam (levels == 0) is one launch.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "hm_pano.hpp"
+
+using namespace hmdev;
+
+namespace {
+
+constexpr int kOverlapPad = 128;  // CanvasManager overlap_pad default (canvasManager.h:65)
+const dim3 kBlock(64, 4, 1);
+
+template <int C>
+__device__ __forceinline__ uint32_t load_src_px(const Surf& s, int b, int x, int y) {
+  const uint8_t* p = row_ptr(s, b, y) + (size_t)x * C;
+  if constexpr (C == 4) return *reinterpret_cast<const uint32_t*>(p);
+  return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16);
+}
+
+template <int C>
+__device__ __forceinline__ void store_px(uint8_t* p, uint32_t v) {
+  if constexpr (C == 4) {
+    *reinterpret_cast<uint32_t*>(p) = v;
+  } else {
+    p[0] = (uint8_t)v;
+    p[1] = (uint8_t)(v >> 8);
+    p[2] = (uint8_t)(v >> 16);
+  }
+}
+
+// BatchedRemapKernelExOffset pixel (src/cuda/cudaRemap.cu:95-122), default
+// pixel all-zero, no_unmapped_write = false: in range -> source pixel, else
+// zero with alpha 0 for the 65535 sentinel and 255 otherwise.
+template <int C>
+__device__ __forceinline__ uint32_t remap_px(const Surf& s, int b, uint32_t m) {
+  const uint32_t sx = m & 0xffffu, sy = m >> 16;
+  if (sx < s.w && sy < s.h) return load_src_px<C>(s, b, (int)sx, (int)sy);
+  if constexpr (C == 4) return sx == (uint32_t)kUnmapped ? 0u : 0xff000000u;
+  return 0u;
+}
+
+// BatchedRemapKernelExOffsetWithDestMap pixel (cudaRemap.cu:235-244):
+// out-of-range or unmapped -> the all-zero default pixel.
+template <int C>
+__device__ __forceinline__ uint32_t remap_px_dm(const Surf& s, int b, uint32_t m) {
+  const uint32_t sx = m & 0xffffu, sy = m >> 16;
+  if (sx < s.w && sy < s.h) return load_src_px<C>(s, b, (int)sx, (int)sy);
+  return 0u;
+}
+
+struct Pano2L0 {
+  Surf src[2];
+  const uint32_t* map[2];
+  int mw[2], mh[2], px[2], py[2];
+  uint8_t* canvas;
+  uint32_t cpitch;
+  int cw, ch;
+  int write_canvas, skip_x0, skip_x1;  // canvas columns [skip_x0, skip_x1) are left to the blend write-back
+  uint8_t* g0[2];
+  int bw, bh;
+  int d[2], ulo[2], uhi[2];  // G_i[u] <- canvas column u + d[i] for u in [ulo[i], uhi[i])
+  int g2_canvas;             // 1: G2 takes the canvas after both remaps; 0: image 2 alone
+};
+
+// One thread per canvas pixel.  c1 = canvas after remap 1, c12 = canvas
+// after remap 2 (which overwrites image 2's whole box, cudaPano.inl:291-303).
+template <int C>
+__global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int b = blockIdx.z;
+  if (X >= a.cw || Y >= a.ch) return;
+  uint32_t r[2] = {0u, 0u};
+  bool valid[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int rx = X - a.px[i], ry = Y - a.py[i];
+    valid[i] = rx >= 0 && ry >= 0 && rx < a.mw[i] && ry < a.mh[i];
+    if (valid[i]) r[i] = remap_px<C>(a.src[i], b, a.map[i][(size_t)ry * a.mw[i] + rx]);
+  }
+  const uint32_t c1 = valid[0] ? r[0] : 0u;
+  const uint32_t c12 = valid[1] ? r[1] : c1;
+  if (a.write_canvas && !(X >= a.skip_x0 && X < a.skip_x1))
+    store_px<C>(a.canvas + ((size_t)b * a.ch + Y) * a.cpitch + (size_t)X * C, c12);
+  if (Y < a.bh) {
+    const int u1 = X - a.d[0];
+    if (u1 >= a.ulo[0] && u1 < a.uhi[0]) store_px<C>(a.g0[0] + (((size_t)b * a.bh + Y) * a.bw + u1) * C, c1);
+    const int u2 = X - a.d[1];
+    if (u2 >= a.ulo[1] && u2 < a.uhi[1])
+      store_px<C>(a.g0[1] + (((size_t)b * a.bh + Y) * a.bw + u2) * C, a.g2_canvas ? c12 : (valid[1] ? r[1] : 0u));
+  }
+}
+
+struct Pano2Hard {
+  Surf src[2];
+  const uint32_t* map[2];
+  int mw[2], mh[2], px[2], py[2];
+  uint8_t* canvas;
+  uint32_t cpitch;
+  int cw, ch;
+  const uint8_t* seam;  // canvas-sized, 1 selects image 1, 0 image 2
+};
+
+// Hard seam: the two dest-map remaps of cudaPano.inl:262-276 and :345-359
+// (image 1 where seam == 1, image 2 where seam == 0) plus the canvas memset,
+// as one pass.
+template <int C>
+__global__ void __launch_bounds__(256) pano2_hard_kernel(Pano2Hard a) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int b = blockIdx.z;
+  if (X >= a.cw || Y >= a.ch) return;
+  const uint8_t s = a.seam[(size_t)Y * a.cw + X];
+  uint32_t v = 0u;
+  const int i = s == 1 ? 0 : (s == 0 ? 1 : -1);
+  if (i >= 0) {
+    const int rx = X - a.px[i], ry = Y - a.py[i];
+    if (rx >= 0 && ry >= 0 && rx < a.mw[i] && ry < a.mh[i])
+      v = remap_px_dm<C>(a.src[i], b, a.map[i][(size_t)ry * a.mw[i] + rx]);
+  }
+  store_px<C>(a.canvas + ((size_t)b * a.ch + Y) * a.cpitch + (size_t)X * C, v);
+}
+
+Surf to_surf(const hmSurface& s) { return Surf{static_cast<uint8_t*>(s.d_ptr), s.width, s.height, s.pitch}; }
+
+}  // namespace
+
+// ===========================================================================
+// shared helpers
+// ===========================================================================
+namespace hm {
+
+hipError_t upload_packed_map(const uint16_t* map_x, const uint16_t* map_y, int w, int h, DevBuf& dst) {
+  if (!map_x || !map_y || w < 1 || h < 1) return hipErrorInvalidValue;
+  std::vector<uint32_t> packed((size_t)w * h);
+  for (size_t i = 0; i < packed.size(); ++i) packed[i] = (uint32_t)map_x[i] | ((uint32_t)map_y[i] << 16);
+  HM_RETURN_IF_ERROR(dst.alloc(packed.size() * sizeof(uint32_t)));
+  return hipMemcpy(dst.p, packed.data(), packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+}
+
+bool pad_replicate(const uint8_t* src, int sw, int sh, int cw, int ch, std::vector<uint8_t>& out) {
+  if (!src || sw < 1 || sh < 1 || sw > cw || sh > ch) return false;
+  out.resize((size_t)cw * ch);
+  for (int y = 0; y < ch; ++y) {
+    const uint8_t* row = src + (size_t)std::min(y, sh - 1) * sw;
+    for (int x = 0; x < cw; ++x) out[(size_t)y * cw + x] = row[std::min(x, sw - 1)];
+  }
+  return true;
+}
+
+bool surface_ok(const hmSurface& s, int channels, bool need_vec_align) {
+  if (!s.d_ptr || s.width == 0 || s.height == 0) return false;
+  if (s.pitch < s.width * (uint32_t)channels) return false;
+  if (need_vec_align && channels == 4 && ((reinterpret_cast<uintptr_t>(s.d_ptr) & 3u) || (s.pitch & 3u)))
+    return false;
+  return true;
+}
+
+}  // namespace hm
+
+hipError_t hmStitchPano::init_pyramid(int kind, int bw, int bh, const std::vector<float>& mask_host) {
+  HM_RETURN_IF_ERROR(pyr.init(kind, n_images, channels, bw, bh, num_levels, batch, debug));
+  HM_RETURN_IF_ERROR(mask0.alloc(mask_host.size() * sizeof(float)));
+  HM_RETURN_IF_ERROR(hipMemcpy(mask0.p, mask_host.data(), mask_host.size() * sizeof(float), hipMemcpyHostToDevice));
+  // The mask pyramid depends only on the seam: built once and kept resident
+  // (the reference rebuilds it every frame with the same kernel arithmetic).
+  HM_RETURN_IF_ERROR(pyr.build_mask_pyramid(mask0.as<float>(), nullptr));
+  HM_RETURN_IF_ERROR(hipStreamSynchronize(nullptr));
+  initialized_pyramid = true;
+  return hipSuccess;
+}
+
+// ===========================================================================
+// CudaStitchPano
+// ===========================================================================
+namespace {
+
+struct Pano2 : hmStitchPano {
+  int x1 = 0, y1 = 0, x2 = 0, y2 = 0;
+  int mw[2] = {0, 0}, mh[2] = {0, 0};
+  bool minimize = false;
+  int bw = 0, bh = 0;
+  hm::DevBuf map[2];
+  hm::DevBuf seam;  // hard seam, canvas-sized
+  std::vector<float> mask_host;
+  Pano2L0 l0{};
+
+  hipError_t setup(const hmControlMasks2& m, int minimize_blend);
+  hipError_t ensure_pyramid();
+  hipError_t process(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s);
+};
+
+hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
+  if (!m.img1_col || !m.img1_row || !m.img2_col || !m.img2_row || !m.whole_seam_mask_image || m.img1_w < 1 ||
+      m.img1_h < 1 || m.img2_w < 1 || m.img2_h < 1)
+    return hipErrorFileNotFound;  // ControlMasks::is_valid() failed (cudaPano.inl:28-31)
+  hard_seam = num_levels == 0;
+  // ControlMasks::canvas_width/height in float, then size_t (controlMasks.cpp:265-271).
+  canvas_w = (int)(size_t)std::max(m.xpos[0] + (float)m.img1_w, m.xpos[1] + (float)m.img2_w);
+  canvas_h = (int)(size_t)std::max(m.ypos[0] + (float)m.img1_h, m.ypos[1] + (float)m.img2_h);
+  // CanvasInfo positions are cv::Point (int) (cudaPano.inl:47-52).
+  x1 = (int)m.xpos[0];
+  y1 = (int)m.ypos[0];
+  x2 = (int)m.xpos[1];
+  y2 = (int)m.ypos[1];
+  mw[0] = m.img1_w;
+  mh[0] = m.img1_h;
+  mw[1] = m.img2_w;
+  mh[1] = m.img2_h;
+  minimize = minimize_blend && !hard_seam;
+
+  std::vector<uint8_t> padded;
+  if (!hm::pad_replicate(m.whole_seam_mask_image, m.seam_w, m.seam_h, canvas_w, canvas_h, padded))
+    return hipErrorInvalidValue;
+  HM_RETURN_IF_ERROR(hm::upload_packed_map(m.img1_col, m.img1_row, mw[0], mh[0], map[0]));
+  HM_RETURN_IF_ERROR(hm::upload_packed_map(m.img2_col, m.img2_row, mw[1], mh[1], map[1]));
+
+  if (hard_seam) {
+    HM_RETURN_IF_ERROR(seam.alloc(padded.size()));
+    return hipMemcpy(seam.p, padded.data(), padded.size(), hipMemcpyHostToDevice);
+  }
+
+  // CanvasManager::updateMinimizeBlend (canvasManager.cpp:16-61).
+  const int overlap = mw[0] - x2;
+  const int blend_width = overlap + 2 * kOverlapPad;
+  int bx0 = 0;
+  l0 = Pano2L0{};
+  if (minimize) {
+    // The reference asserts these (canvasManager.cpp:29, 51-52); the first
+    // ROI must also start inside the canvas.
+    if (overlap <= 0 || x2 - kOverlapPad < 0 || mw[0] + kOverlapPad > canvas_w || x2 - kOverlapPad - x1 < 0)
+      return hipErrorInvalidValue;
+    bx0 = x2 - kOverlapPad;
+    bw = blend_width;
+    bh = canvas_h;
+    const int roi_w = blend_width - kOverlapPad;
+    // full1[x1 + x] <- canvas[(x2 - 128 - x1) + x]; full2[x1 + 128 + x] <- canvas[x2 + x]
+    // (make_full calls, cudaPano.inl:225-236 and :306-317).
+    l0.d[0] = (x2 - kOverlapPad - x1) - x1;
+    l0.ulo[0] = std::max(0, x1);
+    l0.uhi[0] = std::min(bw, x1 + roi_w);
+    l0.d[1] = x2 - (x1 + kOverlapPad);
+    l0.ulo[1] = std::max(0, x1 + kOverlapPad);
+    l0.uhi[1] = std::min(bw, x1 + kOverlapPad + roi_w);
+    l0.g2_canvas = 1;
+    l0.write_canvas = 1;
+    l0.skip_x0 = bx0;
+    l0.skip_x1 = bx0 + bw;
+  } else {
+    bw = canvas_w;
+    bh = canvas_h;
+    l0.d[0] = l0.d[1] = 0;
+    l0.ulo[0] = l0.ulo[1] = 0;
+    l0.uhi[0] = l0.uhi[1] = canvas_w;
+    l0.g2_canvas = 0;
+    l0.write_canvas = 0;
+  }
+  // convertMaskMat crop + convertTo(float) (cudaPano.inl:62-77).
+  mask_host.resize((size_t)bw * bh);
+  for (int y = 0; y < bh; ++y)
+    for (int x = 0; x < bw; ++x) mask_host[(size_t)y * bw + x] = (float)padded[(size_t)y * canvas_w + bx0 + x];
+  for (int i = 0; i < 2; ++i) {
+    HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+    // Columns the per-frame kernel never writes are the zero of the
+    // reference's per-frame memset; clear them once.
+    HM_RETURN_IF_ERROR(hipMemset(g0[i].p, 0, g0[i].bytes));
+  }
+  return hipSuccess;
+}
+
+hipError_t Pano2::ensure_pyramid() {
+  if (hard_seam || initialized_pyramid) return hipSuccess;
+  return init_pyramid(kBlend2, bw, bh, mask_host);
+}
+
+hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s) {
+  if (!hm::surface_ok(in1, channels, true) || !hm::surface_ok(in2, channels, true) ||
+      !hm::surface_ok(canvas, channels, true))
+    return hipErrorInvalidValue;
+  if ((int)canvas.width != canvas_w || (int)canvas.height != canvas_h) return hipErrorInvalidValue;
+  const Surf s1 = to_surf(in1), s2 = to_surf(in2);
+  const dim3 grid = hm::grid_for(canvas_w, canvas_h, batch, kBlock);
+  if (hard_seam) {
+    Pano2Hard a{};
+    a.src[0] = s1;
+    a.src[1] = s2;
+    for (int i = 0; i < 2; ++i) {
+      a.map[i] = map[i].as<uint32_t>();
+      a.mw[i] = mw[i];
+      a.mh[i] = mh[i];
+    }
+    a.px[0] = x1;
+    a.py[0] = y1;
+    a.px[1] = x2;
+    a.py[1] = y2;
+    a.canvas = static_cast<uint8_t*>(canvas.d_ptr);
+    a.cpitch = canvas.pitch;
+    a.cw = canvas_w;
+    a.ch = canvas_h;
+    a.seam = seam.as<uint8_t>();
+    timer.begin(s);
+    if (channels == 4)
+      hipLaunchKernelGGL(pano2_hard_kernel<4>, grid, kBlock, 0, s, a);
+    else
+      hipLaunchKernelGGL(pano2_hard_kernel<3>, grid, kBlock, 0, s, a);
+    timer.end(s, "hard_seam");
+    return hipGetLastError();
+  }
+  HM_RETURN_IF_ERROR(ensure_pyramid());
+  Pano2L0 a = l0;
+  a.src[0] = s1;
+  a.src[1] = s2;
+  for (int i = 0; i < 2; ++i) {
+    a.map[i] = map[i].as<uint32_t>();
+    a.mw[i] = mw[i];
+    a.mh[i] = mh[i];
+    a.g0[i] = g0[i].as<uint8_t>();
+  }
+  a.px[0] = x1;
+  a.py[0] = y1;
+  a.px[1] = x2;
+  a.py[1] = y2;
+  a.canvas = static_cast<uint8_t*>(canvas.d_ptr);
+  a.cpitch = canvas.pitch;
+  a.cw = canvas_w;
+  a.ch = canvas_h;
+  a.bw = bw;
+  a.bh = bh;
+  timer.begin(s);
+  if (channels == 4)
+    hipLaunchKernelGGL(pano2_level0_kernel<4>, grid, kBlock, 0, s, a);
+  else
+    hipLaunchKernelGGL(pano2_level0_kernel<3>, grid, kBlock, 0, s, a);
+  timer.end(s, "remap_l0");
+  HM_RETURN_IF_ERROR(hipGetLastError());
+
+  const void* g[2] = {g0[0].p, g0[1].p};
+  hm::Level0Out out;
+  out.u8 = static_cast<uint8_t*>(canvas.d_ptr);
+  out.u8_pitch = canvas.pitch;
+  out.u8_w = canvas_w;
+  out.u8_h = canvas_h;
+  out.ox = minimize ? x2 - kOverlapPad : 0;  // copy-back offset (cudaPano.inl:408)
+  out.oy = 0;
+  return pyr.run(g, true, out, s, timer_or_null());
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" int hmStitchPanoCreate(int batch_size, int num_levels, const hmControlMasks2* masks, int quiet,
+                                  int minimize_blend, int max_output_width, int channels, hmStitchPano** out) {
+  (void)quiet;
+  (void)max_output_width;  // unused by the reference too (cudaPano.inl:27)
+  if (!out) return hipErrorInvalidValue;
+  auto* p = new Pano2();
+  *out = p;
+  p->n_images = 2;
+  p->batch = batch_size;
+  p->num_levels = num_levels;
+  p->channels = channels;
+  if (!masks) {
+    p->status = hipErrorFileNotFound;
+  } else if (batch_size < 1 || num_levels < 0 || num_levels > hmdev::kMaxLevels || (channels != 3 && channels != 4)) {
+    p->status = hipErrorInvalidValue;
+  } else {
+    p->status = p->setup(*masks, minimize_blend);
+  }
+  return p->status;
+}
+
+extern "C" int hmStitchPanoProcess(hmStitchPano* pano, hmSurface input1, hmSurface input2, hmSurface canvas,
+                                   hipStream_t stream) {
+  if (!pano) return hipErrorInvalidValue;
+  if (pano->status != hipSuccess) return pano->status;
+  auto* p = dynamic_cast<Pano2*>(pano);
+  if (!p) return hipErrorInvalidValue;
+  const hipError_t e = p->process(input1, input2, canvas, stream);
+  if (e != hipSuccess) pano->status = e;  // CudaStitchPano::process makes errors sticky (cudaPano.inl:468-470)
+  return e;
+}

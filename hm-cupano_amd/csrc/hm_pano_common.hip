@@ -1,0 +1,105 @@
+// hm_pano_common.hip -- C-ABI shared by every pano object: status, sizes,
+// pyramid inspection and launch timing.
+#include <string.h>
+
+#include <vector>
+
+#include "hm_pano.hpp"
+
+namespace {
+
+__global__ void u8_to_f32_kernel(const uint8_t* in, float* out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
+}  // namespace
+
+extern "C" int hmStitchPanoStatus(const hmStitchPano* pano) { return pano ? pano->status : hipErrorInvalidValue; }
+
+extern "C" int hmStitchPanoCanvasSize(const hmStitchPano* pano, int* width, int* height) {
+  if (!pano || !width || !height) return hipErrorInvalidValue;
+  *width = pano->canvas_w;
+  *height = pano->canvas_h;
+  return hipSuccess;
+}
+
+extern "C" int hmStitchPanoBatchSize(const hmStitchPano* pano) { return pano ? pano->batch : 0; }
+
+extern "C" void hmStitchPanoDestroy(hmStitchPano* pano) { delete pano; }
+
+extern "C" int hmStitchPanoSetDebug(hmStitchPano* pano, int enable) {
+  if (!pano) return hipErrorInvalidValue;
+  if (pano->initialized_pyramid) return hipErrorNotSupported;  // must precede the first process
+  pano->debug = enable != 0;
+  return hipSuccess;
+}
+
+extern "C" int hmStitchPanoLevels(const hmStitchPano* pano, int* num_levels, int* widths, int* heights) {
+  if (!pano || !num_levels) return hipErrorInvalidValue;
+  if (!pano->initialized_pyramid) {
+    *num_levels = 0;
+    return hipSuccess;
+  }
+  *num_levels = pano->pyr.levels;
+  for (int l = 0; l < pano->pyr.levels; ++l) {
+    if (widths) widths[l] = pano->pyr.W[l];
+    if (heights) heights[l] = pano->pyr.H[l];
+  }
+  return hipSuccess;
+}
+
+extern "C" int hmStitchPanoReadLevel(hmStitchPano* pano, int what, int image, int level, void* host_dst, size_t bytes,
+                                     hipStream_t stream) {
+  if (!pano || !host_dst || !pano->initialized_pyramid) return hipErrorInvalidValue;
+  const hm::Pyramid& p = pano->pyr;
+  if (level < 0 || level >= p.levels || image < 0 || image >= p.n) return hipErrorInvalidValue;
+  const size_t px = (size_t)p.W[level] * p.H[level];
+  const size_t img_floats = px * p.c * p.batch;
+  const void* src = nullptr;
+  size_t need = img_floats * sizeof(float);
+  hm::DevBuf tmp;
+  HM_RETURN_IF_ERROR(hipStreamSynchronize(stream));
+  switch (what) {
+    case HM_LEVEL_GAUSS:
+      if (level == 0) {
+        HM_RETURN_IF_ERROR(tmp.alloc(need));
+        const size_t n = img_floats;
+        hipLaunchKernelGGL(u8_to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                           pano->g0[image].as<uint8_t>(), tmp.as<float>(), n);
+        HM_RETURN_IF_ERROR(hipStreamSynchronize(stream));
+        src = tmp.p;
+      } else {
+        src = p.g[image][level].p;
+      }
+      break;
+    case HM_LEVEL_MASK:
+      need = px * p.mask_channels() * sizeof(float);
+      src = level ? p.mask[level].p : pano->mask0.p;
+      break;
+    case HM_LEVEL_LAPLACIAN: src = p.debug ? p.lap[image][level].p : nullptr; break;
+    case HM_LEVEL_BLEND: src = p.debug ? p.blend[level].p : nullptr; break;
+    case HM_LEVEL_RECONSTRUCT: src = level ? p.r[level].p : (p.debug ? p.r0_debug.p : nullptr); break;
+    default: return hipErrorInvalidValue;
+  }
+  if (!src) return hipErrorNotSupported;
+  if (bytes < need) return hipErrorInvalidValue;
+  return hipMemcpy(host_dst, src, need, hipMemcpyDeviceToHost);
+}
+
+extern "C" int hmStitchPanoSetTiming(hmStitchPano* pano, int enable) {
+  if (!pano) return hipErrorInvalidValue;
+  pano->timer.enable(enable != 0);
+  return hipSuccess;
+}
+
+extern "C" int hmStitchPanoTimingReport(hmStitchPano* pano, char* buf, size_t len, int reset) {
+  if (!pano || !buf || len == 0) return hipErrorInvalidValue;
+  HM_RETURN_IF_ERROR(pano->timer.collect());
+  const std::string r = pano->timer.report();
+  const size_t n = r.size() < len - 1 ? r.size() : len - 1;
+  memcpy(buf, r.data(), n);
+  buf[n] = '\0';
+  if (reset) pano->timer.reset_totals();
+  return hipSuccess;
+}

@@ -1,0 +1,403 @@
+// hm_pyramid.hip -- pyramid engine: kernel dispatch, level management, the
+// kernel-tier blend-context C-ABI and launch timing.
+#include <stdio.h>
+
+#include <sstream>
+
+#include "hm_internal.hpp"
+#include "hm_stitch.h"
+
+namespace hm {
+
+using namespace hmdev;
+
+static const dim3 kBlock(64, 4, 1);  // one 64-lane wavefront per row segment
+
+int pyramid_sizes(int w, int h, int levels, bool clamp_small_sides, int* widths, int* heights) {
+  if (levels < 1 || levels > kMaxLevels || w < 1 || h < 1) return -1;
+  widths[0] = w;
+  heights[0] = h;
+  for (int i = 1; i < levels; ++i) {
+    if (clamp_small_sides && (widths[i - 1] < 2 || heights[i - 1] < 2)) return i;
+    widths[i] = (widths[i - 1] + 1) / 2;
+    heights[i] = (heights[i - 1] + 1) / 2;
+  }
+  return levels;
+}
+
+// Mask pyramid: plain mean of the in-range taps over M interleaved channels
+// (cudaBlend.cu:268-296; cudaBlendN.cu:19-48; cudaBlend3.cu:220-262).
+static __global__ void __launch_bounds__(256) mask_down_kernel(const float* in, int in_w, int in_h, float* out, int out_w,
+                                                        int out_h, int m) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= out_w || y >= out_h) return;
+  float sums[kMaxImages];
+  for (int k = 0; k < m; ++k) sums[k] = 0.0f;
+  int count = 0;
+  for (int dy = 0; dy < 2; ++dy)
+    for (int dx = 0; dx < 2; ++dx) {
+      const int ix = 2 * x + dx, iy = 2 * y + dy;
+      if (ix < in_w && iy < in_h) {
+        const float* p = in + ((size_t)iy * in_w + ix) * m;
+        for (int k = 0; k < m; ++k) sums[k] += p[k];
+        ++count;
+      }
+    }
+  float* o = out + ((size_t)y * out_w + x) * m;
+  for (int k = 0; k < m; ++k) o[k] = sums[k] / (float)count;
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <typename TIn, int C, int N>
+static hipError_t down_t(const void* const* in, float* const* out, int in_w, int in_h, int out_w, int out_h,
+                         int batch, hipStream_t s) {
+  DownArgs<TIn, N> a;
+  for (int i = 0; i < N; ++i) {
+    a.in[i] = static_cast<const TIn*>(in[i]);
+    a.out[i] = out[i];
+  }
+  a.in_w = in_w;
+  a.in_h = in_h;
+  a.out_w = out_w;
+  a.out_h = out_h;
+  hipLaunchKernelGGL((down_kernel<TIn, C, N>), grid_for(out_w, out_h, batch, kBlock), kBlock, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename TIn, int C>
+static hipError_t down_c(int n, const void* const* in, float* const* out, int in_w, int in_h, int out_w, int out_h,
+                         int batch, hipStream_t s) {
+  switch (n) {
+    case 2: return down_t<TIn, C, 2>(in, out, in_w, in_h, out_w, out_h, batch, s);
+    case 3: return down_t<TIn, C, 3>(in, out, in_w, in_h, out_w, out_h, batch, s);
+    case 4: return down_t<TIn, C, 4>(in, out, in_w, in_h, out_w, out_h, batch, s);
+    case 5: return down_t<TIn, C, 5>(in, out, in_w, in_h, out_w, out_h, batch, s);
+    case 6: return down_t<TIn, C, 6>(in, out, in_w, in_h, out_w, out_h, batch, s);
+    case 7: return down_t<TIn, C, 7>(in, out, in_w, in_h, out_w, out_h, batch, s);
+    case 8: return down_t<TIn, C, 8>(in, out, in_w, in_h, out_w, out_h, batch, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_down(int c, int n, bool in_u8, const void* const* in, float* const* out, int in_w, int in_h,
+                       int out_w, int out_h, int batch, hipStream_t s) {
+  if (c == 4) return in_u8 ? down_c<uint8_t, 4>(n, in, out, in_w, in_h, out_w, out_h, batch, s)
+                           : down_c<float, 4>(n, in, out, in_w, in_h, out_w, out_h, batch, s);
+  if (c == 3) return in_u8 ? down_c<uint8_t, 3>(n, in, out, in_w, in_h, out_w, out_h, batch, s)
+                           : down_c<float, 3>(n, in, out, in_w, in_h, out_w, out_h, batch, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_mask_down(const float* in, int in_w, int in_h, float* out, int out_w, int out_h, int m,
+                            hipStream_t s) {
+  if (m < 1 || m > kMaxImages) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(mask_down_kernel, grid_for(out_w, out_h, 1, kBlock), kBlock, 0, s, in, in_w, in_h, out, out_w,
+                     out_h, m);
+  return hipGetLastError();
+}
+
+template <typename THi, int C, int N, int KIND>
+static hipError_t up_t(const UpLaunch& u, int batch, hipStream_t s) {
+  UpArgs<THi, N> a;
+  for (int i = 0; i < N; ++i) {
+    a.g_hi[i] = static_cast<const THi*>(u.g_hi[i]);
+    a.g_lo[i] = u.g_lo[i];
+    a.lap_out[i] = u.lap_out[i];
+  }
+  a.mask = u.mask;
+  a.r_lo = u.r_lo;
+  a.blend_out = u.blend_out;
+  a.r_out = u.r_out;
+  a.u8_out = u.u8_out;
+  a.u8_pitch = u.u8_pitch;
+  a.u8_w = u.u8_w;
+  a.u8_h = u.u8_h;
+  a.ox = u.ox;
+  a.oy = u.oy;
+  a.hi_w = u.hi_w;
+  a.hi_h = u.hi_h;
+  a.lo_w = u.lo_w;
+  a.lo_h = u.lo_h;
+  a.win_x = u.win_x;
+  a.win_y = u.win_y;
+  a.win_w = u.win_w;
+  a.win_h = u.win_h;
+  if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
+  hipLaunchKernelGGL((up_kernel<THi, C, N, KIND>), grid_for(u.win_w, u.win_h, batch, kBlock), kBlock, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename THi, int C>
+static hipError_t up_c(int n, int kind, const UpLaunch& u, int batch, hipStream_t s) {
+  if (kind == kBlend2) return n == 2 ? up_t<THi, C, 2, kBlend2>(u, batch, s) : hipErrorInvalidValue;
+  if (kind == kBlend3) return n == 3 ? up_t<THi, C, 3, kBlend3>(u, batch, s) : hipErrorInvalidValue;
+  switch (n) {
+    case 2: return up_t<THi, C, 2, kBlendN>(u, batch, s);
+    case 3: return up_t<THi, C, 3, kBlendN>(u, batch, s);
+    case 4: return up_t<THi, C, 4, kBlendN>(u, batch, s);
+    case 5: return up_t<THi, C, 5, kBlendN>(u, batch, s);
+    case 6: return up_t<THi, C, 6, kBlendN>(u, batch, s);
+    case 7: return up_t<THi, C, 7, kBlendN>(u, batch, s);
+    case 8: return up_t<THi, C, 8, kBlendN>(u, batch, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_up(int c, int n, int kind, bool hi_u8, const UpLaunch& u, int batch, hipStream_t s) {
+  if (c == 4) return hi_u8 ? up_c<uint8_t, 4>(n, kind, u, batch, s) : up_c<float, 4>(n, kind, u, batch, s);
+  if (c == 3) return hi_u8 ? up_c<uint8_t, 3>(n, kind, u, batch, s) : up_c<float, 3>(n, kind, u, batch, s);
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// launch timing
+// ---------------------------------------------------------------------------
+LaunchTimer::~LaunchTimer() {
+  for (auto& p : pending_) {
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  if (cur_begin_) (void)hipEventDestroy(cur_begin_);
+}
+
+hipEvent_t LaunchTimer::get_event() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void LaunchTimer::begin(hipStream_t s) {
+  if (!enabled_) return;
+  cur_begin_ = get_event();
+  (void)hipEventRecord(cur_begin_, s);
+}
+
+void LaunchTimer::end(hipStream_t s, const char* name) {
+  if (!enabled_ || !cur_begin_) return;
+  hipEvent_t e = get_event();
+  (void)hipEventRecord(e, s);
+  pending_.push_back(Pending{cur_begin_, e, name});
+  cur_begin_ = nullptr;
+}
+
+hipError_t LaunchTimer::collect() {
+  for (auto& p : pending_) {
+    HM_RETURN_IF_ERROR(hipEventSynchronize(p.b));
+    float ms = 0.0f;
+    HM_RETURN_IF_ERROR(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& t = totals_[p.name];
+    t.first += ms;
+    t.second += 1;
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+  return hipSuccess;
+}
+
+std::string LaunchTimer::report() const {
+  std::ostringstream os;
+  for (const auto& kv : totals_) os << kv.first << " " << kv.second.first << " " << kv.second.second << "\n";
+  return os.str();
+}
+
+// ---------------------------------------------------------------------------
+// Pyramid
+// ---------------------------------------------------------------------------
+hipError_t Pyramid::init(int kind_, int n_, int channels, int width, int height, int levels_req, int batch_,
+                         bool debug_) {
+  if ((kind_ != kBlend2 && kind_ != kBlend3 && kind_ != kBlendN) || n_ < 2 || n_ > kMaxImages ||
+      (channels != 3 && channels != 4) || batch_ < 1)
+    return hipErrorInvalidValue;
+  if (kind_ == kBlend2 && n_ != 2) return hipErrorInvalidValue;
+  if (kind_ == kBlend3 && n_ != 3) return hipErrorInvalidValue;
+  kind = kind_;
+  n = n_;
+  c = channels;
+  batch = batch_;
+  debug = debug_;
+  levels = pyramid_sizes(width, height, levels_req, kind != kBlend2, W, H);
+  if (levels < 1) return hipErrorInvalidValue;
+  allocation_size = 0;
+  for (int l = 0; l < levels; ++l) {
+    const size_t px = (size_t)W[l] * H[l];
+    const size_t img = px * c * batch * sizeof(float);
+    if (l > 0) {
+      for (int i = 0; i < n; ++i) HM_RETURN_IF_ERROR(g[i][l].alloc(img));
+      HM_RETURN_IF_ERROR(mask[l].alloc(px * mask_channels() * sizeof(float)));
+      HM_RETURN_IF_ERROR(r[l].alloc(img));
+      allocation_size += n * img + img + px * mask_channels() * sizeof(float);
+    }
+    if (debug) {
+      for (int i = 0; i < n; ++i) HM_RETURN_IF_ERROR(lap[i][l].alloc(img));
+      HM_RETURN_IF_ERROR(blend[l].alloc(img));
+      allocation_size += (n + 1) * img;
+    }
+  }
+  if (debug) {
+    HM_RETURN_IF_ERROR(r0_debug.alloc((size_t)W[0] * H[0] * c * batch * sizeof(float)));
+    allocation_size += r0_debug.bytes;
+  }
+  return hipSuccess;
+}
+
+hipError_t Pyramid::build_mask_pyramid(const float* mask0_, hipStream_t s) {
+  mask0 = mask0_;
+  const float* prev = mask0;
+  for (int l = 0; l + 1 < levels; ++l) {
+    HM_RETURN_IF_ERROR(
+        launch_mask_down(prev, W[l], H[l], mask[l + 1].as<float>(), W[l + 1], H[l + 1], mask_channels(), s));
+    prev = mask[l + 1].as<float>();
+  }
+  return hipSuccess;
+}
+
+hipError_t Pyramid::run(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s, LaunchTimer* timer) {
+  // 1. Down pass: every image's level k -> k+1 in one launch per level.
+  for (int l = 0; l + 1 < levels; ++l) {
+    const void* in[kMaxImages];
+    float* o[kMaxImages];
+    for (int i = 0; i < n; ++i) {
+      in[i] = l == 0 ? g0[i] : g[i][l].p;
+      o[i] = g[i][l + 1].as<float>();
+    }
+    if (timer) timer->begin(s);
+    HM_RETURN_IF_ERROR(launch_down(c, n, l == 0 && g0_u8, in, o, W[l], H[l], W[l + 1], H[l + 1], batch, s));
+    if (timer) timer->end(s, l == 0 ? "down_l0" : "down");
+  }
+  // 2. Up pass from the coarsest level: Laplacian + blend + reconstruct fused.
+  for (int l = levels - 1; l >= 0; --l) {
+    UpLaunch u;
+    const bool coarsest = l == levels - 1;
+    for (int i = 0; i < n; ++i) {
+      u.g_hi[i] = l == 0 ? g0[i] : g[i][l].p;
+      u.g_lo[i] = coarsest ? nullptr : g[i][l + 1].as<float>();
+      u.lap_out[i] = debug ? lap[i][l].as<float>() : nullptr;
+    }
+    u.mask = l == 0 ? mask0 : mask[l].as<float>();
+    u.r_lo = coarsest ? nullptr : r[l + 1].as<float>();
+    u.blend_out = debug ? blend[l].as<float>() : nullptr;
+    u.hi_w = W[l];
+    u.hi_h = H[l];
+    u.lo_w = coarsest ? 0 : W[l + 1];
+    u.lo_h = coarsest ? 0 : H[l + 1];
+    u.win_x = 0;
+    u.win_y = 0;
+    u.win_w = W[l];
+    u.win_h = H[l];
+    if (l > 0) {
+      u.r_out = r[l].as<float>();
+    } else {
+      if (debug) {
+        // Full level-0 evaluation into the debug buffers only; the real
+        // output is written by the (possibly windowed) launch below.
+        u.r_out = r0_debug.as<float>();
+        HM_RETURN_IF_ERROR(launch_up(c, n, kind, g0_u8, u, batch, s));
+        for (int i = 0; i < n; ++i) u.lap_out[i] = nullptr;
+        u.blend_out = nullptr;
+      }
+      u.r_out = out.f32;
+      u.u8_out = out.u8;
+      u.u8_pitch = out.u8_pitch;
+      u.u8_w = out.u8_w;
+      u.u8_h = out.u8_h;
+      u.ox = out.ox;
+      u.oy = out.oy;
+      if (out.win_w >= 0) {
+        u.win_x = out.win_x;
+        u.win_y = out.win_y;
+        u.win_w = out.win_w;
+        u.win_h = out.win_h;
+      }
+    }
+    if (timer) timer->begin(s);
+    HM_RETURN_IF_ERROR(launch_up(c, n, kind, l == 0 && g0_u8, u, batch, s));
+    if (timer) timer->end(s, l == 0 ? "up_l0" : "up");
+  }
+  return hipSuccess;
+}
+
+}  // namespace hm
+
+// ===========================================================================
+// Kernel-tier C-ABI: Laplacian blend contexts
+// ===========================================================================
+struct hmBlendContext {
+  hm::Pyramid pyr;
+  int width = 0, height = 0;
+  hm::LaunchTimer timer;
+};
+
+extern "C" int hmBlendContextCreate(int kind, int n_images, int width, int height, int num_levels, int batch_size,
+                                    int channels, int debug, hmBlendContext** out) {
+  if (!out) return hipErrorInvalidValue;
+  *out = nullptr;
+  if (kind != HM_BLEND_TWO && kind != HM_BLEND_THREE && kind != HM_BLEND_N) return hipErrorInvalidValue;
+  if (width < 1 || height < 1 || num_levels < 1) return hipErrorInvalidValue;
+  auto* ctx = new hmBlendContext();
+  const int k = kind == HM_BLEND_TWO ? hmdev::kBlend2 : (kind == HM_BLEND_THREE ? hmdev::kBlend3 : hmdev::kBlendN);
+  hipError_t e = ctx->pyr.init(k, n_images, channels, width, height, num_levels, batch_size, debug != 0);
+  if (e != hipSuccess) {
+    delete ctx;
+    return e;
+  }
+  ctx->width = width;
+  ctx->height = height;
+  *out = ctx;
+  return hipSuccess;
+}
+
+extern "C" int hmBlendContextLevels(const hmBlendContext* ctx, int* num_levels, int* widths, int* heights) {
+  if (!ctx || !num_levels) return hipErrorInvalidValue;
+  *num_levels = ctx->pyr.levels;
+  for (int l = 0; l < ctx->pyr.levels; ++l) {
+    if (widths) widths[l] = ctx->pyr.W[l];
+    if (heights) heights[l] = ctx->pyr.H[l];
+  }
+  return hipSuccess;
+}
+
+extern "C" int hmBlendRun(hmBlendContext* ctx, const float* const* d_images, const float* d_mask, float* d_output,
+                          hipStream_t stream) {
+  if (!ctx || !d_images || !d_mask || !d_output) return hipErrorInvalidValue;
+  const int n = ctx->pyr.n;
+  const void* g0[hmdev::kMaxImages];
+  for (int i = 0; i < n; ++i) {
+    if (!d_images[i]) return hipErrorInvalidValue;
+    if (ctx->pyr.c == 4 && (reinterpret_cast<uintptr_t>(d_images[i]) & 15u)) return hipErrorInvalidValue;
+    g0[i] = d_images[i];
+  }
+  if (ctx->pyr.c == 4 && (reinterpret_cast<uintptr_t>(d_output) & 15u)) return hipErrorInvalidValue;
+  // The reference recomputes the mask pyramid on every call (cudaBlend.cu:912-923).
+  HM_RETURN_IF_ERROR(ctx->pyr.build_mask_pyramid(d_mask, stream));
+  hm::Level0Out out;
+  out.f32 = d_output;
+  return ctx->pyr.run(g0, false, out, stream, ctx->timer.enabled() ? &ctx->timer : nullptr);
+}
+
+extern "C" int hmBlendContextLevelPtr(const hmBlendContext* ctx, int what, int image, int level, const float** ptr) {
+  if (!ctx || !ptr || level < 0 || level >= ctx->pyr.levels) return hipErrorInvalidValue;
+  const hm::Pyramid& p = ctx->pyr;
+  if (image < 0 || image >= p.n) image = 0;
+  *ptr = nullptr;
+  switch (what) {
+    case HM_LEVEL_GAUSS: *ptr = level ? p.g[image][level].as<float>() : nullptr; break;
+    case HM_LEVEL_MASK: *ptr = level ? p.mask[level].as<float>() : p.mask0; break;
+    case HM_LEVEL_LAPLACIAN: *ptr = p.debug ? p.lap[image][level].as<float>() : nullptr; break;
+    case HM_LEVEL_BLEND: *ptr = p.debug ? p.blend[level].as<float>() : nullptr; break;
+    case HM_LEVEL_RECONSTRUCT: *ptr = level ? p.r[level].as<float>() : (p.debug ? p.r0_debug.as<float>() : nullptr); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipSuccess;
+}
+
+extern "C" void hmBlendContextDestroy(hmBlendContext* ctx) { delete ctx; }

@@ -1,0 +1,310 @@
+"""Python binding of libhm_stitch.so (the C ABI declared in include/hm_stitch.h).
+
+A thin ctypes layer for tests, the benchmark and Python callers.  Device
+memory and streams come from torch (plain plumbing): pass torch uint8/float32
+CUDA tensors shaped [B, H, W, C] (rows may be padded: the row stride becomes
+the surface pitch) and the current torch stream is used unless one is given.
+
+Nothing here falls back to a CPU path: if the shared library is missing or a
+call fails, a RuntimeError is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhm_stitch.so")
+
+HM_PIX_U8C3, HM_PIX_U8C4, HM_PIX_F32C3, HM_PIX_F32C4 = 0, 1, 2, 3
+HM_BLEND_TWO, HM_BLEND_THREE, HM_BLEND_N = 2, 3, 0
+HM_LEVEL_GAUSS, HM_LEVEL_MASK, HM_LEVEL_LAPLACIAN, HM_LEVEL_BLEND, HM_LEVEL_RECONSTRUCT = range(5)
+
+
+class HmError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what} failed with hipError {code}")
+        self.code = code
+
+
+class hmSurface(ctypes.Structure):
+    _fields_ = [("d_ptr", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("pitch", ctypes.c_uint32)]
+
+
+class hmControlMasks2(ctypes.Structure):
+    _fields_ = [
+        ("img1_col", ctypes.c_void_p), ("img1_row", ctypes.c_void_p), ("img1_w", ctypes.c_int), ("img1_h", ctypes.c_int),
+        ("img2_col", ctypes.c_void_p), ("img2_row", ctypes.c_void_p), ("img2_w", ctypes.c_int), ("img2_h", ctypes.c_int),
+        ("whole_seam_mask_image", ctypes.c_void_p), ("seam_w", ctypes.c_int), ("seam_h", ctypes.c_int),
+        ("xpos", ctypes.c_float * 2), ("ypos", ctypes.c_float * 2),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libhm_stitch.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, u32, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_float
+    S = hmSurface
+    sig = {
+        "hmRemapExOffset": (i, [S, i, S, i, vp, vp, i, i, i, i, i, i, vp]),
+        "hmRemapExOffsetRoi": (i, [S, i, S, i, vp, vp, i, i, i, i, i, i, i, i, i, i, vp]),
+        "hmRemapExOffsetWithDestMap": (i, [S, S, i, vp, vp, i, vp, i, i, i, i, i, vp]),
+        "hmRemapHardSeamN": (i, [i, ctypes.POINTER(S), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i),
+                                 ctypes.POINTER(i), vp, S, i, i, vp]),
+        "hmCopyRoi": (i, [S, i, i, i, i, i, S, i, i, i, i, vp]),
+        "hmMakeFull": (i, [S, i, i, i, i, i, i, i, i, S, i, vp]),
+        "hmBlendContextCreate": (i, [i, i, i, i, i, i, i, i, ctypes.POINTER(vp)]),
+        "hmBlendContextLevels": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "hmBlendRun": (i, [vp, ctypes.POINTER(vp), vp, vp, vp]),
+        "hmBlendContextLevelPtr": (i, [vp, i, i, i, ctypes.POINTER(vp)]),
+        "hmBlendContextDestroy": (None, [vp]),
+        "hmStitchPanoCreate": (i, [i, i, ctypes.POINTER(hmControlMasks2), i, i, i, i, ctypes.POINTER(vp)]),
+        "hmStitchPanoProcess": (i, [vp, S, S, S, vp]),
+        "hmStitchPanoStatus": (i, [vp]),
+        "hmStitchPanoCanvasSize": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "hmStitchPanoBatchSize": (i, [vp]),
+        "hmStitchPanoDestroy": (None, [vp]),
+        "hmStitchPanoSetDebug": (i, [vp, i]),
+        "hmStitchPanoLevels": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "hmStitchPanoReadLevel": (i, [vp, i, i, i, vp, ctypes.c_size_t, vp]),
+        "hmStitchPanoSetTiming": (i, [vp, i]),
+        "hmStitchPanoTimingReport": (i, [vp, ctypes.c_char_p, ctypes.c_size_t, i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(code: int, what: str) -> None:
+    if code != 0:
+        raise HmError(code, what)
+
+
+# ---------------------------------------------------------------------------
+# torch plumbing
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch
+
+    return torch
+
+
+def surface(t, channels: Optional[int] = None) -> hmSurface:
+    """hmSurface of a [B, H, W, C] (or [H, W, C]) CUDA tensor; rows may be padded."""
+    if t.dim() == 3:
+        t = t.unsqueeze(0)
+    if t.dim() != 4:
+        raise ValueError("expected a [B, H, W, C] tensor")
+    b, h, w, c = t.shape
+    es = t.element_size()
+    if t.stride(3) != 1 or t.stride(2) != c:
+        raise ValueError("pixels must be packed (stride(2) == C, stride(3) == 1)")
+    if b > 1 and t.stride(0) != t.stride(1) * h:
+        raise ValueError("batch items must be contiguous in rows")
+    return hmSurface(t.data_ptr(), w, h, t.stride(1) * es)
+
+
+def current_stream() -> int:
+    return _torch().cuda.current_stream().cuda_stream
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        return current_stream()
+    return stream if isinstance(stream, int) else stream.cuda_stream
+
+
+def _u16(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint16)
+
+
+# ---------------------------------------------------------------------------
+# pano classes
+# ---------------------------------------------------------------------------
+class StitchPano:
+    """CudaStitchPano<uchar4|uchar3, float4|float3> (src/pano/cudaPano.h:68-115)."""
+
+    def __init__(self, batch_size: int, num_levels: int, rig, minimize_blend: bool = True, channels: int = 4,
+                 debug: bool = False):
+        L = lib()
+        self._keep = [_u16(rig.map_x[0]), _u16(rig.map_y[0]), _u16(rig.map_x[1]), _u16(rig.map_y[1]),
+                      np.ascontiguousarray(rig.seam, dtype=np.uint8)]
+        m = hmControlMasks2()
+        m.img1_col, m.img1_row = self._keep[0].ctypes.data, self._keep[1].ctypes.data
+        m.img1_h, m.img1_w = self._keep[0].shape
+        m.img2_col, m.img2_row = self._keep[2].ctypes.data, self._keep[3].ctypes.data
+        m.img2_h, m.img2_w = self._keep[2].shape
+        m.whole_seam_mask_image = self._keep[4].ctypes.data
+        m.seam_h, m.seam_w = self._keep[4].shape
+        m.xpos[0], m.xpos[1] = float(rig.pos_x[0]), float(rig.pos_x[1])
+        m.ypos[0], m.ypos[1] = float(rig.pos_y[0]), float(rig.pos_y[1])
+        self._masks = m
+        h = ctypes.c_void_p()
+        rc = L.hmStitchPanoCreate(batch_size, num_levels, ctypes.byref(m), 1, int(minimize_blend), 0, channels,
+                                  ctypes.byref(h))
+        if not h.value:
+            raise HmError(rc, "hmStitchPanoCreate")
+        self._h = h
+        self.batch_size = batch_size
+        self.channels = channels
+        self.num_levels = num_levels
+        if debug:
+            check(L.hmStitchPanoSetDebug(h, 1), "hmStitchPanoSetDebug")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hmStitchPanoDestroy(h)
+            self._h = None
+
+    @property
+    def status(self) -> int:
+        return lib().hmStitchPanoStatus(self._h)
+
+    def canvas_size(self):
+        w, h = ctypes.c_int(), ctypes.c_int()
+        check(lib().hmStitchPanoCanvasSize(self._h, ctypes.byref(w), ctypes.byref(h)), "hmStitchPanoCanvasSize")
+        return w.value, h.value
+
+    def alloc_canvas(self, device="cuda"):
+        torch = _torch()
+        w, h = self.canvas_size()
+        return torch.empty((self.batch_size, h, w, self.channels), dtype=torch.uint8, device=device)
+
+    def process(self, in1, in2, canvas, stream=None) -> int:
+        return lib().hmStitchPanoProcess(self._h, surface(in1), surface(in2), surface(canvas), _stream(stream))
+
+    def levels(self):
+        n = ctypes.c_int()
+        ws = (ctypes.c_int * 16)()
+        hs = (ctypes.c_int * 16)()
+        check(lib().hmStitchPanoLevels(self._h, ctypes.byref(n), ws, hs), "hmStitchPanoLevels")
+        return [(ws[i], hs[i]) for i in range(n.value)]
+
+    def read_level(self, what: int, image: int, level: int, stream=None) -> np.ndarray:
+        lv = self.levels()
+        w, h = lv[level]
+        if what == HM_LEVEL_MASK:
+            shape = (h, w, 1 if self._mask_channels() == 1 else self._mask_channels())
+        else:
+            shape = (self.batch_size, h, w, self.channels)
+        out = np.empty(shape, dtype=np.float32)
+        check(lib().hmStitchPanoReadLevel(self._h, what, image, level, out.ctypes.data, out.nbytes, _stream(stream)),
+              "hmStitchPanoReadLevel")
+        return out
+
+    def _mask_channels(self) -> int:
+        return 1
+
+    def set_timing(self, on: bool) -> None:
+        check(lib().hmStitchPanoSetTiming(self._h, int(on)), "hmStitchPanoSetTiming")
+
+    def timing_report(self, reset: bool = True) -> dict:
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().hmStitchPanoTimingReport(self._h, buf, 4096, int(reset)), "hmStitchPanoTimingReport")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, ms, cnt = line.split()
+            out[name] = (float(ms), int(cnt))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# kernel tier
+# ---------------------------------------------------------------------------
+class BlendContext:
+    """CudaBatchLaplacianBlendContext{,3,N} + cudaBatchedLaplacianBlendWithContext{,3,N}."""
+
+    def __init__(self, kind: int, n_images: int, width: int, height: int, num_levels: int, batch_size: int,
+                 channels: int = 4, debug: bool = False):
+        h = ctypes.c_void_p()
+        check(lib().hmBlendContextCreate(kind, n_images, width, height, num_levels, batch_size, channels, int(debug),
+                                         ctypes.byref(h)), "hmBlendContextCreate")
+        self._h = h
+        self.n = n_images
+        self.kind = kind
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hmBlendContextDestroy(h)
+            self._h = None
+
+    def levels(self):
+        n = ctypes.c_int()
+        ws = (ctypes.c_int * 16)()
+        hs = (ctypes.c_int * 16)()
+        check(lib().hmBlendContextLevels(self._h, ctypes.byref(n), ws, hs), "hmBlendContextLevels")
+        return [(ws[i], hs[i]) for i in range(n.value)]
+
+    def run(self, images: Sequence, mask, out, stream=None) -> None:
+        arr = (ctypes.c_void_p * len(images))(*[t.data_ptr() for t in images])
+        check(lib().hmBlendRun(self._h, arr, mask.data_ptr(), out.data_ptr(), _stream(stream)), "hmBlendRun")
+
+    def level_ptr(self, what: int, image: int, level: int) -> int:
+        p = ctypes.c_void_p()
+        check(lib().hmBlendContextLevelPtr(self._h, what, image, level, ctypes.byref(p)), "hmBlendContextLevelPtr")
+        return p.value or 0
+
+
+def remap_ex_offset(src, src_type, dest, dest_type, map_x, map_y, batch, remap_w, remap_h, off_x, off_y,
+                    no_unmapped_write=False, roi=None, stream=None) -> int:
+    L = lib()
+    if roi is None:
+        return L.hmRemapExOffset(surface(src), src_type, surface(dest), dest_type, map_x.data_ptr(), map_y.data_ptr(),
+                                 batch, remap_w, remap_h, off_x, off_y, int(no_unmapped_write), _stream(stream))
+    return L.hmRemapExOffsetRoi(surface(src), src_type, surface(dest), dest_type, map_x.data_ptr(), map_y.data_ptr(),
+                                batch, remap_w, remap_h, off_x, off_y, roi[0], roi[1], roi[2], roi[3],
+                                int(no_unmapped_write), _stream(stream))
+
+
+def remap_dest_map(src, dest, pix_type, map_x, map_y, image_index, dest_map, batch, remap_w, remap_h, off_x, off_y,
+                   stream=None) -> int:
+    return lib().hmRemapExOffsetWithDestMap(surface(src), surface(dest), pix_type, map_x.data_ptr(), map_y.data_ptr(),
+                                            image_index, dest_map.data_ptr(), batch, remap_w, remap_h, off_x, off_y,
+                                            _stream(stream))
+
+
+def remap_hard_seam_n(inputs, maps_x, maps_y, offsets, sizes, dest_map, dest, pix_type, batch, stream=None) -> int:
+    n = len(inputs)
+    surfs = (hmSurface * n)(*[surface(t) for t in inputs])
+    mx = (ctypes.c_void_p * n)(*[m.data_ptr() for m in maps_x])
+    my = (ctypes.c_void_p * n)(*[m.data_ptr() for m in maps_y])
+    off = (ctypes.c_int * (2 * n))(*[v for o in offsets for v in o])
+    sz = (ctypes.c_int * (2 * n))(*[v for s in sizes for v in s])
+    return lib().hmRemapHardSeamN(n, surfs, mx, my, off, sz, dest_map.data_ptr(), surface(dest), pix_type, batch,
+                                  _stream(stream))
+
+
+def copy_roi(src, src_type, region_w, region_h, sx, sy, dest, dest_type, ox, oy, batch, stream=None) -> int:
+    return lib().hmCopyRoi(surface(src), src_type, region_w, region_h, sx, sy, surface(dest), dest_type, ox, oy, batch,
+                           _stream(stream))
+
+
+def make_full(src, src_type, region_w, region_h, sx, sy, ox, oy, batch, dest, dest_type, stream=None) -> int:
+    return lib().hmMakeFull(surface(src), src_type, region_w, region_h, sx, sy, ox, oy, batch, surface(dest),
+                            dest_type, _stream(stream))
+
+
+def exported_symbols() -> List[str]:
+    """Names declared in include/hm_stitch.h (for the export test)."""
+    import re
+
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "hm_stitch.h")
+    text = open(hdr).read()
+    return sorted(set(re.findall(r"^(?:int|void)\s+(hm\w+)\s*\(", text, flags=re.M)))

@@ -1,0 +1,191 @@
+/*
+ * hm_stitch.h -- C ABI of the MI355X (gfx950) stitch library libhm_stitch.so.
+ *
+ * This is the drop-in boundary for hm-cupano's single-frame stitch path.  It
+ * replaces two tiers of the reference (paths relative to the hm-cupano tree):
+ *
+ *   tier 1, kernel host functions  src/cuda/cudaRemap.h, cudaMakeFull.h,
+ *           cudaBlend.h, cudaBlendN.h, cudaBlend3.h   -> hmRemap*, hmCopyRoi,
+ *           hmMakeFull, hmBlend*
+ *   tier 2, pano classes           src/pano/cudaPano.h, cudaPanoN.h,
+ *           cudaPano3.h                                -> hmStitchPano*
+ *
+ * The C++ classes with the reference names (hm::pano::cuda::CudaStitchPano,
+ * CudaStitchPanoN, CudaStitchPano3, hm::CudaMat, CudaStatus ...) are
+ * header-only wrappers over this ABI (hm-cupano_amd/include/cupano/).
+ *
+ * Conventions
+ *   - every function returns an int holding a hipError_t value (0 = success);
+ *     launchers are asynchronous on the given stream, like the reference's;
+ *   - device memory is passed as plain pointers (hmSurface for pitched
+ *     images); host arrays (maps, seams) are plain host pointers;
+ *   - pixel types: pipeline images are uint8 BGRA (4 channels) or BGR
+ *     (3 channels); compute images are float with the same channel count.
+ */
+#ifndef HM_STITCH_H
+#define HM_STITCH_H
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Pitched, batch-major device image: pixel (b,x,y) lives at
+ * d_ptr + (b*height + y)*pitch + x*sizeof(pixel).
+ * Replaces CudaSurface<T> (src/cuda/cudaTypes.h:195-201). */
+typedef struct hmSurface {
+  void* d_ptr;
+  uint32_t width;
+  uint32_t height;
+  uint32_t pitch; /* bytes */
+} hmSurface;
+
+/* Pixel types of the template instantiations the reference ships
+ * (src/cuda/cudaRemap.cu:507-554). */
+typedef enum hmPixelType {
+  HM_PIX_U8C3 = 0,  /* uchar3 */
+  HM_PIX_U8C4 = 1,  /* uchar4 */
+  HM_PIX_F32C3 = 2, /* float3 */
+  HM_PIX_F32C4 = 3  /* float4 */
+} hmPixelType;
+
+/* ------------------------------------------------------------------------- */
+/* Tier 1: remap and ROI copy                                                */
+/* ------------------------------------------------------------------------- */
+
+/* batched_remap_kernel_ex_offset (src/cuda/cudaRemap.h:41-54,
+ * cudaRemap.cu:326-344).  src must be a uint8 type; the default pixel is
+ * all-zero (T_in{}), as every pano class passes. */
+int hmRemapExOffset(hmSurface src, hmPixelType src_type, hmSurface dest, hmPixelType dest_type,
+                    const uint16_t* d_map_x, const uint16_t* d_map_y, int batch_size, int remap_w, int remap_h,
+                    int offset_x, int offset_y, int no_unmapped_write, hipStream_t stream);
+
+/* batched_remap_kernel_ex_offset_roi (src/cuda/cudaRemap.h:72-89,
+ * cudaRemap.cu:347-390), including its ROI validation. */
+int hmRemapExOffsetRoi(hmSurface src, hmPixelType src_type, hmSurface dest, hmPixelType dest_type,
+                       const uint16_t* d_map_x, const uint16_t* d_map_y, int batch_size, int remap_w, int remap_h,
+                       int offset_x, int offset_y, int roi_x, int roi_y, int roi_w, int roi_h,
+                       int no_unmapped_write, hipStream_t stream);
+
+/* batched_remap_kernel_ex_offset_with_dest_map (src/cuda/cudaRemap.h:91-105,
+ * cudaRemap.cu:396-415).  dest_image_map is unpitched, dest.width wide. */
+int hmRemapExOffsetWithDestMap(hmSurface src, hmSurface dest, hmPixelType type, const uint16_t* d_map_x,
+                               const uint16_t* d_map_y, int this_image_index, const uint8_t* d_dest_image_map,
+                               int batch_size, int remap_w, int remap_h, int offset_x, int offset_y,
+                               hipStream_t stream);
+
+/* batched_remap_hard_seam_kernel_n (src/cuda/cudaRemap.h:117-128,
+ * cudaRemap.cu:418-437).  The per-image arrays are HOST arrays (their
+ * contents travel in the kernel arguments; no per-frame H2D copy). */
+int hmRemapHardSeamN(int n_images, const hmSurface* inputs, const uint16_t* const* d_map_x,
+                     const uint16_t* const* d_map_y, const int* offsets_xy, const int* sizes_wh,
+                     const uint8_t* d_dest_image_map, hmSurface dest, hmPixelType type, int batch_size,
+                     hipStream_t stream);
+
+/* copy_roi_batched (src/cuda/cudaMakeFull.h:61-86, cudaMakeFull.cu:241-258):
+ * typed ROI copy, float -> uint8 truncating. */
+int hmCopyRoi(hmSurface src, hmPixelType src_type, int region_w, int region_h, int src_roi_x, int src_roi_y,
+              hmSurface dest, hmPixelType dest_type, int offset_x, int offset_y, int batch_size,
+              hipStream_t stream);
+
+/* simple_make_full_batch (src/cuda/cudaMakeFull.h:32-46,
+ * cudaMakeFull.cu:173-214): zero the destination, then hmCopyRoi. */
+int hmMakeFull(hmSurface src, hmPixelType src_type, int region_w, int region_h, int src_roi_x, int src_roi_y,
+               int dest_offset_x, int dest_offset_y, int batch_size, hmSurface dest, hmPixelType dest_type,
+               hipStream_t stream);
+
+/* ------------------------------------------------------------------------- */
+/* Tier 1: Laplacian blend contexts                                          */
+/* ------------------------------------------------------------------------- */
+typedef struct hmBlendContext hmBlendContext;
+
+typedef enum hmBlendKind {
+  HM_BLEND_TWO = 2,   /* CudaBatchLaplacianBlendContext      (src/cuda/cudaBlend.h:38-141)  */
+  HM_BLEND_THREE = 3, /* CudaBatchLaplacianBlendContext3     (src/cuda/cudaBlend3.h:45-161) */
+  HM_BLEND_N = 0      /* CudaBatchLaplacianBlendContextN<N>  (src/cuda/cudaBlendN.h:15-99)  */
+} hmBlendKind;
+
+typedef enum hmLevelKind {
+  HM_LEVEL_GAUSS = 0,
+  HM_LEVEL_MASK = 1,
+  HM_LEVEL_LAPLACIAN = 2,
+  HM_LEVEL_BLEND = 3,
+  HM_LEVEL_RECONSTRUCT = 4
+} hmLevelKind;
+
+/* Context constructor (cudaBlend.h:52-65, cudaBlendN.h:15-99,
+ * cudaBlend3.h:45-161).  All levels are allocated here, not on first use.
+ * debug != 0 also keeps the Laplacian and blended pyramids for dumps. */
+int hmBlendContextCreate(int kind, int n_images, int width, int height, int num_levels, int batch_size, int channels,
+                         int debug, hmBlendContext** out);
+/* Actual level count and sizes (the N/3-image rule may lower the count). */
+int hmBlendContextLevels(const hmBlendContext* ctx, int* num_levels, int* widths, int* heights);
+/* cudaBatchedLaplacianBlendWithContext (src/cuda/cudaBlend.h:237-245),
+ * ...WithContextN (cudaBlendN.h:182-188), ...WithContext3 (cudaBlend3.h).
+ * d_images is a HOST array of n device pointers (tight B x H x W x C float);
+ * d_mask is H x W x M float (M = 1, 3 or N); d_output may alias d_images[0]. */
+int hmBlendRun(hmBlendContext* ctx, const float* const* d_images, const float* d_mask, float* d_output,
+               hipStream_t stream);
+/* Device pointer of one pyramid level (for dumps); NULL when not kept. */
+int hmBlendContextLevelPtr(const hmBlendContext* ctx, int what, int image, int level, const float** ptr);
+void hmBlendContextDestroy(hmBlendContext* ctx);
+
+/* ------------------------------------------------------------------------- */
+/* Tier 2: pano classes                                                      */
+/* ------------------------------------------------------------------------- */
+
+/* In-memory ControlMasks (src/pano/controlMasks.h:30-66): uint16 CV_16U
+ * remap tables (host, tight), the seam AFTER load_seam_mask's inversion
+ * (controlMasks.cpp:225-251: 1 selects image 1), and the SpatialTiff
+ * positions (already normalised, controlMasks.cpp:42-59). */
+typedef struct hmControlMasks2 {
+  const uint16_t* img1_col;
+  const uint16_t* img1_row;
+  int img1_w, img1_h;
+  const uint16_t* img2_col;
+  const uint16_t* img2_row;
+  int img2_w, img2_h;
+  const uint8_t* whole_seam_mask_image;
+  int seam_w, seam_h;
+  float xpos[2], ypos[2];
+} hmControlMasks2;
+
+typedef struct hmStitchPano hmStitchPano;
+
+/* CudaStitchPano<uchar4|uchar3, float4|float3> ctor (src/pano/cudaPano.h:71-77,
+ * cudaPano.inl:19-94).  channels = 4 (uchar4/float4) or 3 (uchar3/float3).
+ * Like the reference, a failing constructor still returns an object whose
+ * status is sticky; *out is NULL only on allocation failure of the object. */
+int hmStitchPanoCreate(int batch_size, int num_levels, const hmControlMasks2* masks, int quiet, int minimize_blend,
+                       int max_output_width, int channels, hmStitchPano** out);
+/* CudaStitchPano::process (cudaPano.h:95-99, cudaPano.inl:460-472). */
+int hmStitchPanoProcess(hmStitchPano* pano, hmSurface input1, hmSurface input2, hmSurface canvas,
+                        hipStream_t stream);
+int hmStitchPanoStatus(const hmStitchPano* pano);
+int hmStitchPanoCanvasSize(const hmStitchPano* pano, int* width, int* height);
+int hmStitchPanoBatchSize(const hmStitchPano* pano);
+void hmStitchPanoDestroy(hmStitchPano* pano);
+
+/* ------------------------------------------------------------------------- */
+/* Pyramid inspection and launch timing (all pano classes)                   */
+/* ------------------------------------------------------------------------- */
+/* Keep the Laplacian/blend pyramids (call before the first process). */
+int hmStitchPanoSetDebug(hmStitchPano* pano, int enable);
+int hmStitchPanoLevels(const hmStitchPano* pano, int* num_levels, int* widths, int* heights);
+/* Copy one level to host memory (synchronises the stream), the data that
+ * dump_soft_blend_pyramid writes (cudaPano.inl:420-458).  Level-0 Gaussians
+ * are converted from the resident uint8 form to float. */
+int hmStitchPanoReadLevel(hmStitchPano* pano, int what, int image, int level, void* host_dst, size_t bytes,
+                          hipStream_t stream);
+/* Per-launch event timing on the process stream. */
+int hmStitchPanoSetTiming(hmStitchPano* pano, int enable);
+/* Writes "name total_ms count\n" lines accumulated since the last reset. */
+int hmStitchPanoTimingReport(hmStitchPano* pano, char* buf, size_t len, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HM_STITCH_H */
