@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-frames", type=int, default=200)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (scale for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the stitch path shards by rig/frame with no collectives, "
@@ -105,6 +107,17 @@ def main():
     import hm_synth
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.calibrate_copy:
+        n = 1 << 30
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty(n, dtype=torch.uint8, device="cuda")
+        a.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": n, "bytes_written": n, "launches": 5}))
+        return
     hm_stitch.lib()
     log("generating the 8500x3300 rig and frames")
     rig = hm_synth.config_rig("cfg1")

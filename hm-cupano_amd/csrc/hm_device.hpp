@@ -24,6 +24,31 @@ __device__ __forceinline__ uint8_t* row_ptr(const Surf& s, int b, int y) {
   return s.ptr + ((size_t)b * s.h + (size_t)y) * (size_t)s.pitch;
 }
 
+// ---- XCD-aware tile order ------------------------------------------------
+// MI355X dispatches consecutive workgroup ids round-robin over its 8 XCDs,
+// each with its own L2.  Renumbering so that XCD k runs the k-th contiguous
+// eighth of the row-major tile list keeps vertically adjacent tiles (which
+// share source lines and pyramid taps) in one L2.  A bijection on [0, n).
+constexpr int kNumXcd = 8;
+constexpr bool kXcdSwizzle = false;  // measured: no gain for these kernels
+__device__ __forceinline__ int xcd_tile(int bid, int n) {
+  if (!kXcdSwizzle) return bid;
+  const int per = n / kNumXcd, rem = n % kNumXcd;
+  const int xcd = bid % kNumXcd, idx = bid / kNumXcd;
+  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+// Pixel of this thread in a 2-D tiling launched as a 1-D grid of
+// (tiles_x * tiles_y) workgroups of blockDim.x * blockDim.y threads.
+struct TileXY {
+  int x, y;
+};
+__device__ __forceinline__ TileXY tile_pixel(int tiles_x, int tiles_y) {
+  const int t = xcd_tile((int)blockIdx.x, tiles_x * tiles_y);
+  const int tx = t % tiles_x, ty = t / tiles_x;
+  return TileXY{tx * (int)blockDim.x + (int)threadIdx.x, ty * (int)blockDim.y + (int)threadIdx.y};
+}
+
 // ---- pixel load/store (C = 3 or 4), values carried in a float4 ----------
 template <int C>
 __device__ __forceinline__ float4 ld_px(const float* base, size_t pix) {
@@ -97,6 +122,20 @@ __device__ __forceinline__ Taps make_taps(int x, int y, int lw, int lh) {
   return t;
 }
 
+// Correctly rounded x / d for the divisors of the pyramid arithmetic.  A
+// power-of-two divisor (sum of bilinear weights 0.25, 0.5 or 1; a 2x2 tap
+// count of 1, 2 or 4) divides exactly by multiplying with its exact
+// reciprocal, so only the other divisors (0.75, 3) take the IEEE division;
+// the result is bit-identical to x / d either way.
+__device__ __forceinline__ float div_exact(float x, float d) {
+  if (d == 1.0f) return x;
+  if (d == 0.5f) return x * 2.0f;
+  if (d == 0.25f) return x * 4.0f;
+  if (d == 2.0f) return x * 0.5f;
+  if (d == 4.0f) return x * 0.25f;
+  return x / d;
+}
+
 // Alpha-aware bilinear (cudaBlend.cu:416-444, :649-666): taps 00,10,01,11;
 // with C == 4 a tap whose alpha is 0 is skipped; value = sumV/sumW or 0.
 template <int C>
@@ -114,7 +153,7 @@ __device__ __forceinline__ float4 up_alpha(const float* low, size_t row0, size_t
   if (C != 4 || p10.w != 0.0f) { sw += w10; sx += p10.x * w10; sy += p10.y * w10; sz += p10.z * w10; }
   if (C != 4 || p01.w != 0.0f) { sw += w01; sx += p01.x * w01; sy += p01.y * w01; sz += p01.z * w01; }
   if (C != 4 || p11.w != 0.0f) { sw += w11; sx += p11.x * w11; sy += p11.y * w11; sz += p11.z * w11; }
-  if (sw > 0.0f) return make_float4(sx / sw, sy / sw, sz / sw, 0.0f);
+  if (sw > 0.0f) return make_float4(div_exact(sx, sw), div_exact(sy, sw), div_exact(sz, sw), 0.0f);
   return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 

@@ -8,7 +8,7 @@
 #include <utility>
 #include <vector>
 
-#include "hm_pyramid_kernels.hpp"
+#include "hm_pyramid_fused.hpp"
 
 #define HM_RETURN_IF_ERROR(expr)            \
   do {                                      \
@@ -92,6 +92,23 @@ struct UpLaunch {
 };
 hipError_t launch_up(int c, int n, int kind, bool hi_u8, const UpLaunch& a, int batch, hipStream_t s);
 
+// Whole-pyramid kernels (hm_pyramid_fused.hpp).  `lv` is a device table:
+// lv[j] describes the j-th level produced (down) or level j+1 (up).
+hipError_t launch_down_fused(int c, int n, bool in_u8, const void* const* g0, int w0, int h0, int n_out,
+                             const hmdev::LevelDesc* lv, int batch, hipStream_t s);
+struct UpFusedLaunch {
+  const void* g0[hmdev::kMaxImages] = {};
+  const float* mask0 = nullptr;
+  int w0 = 0, h0 = 0, levels = 0;
+  const hmdev::LevelDesc* lv = nullptr;
+  float* f32_out = nullptr;
+  uint8_t* u8_out = nullptr;
+  size_t u8_pitch = 0;
+  int u8_w = 0, u8_h = 0, ox = 0, oy = 0;
+  int win_x = 0, win_y = 0, win_w = 0, win_h = 0;
+};
+hipError_t launch_up_fused(int c, int n, int kind, bool g0_u8, const UpFusedLaunch& u, int batch, hipStream_t s);
+
 // Where the level-0 reconstruction goes.
 struct Level0Out {
   float* f32 = nullptr;   // tight float output (blend-context API)
@@ -122,6 +139,7 @@ class Pyramid {
   DevBuf lap[hmdev::kMaxImages][hmdev::kMaxLevels]; // debug
   DevBuf blend[hmdev::kMaxLevels];                  // debug
   DevBuf r0_debug;                                  // debug copy of level-0 reconstruction
+  DevBuf level_table;                               // LevelDesc of levels 1..levels-1 (device)
   size_t allocation_size = 0;
   int mask_channels() const { return kind == hmdev::kBlend2 ? 1 : n; }
 };
@@ -129,6 +147,19 @@ class Pyramid {
 // Level sizes: the 2-image rule (src/cuda/cudaBlend.cu:860-866) or the
 // N/3-image rule that stops when a side < 2 (src/cuda/cudaBlendN.cu:675-691).
 int pyramid_sizes(int w, int h, int levels, bool clamp_small_sides, int* widths, int* heights);
+
+// 1-D grid of (tiles_x * tiles_y) workgroups (z = batch) for tile_pixel().
+struct TileGrid {
+  dim3 grid;
+  int tiles_x, tiles_y;
+};
+inline TileGrid tile_grid(int w, int h, int batch, dim3 block) {
+  TileGrid g;
+  g.tiles_x = (w + (int)block.x - 1) / (int)block.x;
+  g.tiles_y = (h + (int)block.y - 1) / (int)block.y;
+  g.grid = dim3((unsigned)(g.tiles_x * g.tiles_y), 1u, (unsigned)batch);
+  return g;
+}
 
 inline dim3 grid_for(int w, int h, int batch, dim3 block) {
   return dim3((unsigned)((w + (int)block.x - 1) / (int)block.x), (unsigned)((h + (int)block.y - 1) / (int)block.y),

@@ -74,34 +74,59 @@ struct Pano2L0 {
   int bw, bh;
   int d[2], ulo[2], uhi[2];  // G_i[u] <- canvas column u + d[i] for u in [ulo[i], uhi[i])
   int g2_canvas;             // 1: G2 takes the canvas after both remaps; 0: image 2 alone
+  int tiles_x, tiles_y;
 };
 
-// One thread per canvas pixel.  c1 = canvas after remap 1, c12 = canvas
-// after remap 2 (which overwrites image 2's whole box, cudaPano.inl:291-303).
+// Each thread handles kPx pixels of one canvas row, kBlock.x apart, so a
+// wavefront's loads stay coalesced while every lane keeps 2 * kPx independent
+// map loads (then gathers) in flight.  c1 = canvas after remap 1, c12 =
+// canvas after remap 2 (which overwrites image 2's whole box,
+// cudaPano.inl:291-303).
+constexpr int kPx = 4;
+
 template <int C>
 __global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
-  const int X = blockIdx.x * blockDim.x + threadIdx.x;
-  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
+  const int X0 = (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int Y = (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
   const int b = blockIdx.z;
-  if (X >= a.cw || Y >= a.ch) return;
-  uint32_t r[2] = {0u, 0u};
-  bool valid[2];
+  if (Y >= a.ch) return;
+  uint32_t m[2][kPx];
+  bool valid[2][kPx];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const int rx = X - a.px[i], ry = Y - a.py[i];
-    valid[i] = rx >= 0 && ry >= 0 && rx < a.mw[i] && ry < a.mh[i];
-    if (valid[i]) r[i] = remap_px<C>(a.src[i], b, a.map[i][(size_t)ry * a.mw[i] + rx]);
+    const int ry = Y - a.py[i];
+    const bool row_ok = ry >= 0 && ry < a.mh[i];
+    const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) {
+      const int rx = X0 + j * (int)blockDim.x - a.px[i];
+      valid[i][j] = row_ok && rx >= 0 && rx < a.mw[i] && X0 + j * (int)blockDim.x < a.cw;
+      m[i][j] = valid[i][j] ? mrow[rx] : 0u;
+    }
   }
-  const uint32_t c1 = valid[0] ? r[0] : 0u;
-  const uint32_t c12 = valid[1] ? r[1] : c1;
-  if (a.write_canvas && !(X >= a.skip_x0 && X < a.skip_x1))
-    store_px<C>(a.canvas + ((size_t)b * a.ch + Y) * a.cpitch + (size_t)X * C, c12);
-  if (Y < a.bh) {
-    const int u1 = X - a.d[0];
-    if (u1 >= a.ulo[0] && u1 < a.uhi[0]) store_px<C>(a.g0[0] + (((size_t)b * a.bh + Y) * a.bw + u1) * C, c1);
-    const int u2 = X - a.d[1];
-    if (u2 >= a.ulo[1] && u2 < a.uhi[1])
-      store_px<C>(a.g0[1] + (((size_t)b * a.bh + Y) * a.bw + u2) * C, a.g2_canvas ? c12 : (valid[1] ? r[1] : 0u));
+  uint32_t r[2][kPx];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) r[i][j] = valid[i][j] ? remap_px<C>(a.src[i], b, m[i][j]) : 0u;
+  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+  uint8_t* g0row = a.g0[0] + ((size_t)b * a.bh + Y) * a.bw * C;
+  uint8_t* g1row = a.g0[1] + ((size_t)b * a.bh + Y) * a.bw * C;
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const int X = X0 + j * (int)blockDim.x;
+    if (X >= a.cw) continue;
+    const uint32_t c1 = valid[0][j] ? r[0][j] : 0u;
+    const uint32_t c12 = valid[1][j] ? r[1][j] : c1;
+    if (a.write_canvas && !(X >= a.skip_x0 && X < a.skip_x1)) store_px<C>(crow + (size_t)X * C, c12);
+    if (Y < a.bh) {
+      const int u1 = X - a.d[0];
+      if (u1 >= a.ulo[0] && u1 < a.uhi[0]) store_px<C>(g0row + (size_t)u1 * C, c1);
+      const int u2 = X - a.d[1];
+      if (u2 >= a.ulo[1] && u2 < a.uhi[1])
+        store_px<C>(g1row + (size_t)u2 * C, a.g2_canvas ? c12 : (valid[1][j] ? r[1][j] : 0u));
+    }
   }
 }
 
@@ -113,26 +138,54 @@ struct Pano2Hard {
   uint32_t cpitch;
   int cw, ch;
   const uint8_t* seam;  // canvas-sized, 1 selects image 1, 0 image 2
+  int tiles_x, tiles_y;
 };
 
 // Hard seam: the two dest-map remaps of cudaPano.inl:262-276 and :345-359
 // (image 1 where seam == 1, image 2 where seam == 0) plus the canvas memset,
-// as one pass.
+// as one pass; kPx pixels per thread as above.
 template <int C>
 __global__ void __launch_bounds__(256) pano2_hard_kernel(Pano2Hard a) {
-  const int X = blockIdx.x * blockDim.x + threadIdx.x;
-  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
+  const int X0 = (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int Y = (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
   const int b = blockIdx.z;
-  if (X >= a.cw || Y >= a.ch) return;
-  const uint8_t s = a.seam[(size_t)Y * a.cw + X];
-  uint32_t v = 0u;
-  const int i = s == 1 ? 0 : (s == 0 ? 1 : -1);
-  if (i >= 0) {
-    const int rx = X - a.px[i], ry = Y - a.py[i];
-    if (rx >= 0 && ry >= 0 && rx < a.mw[i] && ry < a.mh[i])
-      v = remap_px_dm<C>(a.src[i], b, a.map[i][(size_t)ry * a.mw[i] + rx]);
+  if (Y >= a.ch) return;
+  const uint8_t* srow = a.seam + (size_t)Y * a.cw;
+  uint32_t m[kPx];
+  int sel[kPx];  // 0: image 1, 1: image 2, -1: nothing
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const int X = X0 + j * (int)blockDim.x;
+    sel[j] = -1;
+    m[j] = 0u;
+    if (X >= a.cw) continue;
+    const uint8_t sv = srow[X];
+    if (sv > 1) continue;
+    // Scalar selects: a dynamic index into the kernel-argument arrays would
+    // force them into scratch memory.
+    const bool first = sv == 1;
+    const int rx = X - (first ? a.px[0] : a.px[1]);
+    const int ry = Y - (first ? a.py[0] : a.py[1]);
+    const int mw = first ? a.mw[0] : a.mw[1];
+    const int mh = first ? a.mh[0] : a.mh[1];
+    if (rx >= 0 && ry >= 0 && rx < mw && ry < mh) {
+      sel[j] = first ? 0 : 1;
+      m[j] = (first ? a.map[0] : a.map[1])[(size_t)ry * mw + rx];
+    }
   }
-  store_px<C>(a.canvas + ((size_t)b * a.ch + Y) * a.cpitch + (size_t)X * C, v);
+  uint32_t v[kPx];
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    v[j] = 0u;
+    if (sel[j] >= 0) v[j] = remap_px_dm<C>(sel[j] == 0 ? a.src[0] : a.src[1], b, m[j]);
+  }
+  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const int X = X0 + j * (int)blockDim.x;
+    if (X < a.cw) store_px<C>(crow + (size_t)X * C, v[j]);
+  }
 }
 
 Surf to_surf(const hmSurface& s) { return Surf{static_cast<uint8_t*>(s.d_ptr), s.width, s.height, s.pitch}; }
@@ -293,7 +346,8 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     return hipErrorInvalidValue;
   if ((int)canvas.width != canvas_w || (int)canvas.height != canvas_h) return hipErrorInvalidValue;
   const Surf s1 = to_surf(in1), s2 = to_surf(in2);
-  const dim3 grid = hm::grid_for(canvas_w, canvas_h, batch, kBlock);
+  const hm::TileGrid tg = hm::tile_grid(canvas_w, canvas_h, batch, dim3(kBlock.x * kPx, kBlock.y, 1));
+  const dim3 grid = tg.grid;
   if (hard_seam) {
     Pano2Hard a{};
     a.src[0] = s1;
@@ -312,6 +366,8 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     a.cw = canvas_w;
     a.ch = canvas_h;
     a.seam = seam.as<uint8_t>();
+    a.tiles_x = tg.tiles_x;
+    a.tiles_y = tg.tiles_y;
     timer.begin(s);
     if (channels == 4)
       hipLaunchKernelGGL(pano2_hard_kernel<4>, grid, kBlock, 0, s, a);
@@ -340,6 +396,8 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   a.ch = canvas_h;
   a.bw = bw;
   a.bh = bh;
+  a.tiles_x = tg.tiles_x;
+  a.tiles_y = tg.tiles_y;
   timer.begin(s);
   if (channels == 4)
     hipLaunchKernelGGL(pano2_level0_kernel<4>, grid, kBlock, 0, s, a);

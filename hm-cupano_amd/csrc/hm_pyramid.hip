@@ -2,7 +2,9 @@
 // kernel-tier blend-context C-ABI and launch timing.
 #include <stdio.h>
 
+#include <algorithm>
 #include <sstream>
+#include <vector>
 
 #include "hm_internal.hpp"
 #include "hm_stitch.h"
@@ -63,7 +65,10 @@ static hipError_t down_t(const void* const* in, float* const* out, int in_w, int
   a.in_h = in_h;
   a.out_w = out_w;
   a.out_h = out_h;
-  hipLaunchKernelGGL((down_kernel<TIn, C, N>), grid_for(out_w, out_h, batch, kBlock), kBlock, 0, s, a);
+  const TileGrid tg = tile_grid(out_w, out_h, batch, kBlock);
+  a.tiles_x = tg.tiles_x;
+  a.tiles_y = tg.tiles_y;
+  hipLaunchKernelGGL((down_kernel<TIn, C, N>), tg.grid, kBlock, 0, s, a);
   return hipGetLastError();
 }
 
@@ -126,7 +131,10 @@ static hipError_t up_t(const UpLaunch& u, int batch, hipStream_t s) {
   a.win_w = u.win_w;
   a.win_h = u.win_h;
   if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
-  hipLaunchKernelGGL((up_kernel<THi, C, N, KIND>), grid_for(u.win_w, u.win_h, batch, kBlock), kBlock, 0, s, a);
+  const TileGrid tg = tile_grid(u.win_w, u.win_h, batch, kBlock);
+  a.tiles_x = tg.tiles_x;
+  a.tiles_y = tg.tiles_y;
+  hipLaunchKernelGGL((up_kernel<THi, C, N, KIND>), tg.grid, kBlock, 0, s, a);
   return hipGetLastError();
 }
 
@@ -149,6 +157,101 @@ static hipError_t up_c(int n, int kind, const UpLaunch& u, int batch, hipStream_
 hipError_t launch_up(int c, int n, int kind, bool hi_u8, const UpLaunch& u, int batch, hipStream_t s) {
   if (c == 4) return hi_u8 ? up_c<uint8_t, 4>(n, kind, u, batch, s) : up_c<float, 4>(n, kind, u, batch, s);
   if (c == 3) return hi_u8 ? up_c<uint8_t, 3>(n, kind, u, batch, s) : up_c<float, 3>(n, kind, u, batch, s);
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// fused whole-pyramid kernels
+// ---------------------------------------------------------------------------
+template <typename T0, int C, int N>
+static hipError_t down_fused_t(const void* const* g0, int w0, int h0, int n_out, const LevelDesc* lv, int batch,
+                               hipStream_t s) {
+  DownFusedArgs<T0, N> a;
+  for (int i = 0; i < N; ++i) a.g0[i] = static_cast<const T0*>(g0[i]);
+  a.w0 = w0;
+  a.h0 = h0;
+  a.n_out = n_out;
+  a.lv = lv;
+  a.tiles_x = (w0 + kDownTile - 1) / kDownTile;
+  a.tiles_y = (h0 + kDownTile - 1) / kDownTile;
+  hipLaunchKernelGGL((down_fused_kernel<T0, C, N>), dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch),
+                     dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T0, int C>
+static hipError_t down_fused_c(int n, const void* const* g0, int w0, int h0, int n_out, const LevelDesc* lv,
+                               int batch, hipStream_t s) {
+  switch (n) {
+    case 2: return down_fused_t<T0, C, 2>(g0, w0, h0, n_out, lv, batch, s);
+    case 3: return down_fused_t<T0, C, 3>(g0, w0, h0, n_out, lv, batch, s);
+    case 4: return down_fused_t<T0, C, 4>(g0, w0, h0, n_out, lv, batch, s);
+    case 5: return down_fused_t<T0, C, 5>(g0, w0, h0, n_out, lv, batch, s);
+    case 6: return down_fused_t<T0, C, 6>(g0, w0, h0, n_out, lv, batch, s);
+    case 7: return down_fused_t<T0, C, 7>(g0, w0, h0, n_out, lv, batch, s);
+    case 8: return down_fused_t<T0, C, 8>(g0, w0, h0, n_out, lv, batch, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_down_fused(int c, int n, bool in_u8, const void* const* g0, int w0, int h0, int n_out,
+                             const LevelDesc* lv, int batch, hipStream_t s) {
+  if (n_out < 1 || n_out > 5) return hipErrorInvalidValue;
+  if (c == 4) return in_u8 ? down_fused_c<uint8_t, 4>(n, g0, w0, h0, n_out, lv, batch, s)
+                           : down_fused_c<float, 4>(n, g0, w0, h0, n_out, lv, batch, s);
+  if (c == 3) return in_u8 ? down_fused_c<uint8_t, 3>(n, g0, w0, h0, n_out, lv, batch, s)
+                           : down_fused_c<float, 3>(n, g0, w0, h0, n_out, lv, batch, s);
+  return hipErrorInvalidValue;
+}
+
+template <typename T0, int C, int N, int KIND>
+static hipError_t up_fused_t(const UpFusedLaunch& u, int batch, hipStream_t s) {
+  if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
+  UpFusedArgs<T0, N> a;
+  for (int i = 0; i < N; ++i) a.g0[i] = static_cast<const T0*>(u.g0[i]);
+  a.mask0 = u.mask0;
+  a.w0 = u.w0;
+  a.h0 = u.h0;
+  a.levels = u.levels;
+  a.lv = u.lv;
+  a.f32_out = u.f32_out;
+  a.u8_out = u.u8_out;
+  a.u8_pitch = u.u8_pitch;
+  a.u8_w = u.u8_w;
+  a.u8_h = u.u8_h;
+  a.ox = u.ox;
+  a.oy = u.oy;
+  a.win_x = u.win_x;
+  a.win_y = u.win_y;
+  a.win_w = u.win_w;
+  a.win_h = u.win_h;
+  a.tiles_x = (u.win_w + kUpTile - 1) / kUpTile;
+  a.tiles_y = (u.win_h + kUpTile - 1) / kUpTile;
+  hipLaunchKernelGGL((up_fused_kernel<T0, C, N, KIND>), dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch),
+                     dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T0, int C>
+static hipError_t up_fused_c(int n, int kind, const UpFusedLaunch& u, int batch, hipStream_t s) {
+  if (kind == kBlend2) return n == 2 ? up_fused_t<T0, C, 2, kBlend2>(u, batch, s) : hipErrorInvalidValue;
+  if (kind == kBlend3) return n == 3 ? up_fused_t<T0, C, 3, kBlend3>(u, batch, s) : hipErrorInvalidValue;
+  switch (n) {
+    case 2: return up_fused_t<T0, C, 2, kBlendN>(u, batch, s);
+    case 3: return up_fused_t<T0, C, 3, kBlendN>(u, batch, s);
+    case 4: return up_fused_t<T0, C, 4, kBlendN>(u, batch, s);
+    case 5: return up_fused_t<T0, C, 5, kBlendN>(u, batch, s);
+    case 6: return up_fused_t<T0, C, 6, kBlendN>(u, batch, s);
+    case 7: return up_fused_t<T0, C, 7, kBlendN>(u, batch, s);
+    case 8: return up_fused_t<T0, C, 8, kBlendN>(u, batch, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_up_fused(int c, int n, int kind, bool g0_u8, const UpFusedLaunch& u, int batch, hipStream_t s) {
+  if (u.levels < 1 || u.levels > kMaxLevels) return hipErrorInvalidValue;
+  if (c == 4) return g0_u8 ? up_fused_c<uint8_t, 4>(n, kind, u, batch, s) : up_fused_c<float, 4>(n, kind, u, batch, s);
+  if (c == 3) return g0_u8 ? up_fused_c<uint8_t, 3>(n, kind, u, batch, s) : up_fused_c<float, 3>(n, kind, u, batch, s);
   return hipErrorInvalidValue;
 }
 
@@ -247,6 +350,22 @@ hipError_t Pyramid::init(int kind_, int n_, int channels, int width, int height,
     HM_RETURN_IF_ERROR(r0_debug.alloc((size_t)W[0] * H[0] * c * batch * sizeof(float)));
     allocation_size += r0_debug.bytes;
   }
+  if (levels > 1) {
+    std::vector<LevelDesc> table(levels - 1);
+    for (int l = 1; l < levels; ++l) {
+      LevelDesc& d = table[l - 1];
+      for (int i = 0; i < kMaxImages; ++i) {
+        d.g[i] = i < n ? g[i][l].as<float>() : nullptr;
+        d.g_out[i] = i < n ? g[i][l].as<float>() : nullptr;
+      }
+      d.mask = mask[l].as<float>();
+      d.w = W[l];
+      d.h = H[l];
+    }
+    HM_RETURN_IF_ERROR(level_table.alloc(table.size() * sizeof(LevelDesc)));
+    HM_RETURN_IF_ERROR(
+        hipMemcpy(level_table.p, table.data(), table.size() * sizeof(LevelDesc), hipMemcpyHostToDevice));
+  }
   return hipSuccess;
 }
 
@@ -262,6 +381,45 @@ hipError_t Pyramid::build_mask_pyramid(const float* mask0_, hipStream_t s) {
 }
 
 hipError_t Pyramid::run(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s, LaunchTimer* timer) {
+  if (!debug) {
+    // Down pass: up to five levels per launch from aligned tiles.
+    const LevelDesc* table = level_table.as<LevelDesc>();
+    int from = 0;
+    while (from + 1 < levels) {
+      const int n_out = std::min(5, levels - 1 - from);
+      const void* in[kMaxImages];
+      for (int i = 0; i < n; ++i) in[i] = from == 0 ? g0[i] : g[i][from].p;
+      if (timer) timer->begin(s);
+      HM_RETURN_IF_ERROR(launch_down_fused(c, n, from == 0 && g0_u8, in, W[from], H[from], n_out, table + from,
+                                           batch, s));
+      if (timer) timer->end(s, "down_fused");
+      from += n_out;
+    }
+    // Up pass: the whole reconstruction per level-0 tile in one launch.
+    UpFusedLaunch u;
+    for (int i = 0; i < n; ++i) u.g0[i] = g0[i];
+    u.mask0 = mask0;
+    u.w0 = W[0];
+    u.h0 = H[0];
+    u.levels = levels;
+    u.lv = table;
+    u.f32_out = out.f32;
+    u.u8_out = out.u8;
+    u.u8_pitch = out.u8_pitch;
+    u.u8_w = out.u8_w;
+    u.u8_h = out.u8_h;
+    u.ox = out.ox;
+    u.oy = out.oy;
+    u.win_x = out.win_w >= 0 ? out.win_x : 0;
+    u.win_y = out.win_w >= 0 ? out.win_y : 0;
+    u.win_w = out.win_w >= 0 ? out.win_w : W[0];
+    u.win_h = out.win_w >= 0 ? out.win_h : H[0];
+    if (timer) timer->begin(s);
+    HM_RETURN_IF_ERROR(launch_up_fused(c, n, kind, g0_u8, u, batch, s));
+    if (timer) timer->end(s, "up_fused");
+    return hipSuccess;
+  }
+  // Debug: per-level launches that also keep the Laplacian and blended levels.
   // 1. Down pass: every image's level k -> k+1 in one launch per level.
   for (int l = 0; l + 1 < levels; ++l) {
     const void* in[kMaxImages];

@@ -30,12 +30,42 @@ struct DownArgs {
   const TIn* in[N];
   float* out[N];
   int in_w, in_h, out_w, out_h;
+  int tiles_x, tiles_y;
 };
+
+// One coarse pixel from its 2x2 block, taps in the reference order (0,0),
+// (1,0), (0,1), (1,1): RGB mean of the in-range taps whose alpha != 0 (0 if
+// none), alpha = max over the in-range taps (cudaBlend.cu:85-134).
+template <int C>
+__device__ __forceinline__ float4 down_px(const float4* p, const bool* in_range) {
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f, alpha = 0.0f;
+  int count = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (!in_range[k]) continue;
+    bool keep = true;
+    if constexpr (C == 4) {
+      alpha = alpha < p[k].w ? p[k].w : alpha;
+      keep = p[k].w != 0.0f;
+    }
+    if (keep) {
+      sx += p[k].x;
+      sy += p[k].y;
+      sz += p[k].z;
+      ++count;
+    }
+  }
+  if (count > 0) {
+    const float fc = (float)count;
+    return make_float4(div_exact(sx, fc), div_exact(sy, fc), div_exact(sz, fc), alpha);
+  }
+  return make_float4(0.0f, 0.0f, 0.0f, alpha);
+}
 
 template <typename TIn, int C, int N>
 __global__ void __launch_bounds__(256) down_kernel(DownArgs<TIn, N> a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const TileXY p0 = tile_pixel(a.tiles_x, a.tiles_y);
+  const int x = p0.x, y = p0.y;
   const int b = blockIdx.z;
   if (x >= a.out_w || y >= a.out_h) return;
   const int ix = 2 * x, iy = 2 * y;
@@ -43,40 +73,15 @@ __global__ void __launch_bounds__(256) down_kernel(DownArgs<TIn, N> a) {
   const size_t r0 = ((size_t)b * a.in_h + iy) * a.in_w;
   const size_t r1 = r0 + a.in_w;
   const size_t o = ((size_t)b * a.out_h + y) * a.out_w + x;
+  const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    // taps in the reference order: (0,0), (1,0), (0,1), (1,1)
     float4 p[4];
-    bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
     p[0] = ld_px<C>(a.in[i], r0 + ix);
     if (has_x1) p[1] = ld_px<C>(a.in[i], r0 + ix + 1);
     if (has_y1) p[2] = ld_px<C>(a.in[i], r1 + ix);
     if (has_x1 && has_y1) p[3] = ld_px<C>(a.in[i], r1 + ix + 1);
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f, alpha = 0.0f;
-    int count = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (!in_range[k]) continue;
-      bool keep = true;
-      if constexpr (C == 4) {
-        alpha = alpha < p[k].w ? p[k].w : alpha;
-        keep = p[k].w != 0.0f;
-      }
-      if (keep) {
-        sx += p[k].x;
-        sy += p[k].y;
-        sz += p[k].z;
-        ++count;
-      }
-    }
-    float4 r;
-    if (count > 0) {
-      const float fc = (float)count;
-      r = make_float4(sx / fc, sy / fc, sz / fc, alpha);
-    } else {
-      r = make_float4(0.0f, 0.0f, 0.0f, alpha);
-    }
-    st_px<C>(a.out[i], o, r);
+    st_px<C>(a.out[i], o, down_px<C>(p, in_range));
   }
 }
 
@@ -207,12 +212,14 @@ struct UpArgs {
   int ox, oy;             // canvas position of level pixel (0,0)
   int hi_w, hi_h, lo_w, lo_h;
   int win_x, win_y, win_w, win_h;  // pixels of level k evaluated
+  int tiles_x, tiles_y;
 };
 
 template <typename THi, int C, int N, int KIND>
 __global__ void __launch_bounds__(256) up_kernel(UpArgs<THi, N> a) {
-  const int x = a.win_x + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int y = a.win_y + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const TileXY p0 = tile_pixel(a.tiles_x, a.tiles_y);
+  const int x = a.win_x + p0.x;
+  const int y = a.win_y + p0.y;
   const int b = blockIdx.z;
   if (x >= a.win_x + a.win_w || y >= a.win_y + a.win_h) return;
   const size_t hpix = ((size_t)b * a.hi_h + y) * a.hi_w + x;

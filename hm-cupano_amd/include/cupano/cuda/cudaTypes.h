@@ -1,0 +1,48 @@
+// cudaTypes.h -- pitched device image descriptor and pixel traits.
+//
+// CudaSurface<T> keeps the field names of hm-cupano's src/cuda/cudaTypes.h:195-201
+// (d_ptr, width, height, pitch in bytes); BaseScalar_t<T> is the channel type.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "hm_stitch.h"
+
+template <typename T>
+struct CudaSurface final {
+  T* d_ptr;
+  uint32_t width;
+  uint32_t height;
+  uint32_t pitch;
+};
+
+template <typename T>
+struct BaseScalar {
+  using type = T;
+};
+template <> struct BaseScalar<uchar3> { using type = unsigned char; };
+template <> struct BaseScalar<uchar4> { using type = unsigned char; };
+template <> struct BaseScalar<float3> { using type = float; };
+template <> struct BaseScalar<float4> { using type = float; };
+template <typename T>
+using BaseScalar_t = typename BaseScalar<T>::type;
+
+template <typename T>
+constexpr int num_channels_v = (int)(sizeof(T) / sizeof(BaseScalar_t<T>));
+
+// The C-ABI pixel code of a pixel type.
+template <typename T>
+constexpr hmPixelType hm_pixel_type() {
+  if constexpr (std::is_same_v<T, uchar3>) return HM_PIX_U8C3;
+  if constexpr (std::is_same_v<T, uchar4>) return HM_PIX_U8C4;
+  if constexpr (std::is_same_v<T, float3>) return HM_PIX_F32C3;
+  return HM_PIX_F32C4;
+}
+
+template <typename T>
+inline hmSurface to_hm_surface(const CudaSurface<T>& s) {
+  return hmSurface{const_cast<void*>(static_cast<const void*>(s.d_ptr)), s.width, s.height, s.pitch};
+}

@@ -907,3 +907,111 @@ int hmo_panoN_process(const hmo_panoN_cfg* cfg, const uint8_t* const* inputs, ui
   free(padded);
   return rc;
 }
+
+/* ------------------------------------------------------------------------- */
+/* 3-image pano                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* ControlMasks3::canvas_width/height (src/pano/controlMasks3.cpp:303-317). */
+int hmo_pano3_canvas_size(const hmo_pano3_cfg* cfg, int* canvas_w, int* canvas_h) {
+  int w = 0, h = 0;
+  for (int i = 0; i < 3; ++i) {
+    w = IMAX(w, cfg->pos_x[i] + cfg->map_w[i]);
+    h = IMAX(h, cfg->pos_y[i] + cfg->map_h[i]);
+  }
+  *canvas_w = w;
+  *canvas_h = h;
+  return 0;
+}
+
+/* FusedRemapToFullKernel3 (cudaFusedKernels3.cu:25-119): over the full
+ * buffers' extent, each image's box writes only mapped, in-range pixels. */
+static void fused_remap_to_full3(const hmo_pano3_cfg* cfg, const uint8_t* const* in, float* const* full, int fw,
+                                 int fh) {
+  const int c = cfg->channels;
+  for (int b = 0; b < cfg->batch; ++b)
+    for (int y = 0; y < fh; ++y)
+      for (int x = 0; x < fw; ++x)
+        for (int i = 0; i < 3; ++i) {
+          const int rx = x - cfg->pos_x[i], ry = y - cfg->pos_y[i];
+          if (rx < 0 || rx >= cfg->map_w[i] || ry < 0 || ry >= cfg->map_h[i]) continue;
+          const size_t mi = (size_t)ry * cfg->map_w[i] + rx;
+          const int sx = cfg->map_x[i][mi], sy = cfg->map_y[i][mi];
+          if (sx == HMO_UNMAPPED || !(sx < cfg->src_w[i] && sy < cfg->src_h[i])) continue;
+          store_cast_u8(in[i] + pix_index(b, sx, sy, cfg->src_w[i], cfg->src_h[i], c), c, full[i], 1,
+                        pix_index(b, x, y, fw, fh, c), c);
+        }
+}
+
+/* FusedRemapHardSeam3 (cudaFusedKernels3.cu:124-217): seam indexed with the
+ * canvas width; unmapped / out-of-range pixels are left untouched. */
+static void fused_hard_seam3(const hmo_pano3_cfg* cfg, const uint8_t* const* in, uint8_t* canvas, int cw, int ch) {
+  const int c = cfg->channels;
+  for (int b = 0; b < cfg->batch; ++b)
+    for (int y = 0; y < ch; ++y)
+      for (int x = 0; x < cw; ++x) {
+        const int i = cfg->seam[(size_t)y * cw + x];
+        if (i > 2) continue;
+        const int rx = x - cfg->pos_x[i], ry = y - cfg->pos_y[i];
+        if (rx < 0 || rx >= cfg->map_w[i] || ry < 0 || ry >= cfg->map_h[i]) continue;
+        const size_t mi = (size_t)ry * cfg->map_w[i] + rx;
+        const int sx = cfg->map_x[i][mi], sy = cfg->map_y[i][mi];
+        if (sx == HMO_UNMAPPED || !(sx < cfg->src_w[i] && sy < cfg->src_h[i])) continue;
+        memcpy(canvas + pix_index(b, x, y, cw, ch, c), in[i] + pix_index(b, sx, sy, cfg->src_w[i], cfg->src_h[i], c),
+               (size_t)c);
+      }
+}
+
+int hmo_pano3_process(const hmo_pano3_cfg* cfg, const uint8_t* const* inputs, uint8_t* canvas, hmo_dump* dump) {
+  const int c = cfg->channels, batch = cfg->batch;
+  if ((c != 3 && c != 4) || batch < 1 || cfg->levels < 0) return -1;
+  int cw, ch;
+  hmo_pano3_canvas_size(cfg, &cw, &ch);
+  const int hard = cfg->levels == 0;
+  /* The hard seam indexes the seam with the canvas width (cudaFusedKernels3.cu:151,
+   * cudaRemap.cu:233); the soft mask is cropped to canvas_h rows
+   * (CanvasManager3::convertMaskMat, canvasManager3.cpp:107-128). */
+  if (hard && (cfg->seam_w != cw || cfg->seam_h < ch)) return -1;
+  if (!hard && cfg->seam_h < ch) return -1;
+  if (c == 4) memset(canvas, 0, (size_t)cw * ch * c * batch); /* cudaPano3_optimized.inl:36-39 */
+  if (hard) {
+    if (cfg->fused) {
+      fused_hard_seam3(cfg, inputs, canvas, cw, ch);
+    } else {
+      for (int i = 0; i < 3; ++i) /* cudaPano3.inl: dest-map remaps in image order */
+        hmo_remap_dest_map(inputs[i], cfg->src_w[i], cfg->src_h[i], c, canvas, cw, ch, cfg->map_x[i], cfg->map_y[i],
+                           cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], i, cfg->seam, batch);
+    }
+    return 0;
+  }
+  const int bw = cfg->seam_w, bh = ch;
+  float* mask = (float*)malloc((size_t)bw * bh * 3 * sizeof(float));
+  for (int y = 0; y < bh; ++y)
+    for (int x = 0; x < bw; ++x) {
+      const int lbl = cfg->seam[(size_t)y * cfg->seam_w + x];
+      for (int i = 0; i < 3; ++i) mask[((size_t)y * bw + x) * 3 + i] = (lbl == i) ? 1.0f : 0.0f;
+    }
+  const size_t fpx = (size_t)bw * bh * c * batch;
+  float* full[3];
+  for (int i = 0; i < 3; ++i) full[i] = (float*)calloc(fpx, sizeof(float));
+  if (cfg->fused) {
+    fused_remap_to_full3(cfg, inputs, full, bw, bh);
+  } else {
+    /* cudaPano3.inl:210-305: remap i onto the canvas, then copy its box into full i. */
+    for (int i = 0; i < 3; ++i) {
+      hmo_remap_offset(inputs[i], cfg->src_w[i], cfg->src_h[i], c, canvas, 0, cw, ch, c, cfg->map_x[i],
+                       cfg->map_y[i], cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], 0, 0, -1, -1, 0,
+                       batch);
+      memset(full[i], 0, fpx * sizeof(float));
+      hmo_copy_roi(canvas, 0, cw, ch, c, cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], full[i], 1, bw,
+                   bh, c, cfg->pos_x[i], cfg->pos_y[i], batch);
+    }
+  }
+  float* out = (float*)calloc(fpx, sizeof(float));
+  int rc = hmo_blend3(full[0], full[1], full[2], mask, out, bw, bh, c, cfg->levels, batch, dump);
+  if (!rc) hmo_copy_roi(out, 1, bw, bh, c, bw, bh, 0, 0, canvas, 0, cw, ch, c, 0, 0, batch);
+  for (int i = 0; i < 3; ++i) free(full[i]);
+  free(out);
+  free(mask);
+  return rc;
+}

@@ -131,6 +131,23 @@ int hmo_panoN_canvas_size(const hmo_panoN_cfg* cfg, int* canvas_w, int* canvas_h
 int hmo_panoN_rois(const hmo_panoN_cfg* cfg, int* write_roi, int* blend_roi);
 int hmo_panoN_process(const hmo_panoN_cfg* cfg, const uint8_t* const* inputs, uint8_t* canvas, hmo_dump* dump);
 
+/* ---- 3-image pano: CudaStitchPano3 ctor + process ----
+ * (src/pano/cudaPano3.inl:19-110, 183-422; cudaPano3_optimized.inl:19-163;
+ *  canvasManager3.cpp:13-130; cudaFusedKernels3.cu:25-217).
+ * The canvas is in/out: a 3-channel pipeline does not clear it. */
+typedef struct hmo_pano3_cfg {
+  int batch, levels, channels, fused;
+  int src_w[3], src_h[3];
+  int map_w[3], map_h[3];
+  const uint16_t* map_x[3];
+  const uint16_t* map_y[3];
+  int pos_x[3], pos_y[3];
+  const uint8_t* seam;       /* labels 0..2, seam_w x seam_h */
+  int seam_w, seam_h;
+} hmo_pano3_cfg;
+int hmo_pano3_canvas_size(const hmo_pano3_cfg* cfg, int* canvas_w, int* canvas_h);
+int hmo_pano3_process(const hmo_pano3_cfg* cfg, const uint8_t* const* inputs, uint8_t* canvas, hmo_dump* dump);
+
 #ifdef __cplusplus
 }
 #endif

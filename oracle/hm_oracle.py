@@ -55,6 +55,15 @@ class hmo_panoN_cfg(ctypes.Structure):
     ]
 
 
+class hmo_pano3_cfg(ctypes.Structure):
+    _fields_ = [
+        ("batch", _I), ("levels", _I), ("channels", _I), ("fused", _I),
+        ("src_w", _I * 3), ("src_h", _I * 3), ("map_w", _I * 3), ("map_h", _I * 3),
+        ("map_x", _P * 3), ("map_y", _P * 3), ("pos_x", _I * 3), ("pos_y", _I * 3),
+        ("seam", _P), ("seam_w", _I), ("seam_h", _I),
+    ]
+
+
 _lib = None
 
 
@@ -90,6 +99,8 @@ def lib():
         L.hmo_panoN_canvas_size.argtypes = [_P, _P, _P]
         L.hmo_panoN_rois.argtypes = [_P, _P, _P]
         L.hmo_panoN_process.argtypes = [_P, _P, _P, _P]
+        L.hmo_pano3_canvas_size.argtypes = [_P, _P, _P]
+        L.hmo_pano3_process.argtypes = [_P, _P, _P, _P]
         _lib = L
     return _lib
 
@@ -296,3 +307,34 @@ def panoN(rig, inputs: List[np.ndarray], levels: int, minimize: bool = True, dum
     if rc != 0:
         raise RuntimeError(f"oracle panoN failed ({rc})")
     return (canvas, d) if dump else canvas
+
+
+def pano3(rig, inputs: List[np.ndarray], levels: int, fused: bool = True, canvas: Optional[np.ndarray] = None,
+          dump: bool = False):
+    """CudaStitchPano3::process on the CPU.  `canvas` (uint8 [B,H,W,C]) is the
+    prior canvas content (a 3-channel pipeline leaves unwritten pixels alone)."""
+    b, _, _, c = inputs[0].shape
+    srcs = [np.ascontiguousarray(x) for x in inputs]
+    cfg = hmo_pano3_cfg()
+    keep = []
+    cfg.batch, cfg.levels, cfg.channels, cfg.fused = b, levels, c, int(fused)
+    for i in range(3):
+        mx = np.ascontiguousarray(rig.map_x[i], np.uint16)
+        my = np.ascontiguousarray(rig.map_y[i], np.uint16)
+        keep += [mx, my]
+        cfg.src_w[i], cfg.src_h[i] = srcs[i].shape[2], srcs[i].shape[1]
+        cfg.map_h[i], cfg.map_w[i] = mx.shape
+        cfg.map_x[i], cfg.map_y[i] = _ptr(mx), _ptr(my)
+        cfg.pos_x[i], cfg.pos_y[i] = int(rig.pos_x[i]), int(rig.pos_y[i])
+    seam = np.ascontiguousarray(rig.seam, np.uint8)
+    cfg.seam = _ptr(seam)
+    cfg.seam_h, cfg.seam_w = seam.shape
+    cw, ch = _I(), _I()
+    lib().hmo_pano3_canvas_size(ctypes.byref(cfg), ctypes.byref(cw), ctypes.byref(ch))
+    out = np.zeros((b, ch.value, cw.value, c), np.uint8) if canvas is None else np.ascontiguousarray(canvas).copy()
+    d = Dump(3, seam.shape[1], ch.value, c, levels, b, True, 3) if (dump and levels > 0) else None
+    arr = (_P * 3)(*[_ptr(x) for x in srcs])
+    rc = lib().hmo_pano3_process(ctypes.byref(cfg), arr, _ptr(out), ctypes.byref(d.st) if d else None)
+    if rc != 0:
+        raise RuntimeError(f"oracle pano3 failed ({rc})")
+    return (out, d) if dump else out

@@ -25,7 +25,7 @@ def rand_images(rng, n, b, h, w, c, holes=True):
             im[..., 3] = 255
             if holes:
                 # transparent regions that differ per image, as remapped footprints do
-                y0, x0 = rng.integers(0, h // 2), rng.integers(0, w // 2)
+                y0, x0 = rng.integers(0, max(1, h // 2)), rng.integers(0, max(1, w // 2))
                 im[:, y0: y0 + h // 3, x0: x0 + w // 3] = 0
                 im[:, :, : i * 3] = 0
         imgs.append(im)
@@ -50,8 +50,10 @@ def test_blend_known_answers_gpu(hm, torch_gpu, case):
 
 @pytest.mark.parametrize("kind,n", [(2, 2), (3, 3), (0, 2), (0, 3), (0, 5), (0, 8)])
 @pytest.mark.parametrize("c", [4, 3])
-@pytest.mark.parametrize("shape,levels", [((37, 53), 5), ((64, 48), 3), ((5, 3), 6), ((1, 1), 1)])
-def test_blend_context_matches_oracle(hm, torch_gpu, oracle, kind, n, c, shape, levels):
+@pytest.mark.parametrize("shape,levels", [((37, 53), 5), ((64, 48), 3), ((5, 3), 6), ((1, 1), 1), ((131, 97), 8),
+                                          ((70, 200), 2)])
+@pytest.mark.parametrize("debug", [False, True], ids=["fused", "per_level"])
+def test_blend_context_matches_oracle(hm, torch_gpu, oracle, kind, n, c, shape, levels, debug):
     torch = torch_gpu
     rng = np.random.default_rng(hash((kind, n, c, shape, levels)) & 0xFFFF)
     h, w = shape
@@ -65,7 +67,7 @@ def test_blend_context_matches_oracle(hm, torch_gpu, oracle, kind, n, c, shape, 
         mask[: h // 2] = (mask[: h // 2] == mask[: h // 2].max(-1, keepdims=True)).astype(np.float32)
         mask[0, 0] = 0
     want, d = oracle.blend(kind, imgs, mask, levels, dump=True)
-    ctx = hm.BlendContext(kind, n, w, h, levels, b, c, debug=True)
+    ctx = hm.BlendContext(kind, n, w, h, levels, b, c, debug=debug)
     assert ctx.levels() == d.sizes
     out = torch.zeros((b, h, w, c), dtype=torch.float32, device="cuda")
     dimgs = [dev(torch, i) for i in imgs]

@@ -42,7 +42,7 @@ def run_pano(hm, torch, rig, frames, levels, minimize=True, channels=4, debug=Fa
     return pano, canvas.cpu().numpy()
 
 
-@pytest.mark.parametrize("levels", [0, 1, 2, 6])
+@pytest.mark.parametrize("levels", [0, 1, 2, 6, 8])
 @pytest.mark.parametrize("minimize", [True, False])
 def test_pano2_small_rig_bit_exact(hm, torch_gpu, oracle, levels, minimize):
     import hm_synth

@@ -452,3 +452,59 @@ def test_panoN_rois_closed_form(oracle):
     assert wr == (499 - 128, 0, 2 + 256, 300)
     # 371-16 = 355 -> aligned down to 352; 629+16 = 645 -> aligned up to 648
     assert br == (352, 0, 648 - 352, 300)
+
+
+def test_pano3_one_hot_selects_middle_image(oracle):
+    # src/pano/cudaPano3_test.cpp:133-196 (uint8 pipeline): 1x1, levels 1, label 1 -> image 1
+    seam = np.ones((1, 1), np.uint8)
+    z = np.zeros((1, 1), np.uint16)
+    rig = IdRig([(1, 1)] * 3, [(0, 0)] * 3, seam)
+    rig.map_x, rig.map_y = [z] * 3, [z] * 3
+    imgs = [const_u8(1, 1, [v, v, v, 255]) for v in (30, 60, 90)]
+    for fused in (True, False):
+        out = oracle.pano3(rig, imgs, 1, fused=fused)
+        np.testing.assert_array_equal(out[0, 0, 0], [60, 60, 60, 255])
+
+
+def test_pano3_hard_seam_and_unmapped_semantics(oracle):
+    w, h = 48, 24
+    seam = np.zeros((h, w), np.uint8)
+    seam[:, 16:32] = 1
+    seam[:, 32:] = 2
+    rig = IdRig([(w, h)] * 3, [(0, 0)] * 3, seam)
+    rig.map_x = [m.copy() for m in rig.map_x]
+    rig.map_x[1][5, 20] = 65535   # unmapped: fused leaves the canvas, the dest-map remap writes zero
+    rig.map_x[2][6, 40] = 60      # out of range (not the sentinel)
+    imgs = [const_u8(w, h, v) for v in ([10, 0, 0, 255], [0, 20, 0, 255], [0, 0, 30, 255])]
+    prior = np.full((1, h, w, 4), 7, np.uint8)
+    out = oracle.pano3(rig, imgs, 0, fused=True, canvas=prior)
+    want = np.zeros((1, h, w, 4), np.uint8)  # 4-channel pipelines clear the canvas first
+    want[:, :, :16] = imgs[0][:, :, :16]
+    want[:, :, 16:32] = imgs[1][:, :, 16:32]
+    want[:, :, 32:] = imgs[2][:, :, 32:]
+    want[0, 5, 20] = 0
+    want[0, 6, 40] = 0
+    np.testing.assert_array_equal(out, want)
+    out_nf = oracle.pano3(rig, imgs, 0, fused=False, canvas=prior)
+    np.testing.assert_array_equal(out_nf, want)
+    # 3-channel pipeline: the canvas is not cleared, untouched pixels keep their value
+    imgs3 = [i[..., :3].copy() for i in imgs]
+    prior3 = np.full((1, h, w, 3), 7, np.uint8)
+    out3 = oracle.pano3(rig, imgs3, 0, fused=True, canvas=prior3)
+    assert tuple(out3[0, 5, 20]) == (7, 7, 7) and tuple(out3[0, 6, 40]) == (7, 7, 7)
+    assert tuple(out3[0, 0, 0]) == (10, 0, 0)
+
+
+def test_pano3_soft_constant_images_blend_to_selection(oracle):
+    # One-hot labels with constant per-image colours: every level's blend selects
+    # the labelled image, so far from label edges the output is that colour.
+    w, h = 96, 40
+    seam = np.zeros((h, w), np.uint8)
+    seam[:, 32:64] = 1
+    seam[:, 64:] = 2
+    rig = IdRig([(w, h)] * 3, [(0, 0)] * 3, seam)
+    imgs = [const_u8(w, h, v) for v in ([200, 0, 0, 255], [0, 150, 0, 255], [0, 0, 100, 255])]
+    out = oracle.pano3(rig, imgs, 4)
+    np.testing.assert_array_equal(out[0, :, 4:20], np.broadcast_to(np.uint8([200, 0, 0, 255]), (h, 16, 4)))
+    np.testing.assert_array_equal(out[0, :, 76:92], np.broadcast_to(np.uint8([0, 0, 100, 255]), (h, 16, 4)))
+    assert (out[..., 3] == 255).all()
