@@ -60,9 +60,14 @@ def algorithmic_bytes(rig, levels, pano_levels):
     for _ in range(1, pano_levels):
         a.append(((a[-1][0] + 1) // 2, (a[-1][1] + 1) // 2))
     A = [w * h for w, h in a]
-    out["remap_l0"] = sum(map_px) * 4 + sum(valid_px) * 4 + (cw - bw) * ch * 4 + 2 * A[0] * 4
-    out["down_l0"] = 2 * A[0] * 4 + 2 * A[1] * 16
-    out["up_l0"] = 2 * A[0] * 4 + 2 * A[1] * 16 + A[0] * 4 + A[1] * 16 + A[0] * 4
+    coarse = sum(A[1:])  # pixels of levels >= 1
+    # remap + Gaussian pyramid: maps, valid source pixels, canvas outside the
+    # blend columns, level 0 of both images (uint8), levels >= 1 (float4)
+    out["remap_down"] = (sum(map_px) * 4 + sum(valid_px) * 4 + (cw - bw) * ch * 4 + 2 * A[0] * 4
+                         + 2 * coarse * 16)
+    # reconstruction: both images' levels, the masks (level 0 uint8, others
+    # float), the blend-region canvas write
+    out["up_fused"] = 2 * A[0] * 4 + 2 * coarse * 16 + A[0] * 1 + coarse * 4 + A[0] * 4
     return out
 
 

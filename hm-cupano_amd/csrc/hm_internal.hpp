@@ -98,7 +98,8 @@ hipError_t launch_down_fused(int c, int n, bool in_u8, const void* const* g0, in
                              const hmdev::LevelDesc* lv, int batch, hipStream_t s);
 struct UpFusedLaunch {
   const void* g0[hmdev::kMaxImages] = {};
-  const float* mask0 = nullptr;
+  const void* mask0 = nullptr;
+  int mask0_mode = hmdev::kMask0F32;
   int w0 = 0, h0 = 0, levels = 0;
   const hmdev::LevelDesc* lv = nullptr;
   float* f32_out = nullptr;
@@ -128,11 +129,18 @@ class Pyramid {
   hipError_t build_mask_pyramid(const float* mask0, hipStream_t s);
   // g0: n level-0 images (uint8 if g0_u8 else float), tight, batch-major.
   hipError_t run(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s, LaunchTimer* timer);
+  // Fused path in two halves: Gaussian levels from level `from` on, then the
+  // whole reconstruction.
+  hipError_t run_down(const void* const* g0, bool g0_u8, int from, hipStream_t s, LaunchTimer* timer);
+  hipError_t run_up(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s, LaunchTimer* timer);
+  const hmdev::LevelDesc* levels_table() const { return level_table.as<hmdev::LevelDesc>(); }
 
   int kind = 0, n = 0, c = 0, levels = 0, batch = 0;
   bool debug = false;
   int W[hmdev::kMaxLevels] = {}, H[hmdev::kMaxLevels] = {};
-  const float* mask0 = nullptr;
+  const float* mask0 = nullptr;                     // float level-0 mask (mask pyramid source)
+  const void* mask0_fast = nullptr;                 // level-0 mask read by the fused up pass
+  int mask0_mode = hmdev::kMask0F32;
   DevBuf g[hmdev::kMaxImages][hmdev::kMaxLevels];  // level >= 1
   DevBuf mask[hmdev::kMaxLevels];                   // level >= 1
   DevBuf r[hmdev::kMaxLevels];                      // level >= 1

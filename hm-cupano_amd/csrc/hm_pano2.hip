@@ -77,56 +77,200 @@ struct Pano2L0 {
   int tiles_x, tiles_y;
 };
 
-// Each thread handles kPx pixels of one canvas row, kBlock.x apart, so a
-// wavefront's loads stay coalesced while every lane keeps 2 * kPx independent
-// map loads (then gathers) in flight.  c1 = canvas after remap 1, c12 =
-// canvas after remap 2 (which overwrites image 2's whole box,
-// cudaPano.inl:291-303).
+// Threads that remap canvas rows handle kPx pixels kBlock.x apart, so a
+// wavefront's loads stay coalesced while every lane keeps several
+// independent map loads (then gathers) in flight.
 constexpr int kPx = 4;
 
+// ---------------------------------------------------------------------------
+// Remap + Gaussian pyramid in one launch.  Workgroups [0, roi_tiles) each take
+// a 32x32 tile of the blend region: they build both level-0 images from the
+// two remaps (the make_full copies of cudaPano.inl:225-236, 306-317), store
+// them as uint8, and reduce them to up to five coarser levels in registers
+// and LDS.  The remaining workgroups remap 256x4 canvas strips outside the
+// blend columns (the final canvas there, cudaPano.inl:210-303).
+// ---------------------------------------------------------------------------
+struct Pano2Fused {
+  Pano2L0 l0;
+  const LevelDesc* lv;  // lv[0] = level 1 ...
+  int n_down;           // Gaussian levels produced here (0..5)
+  int roi_tiles_x, roi_tiles_y;
+  int strips_x0, strips_x1, strips_y;  // canvas strips per row left / right of the blend columns
+};
+
+// Canvas value after remap 1 (c1) at (X, Y), and whether X, Y lies in box i.
 template <int C>
-__global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
-  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
-  const int X0 = (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
-  const int Y = (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
+__device__ __forceinline__ bool in_box(const Pano2L0& a, int i, int X, int Y) {
+  const int rx = X - a.px[i], ry = Y - a.py[i];
+  return rx >= 0 && ry >= 0 && rx < a.mw[i] && ry < a.mh[i];
+}
+template <int C>
+__device__ __forceinline__ uint32_t remap_at(const Pano2L0& a, int i, int b, int X, int Y) {
+  return remap_px<C>(a.src[i], b, a.map[i][(size_t)(Y - a.py[i]) * a.mw[i] + (X - a.px[i])]);
+}
+
+template <int C>
+__global__ void __launch_bounds__(256) pano2_fused_kernel(Pano2Fused f) {
+  __shared__ float4 s1[2][16 * 16];
+  __shared__ float4 s2[2][8 * 8];
+  const Pano2L0& a = f.l0;
   const int b = blockIdx.z;
-  if (Y >= a.ch) return;
-  uint32_t m[2][kPx];
-  bool valid[2][kPx];
+  const int tid = threadIdx.x;
+  const int n_roi = f.roi_tiles_x * f.roi_tiles_y;
+  if ((int)blockIdx.x >= n_roi) {
+    // ---- canvas strip outside the blend columns
+    const int s = (int)blockIdx.x - n_roi;
+    const int per_row = f.strips_x0 + f.strips_x1;
+    const int sy = s / per_row, sx = s % per_row;
+    const int xbase = sx < f.strips_x0 ? sx * 64 * kPx : a.skip_x1 + (sx - f.strips_x0) * 64 * kPx;
+    const int xend = sx < f.strips_x0 ? a.skip_x0 : a.cw;
+    const int Y = sy * 4 + (tid >> 6);
+    if (Y >= a.ch) return;
+    const int X0 = xbase + (tid & 63);
+    uint32_t m[2][kPx];
+    bool valid[2][kPx];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int ry = Y - a.py[i];
-    const bool row_ok = ry >= 0 && ry < a.mh[i];
-    const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
+    for (int i = 0; i < 2; ++i) {
+      const int ry = Y - a.py[i];
+      const bool row_ok = ry >= 0 && ry < a.mh[i];
+      const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
+#pragma unroll
+      for (int j = 0; j < kPx; ++j) {
+        const int X = X0 + 64 * j;
+        const int rx = X - a.px[i];
+        valid[i][j] = row_ok && rx >= 0 && rx < a.mw[i] && X < xend;
+        m[i][j] = valid[i][j] ? mrow[rx] : 0u;
+      }
+    }
+    uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
 #pragma unroll
     for (int j = 0; j < kPx; ++j) {
-      const int rx = X0 + j * (int)blockDim.x - a.px[i];
-      valid[i][j] = row_ok && rx >= 0 && rx < a.mw[i] && X0 + j * (int)blockDim.x < a.cw;
-      m[i][j] = valid[i][j] ? mrow[rx] : 0u;
+      const int X = X0 + 64 * j;
+      if (X >= xend) continue;
+      uint32_t v = 0u;
+      if (valid[1][j])
+        v = remap_px<C>(a.src[1], b, m[1][j]);
+      else if (valid[0][j])
+        v = remap_px<C>(a.src[0], b, m[0][j]);
+      store_px<C>(crow + (size_t)X * C, v);
+    }
+    return;
+  }
+  // ---- blend-region tile: level 0 of both images, then the Gaussian levels
+  const int t = xcd_tile((int)blockIdx.x, n_roi);
+  const int tx = t % f.roi_tiles_x, ty = t / f.roi_tiles_x;
+  const int lx = tid & 15, ly = tid >> 4;
+  const int u0 = tx * 32 + 2 * lx, y0 = ty * 32 + 2 * ly;
+  // Phase 1: every map word this thread needs, issued back to back.
+  //   slot 0: image 1 at X1 (G1 and, when X2 == X1, G2's fallback)
+  //   slot 1: image 2 at X2 (G2)
+  //   slot 2: image 1 at X2 (G2's fallback when X2 != X1)
+  uint32_t mw[3][4];
+  bool use[3][4];
+  bool inr[4], take1[4], take2[4];
+  const bool same_x = a.d[0] == a.d[1];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int u = u0 + (e & 1), y = y0 + (e >> 1);
+    inr[e] = u < a.bw && y < a.bh;
+    const int X1 = u + a.d[0], X2 = u + a.d[1];
+    take1[e] = inr[e] && u >= a.ulo[0] && u < a.uhi[0] && X1 >= 0 && X1 < a.cw && y < a.ch;
+    take2[e] = inr[e] && u >= a.ulo[1] && u < a.uhi[1] && X2 >= 0 && X2 < a.cw && y < a.ch;
+    use[0][e] = take1[e] && in_box<C>(a, 0, X1, y);
+    use[1][e] = take2[e] && in_box<C>(a, 1, X2, y);
+    use[2][e] = take2[e] && !use[1][e] && a.g2_canvas && !(same_x && take1[e]) && in_box<C>(a, 0, X2, y);
+    mw[0][e] = use[0][e] ? a.map[0][(size_t)(y - a.py[0]) * a.mw[0] + (X1 - a.px[0])] : 0u;
+    mw[1][e] = use[1][e] ? a.map[1][(size_t)(y - a.py[1]) * a.mw[1] + (X2 - a.px[1])] : 0u;
+    mw[2][e] = use[2][e] ? a.map[0][(size_t)(y - a.py[0]) * a.mw[0] + (X2 - a.px[0])] : 0u;
+  }
+  // Phase 2: the source gathers.
+  uint32_t rv[3][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    rv[0][e] = use[0][e] ? remap_px<C>(a.src[0], b, mw[0][e]) : 0u;
+    rv[1][e] = use[1][e] ? remap_px<C>(a.src[1], b, mw[1][e]) : 0u;
+    rv[2][e] = use[2][e] ? remap_px<C>(a.src[0], b, mw[2][e]) : 0u;
+  }
+  // Phase 3: the make_full values (cudaPano.inl:225-236, 306-317).
+  uint32_t g[2][4];  // [image][tap (0,0),(1,0),(0,1),(1,1)]
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    g[0][e] = rv[0][e];  // canvas after remap 1 (0 outside image 1's box)
+    uint32_t v2 = 0u;
+    if (use[1][e])
+      v2 = rv[1][e];
+    else if (a.g2_canvas && take2[e])
+      v2 = (same_x && take1[e]) ? rv[0][e] : rv[2][e];  // image 1 shows through outside image 2's box
+    g[1][e] = take2[e] ? v2 : 0u;
+  }
+  // level-0 stores (uint8, exact)
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (!inr[e]) continue;
+    const size_t o = ((size_t)b * a.bh + (y0 + (e >> 1))) * a.bw + (u0 + (e & 1));
+    store_px<C>(a.g0[0] + o * C, g[0][e]);
+    store_px<C>(a.g0[1] + o * C, g[1][e]);
+  }
+  if (f.n_down <= 0) return;
+  // level 1 from the 2x2 block in registers
+  {
+    const LevelDesc& d = f.lv[0];
+    const int x = tx * 16 + lx, y = ty * 16 + ly;
+    if (x < d.w && y < d.h) {
+      const size_t o = ((size_t)b * d.h + y) * d.w + x;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        float4 p[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t v = g[i][e];
+          p[e] = make_float4((float)(v & 0xffu), (float)((v >> 8) & 0xffu), (float)((v >> 16) & 0xffu),
+                             (float)(v >> 24));
+        }
+        const float4 r = down_px<C>(p, inr);
+        s1[i][ly * 16 + lx] = r;
+        st_px<C>(d.g_out[i], o, r);
+      }
     }
   }
-  uint32_t r[2][kPx];
+  // levels 2 .. n_down from LDS
+  int edge = 16;
+  for (int k = 1; k < f.n_down; ++k) {
+    __syncthreads();
+    const LevelDesc& din = f.lv[k - 1];
+    const LevelDesc& d = f.lv[k];
+    const int oedge = edge >> 1;
+    float4 r[2];
+    bool active = false;
+    int qx = 0, qy = 0;
+    if (tid < oedge * oedge) {
+      qx = tid % oedge;
+      qy = tid / oedge;
+      const int x = tx * oedge + qx, y = ty * oedge + qy;
+      if (x < d.w && y < d.h) {
+        active = true;
+        const bool has_x1 = 2 * x + 1 < din.w, has_y1 = 2 * y + 1 < din.h;
+        const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
+        const int sx = 2 * qx, sy = 2 * qy;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < kPx; ++j) r[i][j] = valid[i][j] ? remap_px<C>(a.src[i], b, m[i][j]) : 0u;
-  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
-  uint8_t* g0row = a.g0[0] + ((size_t)b * a.bh + Y) * a.bw * C;
-  uint8_t* g1row = a.g0[1] + ((size_t)b * a.bh + Y) * a.bw * C;
-#pragma unroll
-  for (int j = 0; j < kPx; ++j) {
-    const int X = X0 + j * (int)blockDim.x;
-    if (X >= a.cw) continue;
-    const uint32_t c1 = valid[0][j] ? r[0][j] : 0u;
-    const uint32_t c12 = valid[1][j] ? r[1][j] : c1;
-    if (a.write_canvas && !(X >= a.skip_x0 && X < a.skip_x1)) store_px<C>(crow + (size_t)X * C, c12);
-    if (Y < a.bh) {
-      const int u1 = X - a.d[0];
-      if (u1 >= a.ulo[0] && u1 < a.uhi[0]) store_px<C>(g0row + (size_t)u1 * C, c1);
-      const int u2 = X - a.d[1];
-      if (u2 >= a.ulo[1] && u2 < a.uhi[1])
-        store_px<C>(g1row + (size_t)u2 * C, a.g2_canvas ? c12 : (valid[1][j] ? r[1][j] : 0u));
+        for (int i = 0; i < 2; ++i) {
+          const float4* src = k == 1 ? s1[i] : s2[i];
+          float4 p[4];
+          p[0] = src[sy * edge + sx];
+          if (has_x1) p[1] = src[sy * edge + sx + 1];
+          if (has_y1) p[2] = src[(sy + 1) * edge + sx];
+          if (has_x1 && has_y1) p[3] = src[(sy + 1) * edge + sx + 1];
+          r[i] = down_px<C>(p, in_range);
+          st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r[i]);
+        }
+      }
     }
+    __syncthreads();
+    if (active) {
+      s2[0][qy * oedge + qx] = r[0];
+      s2[1][qy * oedge + qx] = r[1];
+    }
+    edge = oedge;
   }
 }
 
@@ -250,6 +394,8 @@ struct Pano2 : hmStitchPano {
   hm::DevBuf map[2];
   hm::DevBuf seam;  // hard seam, canvas-sized
   std::vector<float> mask_host;
+  std::vector<uint8_t> mask_u8_host;  // the same level-0 mask as uint8 (the seam values)
+  hm::DevBuf mask_u8;
   Pano2L0 l0{};
 
   hipError_t setup(const hmControlMasks2& m, int minimize_blend);
@@ -324,8 +470,13 @@ hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
   }
   // convertMaskMat crop + convertTo(float) (cudaPano.inl:62-77).
   mask_host.resize((size_t)bw * bh);
+  mask_u8_host.resize((size_t)bw * bh);
   for (int y = 0; y < bh; ++y)
-    for (int x = 0; x < bw; ++x) mask_host[(size_t)y * bw + x] = (float)padded[(size_t)y * canvas_w + bx0 + x];
+    for (int x = 0; x < bw; ++x) {
+      const uint8_t v = padded[(size_t)y * canvas_w + bx0 + x];
+      mask_u8_host[(size_t)y * bw + x] = v;
+      mask_host[(size_t)y * bw + x] = (float)v;
+    }
   for (int i = 0; i < 2; ++i) {
     HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
     // Columns the per-frame kernel never writes are the zero of the
@@ -337,7 +488,12 @@ hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
 
 hipError_t Pano2::ensure_pyramid() {
   if (hard_seam || initialized_pyramid) return hipSuccess;
-  return init_pyramid(kBlend2, bw, bh, mask_host);
+  HM_RETURN_IF_ERROR(init_pyramid(kBlend2, bw, bh, mask_host));
+  HM_RETURN_IF_ERROR(mask_u8.alloc(mask_u8_host.size()));
+  HM_RETURN_IF_ERROR(hipMemcpy(mask_u8.p, mask_u8_host.data(), mask_u8_host.size(), hipMemcpyHostToDevice));
+  pyr.mask0_fast = mask_u8.p;  // float(seam byte) == the float mask, 1 byte per pixel
+  pyr.mask0_mode = kMask0U8;
+  return hipSuccess;
 }
 
 hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s) {
@@ -346,9 +502,9 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     return hipErrorInvalidValue;
   if ((int)canvas.width != canvas_w || (int)canvas.height != canvas_h) return hipErrorInvalidValue;
   const Surf s1 = to_surf(in1), s2 = to_surf(in2);
-  const hm::TileGrid tg = hm::tile_grid(canvas_w, canvas_h, batch, dim3(kBlock.x * kPx, kBlock.y, 1));
-  const dim3 grid = tg.grid;
   if (hard_seam) {
+    const hm::TileGrid tg = hm::tile_grid(canvas_w, canvas_h, batch, dim3(kBlock.x * kPx, kBlock.y, 1));
+    const dim3 grid = tg.grid;
     Pano2Hard a{};
     a.src[0] = s1;
     a.src[1] = s2;
@@ -377,7 +533,9 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     return hipGetLastError();
   }
   HM_RETURN_IF_ERROR(ensure_pyramid());
-  Pano2L0 a = l0;
+  Pano2Fused f{};
+  Pano2L0& a = f.l0;
+  a = l0;
   a.src[0] = s1;
   a.src[1] = s2;
   for (int i = 0; i < 2; ++i) {
@@ -396,14 +554,22 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   a.ch = canvas_h;
   a.bw = bw;
   a.bh = bh;
-  a.tiles_x = tg.tiles_x;
-  a.tiles_y = tg.tiles_y;
+  f.lv = pyr.levels_table();
+  f.n_down = std::min(5, pyr.levels - 1);
+  f.roi_tiles_x = (bw + 31) / 32;
+  f.roi_tiles_y = (bh + 31) / 32;
+  const int strip = 64 * kPx;
+  f.strips_x0 = a.write_canvas ? (a.skip_x0 + strip - 1) / strip : 0;
+  f.strips_x1 = a.write_canvas ? (canvas_w - a.skip_x1 + strip - 1) / strip : 0;
+  f.strips_y = (canvas_h + 3) / 4;
+  const int n_strips = (f.strips_x0 + f.strips_x1) * f.strips_y;
+  const dim3 fgrid((unsigned)(f.roi_tiles_x * f.roi_tiles_y + n_strips), 1u, (unsigned)batch);
   timer.begin(s);
   if (channels == 4)
-    hipLaunchKernelGGL(pano2_level0_kernel<4>, grid, kBlock, 0, s, a);
+    hipLaunchKernelGGL(pano2_fused_kernel<4>, fgrid, dim3(256), 0, s, f);
   else
-    hipLaunchKernelGGL(pano2_level0_kernel<3>, grid, kBlock, 0, s, a);
-  timer.end(s, "remap_l0");
+    hipLaunchKernelGGL(pano2_fused_kernel<3>, fgrid, dim3(256), 0, s, f);
+  timer.end(s, "remap_down");
   HM_RETURN_IF_ERROR(hipGetLastError());
 
   const void* g[2] = {g0[0].p, g0[1].p};
@@ -414,7 +580,10 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   out.u8_h = canvas_h;
   out.ox = minimize ? x2 - kOverlapPad : 0;  // copy-back offset (cudaPano.inl:408)
   out.oy = 0;
-  return pyr.run(g, true, out, s, timer_or_null());
+  if (pyr.debug) return pyr.run(g, true, out, s, timer_or_null());  // per-level path keeping every level
+  // Levels beyond the first five continue from the last level produced above.
+  HM_RETURN_IF_ERROR(pyr.run_down(g, true, f.n_down, s, timer_or_null()));
+  return pyr.run_up(g, true, out, s, timer_or_null());
 }
 
 }  // namespace

@@ -204,7 +204,7 @@ hipError_t launch_down_fused(int c, int n, bool in_u8, const void* const* g0, in
   return hipErrorInvalidValue;
 }
 
-template <typename T0, int C, int N, int KIND>
+template <typename T0, int C, int N, int KIND, int MODE>
 static hipError_t up_fused_t(const UpFusedLaunch& u, int batch, hipStream_t s) {
   if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
   UpFusedArgs<T0, N> a;
@@ -227,25 +227,43 @@ static hipError_t up_fused_t(const UpFusedLaunch& u, int batch, hipStream_t s) {
   a.win_h = u.win_h;
   a.tiles_x = (u.win_w + kUpTile - 1) / kUpTile;
   a.tiles_y = (u.win_h + kUpTile - 1) / kUpTile;
-  hipLaunchKernelGGL((up_fused_kernel<T0, C, N, KIND>), dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch),
-                     dim3(256), 0, s, a);
+  hipLaunchKernelGGL((up_fused_kernel<T0, C, N, KIND, MODE>),
+                     dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch), dim3(256), 0, s, a);
   return hipGetLastError();
+}
+
+template <typename T0, int C, int MODE>
+static hipError_t up_fused_n(int n, const UpFusedLaunch& u, int batch, hipStream_t s) {
+  switch (n) {
+    case 2: return up_fused_t<T0, C, 2, kBlendN, MODE>(u, batch, s);
+    case 3: return up_fused_t<T0, C, 3, kBlendN, MODE>(u, batch, s);
+    case 4: return up_fused_t<T0, C, 4, kBlendN, MODE>(u, batch, s);
+    case 5: return up_fused_t<T0, C, 5, kBlendN, MODE>(u, batch, s);
+    case 6: return up_fused_t<T0, C, 6, kBlendN, MODE>(u, batch, s);
+    case 7: return up_fused_t<T0, C, 7, kBlendN, MODE>(u, batch, s);
+    case 8: return up_fused_t<T0, C, 8, kBlendN, MODE>(u, batch, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <typename T0, int C>
 static hipError_t up_fused_c(int n, int kind, const UpFusedLaunch& u, int batch, hipStream_t s) {
-  if (kind == kBlend2) return n == 2 ? up_fused_t<T0, C, 2, kBlend2>(u, batch, s) : hipErrorInvalidValue;
-  if (kind == kBlend3) return n == 3 ? up_fused_t<T0, C, 3, kBlend3>(u, batch, s) : hipErrorInvalidValue;
-  switch (n) {
-    case 2: return up_fused_t<T0, C, 2, kBlendN>(u, batch, s);
-    case 3: return up_fused_t<T0, C, 3, kBlendN>(u, batch, s);
-    case 4: return up_fused_t<T0, C, 4, kBlendN>(u, batch, s);
-    case 5: return up_fused_t<T0, C, 5, kBlendN>(u, batch, s);
-    case 6: return up_fused_t<T0, C, 6, kBlendN>(u, batch, s);
-    case 7: return up_fused_t<T0, C, 7, kBlendN>(u, batch, s);
-    case 8: return up_fused_t<T0, C, 8, kBlendN>(u, batch, s);
-    default: return hipErrorInvalidValue;
+  const int mode = u.mask0_mode;
+  if (kind == kBlend2) {
+    if (n != 2) return hipErrorInvalidValue;
+    if (mode == kMask0U8) return up_fused_t<T0, C, 2, kBlend2, kMask0U8>(u, batch, s);
+    if (mode == kMask0F32) return up_fused_t<T0, C, 2, kBlend2, kMask0F32>(u, batch, s);
+    return hipErrorInvalidValue;
   }
+  if (kind == kBlend3) {
+    if (n != 3) return hipErrorInvalidValue;
+    if (mode == kMask0Label) return up_fused_t<T0, C, 3, kBlend3, kMask0Label>(u, batch, s);
+    if (mode == kMask0F32) return up_fused_t<T0, C, 3, kBlend3, kMask0F32>(u, batch, s);
+    return hipErrorInvalidValue;
+  }
+  if (mode == kMask0Label) return up_fused_n<T0, C, kMask0Label>(n, u, batch, s);
+  if (mode == kMask0F32) return up_fused_n<T0, C, kMask0F32>(n, u, batch, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_up_fused(int c, int n, int kind, bool g0_u8, const UpFusedLaunch& u, int batch, hipStream_t s) {
@@ -380,44 +398,54 @@ hipError_t Pyramid::build_mask_pyramid(const float* mask0_, hipStream_t s) {
   return hipSuccess;
 }
 
+hipError_t Pyramid::run_down(const void* const* g0, bool g0_u8, int from, hipStream_t s, LaunchTimer* timer) {
+  // Up to five levels per launch from aligned tiles.
+  const LevelDesc* table = level_table.as<LevelDesc>();
+  while (from + 1 < levels) {
+    const int n_out = std::min(5, levels - 1 - from);
+    const void* in[kMaxImages];
+    for (int i = 0; i < n; ++i) in[i] = from == 0 ? g0[i] : g[i][from].p;
+    if (timer) timer->begin(s);
+    HM_RETURN_IF_ERROR(
+        launch_down_fused(c, n, from == 0 && g0_u8, in, W[from], H[from], n_out, table + from, batch, s));
+    if (timer) timer->end(s, "down_fused");
+    from += n_out;
+  }
+  return hipSuccess;
+}
+
+hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s,
+                           LaunchTimer* timer) {
+  // The whole reconstruction per level-0 tile in one launch.
+  UpFusedLaunch u;
+  for (int i = 0; i < n; ++i) u.g0[i] = g0[i];
+  u.mask0 = mask0_fast ? mask0_fast : mask0;
+  u.mask0_mode = mask0_fast ? mask0_mode : kMask0F32;
+  u.w0 = W[0];
+  u.h0 = H[0];
+  u.levels = levels;
+  u.lv = level_table.as<LevelDesc>();
+  u.f32_out = out.f32;
+  u.u8_out = out.u8;
+  u.u8_pitch = out.u8_pitch;
+  u.u8_w = out.u8_w;
+  u.u8_h = out.u8_h;
+  u.ox = out.ox;
+  u.oy = out.oy;
+  u.win_x = out.win_w >= 0 ? out.win_x : 0;
+  u.win_y = out.win_w >= 0 ? out.win_y : 0;
+  u.win_w = out.win_w >= 0 ? out.win_w : W[0];
+  u.win_h = out.win_w >= 0 ? out.win_h : H[0];
+  if (timer) timer->begin(s);
+  HM_RETURN_IF_ERROR(launch_up_fused(c, n, kind, g0_u8, u, batch, s));
+  if (timer) timer->end(s, "up_fused");
+  return hipSuccess;
+}
+
 hipError_t Pyramid::run(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s, LaunchTimer* timer) {
   if (!debug) {
-    // Down pass: up to five levels per launch from aligned tiles.
-    const LevelDesc* table = level_table.as<LevelDesc>();
-    int from = 0;
-    while (from + 1 < levels) {
-      const int n_out = std::min(5, levels - 1 - from);
-      const void* in[kMaxImages];
-      for (int i = 0; i < n; ++i) in[i] = from == 0 ? g0[i] : g[i][from].p;
-      if (timer) timer->begin(s);
-      HM_RETURN_IF_ERROR(launch_down_fused(c, n, from == 0 && g0_u8, in, W[from], H[from], n_out, table + from,
-                                           batch, s));
-      if (timer) timer->end(s, "down_fused");
-      from += n_out;
-    }
-    // Up pass: the whole reconstruction per level-0 tile in one launch.
-    UpFusedLaunch u;
-    for (int i = 0; i < n; ++i) u.g0[i] = g0[i];
-    u.mask0 = mask0;
-    u.w0 = W[0];
-    u.h0 = H[0];
-    u.levels = levels;
-    u.lv = table;
-    u.f32_out = out.f32;
-    u.u8_out = out.u8;
-    u.u8_pitch = out.u8_pitch;
-    u.u8_w = out.u8_w;
-    u.u8_h = out.u8_h;
-    u.ox = out.ox;
-    u.oy = out.oy;
-    u.win_x = out.win_w >= 0 ? out.win_x : 0;
-    u.win_y = out.win_w >= 0 ? out.win_y : 0;
-    u.win_w = out.win_w >= 0 ? out.win_w : W[0];
-    u.win_h = out.win_w >= 0 ? out.win_h : H[0];
-    if (timer) timer->begin(s);
-    HM_RETURN_IF_ERROR(launch_up_fused(c, n, kind, g0_u8, u, batch, s));
-    if (timer) timer->end(s, "up_fused");
-    return hipSuccess;
+    HM_RETURN_IF_ERROR(run_down(g0, g0_u8, 0, s, timer));
+    return run_up(g0, g0_u8, out, s, timer);
   }
   // Debug: per-level launches that also keep the Laplacian and blended levels.
   // 1. Down pass: every image's level k -> k+1 in one launch per level.

@@ -9,7 +9,8 @@
 //   the blend and the reconstruction on the tile's footprint at each level
 //   (plus the one-pixel halo the top-left aligned upsample needs), keeping
 //   R_k in LDS; only level 0 is written, straight to the caller's output.
-//   Level 0 is processed in 2x2 quads, whose four pixels share their taps.
+//   Every level is processed in 2x2 quads, whose four pixels share their
+//   taps and have fixed upsample positions (dx, dy) in {0, 1/2}^2.
 // Per-pixel arithmetic is the same device code as the per-level kernels
 // (hm_pyramid_kernels.hpp), in the same order.
 #pragma once
@@ -27,34 +28,78 @@ struct LevelDesc {
 };
 
 // ---------------------------------------------------------------------------
-// tap-value forms of the upsample helpers
+// Upsample of the pixels of a 2x2 quad from its four shared taps.
+// Pixel e of the quad is (e & 1, e >> 1) and sits at the fixed fractional
+// position (dx, dy) = ((e & 1) / 2, (e >> 1) / 2) between taps 00, 10, 01,
+// 11.  These forms give the same bits as the general bilinear with those
+// constants (cudaBlend.cu:416-449, :649-666): a tap of weight 0 adds +0 or -0
+// to a running sum that starts at +0 and, rounding to nearest, never becomes
+// -0, so skipping it never changes the sum; a tap of weight 1 is its own
+// product; and dividing by a power-of-two weight sum is exact.  Values can
+// differ from the general form only in the sign of a zero, which no later
+// step (comparisons, sums with non-zero terms, conversion to uint8) can see.
 // ---------------------------------------------------------------------------
-template <int C>
-__device__ __forceinline__ float4 up_alpha_vals(const float4& p00, const float4& p10, const float4& p01,
-                                                const float4& p11, float dx, float dy) {
-  const float w00 = (1.0f - dx) * (1.0f - dy);
-  const float w10 = dx * (1.0f - dy);
-  const float w01 = (1.0f - dx) * dy;
-  const float w11 = dx * dy;
-  float sw = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  if (C != 4 || p00.w != 0.0f) { sw += w00; sx += p00.x * w00; sy += p00.y * w00; sz += p00.z * w00; }
-  if (C != 4 || p10.w != 0.0f) { sw += w10; sx += p10.x * w10; sy += p10.y * w10; sz += p10.z * w10; }
-  if (C != 4 || p01.w != 0.0f) { sw += w01; sx += p01.x * w01; sy += p01.y * w01; sz += p01.z * w01; }
-  if (C != 4 || p11.w != 0.0f) { sw += w11; sx += p11.x * w11; sy += p11.y * w11; sz += p11.z * w11; }
-  if (sw > 0.0f) return make_float4(div_exact(sx, sw), div_exact(sy, sw), div_exact(sz, sw), 0.0f);
+__device__ __forceinline__ float4 rgb_of(const float4& p) { return make_float4(p.x, p.y, p.z, 0.0f); }
+__device__ __forceinline__ float4 half_pair(const float4& p, const float4& q) {
+  return make_float4(p.x * 0.5f + q.x * 0.5f, p.y * 0.5f + q.y * 0.5f, p.z * 0.5f + q.z * 0.5f, 0.0f);
+}
+// Two taps of weight 1/2 each; a lone valid tap has weight sum 1/2 and gives
+// (v * 0.5) * 2 == v.
+__device__ __forceinline__ float4 pair_alpha(bool a, const float4& p, bool b, const float4& q) {
+  if (a && b) return half_pair(p, q);
+  if (a) return rgb_of(p);
+  if (b) return rgb_of(q);
   return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-template <int C>
-__device__ __forceinline__ float4 up_lap_vals(const float4& p00, const float4& p10, const float4& p01,
-                                              const float4& p11, float dx, float dy) {
-  if constexpr (C == 4) {
-    return up_alpha_vals<4>(p00, p10, p01, p11, dx, dy);
-  } else {
-    return make_float4(lerp2(p00.x, p10.x, p01.x, p11.x, dx, dy), lerp2(p00.y, p10.y, p01.y, p11.y, dx, dy),
-                       lerp2(p00.z, p10.z, p01.z, p11.z, dx, dy), 0.0f);
-  }
+// Alpha-aware form; ALPHA = false takes every tap (3-channel reconstruct).
+template <bool ALPHA>
+__device__ __forceinline__ float4 up_quad_alpha(int e, const float4& p00, const float4& p10, const float4& p01,
+                                                const float4& p11) {
+  const bool a00 = !ALPHA || p00.w != 0.0f, a10 = !ALPHA || p10.w != 0.0f;
+  const bool a01 = !ALPHA || p01.w != 0.0f, a11 = !ALPHA || p11.w != 0.0f;
+  if (e == 0) return a00 ? rgb_of(p00) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (e == 1) return pair_alpha(a00, p00, a10, p10);
+  if (e == 2) return pair_alpha(a00, p00, a01, p01);
+  // every tap with weight 1/4, summed in the order 00, 10, 01, 11
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  int n = 0;
+  if (a00) { sx += p00.x * 0.25f; sy += p00.y * 0.25f; sz += p00.z * 0.25f; ++n; }
+  if (a10) { sx += p10.x * 0.25f; sy += p10.y * 0.25f; sz += p10.z * 0.25f; ++n; }
+  if (a01) { sx += p01.x * 0.25f; sy += p01.y * 0.25f; sz += p01.z * 0.25f; ++n; }
+  if (a11) { sx += p11.x * 0.25f; sy += p11.y * 0.25f; sz += p11.z * 0.25f; ++n; }
+  if (n == 3) return make_float4(sx / 0.75f, sy / 0.75f, sz / 0.75f, 0.0f);
+  const float k = n == 1 ? 4.0f : (n == 2 ? 2.0f : (n == 4 ? 1.0f : 0.0f));  // n == 0: +0 * 0
+  return make_float4(sx * k, sy * k, sz * k, 0.0f);
 }
+
+// The plain two-stage lerp of the 3-channel Laplacian (cudaBlend.cu:446-449).
+__device__ __forceinline__ float4 up_quad_lerp(int e, const float4& p00, const float4& p10, const float4& p01,
+                                               const float4& p11) {
+  if (e == 0) return rgb_of(p00);
+  if (e == 1) return half_pair(p00, p10);
+  if (e == 2) return half_pair(p00, p01);
+  return half_pair(half_pair(p00, p10), half_pair(p01, p11));
+}
+
+// Upsample of the Laplacian (alpha-aware for RGBA, plain for RGB) and of the
+// reconstruction (alpha-aware for RGBA, every tap for RGB).
+template <int C>
+__device__ __forceinline__ float4 up_quad_lap(int e, const float4* p) {
+  if constexpr (C == 4)
+    return up_quad_alpha<true>(e, p[0], p[1], p[2], p[3]);
+  else
+    return up_quad_lerp(e, p[0], p[1], p[2], p[3]);
+}
+template <int C>
+__device__ __forceinline__ float4 up_quad_rec(int e, const float4* p) {
+  return up_quad_alpha<C == 4>(e, p[0], p[1], p[2], p[3]);
+}
+
+// How the level-0 mask is stored.  The values are the same floats the
+// reference's float mask holds: a uint8 seam converted to float is exact, and
+// a one-hot mask is 1.0 for the pixel's label and 0.0 elsewhere.
+enum Mask0Mode { kMask0F32 = 0, kMask0U8 = 1, kMask0Label = 2 };
 
 template <int C, int N, int KIND>
 __device__ __forceinline__ float4 blend_any(const float4* L, const float* mask, size_t mpix) {
@@ -166,10 +211,30 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
 constexpr int kUpTile = 64;                        // level-0 tile edge
 constexpr int kUpRegion = kUpTile / 2 + 2;         // max footprint edge at levels >= 1
 
+template <int C, int N, int KIND, int MODE>
+__device__ __forceinline__ float4 blend_level0(const float4* L, const void* mask, size_t mpix) {
+  if constexpr (MODE == kMask0F32) {
+    return blend_any<C, N, KIND>(L, static_cast<const float*>(mask), mpix);
+  } else if constexpr (MODE == kMask0U8) {
+    static_assert(KIND == kBlend2, "a one-channel uint8 mask belongs to the 2-image blend");
+    return blend2_px<C>(L[0], L[1], (float)static_cast<const uint8_t*>(mask)[mpix]);
+  } else {
+    static_assert(KIND != kBlend2, "label masks belong to the 3- and N-image blends");
+    const int label = static_cast<const uint8_t*>(mask)[mpix];
+    float mp[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) mp[i] = (label == i) ? 1.0f : 0.0f;
+    if constexpr (KIND == kBlend3)
+      return blend3_px<C>(L, mp);
+    else
+      return blendN_px<C, N>(L, mp);
+  }
+}
+
 template <typename T0, int N>
 struct UpFusedArgs {
   const T0* g0[N];            // level-0 images
-  const float* mask0;         // level-0 mask
+  const void* mask0;          // level-0 mask (see Mask0Mode)
   int w0, h0;
   int levels;                 // total levels (>= 1)
   const LevelDesc* lv;        // lv[k - 1] describes level k
@@ -181,7 +246,7 @@ struct UpFusedArgs {
   int tiles_x, tiles_y;
 };
 
-template <typename T0, int C, int N, int KIND>
+template <typename T0, int C, int N, int KIND, int MODE>
 __global__ void __launch_bounds__(256) up_fused_kernel(UpFusedArgs<T0, N> a) {
   __shared__ float4 buf[2][kUpRegion * kUpRegion];
   __shared__ int reg[kMaxLevels][4];  // footprint [x0, x1] x [y0, y1] per level
@@ -207,117 +272,102 @@ __global__ void __launch_bounds__(256) up_fused_kernel(UpFusedArgs<T0, N> a) {
   }
   __syncthreads();
 
-  // ---- levels L-1 .. 1: R_k on the footprint, in LDS
   int cur = 0;
-  for (int k = L - 1; k >= 1; --k) {
+  // ---- coarsest level L-1 >= 1: R = blend of the Gaussians, per pixel
+  if (L > 1) {
+    const int k = L - 1;
     const LevelDesc& d = a.lv[k - 1];
     const int rx0 = reg[k][0], ry0 = reg[k][2];
     const int rw = reg[k][1] - rx0 + 1, rh = reg[k][3] - ry0 + 1;
-    const bool coarsest = k == L - 1;
-    int nx0 = 0, ny0 = 0, nrw = 1;
-    const LevelDesc* dn = nullptr;
-    if (!coarsest) {
-      dn = &a.lv[k];
-      nx0 = reg[k + 1][0];
-      ny0 = reg[k + 1][2];
-      nrw = reg[k + 1][1] - nx0 + 1;
-    }
     for (int idx = tid; idx < rw * rh; idx += blockDim.x) {
       const int x = rx0 + idx % rw, y = ry0 + idx / rw;
       const size_t pix = ((size_t)b * d.h + y) * d.w + x;
       float4 Lv[N];
-      float4 R;
-      if (coarsest) {
 #pragma unroll
-        for (int i = 0; i < N; ++i) Lv[i] = ld_px<C>(d.g[i], pix);
-        R = blend_any<C, N, KIND>(Lv, d.mask, (size_t)y * d.w + x);
-      } else {
-        const int qx = x >> 1, qy = y >> 1;
-        const int qx1 = min(qx + 1, dn->w - 1), qy1 = min(qy + 1, dn->h - 1);
-        const float fdx = (x & 1) ? 0.5f : 0.0f, fdy = (y & 1) ? 0.5f : 0.0f;
-        const size_t l0 = ((size_t)b * dn->h + qy) * dn->w, l1 = ((size_t)b * dn->h + qy1) * dn->w;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          const float4 g = ld_px<C>(d.g[i], pix);
-          const float4 up = up_lap_vals<C>(ld_px<C>(dn->g[i], l0 + qx), ld_px<C>(dn->g[i], l0 + qx1),
-                                           ld_px<C>(dn->g[i], l1 + qx), ld_px<C>(dn->g[i], l1 + qx1), fdx, fdy);
-          Lv[i] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
-        }
-        const float4 B = blend_any<C, N, KIND>(Lv, d.mask, (size_t)y * d.w + x);
-        const float4* prev = buf[cur ^ 1];
-        const int lx = qx - nx0, lx1 = qx1 - nx0, ly = qy - ny0, ly1 = qy1 - ny0;
-        const float4 up = up_alpha_vals<C>(prev[ly * nrw + lx], prev[ly * nrw + lx1], prev[ly1 * nrw + lx],
-                                           prev[ly1 * nrw + lx1], fdx, fdy);
-        R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
-      }
-      buf[cur][idx] = R;
+      for (int i = 0; i < N; ++i) Lv[i] = ld_px<C>(d.g[i], pix);
+      buf[cur][idx] = blend_any<C, N, KIND>(Lv, d.mask, (size_t)y * d.w + x);
     }
     __syncthreads();
     cur ^= 1;
   }
 
-  // ---- level 0 in 2x2 quads sharing their level-1 taps
-  const int qx0 = ax0 >> 1, qy0 = ay0 >> 1;
-  const int nqx = (bx0 >> 1) - qx0 + 1, nqy = (by0 >> 1) - qy0 + 1;
-  const float4* prev = buf[cur ^ 1];
-  int nx0 = 0, ny0 = 0, nrw = 1;
-  const LevelDesc* dn = nullptr;
-  if (L > 1) {
-    dn = &a.lv[0];
-    nx0 = reg[1][0];
-    ny0 = reg[1][2];
-    nrw = reg[1][1] - nx0 + 1;
-  }
-  for (int q = tid; q < nqx * nqy; q += blockDim.x) {
-    const int qx = qx0 + q % nqx, qy = qy0 + q / nqx;
-    float4 gt[N][4];
-    float4 rt[4];
-    if (L > 1) {
-      const int qx1 = min(qx + 1, dn->w - 1), qy1 = min(qy + 1, dn->h - 1);
-      const size_t l0 = ((size_t)b * dn->h + qy) * dn->w, l1 = ((size_t)b * dn->h + qy1) * dn->w;
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        gt[i][0] = ld_px<C>(dn->g[i], l0 + qx);
-        gt[i][1] = ld_px<C>(dn->g[i], l0 + qx1);
-        gt[i][2] = ld_px<C>(dn->g[i], l1 + qx);
-        gt[i][3] = ld_px<C>(dn->g[i], l1 + qx1);
-      }
-      const int lx = qx - nx0, lx1 = qx1 - nx0, ly = qy - ny0, ly1 = qy1 - ny0;
-      rt[0] = prev[ly * nrw + lx];
-      rt[1] = prev[ly * nrw + lx1];
-      rt[2] = prev[ly1 * nrw + lx];
-      rt[3] = prev[ly1 * nrw + lx1];
+  // ---- levels L-2 .. 0 in 2x2 quads; level k reads level k+1's Gaussian
+  // taps from global memory and its R taps from LDS.
+  for (int level = L > 1 ? L - 2 : 0; level >= 0; --level) {
+    const bool has_next = level + 1 < L;  // a coarser level exists
+    const LevelDesc* d = level > 0 ? &a.lv[level - 1] : nullptr;
+    const LevelDesc* dn = has_next ? &a.lv[level] : nullptr;
+    const int lw = level > 0 ? d->w : a.w0, lh = level > 0 ? d->h : a.h0;
+    const int rx0 = reg[level][0], rx1 = reg[level][1], ry0 = reg[level][2], ry1 = reg[level][3];
+    const int rw = rx1 - rx0 + 1;
+    const int qx0 = rx0 >> 1, qy0 = ry0 >> 1;
+    const int nqx = (rx1 >> 1) - qx0 + 1, nqy = (ry1 >> 1) - qy0 + 1;
+    int nx0 = 0, ny0 = 0, nrw = 1;
+    if (has_next) {
+      nx0 = reg[level + 1][0];
+      ny0 = reg[level + 1][2];
+      nrw = reg[level + 1][1] - nx0 + 1;
     }
+    const float4* prev = buf[cur ^ 1];
+    for (int q = tid; q < nqx * nqy; q += blockDim.x) {
+      const int qx = qx0 + q % nqx, qy = qy0 + q / nqx;
+      float4 gt[N][4];
+      float4 rt[4];
+      if (has_next) {
+        const int qx1 = min(qx + 1, dn->w - 1), qy1 = min(qy + 1, dn->h - 1);
+        const size_t l0 = ((size_t)b * dn->h + qy) * dn->w, l1 = ((size_t)b * dn->h + qy1) * dn->w;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
-      if (x < ax0 || x > bx0 || y < ay0 || y > by0) continue;
-      const float fdx = (e & 1) ? 0.5f : 0.0f, fdy = (e >> 1) ? 0.5f : 0.0f;
-      const size_t pix = ((size_t)b * a.h0 + y) * a.w0 + x;
-      float4 Lv[N];
+        for (int i = 0; i < N; ++i) {
+          gt[i][0] = ld_px<C>(dn->g[i], l0 + qx);
+          gt[i][1] = ld_px<C>(dn->g[i], l0 + qx1);
+          gt[i][2] = ld_px<C>(dn->g[i], l1 + qx);
+          gt[i][3] = ld_px<C>(dn->g[i], l1 + qx1);
+        }
+        const int lx = qx - nx0, lx1 = qx1 - nx0, ly = qy - ny0, ly1 = qy1 - ny0;
+        rt[0] = prev[ly * nrw + lx];
+        rt[1] = prev[ly * nrw + lx1];
+        rt[2] = prev[ly1 * nrw + lx];
+        rt[3] = prev[ly1 * nrw + lx1];
+      }
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const float4 g = ld_px<C>(a.g0[i], pix);
-        if (L > 1) {
-          const float4 up = up_lap_vals<C>(gt[i][0], gt[i][1], gt[i][2], gt[i][3], fdx, fdy);
-          Lv[i] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+      for (int e = 0; e < 4; ++e) {
+        const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+        if (x < rx0 || x > rx1 || y < ry0 || y > ry1) continue;
+        const size_t pix = ((size_t)b * lh + y) * lw + x;
+        float4 Lv[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const float4 g = level > 0 ? ld_px<C>(d->g[i], pix) : ld_px<C>(a.g0[i], pix);
+          if (has_next) {
+            const float4 up = up_quad_lap<C>(e, gt[i]);
+            Lv[i] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+          } else {
+            Lv[i] = g;
+          }
+        }
+        const size_t mpix = (size_t)y * lw + x;
+        const float4 B = level > 0 ? blend_any<C, N, KIND>(Lv, d->mask, mpix)
+                                   : blend_level0<C, N, KIND, MODE>(Lv, a.mask0, mpix);
+        float4 R = B;
+        if (has_next) {
+          const float4 up = up_quad_rec<C>(e, rt);
+          R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+        }
+        if (level > 0) {
+          buf[cur][(y - ry0) * rw + (x - rx0)] = R;
         } else {
-          Lv[i] = g;
+          if (a.f32_out) st_px<C>(a.f32_out, pix, R);
+          if (a.u8_out) {
+            const int X = x + a.ox, Y = y + a.oy;
+            if (X < a.u8_w && Y < a.u8_h)
+              st_px_u8<C>(a.u8_out + ((size_t)b * a.u8_h + Y) * a.u8_pitch + (size_t)X * C, R);
+          }
         }
       }
-      const float4 B = blend_any<C, N, KIND>(Lv, a.mask0, (size_t)y * a.w0 + x);
-      float4 R = B;
-      if (L > 1) {
-        const float4 up = up_alpha_vals<C>(rt[0], rt[1], rt[2], rt[3], fdx, fdy);
-        R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
-      }
-      if (a.f32_out) st_px<C>(a.f32_out, pix, R);
-      if (a.u8_out) {
-        const int X = x + a.ox, Y = y + a.oy;
-        if (X < a.u8_w && Y < a.u8_h)
-          st_px_u8<C>(a.u8_out + ((size_t)b * a.u8_h + Y) * a.u8_pitch + (size_t)X * C, R);
-      }
     }
+    if (level == 0) break;
+    __syncthreads();
+    cur ^= 1;
   }
 }
 
