@@ -56,18 +56,40 @@ def algorithmic_bytes(rig, levels, pano_levels):
     w1 = rig.map_x[0].shape[1]
     bw = w1 - x2 + 256
     bh = ch
+    rx0, rx1 = x2 - 128, x2 - 128 + bw  # canvas columns feeding the blend inputs
     a = [(bw, bh)]
     for _ in range(1, pano_levels):
         a.append(((a[-1][0] + 1) // 2, (a[-1][1] + 1) // 2))
     A = [w * h for w, h in a]
     coarse = sum(A[1:])  # pixels of levels >= 1
-    # remap + Gaussian pyramid: maps, valid source pixels, canvas outside the
-    # blend columns, level 0 of both images (uint8), levels >= 1 (float4)
-    out["remap_down"] = (sum(map_px) * 4 + sum(valid_px) * 4 + (cw - bw) * ch * 4 + 2 * A[0] * 4
-                         + 2 * coarse * 16)
-    # reconstruction: both images' levels, the masks (level 0 uint8, others
-    # float), the blend-region canvas write
-    out["up_fused"] = 2 * A[0] * 4 + 2 * coarse * 16 + A[0] * 1 + coarse * 4 + A[0] * 4
+    # map words and valid source pixels inside / outside the blend columns
+    roi_map = roi_valid = 0
+    for i in range(2):
+        c0 = max(0, rx0 - rig.pos_x[i])
+        c1 = min(rig.map_x[i].shape[1], rx1 - rig.pos_x[i])
+        if c1 > c0:
+            sub = rig.map_x[i][:, c0:c1]
+            roi_map += sub.size
+            roi_valid += int((sub != 65535).sum())
+    # remap of the blend columns: maps, valid source pixels, both level-0 images (uint8)
+    out["remap_roi"] = roi_map * 4 + roi_valid * 4 + 2 * A[0] * 4
+    # remap of the rest of the canvas: maps, valid source pixels, canvas writes
+    out["remap_canvas"] = ((sum(map_px) - roi_map) * 4 + (sum(valid_px) - roi_valid) * 4
+                           + (cw - (rx1 - rx0)) * ch * 4)
+    # Gaussian levels >= 1 of both images from level 0
+    out["down_fused"] = 2 * A[0] * 4 + 2 * coarse * 16
+    if pano_levels > 3:
+        # levels >= 2: both images' levels and masks in, R_2 out
+        c2 = sum(A[2:])
+        out["up_coarse"] = 2 * c2 * 16 + c2 * 4 + A[2] * 16
+    if pano_levels > 2:
+        # level 1: both images' level-1 pixels, mask, level-2 taps, R_2 in; R_1 out
+        out["up_quad"] = 2 * A[1] * 16 + A[1] * 4 + 2 * A[2] * 16 + A[2] * 16 + A[1] * 16
+    if pano_levels > 1:
+        # level 0: both level-0 images, the uint8 mask, level-1 taps and R_1 in, canvas out
+        out["up_level0"] = 2 * A[0] * 4 + A[0] * 1 + 2 * A[1] * 16 + A[1] * 16 + A[0] * 4
+    else:
+        out["up_level0"] = 2 * A[0] * 4 + A[0] * 1 + A[0] * 4
     return out
 
 

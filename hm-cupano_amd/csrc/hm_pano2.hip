@@ -4,10 +4,14 @@
 // src/pano/canvasManager.cpp (:16-101).  The reference runs, per frame,
 // memset(canvas, full1, full2) -> remap1 -> make_full1 -> remap2 -> make_full2
 // -> 34-launch Laplacian blend -> copy-back (about 44 stream operations).
-// Here one kernel produces, from the two remaps, both the canvas outside the
-// blend region and the level-0 blend inputs (kept as exact uint8), the
-// pyramid runs one down launch and one fused up launch per level, and the
-// level-0 up launch writes the truncated result straight into the canvas.
+// Here, per frame:
+//   1. one launch remaps both images over the canvas columns that feed the
+//      blend and stores the two level-0 blend inputs as exact uint8;
+//   2. the rest of the canvas (which does not depend on the blend) is
+//      remapped on a second stream, overlapping the pyramid;
+//   3. the Gaussian levels come from one tile-fused down launch, and the
+//      reconstruction from one launch per level in 2x2 quads, the level-0
+//      launch writing the truncated result straight into the canvas.
 // The hard seam (levels == 0) is one launch.
 #include <math.h>
 #include <string.h>
@@ -74,203 +78,69 @@ struct Pano2L0 {
   int bw, bh;
   int d[2], ulo[2], uhi[2];  // G_i[u] <- canvas column u + d[i] for u in [ulo[i], uhi[i])
   int g2_canvas;             // 1: G2 takes the canvas after both remaps; 0: image 2 alone
-  int tiles_x, tiles_y;
+  // Canvas column spans of one launch, [span_x0[k], span_x1[k]), cut into
+  // span_tiles[k] tiles of kBlock.x * kPx columns.
+  int span_x0[2], span_x1[2], span_tiles[2];
+  int tiles_x, tiles_y;  // tiles_x = span_tiles[0] + span_tiles[1]
 };
 
-// Threads that remap canvas rows handle kPx pixels kBlock.x apart, so a
-// wavefront's loads stay coalesced while every lane keeps several
-// independent map loads (then gathers) in flight.
+// Threads handle kPx pixels of one canvas row, kBlock.x apart, so a
+// wavefront's loads stay coalesced while every lane keeps 2 * kPx independent
+// map loads (then gathers) in flight.
 constexpr int kPx = 4;
 
-// ---------------------------------------------------------------------------
-// Remap + Gaussian pyramid in one launch.  Workgroups [0, roi_tiles) each take
-// a 32x32 tile of the blend region: they build both level-0 images from the
-// two remaps (the make_full copies of cudaPano.inl:225-236, 306-317), store
-// them as uint8, and reduce them to up to five coarser levels in registers
-// and LDS.  The remaining workgroups remap 256x4 canvas strips outside the
-// blend columns (the final canvas there, cudaPano.inl:210-303).
-// ---------------------------------------------------------------------------
-struct Pano2Fused {
-  Pano2L0 l0;
-  const LevelDesc* lv;  // lv[0] = level 1 ...
-  int n_down;           // Gaussian levels produced here (0..5)
-  int roi_tiles_x, roi_tiles_y;
-  int strips_x0, strips_x1, strips_y;  // canvas strips per row left / right of the blend columns
-};
-
-// Canvas value after remap 1 (c1) at (X, Y), and whether X, Y lies in box i.
+// The canvas after both remaps (cudaPano.inl:210-303) outside the blend
+// columns, and the two level-0 blend inputs: G1 = canvas after remap 1 and
+// G2 = canvas after remap 2, which overwrites image 2's whole box (the
+// make_full copies, cudaPano.inl:225-236, 306-317), or image 2 alone when the
+// blend region is the whole canvas.
 template <int C>
-__device__ __forceinline__ bool in_box(const Pano2L0& a, int i, int X, int Y) {
-  const int rx = X - a.px[i], ry = Y - a.py[i];
-  return rx >= 0 && ry >= 0 && rx < a.mw[i] && ry < a.mh[i];
-}
-template <int C>
-__device__ __forceinline__ uint32_t remap_at(const Pano2L0& a, int i, int b, int X, int Y) {
-  return remap_px<C>(a.src[i], b, a.map[i][(size_t)(Y - a.py[i]) * a.mw[i] + (X - a.px[i])]);
-}
-
-template <int C>
-__global__ void __launch_bounds__(256) pano2_fused_kernel(Pano2Fused f) {
-  __shared__ float4 s1[2][16 * 16];
-  __shared__ float4 s2[2][8 * 8];
-  const Pano2L0& a = f.l0;
+__global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
+  const int t = (int)blockIdx.x;
+  const int txa = t % a.tiles_x;
+  const int k = txa < a.span_tiles[0] ? 0 : 1;
+  const int tx = txa - (k ? a.span_tiles[0] : 0);
+  const int X0 = a.span_x0[k] + tx * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int Xend = a.span_x1[k];
+  const int Y = (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
   const int b = blockIdx.z;
-  const int tid = threadIdx.x;
-  const int n_roi = f.roi_tiles_x * f.roi_tiles_y;
-  if ((int)blockIdx.x >= n_roi) {
-    // ---- canvas strip outside the blend columns
-    const int s = (int)blockIdx.x - n_roi;
-    const int per_row = f.strips_x0 + f.strips_x1;
-    const int sy = s / per_row, sx = s % per_row;
-    const int xbase = sx < f.strips_x0 ? sx * 64 * kPx : a.skip_x1 + (sx - f.strips_x0) * 64 * kPx;
-    const int xend = sx < f.strips_x0 ? a.skip_x0 : a.cw;
-    const int Y = sy * 4 + (tid >> 6);
-    if (Y >= a.ch) return;
-    const int X0 = xbase + (tid & 63);
-    uint32_t m[2][kPx];
-    bool valid[2][kPx];
+  if (Y >= a.ch) return;
+  uint32_t m[2][kPx];
+  bool valid[2][kPx];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int ry = Y - a.py[i];
-      const bool row_ok = ry >= 0 && ry < a.mh[i];
-      const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
-#pragma unroll
-      for (int j = 0; j < kPx; ++j) {
-        const int X = X0 + 64 * j;
-        const int rx = X - a.px[i];
-        valid[i][j] = row_ok && rx >= 0 && rx < a.mw[i] && X < xend;
-        m[i][j] = valid[i][j] ? mrow[rx] : 0u;
-      }
-    }
-    uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+  for (int i = 0; i < 2; ++i) {
+    const int ry = Y - a.py[i];
+    const bool row_ok = ry >= 0 && ry < a.mh[i];
+    const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
 #pragma unroll
     for (int j = 0; j < kPx; ++j) {
-      const int X = X0 + 64 * j;
-      if (X >= xend) continue;
-      uint32_t v = 0u;
-      if (valid[1][j])
-        v = remap_px<C>(a.src[1], b, m[1][j]);
-      else if (valid[0][j])
-        v = remap_px<C>(a.src[0], b, m[0][j]);
-      store_px<C>(crow + (size_t)X * C, v);
-    }
-    return;
-  }
-  // ---- blend-region tile: level 0 of both images, then the Gaussian levels
-  const int t = xcd_tile((int)blockIdx.x, n_roi);
-  const int tx = t % f.roi_tiles_x, ty = t / f.roi_tiles_x;
-  const int lx = tid & 15, ly = tid >> 4;
-  const int u0 = tx * 32 + 2 * lx, y0 = ty * 32 + 2 * ly;
-  // Phase 1: every map word this thread needs, issued back to back.
-  //   slot 0: image 1 at X1 (G1 and, when X2 == X1, G2's fallback)
-  //   slot 1: image 2 at X2 (G2)
-  //   slot 2: image 1 at X2 (G2's fallback when X2 != X1)
-  uint32_t mw[3][4];
-  bool use[3][4];
-  bool inr[4], take1[4], take2[4];
-  const bool same_x = a.d[0] == a.d[1];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const int u = u0 + (e & 1), y = y0 + (e >> 1);
-    inr[e] = u < a.bw && y < a.bh;
-    const int X1 = u + a.d[0], X2 = u + a.d[1];
-    take1[e] = inr[e] && u >= a.ulo[0] && u < a.uhi[0] && X1 >= 0 && X1 < a.cw && y < a.ch;
-    take2[e] = inr[e] && u >= a.ulo[1] && u < a.uhi[1] && X2 >= 0 && X2 < a.cw && y < a.ch;
-    use[0][e] = take1[e] && in_box<C>(a, 0, X1, y);
-    use[1][e] = take2[e] && in_box<C>(a, 1, X2, y);
-    use[2][e] = take2[e] && !use[1][e] && a.g2_canvas && !(same_x && take1[e]) && in_box<C>(a, 0, X2, y);
-    mw[0][e] = use[0][e] ? a.map[0][(size_t)(y - a.py[0]) * a.mw[0] + (X1 - a.px[0])] : 0u;
-    mw[1][e] = use[1][e] ? a.map[1][(size_t)(y - a.py[1]) * a.mw[1] + (X2 - a.px[1])] : 0u;
-    mw[2][e] = use[2][e] ? a.map[0][(size_t)(y - a.py[0]) * a.mw[0] + (X2 - a.px[0])] : 0u;
-  }
-  // Phase 2: the source gathers.
-  uint32_t rv[3][4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    rv[0][e] = use[0][e] ? remap_px<C>(a.src[0], b, mw[0][e]) : 0u;
-    rv[1][e] = use[1][e] ? remap_px<C>(a.src[1], b, mw[1][e]) : 0u;
-    rv[2][e] = use[2][e] ? remap_px<C>(a.src[0], b, mw[2][e]) : 0u;
-  }
-  // Phase 3: the make_full values (cudaPano.inl:225-236, 306-317).
-  uint32_t g[2][4];  // [image][tap (0,0),(1,0),(0,1),(1,1)]
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    g[0][e] = rv[0][e];  // canvas after remap 1 (0 outside image 1's box)
-    uint32_t v2 = 0u;
-    if (use[1][e])
-      v2 = rv[1][e];
-    else if (a.g2_canvas && take2[e])
-      v2 = (same_x && take1[e]) ? rv[0][e] : rv[2][e];  // image 1 shows through outside image 2's box
-    g[1][e] = take2[e] ? v2 : 0u;
-  }
-  // level-0 stores (uint8, exact)
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    if (!inr[e]) continue;
-    const size_t o = ((size_t)b * a.bh + (y0 + (e >> 1))) * a.bw + (u0 + (e & 1));
-    store_px<C>(a.g0[0] + o * C, g[0][e]);
-    store_px<C>(a.g0[1] + o * C, g[1][e]);
-  }
-  if (f.n_down <= 0) return;
-  // level 1 from the 2x2 block in registers
-  {
-    const LevelDesc& d = f.lv[0];
-    const int x = tx * 16 + lx, y = ty * 16 + ly;
-    if (x < d.w && y < d.h) {
-      const size_t o = ((size_t)b * d.h + y) * d.w + x;
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        float4 p[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const uint32_t v = g[i][e];
-          p[e] = make_float4((float)(v & 0xffu), (float)((v >> 8) & 0xffu), (float)((v >> 16) & 0xffu),
-                             (float)(v >> 24));
-        }
-        const float4 r = down_px<C>(p, inr);
-        s1[i][ly * 16 + lx] = r;
-        st_px<C>(d.g_out[i], o, r);
-      }
+      const int rx = X0 + j * (int)blockDim.x - a.px[i];
+      valid[i][j] = row_ok && rx >= 0 && rx < a.mw[i] && X0 + j * (int)blockDim.x < Xend;
+      m[i][j] = valid[i][j] ? mrow[rx] : 0u;
     }
   }
-  // levels 2 .. n_down from LDS
-  int edge = 16;
-  for (int k = 1; k < f.n_down; ++k) {
-    __syncthreads();
-    const LevelDesc& din = f.lv[k - 1];
-    const LevelDesc& d = f.lv[k];
-    const int oedge = edge >> 1;
-    float4 r[2];
-    bool active = false;
-    int qx = 0, qy = 0;
-    if (tid < oedge * oedge) {
-      qx = tid % oedge;
-      qy = tid / oedge;
-      const int x = tx * oedge + qx, y = ty * oedge + qy;
-      if (x < d.w && y < d.h) {
-        active = true;
-        const bool has_x1 = 2 * x + 1 < din.w, has_y1 = 2 * y + 1 < din.h;
-        const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
-        const int sx = 2 * qx, sy = 2 * qy;
+  uint32_t r[2][kPx];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          const float4* src = k == 1 ? s1[i] : s2[i];
-          float4 p[4];
-          p[0] = src[sy * edge + sx];
-          if (has_x1) p[1] = src[sy * edge + sx + 1];
-          if (has_y1) p[2] = src[(sy + 1) * edge + sx];
-          if (has_x1 && has_y1) p[3] = src[(sy + 1) * edge + sx + 1];
-          r[i] = down_px<C>(p, in_range);
-          st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r[i]);
-        }
-      }
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) r[i][j] = valid[i][j] ? remap_px<C>(a.src[i], b, m[i][j]) : 0u;
+  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+  uint8_t* g0row = a.g0[0] + ((size_t)b * a.bh + Y) * a.bw * C;
+  uint8_t* g1row = a.g0[1] + ((size_t)b * a.bh + Y) * a.bw * C;
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const int X = X0 + j * (int)blockDim.x;
+    if (X >= Xend) continue;
+    const uint32_t c1 = valid[0][j] ? r[0][j] : 0u;
+    const uint32_t c12 = valid[1][j] ? r[1][j] : c1;
+    if (a.write_canvas && !(X >= a.skip_x0 && X < a.skip_x1)) store_px<C>(crow + (size_t)X * C, c12);
+    if (Y < a.bh) {
+      const int u1 = X - a.d[0];
+      if (u1 >= a.ulo[0] && u1 < a.uhi[0]) store_px<C>(g0row + (size_t)u1 * C, c1);
+      const int u2 = X - a.d[1];
+      if (u2 >= a.ulo[1] && u2 < a.uhi[1])
+        store_px<C>(g1row + (size_t)u2 * C, a.g2_canvas ? c12 : (valid[1][j] ? r[1][j] : 0u));
     }
-    __syncthreads();
-    if (active) {
-      s2[0][qy * oedge + qx] = r[0];
-      s2[1][qy * oedge + qx] = r[1];
-    }
-    edge = oedge;
   }
 }
 
@@ -397,10 +267,22 @@ struct Pano2 : hmStitchPano {
   std::vector<uint8_t> mask_u8_host;  // the same level-0 mask as uint8 (the seam values)
   hm::DevBuf mask_u8;
   Pano2L0 l0{};
+  int roi_x0 = 0, roi_x1 = 0;  // canvas columns whose remap feeds the blend inputs
+  // The canvas outside the blend columns does not depend on the pyramid: it
+  // is remapped on a second stream, forked from and joined back into the
+  // caller's stream, so it overlaps the pyramid's latency-bound launches.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
+  ~Pano2() override {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
+  }
   hipError_t setup(const hmControlMasks2& m, int minimize_blend);
   hipError_t ensure_pyramid();
   hipError_t process(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s);
+  hipError_t run_blend(const hmSurface& canvas, hipStream_t s);
 };
 
 hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
@@ -459,6 +341,8 @@ hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
     l0.write_canvas = 1;
     l0.skip_x0 = bx0;
     l0.skip_x1 = bx0 + bw;
+    roi_x0 = std::max(0, std::min(l0.ulo[0] + l0.d[0], l0.ulo[1] + l0.d[1]));
+    roi_x1 = std::min(canvas_w, std::max(l0.uhi[0] + l0.d[0], l0.uhi[1] + l0.d[1]));
   } else {
     bw = canvas_w;
     bh = canvas_h;
@@ -467,6 +351,8 @@ hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
     l0.uhi[0] = l0.uhi[1] = canvas_w;
     l0.g2_canvas = 0;
     l0.write_canvas = 0;
+    roi_x0 = 0;
+    roi_x1 = canvas_w;
   }
   // convertMaskMat crop + convertTo(float) (cudaPano.inl:62-77).
   mask_host.resize((size_t)bw * bh);
@@ -493,6 +379,11 @@ hipError_t Pano2::ensure_pyramid() {
   HM_RETURN_IF_ERROR(hipMemcpy(mask_u8.p, mask_u8_host.data(), mask_u8_host.size(), hipMemcpyHostToDevice));
   pyr.mask0_fast = mask_u8.p;  // float(seam byte) == the float mask, 1 byte per pixel
   pyr.mask0_mode = kMask0U8;
+  if (l0.write_canvas) {
+    HM_RETURN_IF_ERROR(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HM_RETURN_IF_ERROR(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HM_RETURN_IF_ERROR(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  }
   return hipSuccess;
 }
 
@@ -533,9 +424,7 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     return hipGetLastError();
   }
   HM_RETURN_IF_ERROR(ensure_pyramid());
-  Pano2Fused f{};
-  Pano2L0& a = f.l0;
-  a = l0;
+  Pano2L0 a = l0;
   a.src[0] = s1;
   a.src[1] = s2;
   for (int i = 0; i < 2; ++i) {
@@ -554,24 +443,46 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   a.ch = canvas_h;
   a.bw = bw;
   a.bh = bh;
-  f.lv = pyr.levels_table();
-  f.n_down = std::min(5, pyr.levels - 1);
-  f.roi_tiles_x = (bw + 31) / 32;
-  f.roi_tiles_y = (bh + 31) / 32;
-  const int strip = 64 * kPx;
-  f.strips_x0 = a.write_canvas ? (a.skip_x0 + strip - 1) / strip : 0;
-  f.strips_x1 = a.write_canvas ? (canvas_w - a.skip_x1 + strip - 1) / strip : 0;
-  f.strips_y = (canvas_h + 3) / 4;
-  const int n_strips = (f.strips_x0 + f.strips_x1) * f.strips_y;
-  const dim3 fgrid((unsigned)(f.roi_tiles_x * f.roi_tiles_y + n_strips), 1u, (unsigned)batch);
-  timer.begin(s);
-  if (channels == 4)
-    hipLaunchKernelGGL(pano2_fused_kernel<4>, fgrid, dim3(256), 0, s, f);
-  else
-    hipLaunchKernelGGL(pano2_fused_kernel<3>, fgrid, dim3(256), 0, s, f);
-  timer.end(s, "remap_down");
-  HM_RETURN_IF_ERROR(hipGetLastError());
+  const int tile_w = (int)kBlock.x * kPx;
+  a.tiles_y = (canvas_h + (int)kBlock.y - 1) / (int)kBlock.y;
+  auto launch_spans = [&](int x0a, int x1a, int x0b, int x1b, hipStream_t st, const char* name) -> hipError_t {
+    Pano2L0 k = a;
+    k.span_x0[0] = x0a;
+    k.span_x1[0] = x1a;
+    k.span_x0[1] = x0b;
+    k.span_x1[1] = x1b;
+    k.span_tiles[0] = std::max(0, (x1a - x0a + tile_w - 1) / tile_w);
+    k.span_tiles[1] = std::max(0, (x1b - x0b + tile_w - 1) / tile_w);
+    k.tiles_x = k.span_tiles[0] + k.span_tiles[1];
+    if (k.tiles_x == 0) return hipSuccess;
+    const dim3 grid((unsigned)(k.tiles_x * k.tiles_y), 1u, (unsigned)batch);
+    timer.begin(st);
+    if (channels == 4)
+      hipLaunchKernelGGL(pano2_level0_kernel<4>, grid, kBlock, 0, st, k);
+    else
+      hipLaunchKernelGGL(pano2_level0_kernel<3>, grid, kBlock, 0, st, k);
+    timer.end(st, name);
+    return hipGetLastError();
+  };
+  // Blend-input columns first, on the caller's stream: the pyramid waits on
+  // them.  The canvas remap forks after them, so it does not delay them.
+  HM_RETURN_IF_ERROR(launch_spans(roi_x0, roi_x1, 0, 0, s, "remap_roi"));
+  if (side) {
+    HM_RETURN_IF_ERROR(hipEventRecord(ev_fork, s));
+    HM_RETURN_IF_ERROR(hipStreamWaitEvent(side, ev_fork, 0));
+    HM_RETURN_IF_ERROR(launch_spans(0, roi_x0, roi_x1, canvas_w, side, "remap_canvas"));
+    HM_RETURN_IF_ERROR(hipEventRecord(ev_join, side));
+  }
+  const hipError_t e = run_blend(canvas, s);
+  // Join even after an error, so the caller's stream still orders the side work.
+  if (side) {
+    const hipError_t j = hipStreamWaitEvent(s, ev_join, 0);
+    if (e == hipSuccess) return j;
+  }
+  return e;
+}
 
+hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s) {
   const void* g[2] = {g0[0].p, g0[1].p};
   hm::Level0Out out;
   out.u8 = static_cast<uint8_t*>(canvas.d_ptr);
@@ -580,10 +491,8 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   out.u8_h = canvas_h;
   out.ox = minimize ? x2 - kOverlapPad : 0;  // copy-back offset (cudaPano.inl:408)
   out.oy = 0;
-  if (pyr.debug) return pyr.run(g, true, out, s, timer_or_null());  // per-level path keeping every level
-  // Levels beyond the first five continue from the last level produced above.
-  HM_RETURN_IF_ERROR(pyr.run_down(g, true, f.n_down, s, timer_or_null()));
-  return pyr.run_up(g, true, out, s, timer_or_null());
+  // Debug: the per-level path keeping every level; else the fused passes.
+  return pyr.run(g, true, out, s, timer_or_null());
 }
 
 }  // namespace

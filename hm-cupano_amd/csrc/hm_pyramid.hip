@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <sstream>
+#include <type_traits>
 #include <vector>
 
 #include "hm_internal.hpp"
@@ -204,73 +205,115 @@ hipError_t launch_down_fused(int c, int n, bool in_u8, const void* const* g0, in
   return hipErrorInvalidValue;
 }
 
-template <typename T0, int C, int N, int KIND, int MODE>
-static hipError_t up_fused_t(const UpFusedLaunch& u, int batch, hipStream_t s) {
-  if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
-  UpFusedArgs<T0, N> a;
-  for (int i = 0; i < N; ++i) a.g0[i] = static_cast<const T0*>(u.g0[i]);
-  a.mask0 = u.mask0;
-  a.w0 = u.w0;
-  a.h0 = u.h0;
-  a.levels = u.levels;
-  a.lv = u.lv;
-  a.f32_out = u.f32_out;
-  a.u8_out = u.u8_out;
-  a.u8_pitch = u.u8_pitch;
-  a.u8_w = u.u8_w;
-  a.u8_h = u.u8_h;
-  a.ox = u.ox;
-  a.oy = u.oy;
-  a.win_x = u.win_x;
-  a.win_y = u.win_y;
-  a.win_w = u.win_w;
-  a.win_h = u.win_h;
-  a.tiles_x = (u.win_w + kUpTile - 1) / kUpTile;
-  a.tiles_y = (u.win_h + kUpTile - 1) / kUpTile;
-  hipLaunchKernelGGL((up_fused_kernel<T0, C, N, KIND, MODE>),
-                     dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-template <typename T0, int C, int MODE>
-static hipError_t up_fused_n(int n, const UpFusedLaunch& u, int batch, hipStream_t s) {
-  switch (n) {
-    case 2: return up_fused_t<T0, C, 2, kBlendN, MODE>(u, batch, s);
-    case 3: return up_fused_t<T0, C, 3, kBlendN, MODE>(u, batch, s);
-    case 4: return up_fused_t<T0, C, 4, kBlendN, MODE>(u, batch, s);
-    case 5: return up_fused_t<T0, C, 5, kBlendN, MODE>(u, batch, s);
-    case 6: return up_fused_t<T0, C, 6, kBlendN, MODE>(u, batch, s);
-    case 7: return up_fused_t<T0, C, 7, kBlendN, MODE>(u, batch, s);
-    case 8: return up_fused_t<T0, C, 8, kBlendN, MODE>(u, batch, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-template <typename T0, int C>
-static hipError_t up_fused_c(int n, int kind, const UpFusedLaunch& u, int batch, hipStream_t s) {
-  const int mode = u.mask0_mode;
-  if (kind == kBlend2) {
-    if (n != 2) return hipErrorInvalidValue;
-    if (mode == kMask0U8) return up_fused_t<T0, C, 2, kBlend2, kMask0U8>(u, batch, s);
-    if (mode == kMask0F32) return up_fused_t<T0, C, 2, kBlend2, kMask0F32>(u, batch, s);
-    return hipErrorInvalidValue;
-  }
-  if (kind == kBlend3) {
-    if (n != 3) return hipErrorInvalidValue;
-    if (mode == kMask0Label) return up_fused_t<T0, C, 3, kBlend3, kMask0Label>(u, batch, s);
-    if (mode == kMask0F32) return up_fused_t<T0, C, 3, kBlend3, kMask0F32>(u, batch, s);
-    return hipErrorInvalidValue;
-  }
-  if (mode == kMask0Label) return up_fused_n<T0, C, kMask0Label>(n, u, batch, s);
-  if (mode == kMask0F32) return up_fused_n<T0, C, kMask0F32>(n, u, batch, s);
+// Calls f(C, N, KIND) with std::integral_constant arguments for every
+// supported (channels, images, blend kind).
+template <typename F>
+static hipError_t dispatch_cnk(int c, int n, int kind, F&& f) {
+  using std::integral_constant;
+  auto per_c = [&](auto C) -> hipError_t {
+    if (kind == kBlend2)
+      return n == 2 ? f(C, integral_constant<int, 2>{}, integral_constant<int, kBlend2>{}) : hipErrorInvalidValue;
+    if (kind == kBlend3)
+      return n == 3 ? f(C, integral_constant<int, 3>{}, integral_constant<int, kBlend3>{}) : hipErrorInvalidValue;
+    if (kind != kBlendN) return hipErrorInvalidValue;
+    const integral_constant<int, kBlendN> K{};
+    switch (n) {
+      case 2: return f(C, integral_constant<int, 2>{}, K);
+      case 3: return f(C, integral_constant<int, 3>{}, K);
+      case 4: return f(C, integral_constant<int, 4>{}, K);
+      case 5: return f(C, integral_constant<int, 5>{}, K);
+      case 6: return f(C, integral_constant<int, 6>{}, K);
+      case 7: return f(C, integral_constant<int, 7>{}, K);
+      case 8: return f(C, integral_constant<int, 8>{}, K);
+      default: return hipErrorInvalidValue;
+    }
+  };
+  if (c == 4) return per_c(integral_constant<int, 4>{});
+  if (c == 3) return per_c(integral_constant<int, 3>{});
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_up_fused(int c, int n, int kind, bool g0_u8, const UpFusedLaunch& u, int batch, hipStream_t s) {
+hipError_t launch_up_coarse(int c, int n, int kind, const UpFusedLaunch& u, int batch, hipStream_t s) {
   if (u.levels < 1 || u.levels > kMaxLevels) return hipErrorInvalidValue;
-  if (c == 4) return g0_u8 ? up_fused_c<uint8_t, 4>(n, kind, u, batch, s) : up_fused_c<float, 4>(n, kind, u, batch, s);
-  if (c == 3) return g0_u8 ? up_fused_c<uint8_t, 3>(n, kind, u, batch, s) : up_fused_c<float, 3>(n, kind, u, batch, s);
-  return hipErrorInvalidValue;
+  if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
+  return dispatch_cnk(c, n, kind, [&](auto C, auto N, auto K) -> hipError_t {
+    constexpr int kC = decltype(C)::value, kN = decltype(N)::value, kK = decltype(K)::value;
+    UpFusedArgs<float, kN> a;
+    for (int i = 0; i < kN; ++i) a.g0[i] = static_cast<const float*>(u.g0[i]);
+    a.mask0 = u.mask0;
+    a.w0 = u.w0;
+    a.h0 = u.h0;
+    a.levels = u.levels;
+    a.lv = u.lv;
+    a.f32_out = u.f32_out;
+    a.u8_out = nullptr;
+    a.u8_pitch = 0;
+    a.u8_w = a.u8_h = a.ox = a.oy = 0;
+    a.win_x = u.win_x;
+    a.win_y = u.win_y;
+    a.win_w = u.win_w;
+    a.win_h = u.win_h;
+    a.tiles_x = (u.win_w + kUpTile - 1) / kUpTile;
+    a.tiles_y = (u.win_h + kUpTile - 1) / kUpTile;
+    hipLaunchKernelGGL((up_fused_kernel<float, kC, kN, kK, kMask0F32>),
+                       dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch), dim3(256), 0, s, a);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch& u, int batch,
+                            hipStream_t s) {
+  if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
+  const int mode = u.mask0_mode;
+  if (kind == kBlend2 ? (mode != kMask0F32 && mode != kMask0U8) : (mode != kMask0F32 && mode != kMask0Label))
+    return hipErrorInvalidValue;
+  return dispatch_cnk(c, n, kind, [&](auto C, auto N, auto K) -> hipError_t {
+    constexpr int kC = decltype(C)::value, kN = decltype(N)::value, kK = decltype(K)::value;
+    auto go = [&](auto T0v, auto M) -> hipError_t {
+      using T0 = decltype(T0v);
+      UpQuadArgs<T0, kN> a;
+      for (int i = 0; i < kN; ++i) {
+        a.g0[i] = static_cast<const T0*>(u.g0[i]);
+        a.g1[i] = u.g1[i];
+      }
+      a.mask0 = u.mask0;
+      a.r1 = u.r1;
+      a.w0 = u.w0;
+      a.h0 = u.h0;
+      a.w1 = u.w1;
+      a.h1 = u.h1;
+      a.f32_out = u.f32_out;
+      a.u8_out = u.u8_out;
+      a.u8_pitch = u.u8_pitch;
+      a.u8_w = u.u8_w;
+      a.u8_h = u.u8_h;
+      a.ox = u.ox;
+      a.oy = u.oy;
+      a.win_x0 = u.win_x;
+      a.win_y0 = u.win_y;
+      a.win_x1 = u.win_x + u.win_w - 1;
+      a.win_y1 = u.win_y + u.win_h - 1;
+      a.qx0 = a.win_x0 >> 1;
+      a.qy0 = a.win_y0 >> 1;
+      a.nqx = (a.win_x1 >> 1) - a.qx0 + 1;
+      a.nqy = (a.win_y1 >> 1) - a.qy0 + 1;
+      a.tiles_x = (a.nqx + 63) / 64;
+      a.tiles_y = (a.nqy + 3) / 4;
+      hipLaunchKernelGGL((up_quad_kernel<T0, kC, kN, kK, decltype(M)::value>),
+                         dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch), dim3(64, 4, 1), 0, s, a);
+      return hipGetLastError();
+    };
+    using std::integral_constant;
+    if constexpr (kK == kBlend2) {
+      if (mode == kMask0U8)
+        return g0_u8 ? go(uint8_t{}, integral_constant<int, kMask0U8>{}) : go(float{}, integral_constant<int, kMask0U8>{});
+    } else {
+      if (mode == kMask0Label)
+        return g0_u8 ? go(uint8_t{}, integral_constant<int, kMask0Label>{})
+                     : go(float{}, integral_constant<int, kMask0Label>{});
+    }
+    return g0_u8 ? go(uint8_t{}, integral_constant<int, kMask0F32>{}) : go(float{}, integral_constant<int, kMask0F32>{});
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -416,29 +459,86 @@ hipError_t Pyramid::run_down(const void* const* g0, bool g0_u8, int from, hipStr
 
 hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s,
                            LaunchTimer* timer) {
-  // The whole reconstruction per level-0 tile in one launch.
-  UpFusedLaunch u;
-  for (int i = 0; i < n; ++i) u.g0[i] = g0[i];
-  u.mask0 = mask0_fast ? mask0_fast : mask0;
-  u.mask0_mode = mask0_fast ? mask0_mode : kMask0F32;
-  u.w0 = W[0];
-  u.h0 = H[0];
-  u.levels = levels;
-  u.lv = level_table.as<LevelDesc>();
-  u.f32_out = out.f32;
-  u.u8_out = out.u8;
-  u.u8_pitch = out.u8_pitch;
-  u.u8_w = out.u8_w;
-  u.u8_h = out.u8_h;
-  u.ox = out.ox;
-  u.oy = out.oy;
-  u.win_x = out.win_w >= 0 ? out.win_x : 0;
-  u.win_y = out.win_w >= 0 ? out.win_y : 0;
-  u.win_w = out.win_w >= 0 ? out.win_w : W[0];
-  u.win_h = out.win_w >= 0 ? out.win_h : H[0];
-  if (timer) timer->begin(s);
-  HM_RETURN_IF_ERROR(launch_up_fused(c, n, kind, g0_u8, u, batch, s));
-  if (timer) timer->end(s, "up_fused");
+  const int wx = out.win_w >= 0 ? out.win_x : 0, wy = out.win_w >= 0 ? out.win_y : 0;
+  const int ww = out.win_w >= 0 ? out.win_w : W[0], wh = out.win_w >= 0 ? out.win_h : H[0];
+  if (ww <= 0 || wh <= 0) return hipSuccess;
+  // Footprint of the level-0 window at every level: the pixels whose R the
+  // next finer level's taps read.
+  int fx0[kMaxLevels], fx1[kMaxLevels], fy0[kMaxLevels], fy1[kMaxLevels];
+  fx0[0] = wx;
+  fy0[0] = wy;
+  fx1[0] = wx + ww - 1;
+  fy1[0] = wy + wh - 1;
+  for (int k = 1; k < levels; ++k) {
+    fx0[k] = fx0[k - 1] >> 1;
+    fy0[k] = fy0[k - 1] >> 1;
+    fx1[k] = std::min((fx1[k - 1] >> 1) + 1, W[k] - 1);
+    fy1[k] = std::min((fy1[k - 1] >> 1) + 1, H[k] - 1);
+  }
+  // 1. The small levels from kQuadLevels on, per base-level tile in one
+  //    launch, giving R at that level; their chain of dependent steps stays
+  //    inside a workgroup.
+  int top = levels - 1;  // first level run in quads
+  if (levels - 1 > kQuadLevels) {
+    const int base = kQuadLevels;
+    UpFusedLaunch u;
+    for (int i = 0; i < n; ++i) u.g0[i] = g[i][base].p;
+    u.mask0 = mask[base].p;
+    u.mask0_mode = kMask0F32;
+    u.w0 = W[base];
+    u.h0 = H[base];
+    u.levels = levels - base;
+    u.lv = level_table.as<LevelDesc>() + base;
+    u.f32_out = r[base].as<float>();
+    u.win_x = fx0[base];
+    u.win_y = fy0[base];
+    u.win_w = fx1[base] - fx0[base] + 1;
+    u.win_h = fy1[base] - fy0[base] + 1;
+    if (timer) timer->begin(s);
+    HM_RETURN_IF_ERROR(launch_up_coarse(c, n, kind, u, batch, s));
+    if (timer) timer->end(s, "up_coarse");
+    top = base - 1;
+  }
+  // 2. The remaining levels one launch each, one thread per 2x2 quad; level 0
+  //    goes straight to the output.
+  for (int k = top; k >= 0; --k) {
+    const bool has_next = k + 1 < levels;
+    UpQuadLaunch u;
+    for (int i = 0; i < n; ++i) {
+      u.g0[i] = k == 0 ? g0[i] : g[i][k].p;
+      u.g1[i] = has_next ? g[i][k + 1].as<float>() : nullptr;
+    }
+    if (k == 0) {
+      u.mask0 = mask0_fast ? mask0_fast : mask0;
+      u.mask0_mode = mask0_fast ? mask0_mode : kMask0F32;
+    } else {
+      u.mask0 = mask[k].p;
+      u.mask0_mode = kMask0F32;
+    }
+    u.r1 = has_next ? r[k + 1].as<float>() : nullptr;
+    u.w0 = W[k];
+    u.h0 = H[k];
+    u.w1 = has_next ? W[k + 1] : 0;
+    u.h1 = has_next ? H[k + 1] : 0;
+    if (k == 0) {
+      u.f32_out = out.f32;
+      u.u8_out = out.u8;
+      u.u8_pitch = out.u8_pitch;
+      u.u8_w = out.u8_w;
+      u.u8_h = out.u8_h;
+      u.ox = out.ox;
+      u.oy = out.oy;
+    } else {
+      u.f32_out = r[k].as<float>();
+    }
+    u.win_x = fx0[k];
+    u.win_y = fy0[k];
+    u.win_w = fx1[k] - fx0[k] + 1;
+    u.win_h = fy1[k] - fy0[k] + 1;
+    if (timer) timer->begin(s);
+    HM_RETURN_IF_ERROR(launch_up_quad(c, n, kind, k == 0 && g0_u8, u, batch, s));
+    if (timer) timer->end(s, k == 0 ? "up_level0" : "up_quad");
+  }
   return hipSuccess;
 }
 

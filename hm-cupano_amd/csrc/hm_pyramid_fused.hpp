@@ -4,14 +4,20 @@
 //   matching 16x16 .. 1x1 tiles of up to five coarser levels in LDS and writes
 //   them once.  The 2x2 mean never looks outside its own block, so aligned
 //   tiles need no halo and the result is the per-level kernel's exactly.
-// up_fused_kernel: one workgroup reconstructs a 64x64 level-0 tile.  Going
-//   from the coarsest level down, it evaluates the Laplacian of every image,
-//   the blend and the reconstruction on the tile's footprint at each level
-//   (plus the one-pixel halo the top-left aligned upsample needs), keeping
-//   R_k in LDS; only level 0 is written, straight to the caller's output.
-//   Every level is processed in 2x2 quads, whose four pixels share their
-//   taps and have fixed upsample positions (dx, dy) in {0, 1/2}^2.
-// Per-pixel arithmetic is the same device code as the per-level kernels
+// up_fused_kernel: one workgroup reconstructs a 32x32 tile of its base level.
+//   Going from the coarsest level down, it evaluates the Laplacian of every
+//   image, the blend and the reconstruction on the tile's footprint at each
+//   level (plus the one-pixel halo the top-left aligned upsample needs),
+//   keeping R_k in LDS; only the base level is written.  Every level is
+//   processed in 2x2 quads, whose four pixels share their taps and have fixed
+//   upsample positions (dx, dy) in {0, 1/2}^2.  The pyramid runs it on the
+//   small levels from 2 on, producing R_2.
+// up_quad_kernel: one level, one thread per 2x2 quad: the Laplacian of every
+//   image from the level's pixels and the shared taps of the next level, the
+//   blend, and the reconstruction from the shared R taps of the next level.
+//   It keeps no state, so it runs at full occupancy.  Levels 1 and 0 run it;
+//   level 0 writes straight to the output.
+// Per-pixel arithmetic is the same as the per-level kernels
 // (hm_pyramid_kernels.hpp), in the same order.
 #pragma once
 
@@ -208,7 +214,7 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
 // ---------------------------------------------------------------------------
 // up pass
 // ---------------------------------------------------------------------------
-constexpr int kUpTile = 64;                        // level-0 tile edge
+constexpr int kUpTile = 32;                        // base-level tile edge
 constexpr int kUpRegion = kUpTile / 2 + 2;         // max footprint edge at levels >= 1
 
 template <int C, int N, int KIND, int MODE>
@@ -368,6 +374,90 @@ __global__ void __launch_bounds__(256) up_fused_kernel(UpFusedArgs<T0, N> a) {
     if (level == 0) break;
     __syncthreads();
     cur ^= 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// one level in 2x2 quads (fields named as level 0 and its next level 1)
+// ---------------------------------------------------------------------------
+template <typename T0, int N>
+struct UpQuadArgs {
+  const T0* g0[N];            // the level's images (tight, batch-major)
+  const void* mask0;          // the level's mask (see Mask0Mode)
+  const float* g1[N];         // next level's Gaussians
+  const float* r1;            // next level's reconstruction, or nullptr: coarsest level, blend only
+  int w0, h0, w1, h1;
+  float* f32_out;             // float output (tight), or nullptr
+  uint8_t* u8_out;            // uint8 canvas, or nullptr
+  size_t u8_pitch;
+  int u8_w, u8_h, ox, oy;
+  int win_x0, win_y0, win_x1, win_y1;  // pixels produced: [x0, x1] x [y0, y1]
+  int qx0, qy0, nqx, nqy;              // quads covering the window
+  int tiles_x, tiles_y;                // 64 x 4 quads per workgroup
+};
+
+template <typename T0, int C, int N, int KIND, int MODE>
+__global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
+  const bool has_next = a.r1 != nullptr;  // uniform across the launch
+  const int t = (int)blockIdx.x;
+  const int qx = a.qx0 + (t % a.tiles_x) * 64 + (int)threadIdx.x;
+  const int qy = a.qy0 + (t / a.tiles_x) * 4 + (int)threadIdx.y;
+  const int b = blockIdx.z;
+  if (qx >= a.qx0 + a.nqx || qy >= a.qy0 + a.nqy) return;
+  float4 gt[N][4];
+  float4 rt[4];
+  if (has_next) {
+    const int qx1 = min(qx + 1, a.w1 - 1), qy1 = min(qy + 1, a.h1 - 1);
+    const size_t l0 = ((size_t)b * a.h1 + qy) * a.w1, l1 = ((size_t)b * a.h1 + qy1) * a.w1;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      gt[i][0] = ld_px<C>(a.g1[i], l0 + qx);
+      gt[i][1] = ld_px<C>(a.g1[i], l0 + qx1);
+      gt[i][2] = ld_px<C>(a.g1[i], l1 + qx);
+      gt[i][3] = ld_px<C>(a.g1[i], l1 + qx1);
+    }
+    rt[0] = ld_px<C>(a.r1, l0 + qx);
+    rt[1] = ld_px<C>(a.r1, l0 + qx1);
+    rt[2] = ld_px<C>(a.r1, l1 + qx);
+    rt[3] = ld_px<C>(a.r1, l1 + qx1);
+  }
+  // level-0 pixels of the quad, all loaded before any arithmetic
+  float4 g[4][N];
+  bool in[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+    in[e] = x >= a.win_x0 && x <= a.win_x1 && y >= a.win_y0 && y <= a.win_y1;
+    const size_t pix = ((size_t)b * a.h0 + (in[e] ? y : a.win_y0)) * a.w0 + (in[e] ? x : a.win_x0);
+#pragma unroll
+    for (int i = 0; i < N; ++i) g[e][i] = ld_px<C>(a.g0[i], pix);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (!in[e]) continue;
+    const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+    float4 Lv[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (has_next) {
+        const float4 up = up_quad_lap<C>(e, gt[i]);
+        Lv[i] = make_float4(g[e][i].x - up.x, g[e][i].y - up.y, g[e][i].z - up.z, g[e][i].w);
+      } else {
+        Lv[i] = g[e][i];
+      }
+    }
+    const float4 B = blend_level0<C, N, KIND, MODE>(Lv, a.mask0, (size_t)y * a.w0 + x);
+    float4 R = B;
+    if (has_next) {
+      const float4 up = up_quad_rec<C>(e, rt);
+      R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+    }
+    if (a.f32_out) st_px<C>(a.f32_out, ((size_t)b * a.h0 + y) * a.w0 + x, R);
+    if (a.u8_out) {
+      const int X = x + a.ox, Y = y + a.oy;
+      if (X < a.u8_w && Y < a.u8_h)
+        st_px_u8<C>(a.u8_out + ((size_t)b * a.u8_h + Y) * a.u8_pitch + (size_t)X * C, R);
+    }
   }
 }
 
