@@ -8,7 +8,7 @@
 #include "hm_stitch.h"
 
 struct hmStitchPano {
-  virtual ~hmStitchPano() = default;
+  virtual ~hmStitchPano();
 
   int status = hipSuccess;  // sticky, like CudaStitchPano::status_
   int n_images = 0;
@@ -26,6 +26,15 @@ struct hmStitchPano {
   hm::LaunchTimer timer;
 
   hm::LaunchTimer* timer_or_null() { return timer.enabled() ? &timer : nullptr; }
+
+  // Work that does not feed the blend (the canvas outside the written-back
+  // region) runs on a second stream, forked from and joined back into the
+  // caller's stream, so it overlaps the pyramid's latency-bound launches.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipError_t ensure_side();
+  hipError_t fork(hipStream_t s);  // side waits for the caller's stream
+  hipError_t join(hipStream_t s);  // the caller's stream waits for side
 
   // Allocates the pyramid and the resident mask pyramid for a bw x bh blend
   // region; mask_host is bw x bh x M float.

@@ -20,52 +20,13 @@
 #include <vector>
 
 #include "hm_pano.hpp"
+#include "hm_pano_device.hpp"
 
 using namespace hmdev;
 
 namespace {
 
 constexpr int kOverlapPad = 128;  // CanvasManager overlap_pad default (canvasManager.h:65)
-const dim3 kBlock(64, 4, 1);
-
-template <int C>
-__device__ __forceinline__ uint32_t load_src_px(const Surf& s, int b, int x, int y) {
-  const uint8_t* p = row_ptr(s, b, y) + (size_t)x * C;
-  if constexpr (C == 4) return *reinterpret_cast<const uint32_t*>(p);
-  return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16);
-}
-
-template <int C>
-__device__ __forceinline__ void store_px(uint8_t* p, uint32_t v) {
-  if constexpr (C == 4) {
-    *reinterpret_cast<uint32_t*>(p) = v;
-  } else {
-    p[0] = (uint8_t)v;
-    p[1] = (uint8_t)(v >> 8);
-    p[2] = (uint8_t)(v >> 16);
-  }
-}
-
-// BatchedRemapKernelExOffset pixel (src/cuda/cudaRemap.cu:95-122), default
-// pixel all-zero, no_unmapped_write = false: in range -> source pixel, else
-// zero with alpha 0 for the 65535 sentinel and 255 otherwise.
-template <int C>
-__device__ __forceinline__ uint32_t remap_px(const Surf& s, int b, uint32_t m) {
-  const uint32_t sx = m & 0xffffu, sy = m >> 16;
-  if (sx < s.w && sy < s.h) return load_src_px<C>(s, b, (int)sx, (int)sy);
-  if constexpr (C == 4) return sx == (uint32_t)kUnmapped ? 0u : 0xff000000u;
-  return 0u;
-}
-
-// BatchedRemapKernelExOffsetWithDestMap pixel (cudaRemap.cu:235-244):
-// out-of-range or unmapped -> the all-zero default pixel.
-template <int C>
-__device__ __forceinline__ uint32_t remap_px_dm(const Surf& s, int b, uint32_t m) {
-  const uint32_t sx = m & 0xffffu, sy = m >> 16;
-  if (sx < s.w && sy < s.h) return load_src_px<C>(s, b, (int)sx, (int)sy);
-  return 0u;
-}
-
 struct Pano2L0 {
   Surf src[2];
   const uint32_t* map[2];
@@ -83,11 +44,6 @@ struct Pano2L0 {
   int span_x0[2], span_x1[2], span_tiles[2];
   int tiles_x, tiles_y;  // tiles_x = span_tiles[0] + span_tiles[1]
 };
-
-// Threads handle kPx pixels of one canvas row, kBlock.x apart, so a
-// wavefront's loads stay coalesced while every lane keeps 2 * kPx independent
-// map loads (then gathers) in flight.
-constexpr int kPx = 4;
 
 // The canvas after both remaps (cudaPano.inl:210-303) outside the blend
 // columns, and the two level-0 blend inputs: G1 = canvas after remap 1 and
@@ -202,54 +158,7 @@ __global__ void __launch_bounds__(256) pano2_hard_kernel(Pano2Hard a) {
   }
 }
 
-Surf to_surf(const hmSurface& s) { return Surf{static_cast<uint8_t*>(s.d_ptr), s.width, s.height, s.pitch}; }
-
 }  // namespace
-
-// ===========================================================================
-// shared helpers
-// ===========================================================================
-namespace hm {
-
-hipError_t upload_packed_map(const uint16_t* map_x, const uint16_t* map_y, int w, int h, DevBuf& dst) {
-  if (!map_x || !map_y || w < 1 || h < 1) return hipErrorInvalidValue;
-  std::vector<uint32_t> packed((size_t)w * h);
-  for (size_t i = 0; i < packed.size(); ++i) packed[i] = (uint32_t)map_x[i] | ((uint32_t)map_y[i] << 16);
-  HM_RETURN_IF_ERROR(dst.alloc(packed.size() * sizeof(uint32_t)));
-  return hipMemcpy(dst.p, packed.data(), packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-}
-
-bool pad_replicate(const uint8_t* src, int sw, int sh, int cw, int ch, std::vector<uint8_t>& out) {
-  if (!src || sw < 1 || sh < 1 || sw > cw || sh > ch) return false;
-  out.resize((size_t)cw * ch);
-  for (int y = 0; y < ch; ++y) {
-    const uint8_t* row = src + (size_t)std::min(y, sh - 1) * sw;
-    for (int x = 0; x < cw; ++x) out[(size_t)y * cw + x] = row[std::min(x, sw - 1)];
-  }
-  return true;
-}
-
-bool surface_ok(const hmSurface& s, int channels, bool need_vec_align) {
-  if (!s.d_ptr || s.width == 0 || s.height == 0) return false;
-  if (s.pitch < s.width * (uint32_t)channels) return false;
-  if (need_vec_align && channels == 4 && ((reinterpret_cast<uintptr_t>(s.d_ptr) & 3u) || (s.pitch & 3u)))
-    return false;
-  return true;
-}
-
-}  // namespace hm
-
-hipError_t hmStitchPano::init_pyramid(int kind, int bw, int bh, const std::vector<float>& mask_host) {
-  HM_RETURN_IF_ERROR(pyr.init(kind, n_images, channels, bw, bh, num_levels, batch, debug));
-  HM_RETURN_IF_ERROR(mask0.alloc(mask_host.size() * sizeof(float)));
-  HM_RETURN_IF_ERROR(hipMemcpy(mask0.p, mask_host.data(), mask_host.size() * sizeof(float), hipMemcpyHostToDevice));
-  // The mask pyramid depends only on the seam: built once and kept resident
-  // (the reference rebuilds it every frame with the same kernel arithmetic).
-  HM_RETURN_IF_ERROR(pyr.build_mask_pyramid(mask0.as<float>(), nullptr));
-  HM_RETURN_IF_ERROR(hipStreamSynchronize(nullptr));
-  initialized_pyramid = true;
-  return hipSuccess;
-}
 
 // ===========================================================================
 // CudaStitchPano
@@ -268,17 +177,7 @@ struct Pano2 : hmStitchPano {
   hm::DevBuf mask_u8;
   Pano2L0 l0{};
   int roi_x0 = 0, roi_x1 = 0;  // canvas columns whose remap feeds the blend inputs
-  // The canvas outside the blend columns does not depend on the pyramid: it
-  // is remapped on a second stream, forked from and joined back into the
-  // caller's stream, so it overlaps the pyramid's latency-bound launches.
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
-  ~Pano2() override {
-    if (ev_fork) (void)hipEventDestroy(ev_fork);
-    if (ev_join) (void)hipEventDestroy(ev_join);
-    if (side) (void)hipStreamDestroy(side);
-  }
   hipError_t setup(const hmControlMasks2& m, int minimize_blend);
   hipError_t ensure_pyramid();
   hipError_t process(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s);
@@ -379,11 +278,7 @@ hipError_t Pano2::ensure_pyramid() {
   HM_RETURN_IF_ERROR(hipMemcpy(mask_u8.p, mask_u8_host.data(), mask_u8_host.size(), hipMemcpyHostToDevice));
   pyr.mask0_fast = mask_u8.p;  // float(seam byte) == the float mask, 1 byte per pixel
   pyr.mask0_mode = kMask0U8;
-  if (l0.write_canvas) {
-    HM_RETURN_IF_ERROR(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    HM_RETURN_IF_ERROR(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    HM_RETURN_IF_ERROR(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-  }
+  if (l0.write_canvas) HM_RETURN_IF_ERROR(ensure_side());
   return hipSuccess;
 }
 
@@ -468,18 +363,12 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   // them.  The canvas remap forks after them, so it does not delay them.
   HM_RETURN_IF_ERROR(launch_spans(roi_x0, roi_x1, 0, 0, s, "remap_roi"));
   if (side) {
-    HM_RETURN_IF_ERROR(hipEventRecord(ev_fork, s));
-    HM_RETURN_IF_ERROR(hipStreamWaitEvent(side, ev_fork, 0));
-    HM_RETURN_IF_ERROR(launch_spans(0, roi_x0, roi_x1, canvas_w, side, "remap_canvas"));
-    HM_RETURN_IF_ERROR(hipEventRecord(ev_join, side));
+    HM_RETURN_IF_ERROR(fork(s));
+    const hipError_t e = launch_spans(0, roi_x0, roi_x1, canvas_w, side, "remap_canvas");
+    HM_RETURN_IF_ERROR(join(s));
+    HM_RETURN_IF_ERROR(e);
   }
-  const hipError_t e = run_blend(canvas, s);
-  // Join even after an error, so the caller's stream still orders the side work.
-  if (side) {
-    const hipError_t j = hipStreamWaitEvent(s, ev_join, 0);
-    if (e == hipSuccess) return j;
-  }
-  return e;
+  return run_blend(canvas, s);
 }
 
 hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s) {

@@ -1,0 +1,397 @@
+// hm_panoN.hip -- CudaStitchPanoN (2..8 images) on gfx950.
+//
+// Reference: src/pano/cudaPanoN.inl (ctor :108-370, process :489-685),
+// src/pano/canvasManagerN.h (:15-79), src/pano/controlMasksN.cpp (:195-232).
+// Per frame the reference clears the canvas, runs the N-image hard seam over
+// it, remaps every image into a float buffer over the blend ROI, runs the
+// N-image Laplacian blend (about 7 launches per level) and copies the write
+// ROI back.  Here:
+//   1. one launch remaps all N images over the blend ROI into exact uint8
+//      level-0 blend inputs (zero outside each image's box);
+//   2. one launch on a second stream writes the canvas outside the write ROI
+//      (what the clear + hard seam leave there), overlapping the pyramid;
+//   3. the pyramid (tile-fused down, per-level quad up, one-hot level-0 mask
+//      read as one label byte per pixel) writes the write ROI of the
+//      reconstruction straight into the canvas.
+// levels == 0: one launch, the hard seam with the cleared default.
+#include <algorithm>
+#include <vector>
+
+#include "hm_pano.hpp"
+#include "hm_pano_device.hpp"
+
+using namespace hmdev;
+
+namespace {
+
+constexpr int kOverlapPadN = 128;  // CanvasManagerN overlap_pad default (canvasManagerN.h:15)
+
+struct Rect {
+  int x = 0, y = 0, w = 0, h = 0;
+  bool empty() const { return w <= 0 || h <= 0; }
+};
+
+// detailN::pyramid_margin / pyramid_alignment (cudaPanoN.inl:20-36).
+int pyramid_margin(int levels) { return levels <= 0 ? 0 : 1 << std::min(levels, 30); }
+int pyramid_alignment(int levels) { return levels <= 1 ? 1 : 1 << std::min(levels - 1, 30); }
+
+// detailN::seam_boundary_bbox (cudaPanoN.inl:38-77): the box around every
+// pixel whose label differs from its right or lower neighbour, together with
+// that neighbour.
+bool seam_boundary_bbox(const std::vector<uint8_t>& s, int w, int h, Rect& out) {
+  if (w <= 1 || h <= 1) return false;
+  int x0 = w, y0 = h, x1 = -1, y1 = -1;
+  for (int y = 0; y < h; ++y) {
+    const uint8_t* r = &s[(size_t)y * w];
+    const uint8_t* d = y + 1 < h ? r + w : nullptr;
+    for (int x = 0; x < w; ++x) {
+      const bool right = x + 1 < w && r[x] != r[x + 1];
+      const bool down = d && r[x] != d[x];
+      if (!right && !down) continue;
+      x0 = std::min(x0, x);
+      y0 = std::min(y0, y);
+      x1 = std::max(x1, right ? x + 1 : x);
+      y1 = std::max(y1, down ? y + 1 : y);
+    }
+  }
+  if (x1 < 0 || y1 < 0) return false;
+  out = Rect{x0, y0, x1 - x0 + 1, y1 - y0 + 1};
+  return true;
+}
+
+// detailN::expand_and_clamp / align_and_clamp (cudaPanoN.inl:79-105).
+Rect expand_and_clamp(const Rect& r, int pad, int max_w, int max_h) {
+  if (r.empty()) return Rect{};
+  const int x0 = std::max(0, r.x - pad), y0 = std::max(0, r.y - pad);
+  const int x1 = std::min(max_w, r.x + r.w + pad), y1 = std::min(max_h, r.y + r.h + pad);
+  return Rect{x0, y0, std::max(0, x1 - x0), std::max(0, y1 - y0)};
+}
+Rect align_and_clamp(const Rect& r, int align, int max_w, int max_h) {
+  if (r.empty()) return Rect{};
+  if (align <= 1) return r;
+  const int x0 = std::max(0, (r.x / align) * align), y0 = std::max(0, (r.y / align) * align);
+  const int x1 = std::min(max_w, ((r.x + r.w + align - 1) / align) * align);
+  const int y1 = std::min(max_h, ((r.y + r.h + align - 1) / align) * align);
+  return Rect{x0, y0, std::max(0, x1 - x0), std::max(0, y1 - y0)};
+}
+
+template <int N>
+struct PanoNArgs {
+  Surf src[N];
+  const uint32_t* map[N];  // packed x | y << 16
+  int mw[N], mh[N], px[N], py[N];
+  int x0, x1, y0, y1;      // canvas region of this launch, [x0, x1) x [y0, y1)
+  int tiles_x;
+  // blend inputs: uint8 bw x bh images whose origin is canvas (x0, y0)
+  uint8_t* g0[N];
+  int bw, bh;
+  // canvas
+  uint8_t* canvas;
+  uint32_t cpitch;
+  int cw, ch;
+  const uint8_t* seam;                     // canvas-sized labels
+  int skip_x0, skip_x1, skip_y0, skip_y1;  // left to the blend write-back
+};
+
+// Level-0 blend input i = image i remapped at its position
+// (BatchedRemapKernelExOffset(ROI), cudaRemap.cu:95-122 / :129-198, default
+// pixel all-zero) inside its box, and the cleared buffer's zero outside.
+template <int C, int N>
+__global__ void __launch_bounds__(256) panoN_inputs_kernel(PanoNArgs<N> a) {
+  const int t = (int)blockIdx.x;
+  const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
+  const int b = blockIdx.z;
+  if (Y >= a.y1) return;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int ry = Y - a.py[i];
+    const bool row_ok = ry >= 0 && ry < a.mh[i];
+    const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
+    uint32_t m[kPx];
+    bool in[kPx];
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) {
+      const int X = X0 + j * (int)blockDim.x;
+      const int rx = X - a.px[i];
+      in[j] = row_ok && rx >= 0 && rx < a.mw[i] && X < a.x1;
+      m[j] = in[j] ? mrow[rx] : 0u;
+    }
+    uint32_t r[kPx];
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) r[j] = in[j] ? remap_px<C>(a.src[i], b, m[j]) : 0u;
+    uint8_t* grow = a.g0[i] + ((size_t)b * a.bh + (Y - a.y0)) * a.bw * C;
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) {
+      const int X = X0 + j * (int)blockDim.x;
+      if (X < a.x1) store_px<C>(grow + (size_t)(X - a.x0) * C, r[j]);
+    }
+  }
+}
+
+// The cleared canvas after batched_remap_hard_seam_kernel_n
+// (cudaRemap.cu:252-294): the labelled image's source pixel where its map is
+// in range, else zero; pixels inside the skip rectangle are not written.
+template <int C, int N>
+__global__ void __launch_bounds__(256) panoN_canvas_kernel(PanoNArgs<N> a) {
+  const int t = (int)blockIdx.x;
+  const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
+  const int b = blockIdx.z;
+  if (Y >= a.y1) return;
+  const uint8_t* srow = a.seam + (size_t)Y * a.cw;
+  const bool skip_row = Y >= a.skip_y0 && Y < a.skip_y1;
+  int lab[kPx];
+  bool wr[kPx];
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const int X = X0 + j * (int)blockDim.x;
+    wr[j] = X < a.x1 && !(skip_row && X >= a.skip_x0 && X < a.skip_x1);
+    lab[j] = wr[j] ? (int)srow[X] : -1;
+  }
+  uint32_t v[kPx] = {};
+  // One branch per image with compile-time kernel-argument indices; a
+  // wavefront runs only the branches of the labels its lanes hold.
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    uint32_t m[kPx];
+    bool hit[kPx];
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) {
+      const int rx = X0 + j * (int)blockDim.x - a.px[i], ry = Y - a.py[i];
+      hit[j] = lab[j] == i && rx >= 0 && ry >= 0 && rx < a.mw[i] && ry < a.mh[i];
+      m[j] = hit[j] ? a.map[i][(size_t)ry * a.mw[i] + rx] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kPx; ++j)
+      if (hit[j]) v[j] = remap_px_dm<C>(a.src[i], b, m[j]);
+  }
+  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+#pragma unroll
+  for (int j = 0; j < kPx; ++j)
+    if (wr[j]) store_px<C>(crow + (size_t)(X0 + j * (int)blockDim.x) * C, v[j]);
+}
+
+struct PanoN : hmStitchPano {
+  int pos_x[kMaxImages] = {}, pos_y[kMaxImages] = {}, mw[kMaxImages] = {}, mh[kMaxImages] = {};
+  hm::DevBuf map[kMaxImages];
+  hm::DevBuf seam;  // padded labels, canvas-sized (hard seam and the canvas outside the write ROI)
+  bool use_min = false;
+  Rect write_roi, blend_roi;
+  int bx0 = 0, by0 = 0, bw = 0, bh = 0;
+  std::vector<float> mask_host;      // one-hot, bw x bh x n
+  std::vector<uint8_t> label_host;   // the same mask as one label byte per pixel
+  hm::DevBuf labels;
+
+  hipError_t setup(const hmControlMasksN& m, int minimize_blend);
+  hipError_t ensure_pyramid();
+  hipError_t process(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, hipStream_t s);
+  template <int N>
+  hipError_t launch_region(bool inputs_kernel, const hmSurface* inputs, const hmSurface& canvas, int x0, int x1,
+                           int y0, int y1, const Rect& skip, hipStream_t s, const char* name);
+};
+
+hipError_t PanoN::setup(const hmControlMasksN& m, int minimize_blend) {
+  const int n = m.n_images;
+  if (n < 2 || n > kMaxImages) return hipErrorInvalidValue;  // cudaBlendN dispatches N in [2, 8]
+  // ControlMasksN::is_valid (controlMasksN.cpp:195-203).
+  if (!m.whole_seam_mask_indexed || m.seam_w < 1 || m.seam_h < 1) return hipErrorFileNotFound;
+  for (int i = 0; i < n; ++i)
+    if (!m.img_col[i] || !m.img_row[i] || m.img_w[i] < 1 || m.img_h[i] < 1) return hipErrorFileNotFound;
+  n_images = n;
+  hard_seam = num_levels == 0;
+  // ControlMasksN::canvas_width/height: float max, then size_t
+  // (controlMasksN.cpp:205-219); positions become cv::Point (int).
+  float maxw = 0.0f, maxh = 0.0f;
+  for (int i = 0; i < n; ++i) {
+    maxw = std::max(maxw, m.xpos[i] + (float)m.img_w[i]);
+    maxh = std::max(maxh, m.ypos[i] + (float)m.img_h[i]);
+    pos_x[i] = (int)m.xpos[i];
+    pos_y[i] = (int)m.ypos[i];
+    mw[i] = m.img_w[i];
+    mh[i] = m.img_h[i];
+  }
+  canvas_w = (int)(size_t)maxw;
+  canvas_h = (int)(size_t)maxh;
+  for (int i = 0; i < n; ++i) HM_RETURN_IF_ERROR(hm::upload_packed_map(m.img_col[i], m.img_row[i], mw[i], mh[i], map[i]));
+  // CanvasManagerN::convertMaskMat (canvasManagerN.h:33-55).
+  std::vector<uint8_t> padded;
+  if (!hm::pad_replicate(m.whole_seam_mask_indexed, m.seam_w, m.seam_h, canvas_w, canvas_h, padded))
+    return hipErrorInvalidValue;
+  HM_RETURN_IF_ERROR(seam.alloc(padded.size()));
+  HM_RETURN_IF_ERROR(hipMemcpy(seam.p, padded.data(), padded.size(), hipMemcpyHostToDevice));
+  if (hard_seam) return hipSuccess;
+
+  // ROI choice of the ctor (cudaPanoN.inl:151-190).
+  const bool minimize = minimize_blend != 0;  // minimize_blend_ = minimize_blend && num_levels > 0
+  Rect bb;
+  if (minimize && seam_boundary_bbox(padded, canvas_w, canvas_h, bb)) {
+    write_roi = expand_and_clamp(bb, kOverlapPadN, canvas_w, canvas_h);
+    blend_roi = expand_and_clamp(write_roi, pyramid_margin(num_levels), canvas_w, canvas_h);
+    blend_roi = align_and_clamp(blend_roi, pyramid_alignment(num_levels), canvas_w, canvas_h);
+  }
+  // process: use_minimized_blend (cudaPanoN.inl:541-543).
+  use_min = minimize && !blend_roi.empty() && !write_roi.empty();
+  bx0 = use_min ? blend_roi.x : 0;
+  by0 = use_min ? blend_roi.y : 0;
+  bw = use_min ? blend_roi.w : canvas_w;
+  bh = use_min ? blend_roi.h : canvas_h;
+  // split_to_channels + convertTo(float) of the (cropped) seam
+  // (controlMasksN.cpp:221-232, cudaPanoN.inl:192-194).
+  mask_host.assign((size_t)bw * bh * n, 0.0f);
+  label_host.resize((size_t)bw * bh);
+  for (int y = 0; y < bh; ++y)
+    for (int x = 0; x < bw; ++x) {
+      const uint8_t l = padded[(size_t)(by0 + y) * canvas_w + bx0 + x];
+      label_host[(size_t)y * bw + x] = l;
+      if (l < n) mask_host[((size_t)y * bw + x) * n + l] = 1.0f;
+    }
+  for (int i = 0; i < n; ++i) HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+  return hipSuccess;
+}
+
+hipError_t PanoN::ensure_pyramid() {
+  if (hard_seam || initialized_pyramid) return hipSuccess;
+  HM_RETURN_IF_ERROR(init_pyramid(kBlendN, bw, bh, mask_host));
+  HM_RETURN_IF_ERROR(labels.alloc(label_host.size()));
+  HM_RETURN_IF_ERROR(hipMemcpy(labels.p, label_host.data(), label_host.size(), hipMemcpyHostToDevice));
+  pyr.mask0_fast = labels.p;  // one-hot of the label: the float mask's values
+  pyr.mask0_mode = kMask0Label;
+  if (use_min) HM_RETURN_IF_ERROR(ensure_side());
+  return hipSuccess;
+}
+
+template <int N>
+hipError_t PanoN::launch_region(bool inputs_kernel, const hmSurface* inputs, const hmSurface& canvas, int x0, int x1,
+                                int y0, int y1, const Rect& skip, hipStream_t s, const char* name) {
+  if (x1 <= x0 || y1 <= y0) return hipSuccess;
+  PanoNArgs<N> a{};
+  for (int i = 0; i < N; ++i) {
+    a.src[i] = to_surf(inputs[i]);
+    a.map[i] = map[i].as<uint32_t>();
+    a.mw[i] = mw[i];
+    a.mh[i] = mh[i];
+    a.px[i] = pos_x[i];
+    a.py[i] = pos_y[i];
+    a.g0[i] = g0[i].as<uint8_t>();
+  }
+  a.x0 = x0;
+  a.x1 = x1;
+  a.y0 = y0;
+  a.y1 = y1;
+  const int tile_w = (int)kBlock.x * kPx;
+  a.tiles_x = (x1 - x0 + tile_w - 1) / tile_w;
+  const int tiles_y = (y1 - y0 + (int)kBlock.y - 1) / (int)kBlock.y;
+  a.bw = bw;
+  a.bh = bh;
+  a.canvas = static_cast<uint8_t*>(canvas.d_ptr);
+  a.cpitch = canvas.pitch;
+  a.cw = canvas_w;
+  a.ch = canvas_h;
+  a.seam = seam.as<uint8_t>();
+  a.skip_x0 = skip.x;
+  a.skip_x1 = skip.x + skip.w;
+  a.skip_y0 = skip.y;
+  a.skip_y1 = skip.y + skip.h;
+  const dim3 grid((unsigned)(a.tiles_x * tiles_y), 1u, (unsigned)batch);
+  timer.begin(s);
+  if (inputs_kernel) {
+    if (channels == 4)
+      hipLaunchKernelGGL((panoN_inputs_kernel<4, N>), grid, kBlock, 0, s, a);
+    else
+      hipLaunchKernelGGL((panoN_inputs_kernel<3, N>), grid, kBlock, 0, s, a);
+  } else {
+    if (channels == 4)
+      hipLaunchKernelGGL((panoN_canvas_kernel<4, N>), grid, kBlock, 0, s, a);
+    else
+      hipLaunchKernelGGL((panoN_canvas_kernel<3, N>), grid, kBlock, 0, s, a);
+  }
+  timer.end(s, name);
+  return hipGetLastError();
+}
+
+hipError_t PanoN::process(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, hipStream_t s) {
+  // Validation errors of CudaStitchPanoN::process (cudaPanoN.inl:494-504).
+  if (!canvas.d_ptr) return hipErrorInvalidDevicePointer;
+  if (!inputs || n_inputs != n_images) return hipErrorInvalidValue;
+  for (int i = 0; i < n_images; ++i)
+    if (!hm::surface_ok(inputs[i], channels, true)) return hipErrorInvalidValue;
+  if (!hm::surface_ok(canvas, channels, true) || (int)canvas.width != canvas_w || (int)canvas.height != canvas_h)
+    return hipErrorInvalidValue;
+  auto per_n = [&](auto NV) -> hipError_t {
+    constexpr int N = decltype(NV)::value;
+    if (hard_seam)  // cudaPanoN.inl:508-541: clear + hard seam, one pass
+      return launch_region<N>(false, inputs, canvas, 0, canvas_w, 0, canvas_h, Rect{}, s, "hard_seam");
+    HM_RETURN_IF_ERROR(ensure_pyramid());
+    HM_RETURN_IF_ERROR(
+        launch_region<N>(true, inputs, canvas, bx0, bx0 + bw, by0, by0 + bh, Rect{}, s, "remap_inputs"));
+    if (use_min) {
+      HM_RETURN_IF_ERROR(fork(s));
+      const hipError_t e =
+          launch_region<N>(false, inputs, canvas, 0, canvas_w, 0, canvas_h, write_roi, side, "remap_canvas");
+      HM_RETURN_IF_ERROR(join(s));
+      HM_RETURN_IF_ERROR(e);
+    }
+    const void* g[kMaxImages];
+    for (int i = 0; i < n_images; ++i) g[i] = g0[i].p;
+    hm::Level0Out out;
+    out.u8 = static_cast<uint8_t*>(canvas.d_ptr);
+    out.u8_pitch = canvas.pitch;
+    out.u8_w = canvas_w;
+    out.u8_h = canvas_h;
+    out.ox = bx0;
+    out.oy = by0;
+    if (use_min) {  // copy-back of the write ROI (cudaPanoN.inl:628-640)
+      out.win_x = write_roi.x - bx0;
+      out.win_y = write_roi.y - by0;
+      out.win_w = write_roi.w;
+      out.win_h = write_roi.h;
+    }
+    return pyr.run(g, true, out, s, timer_or_null());
+  };
+  using std::integral_constant;
+  switch (n_images) {
+    case 2: return per_n(integral_constant<int, 2>{});
+    case 3: return per_n(integral_constant<int, 3>{});
+    case 4: return per_n(integral_constant<int, 4>{});
+    case 5: return per_n(integral_constant<int, 5>{});
+    case 6: return per_n(integral_constant<int, 6>{});
+    case 7: return per_n(integral_constant<int, 7>{});
+    case 8: return per_n(integral_constant<int, 8>{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" int hmStitchPanoNCreate(int batch_size, int num_levels, const hmControlMasksN* masks, int minimize_blend,
+                                   int quiet, int channels, hmStitchPano** out) {
+  (void)quiet;
+  if (!out) return hipErrorInvalidValue;
+  auto* p = new PanoN();
+  *out = p;
+  p->batch = batch_size;
+  p->num_levels = num_levels;
+  p->channels = channels;
+  if (!masks) {
+    p->status = hipErrorFileNotFound;
+  } else if (batch_size < 1 || num_levels < 0 || num_levels > kMaxLevels || (channels != 3 && channels != 4)) {
+    p->status = hipErrorInvalidValue;
+  } else {
+    p->status = p->setup(*masks, minimize_blend && num_levels > 0);
+  }
+  return p->status;
+}
+
+extern "C" int hmStitchPanoNProcess(hmStitchPano* pano, const hmSurface* inputs, int num_inputs, hmSurface canvas,
+                                    hipStream_t stream) {
+  if (!pano) return hipErrorInvalidValue;
+  if (pano->status != hipSuccess) return pano->status;
+  auto* p = dynamic_cast<PanoN*>(pano);
+  if (!p) return hipErrorInvalidValue;
+  // CudaStitchPanoN::process returns errors without making them sticky.
+  return p->process(inputs, num_inputs, canvas, stream);
+}

@@ -2,6 +2,7 @@
 // pyramid inspection and launch timing.
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "hm_pano.hpp"
@@ -14,6 +15,78 @@ __global__ void u8_to_f32_kernel(const uint8_t* in, float* out, size_t n) {
 }
 
 }  // namespace
+
+// ===========================================================================
+// shared helpers
+// ===========================================================================
+namespace hm {
+
+hipError_t upload_packed_map(const uint16_t* map_x, const uint16_t* map_y, int w, int h, DevBuf& dst) {
+  if (!map_x || !map_y || w < 1 || h < 1) return hipErrorInvalidValue;
+  std::vector<uint32_t> packed((size_t)w * h);
+  for (size_t i = 0; i < packed.size(); ++i) packed[i] = (uint32_t)map_x[i] | ((uint32_t)map_y[i] << 16);
+  HM_RETURN_IF_ERROR(dst.alloc(packed.size() * sizeof(uint32_t)));
+  return hipMemcpy(dst.p, packed.data(), packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+}
+
+bool pad_replicate(const uint8_t* src, int sw, int sh, int cw, int ch, std::vector<uint8_t>& out) {
+  if (!src || sw < 1 || sh < 1 || sw > cw || sh > ch) return false;
+  out.resize((size_t)cw * ch);
+  for (int y = 0; y < ch; ++y) {
+    const uint8_t* row = src + (size_t)std::min(y, sh - 1) * sw;
+    for (int x = 0; x < cw; ++x) out[(size_t)y * cw + x] = row[std::min(x, sw - 1)];
+  }
+  return true;
+}
+
+bool surface_ok(const hmSurface& s, int channels, bool need_vec_align) {
+  if (!s.d_ptr || s.width == 0 || s.height == 0) return false;
+  if (s.pitch < s.width * (uint32_t)channels) return false;
+  if (need_vec_align && channels == 4 && ((reinterpret_cast<uintptr_t>(s.d_ptr) & 3u) || (s.pitch & 3u)))
+    return false;
+  return true;
+}
+
+}  // namespace hm
+
+hipError_t hmStitchPano::init_pyramid(int kind, int bw, int bh, const std::vector<float>& mask_host) {
+  HM_RETURN_IF_ERROR(pyr.init(kind, n_images, channels, bw, bh, num_levels, batch, debug));
+  HM_RETURN_IF_ERROR(mask0.alloc(mask_host.size() * sizeof(float)));
+  HM_RETURN_IF_ERROR(hipMemcpy(mask0.p, mask_host.data(), mask_host.size() * sizeof(float), hipMemcpyHostToDevice));
+  // The mask pyramid depends only on the seam: built once and kept resident
+  // (the reference rebuilds it every frame with the same kernel arithmetic).
+  HM_RETURN_IF_ERROR(pyr.build_mask_pyramid(mask0.as<float>(), nullptr));
+  HM_RETURN_IF_ERROR(hipStreamSynchronize(nullptr));
+  initialized_pyramid = true;
+  return hipSuccess;
+}
+
+
+// ===========================================================================
+// side stream
+// ===========================================================================
+hmStitchPano::~hmStitchPano() {
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
+}
+
+hipError_t hmStitchPano::ensure_side() {
+  if (side) return hipSuccess;
+  HM_RETURN_IF_ERROR(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  HM_RETURN_IF_ERROR(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  return hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+}
+
+hipError_t hmStitchPano::fork(hipStream_t s) {
+  HM_RETURN_IF_ERROR(hipEventRecord(ev_fork, s));
+  return hipStreamWaitEvent(side, ev_fork, 0);
+}
+
+hipError_t hmStitchPano::join(hipStream_t s) {
+  HM_RETURN_IF_ERROR(hipEventRecord(ev_join, side));
+  return hipStreamWaitEvent(s, ev_join, 0);
+}
 
 extern "C" int hmStitchPanoStatus(const hmStitchPano* pano) { return pano ? pano->status : hipErrorInvalidValue; }
 

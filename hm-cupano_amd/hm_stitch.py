@@ -44,6 +44,19 @@ class hmControlMasks2(ctypes.Structure):
     ]
 
 
+HM_MAX_IMAGES = 8
+
+
+class hmControlMasksN(ctypes.Structure):
+    _fields_ = [
+        ("n_images", ctypes.c_int),
+        ("img_col", ctypes.c_void_p * HM_MAX_IMAGES), ("img_row", ctypes.c_void_p * HM_MAX_IMAGES),
+        ("img_w", ctypes.c_int * HM_MAX_IMAGES), ("img_h", ctypes.c_int * HM_MAX_IMAGES),
+        ("whole_seam_mask_indexed", ctypes.c_void_p), ("seam_w", ctypes.c_int), ("seam_h", ctypes.c_int),
+        ("xpos", ctypes.c_float * HM_MAX_IMAGES), ("ypos", ctypes.c_float * HM_MAX_IMAGES),
+    ]
+
+
 _lib = None
 
 
@@ -72,6 +85,8 @@ def lib() -> ctypes.CDLL:
         "hmBlendContextDestroy": (None, [vp]),
         "hmStitchPanoCreate": (i, [i, i, ctypes.POINTER(hmControlMasks2), i, i, i, i, ctypes.POINTER(vp)]),
         "hmStitchPanoProcess": (i, [vp, S, S, S, vp]),
+        "hmStitchPanoNCreate": (i, [i, i, ctypes.POINTER(hmControlMasksN), i, i, i, ctypes.POINTER(vp)]),
+        "hmStitchPanoNProcess": (i, [vp, ctypes.POINTER(S), i, S, vp]),
         "hmStitchPanoStatus": (i, [vp]),
         "hmStitchPanoCanvasSize": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i)]),
         "hmStitchPanoBatchSize": (i, [vp]),
@@ -222,6 +237,51 @@ class StitchPano:
             name, ms, cnt = line.split()
             out[name] = (float(ms), int(cnt))
         return out
+
+
+class StitchPanoN(StitchPano):
+    """CudaStitchPanoN<uchar4|uchar3, float4|float3> (src/pano/cudaPanoN.h:81-146).
+
+    `rig` carries n maps, integer positions and the indexed seam (labels
+    0..n-1)."""
+
+    def __init__(self, batch_size: int, num_levels: int, rig, minimize_blend: bool = True, channels: int = 4,
+                 debug: bool = False):
+        L = lib()
+        n = len(rig.map_x)
+        m = hmControlMasksN()
+        m.n_images = n
+        self._keep = []
+        for k in range(n):
+            mx, my = _u16(rig.map_x[k]), _u16(rig.map_y[k])
+            self._keep += [mx, my]
+            m.img_col[k], m.img_row[k] = mx.ctypes.data, my.ctypes.data
+            m.img_h[k], m.img_w[k] = mx.shape
+            m.xpos[k], m.ypos[k] = float(rig.pos_x[k]), float(rig.pos_y[k])
+        seam = np.ascontiguousarray(rig.seam, dtype=np.uint8)
+        self._keep.append(seam)
+        m.whole_seam_mask_indexed = seam.ctypes.data
+        m.seam_h, m.seam_w = seam.shape
+        self._masks = m
+        self.n_images = n
+        h = ctypes.c_void_p()
+        rc = L.hmStitchPanoNCreate(batch_size, num_levels, ctypes.byref(m), int(minimize_blend), 1, channels,
+                                   ctypes.byref(h))
+        if not h.value:
+            raise HmError(rc, "hmStitchPanoNCreate")
+        self._h = h
+        self.batch_size = batch_size
+        self.channels = channels
+        self.num_levels = num_levels
+        if debug:
+            check(L.hmStitchPanoSetDebug(h, 1), "hmStitchPanoSetDebug")
+
+    def process(self, inputs, canvas, stream=None) -> int:
+        arr = (hmSurface * len(inputs))(*[surface(t) for t in inputs])
+        return lib().hmStitchPanoNProcess(self._h, arr, len(inputs), surface(canvas), _stream(stream))
+
+    def _mask_channels(self) -> int:
+        return self.n_images
 
 
 # ---------------------------------------------------------------------------

@@ -164,10 +164,41 @@ int hmStitchPanoCreate(int batch_size, int num_levels, const hmControlMasks2* ma
 /* CudaStitchPano::process (cudaPano.h:95-99, cudaPano.inl:460-472). */
 int hmStitchPanoProcess(hmStitchPano* pano, hmSurface input1, hmSurface input2, hmSurface canvas,
                         hipStream_t stream);
+/* Shared by every pano object (2-, 3- and N-image):
+ * status() / canvas_width() / canvas_height() / batch_size()
+ * (cudaPano.h:79-93, cudaPanoN.h:84-95, cudaPano3.h:72-83); Destroy replaces
+ * the destructor. */
 int hmStitchPanoStatus(const hmStitchPano* pano);
 int hmStitchPanoCanvasSize(const hmStitchPano* pano, int* width, int* height);
 int hmStitchPanoBatchSize(const hmStitchPano* pano);
 void hmStitchPanoDestroy(hmStitchPano* pano);
+
+#define HM_MAX_IMAGES 8
+
+/* In-memory ControlMasksN (src/pano/controlMasksN.h:13-38): per-image CV_16U
+ * remap tables (host, tight), the indexed seam with labels 0..n-1 (as
+ * ControlMasksN::load leaves it, controlMasksN.cpp:125-193), and the
+ * normalised SpatialTiff positions. */
+typedef struct hmControlMasksN {
+  int n_images;
+  const uint16_t* img_col[HM_MAX_IMAGES];
+  const uint16_t* img_row[HM_MAX_IMAGES];
+  int img_w[HM_MAX_IMAGES], img_h[HM_MAX_IMAGES];
+  const uint8_t* whole_seam_mask_indexed;
+  int seam_w, seam_h;
+  float xpos[HM_MAX_IMAGES], ypos[HM_MAX_IMAGES];
+} hmControlMasksN;
+
+/* CudaStitchPanoN<uchar4|uchar3, float4|float3> ctor (src/pano/cudaPanoN.h:83,
+ * cudaPanoN.inl:108-370), same argument order (minimize_blend before quiet).
+ * Constructor errors are sticky, as for the 2-image class. */
+int hmStitchPanoNCreate(int batch_size, int num_levels, const hmControlMasksN* masks, int minimize_blend, int quiet,
+                        int channels, hmStitchPano** out);
+/* CudaStitchPanoN::process (cudaPanoN.h:97-101, cudaPanoN.inl:489-685):
+ * inputs[0..num_inputs) are the N pitched pipeline images.  Like the
+ * reference, validation errors are returned without becoming sticky. */
+int hmStitchPanoNProcess(hmStitchPano* pano, const hmSurface* inputs, int num_inputs, hmSurface canvas,
+                         hipStream_t stream);
 
 /* ------------------------------------------------------------------------- */
 /* Pyramid inspection and launch timing (all pano classes)                   */

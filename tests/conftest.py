@@ -51,3 +51,35 @@ def frames_for(rig, batch, seed, channels=4):
                        for b in range(batch)])
         out.append(fr)
     return out
+
+
+# ---- small rigs and images of the reference's pano tests ----
+def identity_maps(w, h):
+    return (np.tile(np.arange(w, dtype=np.uint16), (h, 1)), np.tile(np.arange(h, dtype=np.uint16)[:, None], (1, w)))
+
+
+class IdRig:
+    """Identity-map rig of the reference's pano tests."""
+
+    def __init__(self, sizes, positions, seam):
+        self.map_x = [identity_maps(w, h)[0] for w, h in sizes]
+        self.map_y = [identity_maps(w, h)[1] for w, h in sizes]
+        self.pos_x = [p[0] for p in positions]
+        self.pos_y = [p[1] for p in positions]
+        self.seam = seam
+        self.n = len(sizes)
+
+
+def const_u8(w, h, rgba):
+    return np.broadcast_to(np.uint8(rgba), (1, h, w, 4)).copy()
+
+
+def patterned_u8(w, h, idx):
+    ys, xs = np.mgrid[0:h, 0:w]
+    base = idx * 50
+    img = np.zeros((1, h, w, 4), np.uint8)
+    img[0, ..., 0] = base + xs % 17
+    img[0, ..., 1] = base + ys % 19
+    img[0, ..., 2] = base + (xs + ys) % 23
+    img[0, ..., 3] = 255
+    return img

@@ -1,0 +1,162 @@
+"""GPU parity of the N-image pano (CudaStitchPanoN) against the CPU oracle.
+
+Bit-exact throughout: the remaps and hard seam are integer work; the N-image
+blend runs the reference's float arithmetic in the reference's order with no
+fused multiply-add on either side."""
+import numpy as np
+import pytest
+
+from conftest import IdRig, const_u8, frames_for, patterned_u8
+
+pytestmark = pytest.mark.gpu
+
+
+def run_panoN(hm, torch, rig, frames, levels, minimize=True, channels=4, debug=False, pitch_pad=0, fill=77):
+    b = frames[0].shape[0]
+    pano = hm.StitchPanoN(b, levels, rig, minimize_blend=minimize, channels=channels, debug=debug)
+    assert pano.status == 0, pano.status
+    ins = []
+    for f in frames:
+        t = torch.from_numpy(np.ascontiguousarray(f)).cuda()
+        if pitch_pad:
+            p = torch.zeros((b, f.shape[1], f.shape[2] + pitch_pad, channels), dtype=torch.uint8, device="cuda")
+            p[:, :, : f.shape[2]] = t
+            t = p[:, :, : f.shape[2]]
+        ins.append(t)
+    w, h = pano.canvas_size()
+    cbuf = torch.full((b, h, w + pitch_pad, channels), fill, dtype=torch.uint8, device="cuda")
+    canvas = cbuf[:, :, :w]
+    rc = pano.process(ins, canvas)
+    assert rc == 0, rc
+    torch.cuda.synchronize()
+    return pano, canvas.cpu().numpy()
+
+
+def small_n_rig(n, seed):
+    import hm_synth
+
+    return hm_synth.small_rig(n=n, seed=seed, src=(120, 56), canvas=(90 * n + 60, 64))
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8])
+@pytest.mark.parametrize("levels", [0, 1, 3, 6])
+@pytest.mark.parametrize("minimize", [True, False])
+def test_panoN_small_rig_bit_exact(hm, torch_gpu, oracle, n, levels, minimize):
+    rig = small_n_rig(n, 40 + n)
+    frames = frames_for(rig, 1, 3 + n)
+    _, got = run_panoN(hm, torch_gpu, rig, frames, levels, minimize)
+    want = oracle.panoN(rig, frames, levels, minimize=minimize)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("levels", [0, 4])
+def test_panoN_three_channel_batch_pitched(hm, torch_gpu, oracle, levels):
+    rig = small_n_rig(3, 9)
+    frames = frames_for(rig, 2, 5, channels=3)
+    _, got = run_panoN(hm, torch_gpu, rig, frames, levels, True, channels=3, pitch_pad=7)
+    want = oracle.panoN(rig, frames, levels)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_panoN_hard_seam_selection_identity_rig(hm, torch_gpu):
+    """Three identity-mapped images and a 3-band seam select exactly (levels 0)."""
+    w, h = 48, 24
+    seam = np.zeros((h, w), np.uint8)
+    seam[:, 16:32] = 1
+    seam[:, 32:] = 2
+    rig = IdRig([(w, h)] * 3, [(0, 0)] * 3, seam)
+    imgs = [const_u8(w, h, v) for v in ([10, 0, 0, 255], [0, 20, 0, 255], [0, 0, 30, 255])]
+    _, got = run_panoN(hm, torch_gpu, rig, imgs, 0)
+    want = np.zeros((1, h, w, 4), np.uint8)
+    want[:, :, :16] = imgs[0][:, :, :16]
+    want[:, :, 16:32] = imgs[1][:, :, 16:32]
+    want[:, :, 32:] = imgs[2][:, :, 32:]
+    np.testing.assert_array_equal(got, want)
+
+
+def test_panoN_minimize_matches_full_identity_rig(hm, torch_gpu, oracle):
+    """tests/python/test_pytorch_pano.py:204-220: the minimized blend equals the full one (<= 1 LSB)."""
+    w = h = 64
+    seam = np.zeros((h, w), np.uint8)
+    seam[:, w // 2:] = 1
+    rig = IdRig([(w, h), (w, h)], [(0, 0), (0, 0)], seam)
+    imgs = [patterned_u8(w, h, 0), patterned_u8(w, h, 1)]
+    _, full = run_panoN(hm, torch_gpu, rig, imgs, 4, minimize=False)
+    _, mini = run_panoN(hm, torch_gpu, rig, imgs, 4, minimize=True)
+    np.testing.assert_array_equal(full, oracle.panoN(rig, imgs, 4, minimize=False))
+    np.testing.assert_array_equal(mini, oracle.panoN(rig, imgs, 4, minimize=True))
+    assert np.abs(full.astype(int) - mini.astype(int)).max() <= 1
+
+
+def test_panoN_single_label_seam_falls_back_to_full_blend(hm, torch_gpu, oracle):
+    """No seam boundary: the minimized path is skipped (cudaPanoN.inl:169-176)."""
+    rig = small_n_rig(3, 12)
+    rig.seam = np.full_like(rig.seam, 1)
+    frames = frames_for(rig, 1, 2)
+    _, got = run_panoN(hm, torch_gpu, rig, frames, 3, True)
+    np.testing.assert_array_equal(got, oracle.panoN(rig, frames, 3, minimize=True))
+
+
+def test_panoN_pyramid_levels_match_oracle(hm, torch_gpu, oracle):
+    rig = small_n_rig(3, 21)
+    frames = frames_for(rig, 1, 6)
+    pano, got = run_panoN(hm, torch_gpu, rig, frames, 4, True, debug=True)
+    want, d = oracle.panoN(rig, frames, 4, minimize=True, dump=True)
+    np.testing.assert_array_equal(got, want)
+    lv = pano.levels()
+    assert lv == d.sizes
+    for l in range(len(lv)):
+        for i in range(3):
+            np.testing.assert_array_equal(pano.read_level(hm.HM_LEVEL_GAUSS, i, l), d.gauss[i][l], err_msg=f"g{i} L{l}")
+            np.testing.assert_array_equal(pano.read_level(hm.HM_LEVEL_LAPLACIAN, i, l), d.lap[i][l],
+                                          err_msg=f"lap{i} L{l}")
+        np.testing.assert_array_equal(pano.read_level(hm.HM_LEVEL_MASK, 0, l), d.mask[l], err_msg=f"mask L{l}")
+        np.testing.assert_array_equal(pano.read_level(hm.HM_LEVEL_BLEND, 0, l), d.blend[l], err_msg=f"blend L{l}")
+        np.testing.assert_array_equal(pano.read_level(hm.HM_LEVEL_RECONSTRUCT, 0, l), d.reconstruct[l],
+                                      err_msg=f"rec L{l}")
+
+
+def test_panoN_errors_are_not_sticky(hm, torch_gpu, oracle):
+    """CudaStitchPanoN::process returns validation errors without storing them."""
+    torch = torch_gpu
+    rig = small_n_rig(3, 3)
+    frames = frames_for(rig, 1, 1)
+    pano = hm.StitchPanoN(1, 2, rig)
+    ins = [torch.from_numpy(f).cuda() for f in frames]
+    canvas = pano.alloc_canvas()
+    assert pano.process(ins[:2], canvas) != 0  # wrong input count
+    assert pano.status == 0
+    assert pano.process(ins, canvas) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(canvas.cpu().numpy(), oracle.panoN(rig, frames, 2))
+
+
+def test_panoN_invalid_masks_status(hm, torch_gpu):
+    rig = small_n_rig(3, 3)
+    rig.seam = np.zeros((0, 0), np.uint8)
+    assert hm.StitchPanoN(1, 2, rig).status != 0  # ControlMasksN::is_valid fails: sticky status
+
+
+@pytest.mark.parametrize("levels", [0, 6])
+def test_panoN_cfg2_n3_full_size_bit_exact(hm, torch_gpu, oracle, levels):
+    """3 x 3840x2160 -> 6200x2300 against the oracle."""
+    import hm_synth
+
+    rig = hm_synth.config_rig("cfg2_n3")
+    frames = frames_for(rig, 1, 31)
+    _, got = run_panoN(hm, torch_gpu, rig, frames, levels)
+    want = oracle.panoN(rig, frames, levels)
+    diff = got != want
+    assert not diff.any(), f"{diff.any(-1).sum()} pixels differ"
+
+
+def test_panoN_cfg2_n8_hard_seam_bit_exact(hm, torch_gpu, oracle):
+    """8 x 3840x2160 -> 14000x2600, levels 0."""
+    import hm_synth
+
+    rig = hm_synth.config_rig("cfg2_n8")
+    frames = frames_for(rig, 1, 37)
+    _, got = run_panoN(hm, torch_gpu, rig, frames, 0)
+    want = oracle.panoN(rig, frames, 0)
+    diff = got != want
+    assert not diff.any(), f"{diff.any(-1).sum()} pixels differ"

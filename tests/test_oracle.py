@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import frames_for
+from conftest import IdRig, const_u8, frames_for, identity_maps, patterned_u8
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")
 KAT = json.load(open(GOLDEN))
@@ -225,10 +225,6 @@ def test_blend_perfect_reconstruction(oracle):
 # ---------------------------------------------------------------------------
 # remap / copy
 # ---------------------------------------------------------------------------
-def identity_maps(w, h):
-    return (np.tile(np.arange(w, dtype=np.uint16), (h, 1)), np.tile(np.arange(h, dtype=np.uint16)[:, None], (1, w)))
-
-
 def test_remap_identity_copy_and_sentinels(oracle):
     rng = np.random.default_rng(1)
     src = rng.integers(0, 256, (2, 6, 8, 4), dtype=np.uint8)
@@ -321,33 +317,6 @@ def test_copy_roi_casts(oracle):
 # ---------------------------------------------------------------------------
 # pano classes (tests/python/test_pytorch_pano.py and src/pano/*_test.cpp restated)
 # ---------------------------------------------------------------------------
-class IdRig:
-    """Identity-map rig of the reference's pano tests."""
-
-    def __init__(self, sizes, positions, seam):
-        self.map_x = [identity_maps(w, h)[0] for w, h in sizes]
-        self.map_y = [identity_maps(w, h)[1] for w, h in sizes]
-        self.pos_x = [p[0] for p in positions]
-        self.pos_y = [p[1] for p in positions]
-        self.seam = seam
-        self.n = len(sizes)
-
-
-def const_u8(w, h, rgba):
-    return np.broadcast_to(np.uint8(rgba), (1, h, w, 4)).copy()
-
-
-def patterned_u8(w, h, idx):
-    ys, xs = np.mgrid[0:h, 0:w]
-    base = idx * 50
-    img = np.zeros((1, h, w, 4), np.uint8)
-    img[0, ..., 0] = base + xs % 17
-    img[0, ..., 1] = base + ys % 19
-    img[0, ..., 2] = base + (xs + ys) % 23
-    img[0, ..., 3] = 255
-    return img
-
-
 def test_pano2_hard_seam_selection(oracle):
     # tests/python/test_pytorch_pano.py:109-128
     w, h, x2 = 256, 32, 128
