@@ -362,13 +362,14 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   // Blend-input columns first, on the caller's stream: the pyramid waits on
   // them.  The canvas remap forks after them, so it does not delay them.
   HM_RETURN_IF_ERROR(launch_spans(roi_x0, roi_x1, 0, 0, s, "remap_roi"));
-  if (side) {
-    HM_RETURN_IF_ERROR(fork(s));
-    const hipError_t e = launch_spans(0, roi_x0, roi_x1, canvas_w, side, "remap_canvas");
-    HM_RETURN_IF_ERROR(join(s));
-    HM_RETURN_IF_ERROR(e);
-  }
-  return run_blend(canvas, s);
+  if (!side) return run_blend(canvas, s);
+  HM_RETURN_IF_ERROR(fork(s));
+  const hipError_t e1 = launch_spans(0, roi_x0, roi_x1, canvas_w, side, "remap_canvas");
+  const hipError_t e2 = e1 == hipSuccess ? run_blend(canvas, s) : e1;
+  // Join after the pyramid's launches (and after an error too, so the
+  // caller's stream still orders the side work).
+  const hipError_t j = join(s);
+  return e2 != hipSuccess ? e2 : j;
 }
 
 hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s) {

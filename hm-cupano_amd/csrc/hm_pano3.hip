@@ -1,0 +1,212 @@
+// hm_pano3.hip -- CudaStitchPano3 (3 images) on gfx950.
+//
+// Reference: src/pano/cudaPano3.inl (ctor :26-110, process_impl :183-402,
+// process :404-422), src/pano/cudaPano3_optimized.inl (:19-163, the default
+// fused path), src/cuda/cudaFusedKernels3.cu (:25-217),
+// src/pano/canvasManager3.cpp, src/pano/controlMasks3.cpp (:297-317).
+// The 3-image class blends over the whole seam extent (seam width x canvas
+// height).  Per frame the reference clears the canvas (RGBA only), fills three
+// float "full" buffers (one fused kernel, or remap-to-canvas + box copies),
+// runs the 3-image Laplacian blend and copies the result back.  Here:
+//   1. one launch remaps all three images into exact uint8 level-0 blend
+//      inputs (with the fused kernel's or the remap's default pixel);
+//   2. canvas columns beyond the seam width (if any) get what the reference
+//      leaves there, on a second stream;
+//   3. the pyramid writes the reconstruction straight into the canvas.
+// levels == 0: one launch of the hard seam, with the fused kernel's or the
+// dest-map remaps' handling of unmapped pixels.
+#include <algorithm>
+#include <vector>
+
+#include "hm_pano.hpp"
+#include "hm_pano_device.hpp"
+
+using namespace hmdev;
+
+namespace {
+
+struct Pano3 : hmStitchPano {
+  int pos_x[3] = {}, pos_y[3] = {}, mw[3] = {}, mh[3] = {};
+  hm::DevBuf map[3];
+  hm::DevBuf seam;  // hard seam: canvas-sized labels
+  int bw = 0, bh = 0;
+  std::vector<float> mask_host;     // one-hot, bw x bh x 3
+  std::vector<uint8_t> label_host;  // the same mask as one label byte per pixel
+  hm::DevBuf labels;
+
+  hipError_t setup(const hmControlMasks3& m);
+  hipError_t ensure_pyramid();
+  hipError_t process(const hmSurface* inputs, const hmSurface& canvas, bool fused, hipStream_t s);
+  MultiGeom geom() const;
+};
+
+hipError_t Pano3::setup(const hmControlMasks3& m) {
+  // ControlMasks3::is_valid (controlMasks3.cpp:297-301).
+  if (!m.whole_seam_mask_image || m.seam_w < 1 || m.seam_h < 1) return hipErrorFileNotFound;
+  for (int i = 0; i < 3; ++i)
+    if (!m.img_col[i] || !m.img_row[i] || m.img_w[i] < 1 || m.img_h[i] < 1) return hipErrorFileNotFound;
+  n_images = 3;
+  hard_seam = num_levels == 0;
+  // ControlMasks3::canvas_width/height (controlMasks3.cpp:303-317); the
+  // positions become cv::Point (int) (cudaPano3.inl:49-56).
+  float maxw = 0.0f, maxh = 0.0f;
+  for (int i = 0; i < 3; ++i) {
+    maxw = std::max(maxw, m.xpos[i] + (float)m.img_w[i]);
+    maxh = std::max(maxh, m.ypos[i] + (float)m.img_h[i]);
+    pos_x[i] = (int)m.xpos[i];
+    pos_y[i] = (int)m.ypos[i];
+    mw[i] = m.img_w[i];
+    mh[i] = m.img_h[i];
+  }
+  canvas_w = (int)(size_t)maxw;
+  canvas_h = (int)(size_t)maxh;
+  for (int i = 0; i < 3; ++i) HM_RETURN_IF_ERROR(hm::upload_packed_map(m.img_col[i], m.img_row[i], mw[i], mh[i], map[i]));
+  if (hard_seam) {
+    // The hard-seam kernels index the seam with the canvas width
+    // (cudaFusedKernels3.cu:151, cudaRemap.cu:233): it must be canvas-sized.
+    if (m.seam_w != canvas_w || m.seam_h < canvas_h) return hipErrorInvalidValue;
+    const size_t bytes = (size_t)canvas_w * canvas_h;
+    HM_RETURN_IF_ERROR(seam.alloc(bytes));
+    return hipMemcpy(seam.p, m.whole_seam_mask_image, bytes, hipMemcpyHostToDevice);
+  }
+  // The full buffers and the blend take the seam's width and the canvas
+  // height (cudaPano3.inl:74-89; CanvasManager3::convertMaskMat crops rows).
+  if (m.seam_h < canvas_h) return hipErrorInvalidValue;
+  bw = m.seam_w;
+  bh = canvas_h;
+  mask_host.assign((size_t)bw * bh * 3, 0.0f);
+  label_host.resize((size_t)bw * bh);
+  for (int y = 0; y < bh; ++y)
+    for (int x = 0; x < bw; ++x) {
+      const uint8_t l = m.whole_seam_mask_image[(size_t)y * m.seam_w + x];
+      label_host[(size_t)y * bw + x] = l;
+      if (l < 3) mask_host[((size_t)y * bw + x) * 3 + l] = 1.0f;  // split_to_channels (controlMasks3.cpp:187-224)
+    }
+  for (int i = 0; i < 3; ++i) HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+  return hipSuccess;
+}
+
+hipError_t Pano3::ensure_pyramid() {
+  if (hard_seam || initialized_pyramid) return hipSuccess;
+  HM_RETURN_IF_ERROR(init_pyramid(kBlend3, bw, bh, mask_host));
+  HM_RETURN_IF_ERROR(labels.alloc(label_host.size()));
+  HM_RETURN_IF_ERROR(hipMemcpy(labels.p, label_host.data(), label_host.size(), hipMemcpyHostToDevice));
+  pyr.mask0_fast = labels.p;  // one-hot of the label: the float mask's values
+  pyr.mask0_mode = kMask0Label;
+  if (bw < canvas_w) HM_RETURN_IF_ERROR(ensure_side());
+  return hipSuccess;
+}
+
+MultiGeom Pano3::geom() const {
+  MultiGeom g;
+  for (int i = 0; i < 3; ++i) {
+    g.map[i] = map[i].as<uint32_t>();
+    g.mw[i] = mw[i];
+    g.mh[i] = mh[i];
+    g.px[i] = pos_x[i];
+    g.py[i] = pos_y[i];
+    g.g0[i] = const_cast<uint8_t*>(g0[i].as<uint8_t>());
+  }
+  g.bw = bw;
+  g.bh = bh;
+  g.seam = seam.as<uint8_t>();
+  g.cw = canvas_w;
+  g.ch = canvas_h;
+  return g;
+}
+
+hipError_t Pano3::process(const hmSurface* inputs, const hmSurface& canvas, bool fused, hipStream_t s) {
+  for (int i = 0; i < 3; ++i)
+    if (!hm::surface_ok(inputs[i], channels, true)) return hipErrorInvalidValue;
+  if (!hm::surface_ok(canvas, channels, true) || (int)canvas.width != canvas_w || (int)canvas.height != canvas_h)
+    return hipErrorInvalidValue;
+  const MultiGeom g = geom();
+  const int clear = channels == 4;  // RGBA canvases are cleared first (cudaPano3_optimized.inl:35-39)
+  if (hard_seam) {
+    MultiRegion r;
+    r.kernel = kMultiCanvas;
+    r.x1 = canvas_w;
+    r.y1 = canvas_h;
+    r.mode = kCanvasLabel;
+    // Fused (FusedRemapHardSeam3) leaves unmapped pixels alone; the dest-map
+    // remaps (cudaPano3.inl:330-380) write their zero default pixel.
+    r.invalid_zero = fused ? clear : 1;
+    r.uncovered_zero = clear;
+    timer.begin(s);
+    const hipError_t e = launch_multi<3>(g, r, channels, inputs, canvas, batch, s);
+    timer.end(s, "hard_seam");
+    return e;
+  }
+  HM_RETURN_IF_ERROR(ensure_pyramid());
+  MultiRegion ir;
+  ir.kernel = fused ? kMultiInputsDM : kMultiInputs;
+  ir.x1 = bw;
+  ir.y1 = bh;
+  timer.begin(s);
+  HM_RETURN_IF_ERROR(launch_multi<3>(g, ir, channels, inputs, canvas, batch, s));
+  timer.end(s, "remap_inputs");
+  const bool extra = bw < canvas_w;
+  hipError_t e_side = hipSuccess;
+  if (extra) {
+    // Columns the blend does not write back: cleared (RGBA), and on the
+    // unfused path overwritten by the remaps of every image in order.
+    MultiRegion cr;
+    cr.kernel = kMultiCanvas;
+    cr.x0 = bw;
+    cr.x1 = canvas_w;
+    cr.y1 = canvas_h;
+    cr.mode = fused ? kCanvasClear : kCanvasLast;
+    cr.uncovered_zero = clear;
+    HM_RETURN_IF_ERROR(fork(s));
+    timer.begin(side);
+    e_side = launch_multi<3>(g, cr, channels, inputs, canvas, batch, side);
+    timer.end(side, "remap_canvas");
+  }
+  const void* g0p[3] = {g0[0].p, g0[1].p, g0[2].p};
+  hm::Level0Out out;
+  out.u8 = static_cast<uint8_t*>(canvas.d_ptr);
+  out.u8_pitch = canvas.pitch;
+  out.u8_w = canvas_w;
+  out.u8_h = canvas_h;
+  const hipError_t e = e_side == hipSuccess ? pyr.run(g0p, true, out, s, timer_or_null()) : e_side;
+  if (!extra) return e;
+  // Join after the pyramid's launches (and after an error too).
+  const hipError_t j = join(s);
+  return e != hipSuccess ? e : j;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" int hmStitchPano3Create(int batch_size, int num_levels, const hmControlMasks3* masks, int quiet,
+                                   int channels, hmStitchPano** out) {
+  (void)quiet;
+  if (!out) return hipErrorInvalidValue;
+  auto* p = new Pano3();
+  *out = p;
+  p->batch = batch_size;
+  p->num_levels = num_levels;
+  p->channels = channels;
+  if (!masks) {
+    p->status = hipErrorFileNotFound;
+  } else if (batch_size < 1 || num_levels < 0 || num_levels > kMaxLevels || (channels != 3 && channels != 4)) {
+    p->status = hipErrorInvalidValue;
+  } else {
+    p->status = p->setup(*masks);
+  }
+  return p->status;
+}
+
+extern "C" int hmStitchPano3Process(hmStitchPano* pano, hmSurface input0, hmSurface input1, hmSurface input2,
+                                    hmSurface canvas, int fused, hipStream_t stream) {
+  if (!pano) return hipErrorInvalidValue;
+  if (pano->status != hipSuccess) return pano->status;
+  auto* p = dynamic_cast<Pano3*>(pano);
+  if (!p) return hipErrorInvalidValue;
+  const hmSurface in[3] = {input0, input1, input2};
+  const hipError_t e = p->process(in, canvas, fused != 0, stream);
+  if (e != hipSuccess) pano->status = e;  // both process paths make errors sticky (cudaPano3.inl:415-421)
+  return e;
+}

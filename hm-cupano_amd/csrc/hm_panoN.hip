@@ -75,103 +75,6 @@ Rect align_and_clamp(const Rect& r, int align, int max_w, int max_h) {
   return Rect{x0, y0, std::max(0, x1 - x0), std::max(0, y1 - y0)};
 }
 
-template <int N>
-struct PanoNArgs {
-  Surf src[N];
-  const uint32_t* map[N];  // packed x | y << 16
-  int mw[N], mh[N], px[N], py[N];
-  int x0, x1, y0, y1;      // canvas region of this launch, [x0, x1) x [y0, y1)
-  int tiles_x;
-  // blend inputs: uint8 bw x bh images whose origin is canvas (x0, y0)
-  uint8_t* g0[N];
-  int bw, bh;
-  // canvas
-  uint8_t* canvas;
-  uint32_t cpitch;
-  int cw, ch;
-  const uint8_t* seam;                     // canvas-sized labels
-  int skip_x0, skip_x1, skip_y0, skip_y1;  // left to the blend write-back
-};
-
-// Level-0 blend input i = image i remapped at its position
-// (BatchedRemapKernelExOffset(ROI), cudaRemap.cu:95-122 / :129-198, default
-// pixel all-zero) inside its box, and the cleared buffer's zero outside.
-template <int C, int N>
-__global__ void __launch_bounds__(256) panoN_inputs_kernel(PanoNArgs<N> a) {
-  const int t = (int)blockIdx.x;
-  const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
-  const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
-  const int b = blockIdx.z;
-  if (Y >= a.y1) return;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const int ry = Y - a.py[i];
-    const bool row_ok = ry >= 0 && ry < a.mh[i];
-    const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
-    uint32_t m[kPx];
-    bool in[kPx];
-#pragma unroll
-    for (int j = 0; j < kPx; ++j) {
-      const int X = X0 + j * (int)blockDim.x;
-      const int rx = X - a.px[i];
-      in[j] = row_ok && rx >= 0 && rx < a.mw[i] && X < a.x1;
-      m[j] = in[j] ? mrow[rx] : 0u;
-    }
-    uint32_t r[kPx];
-#pragma unroll
-    for (int j = 0; j < kPx; ++j) r[j] = in[j] ? remap_px<C>(a.src[i], b, m[j]) : 0u;
-    uint8_t* grow = a.g0[i] + ((size_t)b * a.bh + (Y - a.y0)) * a.bw * C;
-#pragma unroll
-    for (int j = 0; j < kPx; ++j) {
-      const int X = X0 + j * (int)blockDim.x;
-      if (X < a.x1) store_px<C>(grow + (size_t)(X - a.x0) * C, r[j]);
-    }
-  }
-}
-
-// The cleared canvas after batched_remap_hard_seam_kernel_n
-// (cudaRemap.cu:252-294): the labelled image's source pixel where its map is
-// in range, else zero; pixels inside the skip rectangle are not written.
-template <int C, int N>
-__global__ void __launch_bounds__(256) panoN_canvas_kernel(PanoNArgs<N> a) {
-  const int t = (int)blockIdx.x;
-  const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
-  const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
-  const int b = blockIdx.z;
-  if (Y >= a.y1) return;
-  const uint8_t* srow = a.seam + (size_t)Y * a.cw;
-  const bool skip_row = Y >= a.skip_y0 && Y < a.skip_y1;
-  int lab[kPx];
-  bool wr[kPx];
-#pragma unroll
-  for (int j = 0; j < kPx; ++j) {
-    const int X = X0 + j * (int)blockDim.x;
-    wr[j] = X < a.x1 && !(skip_row && X >= a.skip_x0 && X < a.skip_x1);
-    lab[j] = wr[j] ? (int)srow[X] : -1;
-  }
-  uint32_t v[kPx] = {};
-  // One branch per image with compile-time kernel-argument indices; a
-  // wavefront runs only the branches of the labels its lanes hold.
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    uint32_t m[kPx];
-    bool hit[kPx];
-#pragma unroll
-    for (int j = 0; j < kPx; ++j) {
-      const int rx = X0 + j * (int)blockDim.x - a.px[i], ry = Y - a.py[i];
-      hit[j] = lab[j] == i && rx >= 0 && ry >= 0 && rx < a.mw[i] && ry < a.mh[i];
-      m[j] = hit[j] ? a.map[i][(size_t)ry * a.mw[i] + rx] : 0u;
-    }
-#pragma unroll
-    for (int j = 0; j < kPx; ++j)
-      if (hit[j]) v[j] = remap_px_dm<C>(a.src[i], b, m[j]);
-  }
-  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
-#pragma unroll
-  for (int j = 0; j < kPx; ++j)
-    if (wr[j]) store_px<C>(crow + (size_t)(X0 + j * (int)blockDim.x) * C, v[j]);
-}
-
 struct PanoN : hmStitchPano {
   int pos_x[kMaxImages] = {}, pos_y[kMaxImages] = {}, mw[kMaxImages] = {}, mh[kMaxImages] = {};
   hm::DevBuf map[kMaxImages];
@@ -187,8 +90,8 @@ struct PanoN : hmStitchPano {
   hipError_t ensure_pyramid();
   hipError_t process(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, hipStream_t s);
   template <int N>
-  hipError_t launch_region(bool inputs_kernel, const hmSurface* inputs, const hmSurface& canvas, int x0, int x1,
-                           int y0, int y1, const Rect& skip, hipStream_t s, const char* name);
+  hipError_t launch_region(const MultiRegion& r, const hmSurface* inputs, const hmSurface& canvas, hipStream_t s,
+                           const char* name);
 };
 
 hipError_t PanoN::setup(const hmControlMasksN& m, int minimize_blend) {
@@ -262,52 +165,26 @@ hipError_t PanoN::ensure_pyramid() {
 }
 
 template <int N>
-hipError_t PanoN::launch_region(bool inputs_kernel, const hmSurface* inputs, const hmSurface& canvas, int x0, int x1,
-                                int y0, int y1, const Rect& skip, hipStream_t s, const char* name) {
-  if (x1 <= x0 || y1 <= y0) return hipSuccess;
-  PanoNArgs<N> a{};
+hipError_t PanoN::launch_region(const MultiRegion& r, const hmSurface* inputs, const hmSurface& canvas, hipStream_t s,
+                                const char* name) {
+  MultiGeom g;
   for (int i = 0; i < N; ++i) {
-    a.src[i] = to_surf(inputs[i]);
-    a.map[i] = map[i].as<uint32_t>();
-    a.mw[i] = mw[i];
-    a.mh[i] = mh[i];
-    a.px[i] = pos_x[i];
-    a.py[i] = pos_y[i];
-    a.g0[i] = g0[i].as<uint8_t>();
+    g.map[i] = map[i].as<uint32_t>();
+    g.mw[i] = mw[i];
+    g.mh[i] = mh[i];
+    g.px[i] = pos_x[i];
+    g.py[i] = pos_y[i];
+    g.g0[i] = g0[i].as<uint8_t>();
   }
-  a.x0 = x0;
-  a.x1 = x1;
-  a.y0 = y0;
-  a.y1 = y1;
-  const int tile_w = (int)kBlock.x * kPx;
-  a.tiles_x = (x1 - x0 + tile_w - 1) / tile_w;
-  const int tiles_y = (y1 - y0 + (int)kBlock.y - 1) / (int)kBlock.y;
-  a.bw = bw;
-  a.bh = bh;
-  a.canvas = static_cast<uint8_t*>(canvas.d_ptr);
-  a.cpitch = canvas.pitch;
-  a.cw = canvas_w;
-  a.ch = canvas_h;
-  a.seam = seam.as<uint8_t>();
-  a.skip_x0 = skip.x;
-  a.skip_x1 = skip.x + skip.w;
-  a.skip_y0 = skip.y;
-  a.skip_y1 = skip.y + skip.h;
-  const dim3 grid((unsigned)(a.tiles_x * tiles_y), 1u, (unsigned)batch);
+  g.bw = bw;
+  g.bh = bh;
+  g.seam = seam.as<uint8_t>();
+  g.cw = canvas_w;
+  g.ch = canvas_h;
   timer.begin(s);
-  if (inputs_kernel) {
-    if (channels == 4)
-      hipLaunchKernelGGL((panoN_inputs_kernel<4, N>), grid, kBlock, 0, s, a);
-    else
-      hipLaunchKernelGGL((panoN_inputs_kernel<3, N>), grid, kBlock, 0, s, a);
-  } else {
-    if (channels == 4)
-      hipLaunchKernelGGL((panoN_canvas_kernel<4, N>), grid, kBlock, 0, s, a);
-    else
-      hipLaunchKernelGGL((panoN_canvas_kernel<3, N>), grid, kBlock, 0, s, a);
-  }
+  const hipError_t e = launch_multi<N>(g, r, channels, inputs, canvas, batch, s);
   timer.end(s, name);
-  return hipGetLastError();
+  return e;
 }
 
 hipError_t PanoN::process(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, hipStream_t s) {
@@ -320,17 +197,28 @@ hipError_t PanoN::process(const hmSurface* inputs, int n_inputs, const hmSurface
     return hipErrorInvalidValue;
   auto per_n = [&](auto NV) -> hipError_t {
     constexpr int N = decltype(NV)::value;
-    if (hard_seam)  // cudaPanoN.inl:508-541: clear + hard seam, one pass
-      return launch_region<N>(false, inputs, canvas, 0, canvas_w, 0, canvas_h, Rect{}, s, "hard_seam");
+    // The canvas: clear + hard seam (cudaPanoN.inl:508-541, :545-560).
+    MultiRegion cr;
+    cr.kernel = kMultiCanvas;
+    cr.x1 = canvas_w;
+    cr.y1 = canvas_h;
+    if (hard_seam) return launch_region<N>(cr, inputs, canvas, s, "hard_seam");
     HM_RETURN_IF_ERROR(ensure_pyramid());
-    HM_RETURN_IF_ERROR(
-        launch_region<N>(true, inputs, canvas, bx0, bx0 + bw, by0, by0 + bh, Rect{}, s, "remap_inputs"));
-    if (use_min) {
+    MultiRegion ir;
+    ir.kernel = kMultiInputs;
+    ir.x0 = bx0;
+    ir.x1 = bx0 + bw;
+    ir.y0 = by0;
+    ir.y1 = by0 + bh;
+    HM_RETURN_IF_ERROR(launch_region<N>(ir, inputs, canvas, s, "remap_inputs"));
+    hipError_t e_side = hipSuccess;
+    if (use_min) {  // outside the write ROI, which the blend writes back
+      cr.skip_x = write_roi.x;
+      cr.skip_y = write_roi.y;
+      cr.skip_w = write_roi.w;
+      cr.skip_h = write_roi.h;
       HM_RETURN_IF_ERROR(fork(s));
-      const hipError_t e =
-          launch_region<N>(false, inputs, canvas, 0, canvas_w, 0, canvas_h, write_roi, side, "remap_canvas");
-      HM_RETURN_IF_ERROR(join(s));
-      HM_RETURN_IF_ERROR(e);
+      e_side = launch_region<N>(cr, inputs, canvas, side, "remap_canvas");
     }
     const void* g[kMaxImages];
     for (int i = 0; i < n_images; ++i) g[i] = g0[i].p;
@@ -347,7 +235,11 @@ hipError_t PanoN::process(const hmSurface* inputs, int n_inputs, const hmSurface
       out.win_w = write_roi.w;
       out.win_h = write_roi.h;
     }
-    return pyr.run(g, true, out, s, timer_or_null());
+    const hipError_t e = e_side == hipSuccess ? pyr.run(g, true, out, s, timer_or_null()) : e_side;
+    if (!use_min) return e;
+    // Join after the pyramid's launches (and after an error too).
+    const hipError_t j = join(s);
+    return e != hipSuccess ? e : j;
   };
   using std::integral_constant;
   switch (n_images) {

@@ -54,4 +54,217 @@ inline Surf to_surf(const hmSurface& s) {
   return Surf{static_cast<uint8_t*>(s.d_ptr), s.width, s.height, s.pitch};
 }
 
+// ---------------------------------------------------------------------------
+// Multi-image (N- and 3-image) remap kernels
+// ---------------------------------------------------------------------------
+template <int N>
+struct MultiArgs {
+  Surf src[N];
+  const uint32_t* map[N];  // packed x | y << 16
+  int mw[N], mh[N], px[N], py[N];
+  int x0, x1, y0, y1;      // canvas region of this launch, [x0, x1) x [y0, y1)
+  int tiles_x;
+  // blend inputs: uint8 bw x bh images whose origin is canvas (x0, y0)
+  uint8_t* g0[N];
+  int bw, bh;
+  // canvas
+  uint8_t* canvas;
+  uint32_t cpitch;
+  int cw, ch;
+  const uint8_t* seam;                     // canvas-sized labels
+  int skip_x0, skip_x1, skip_y0, skip_y1;  // left to the blend write-back
+  int mode;                                // kCanvasLabel, kCanvasLast or kCanvasClear
+  int invalid_zero;    // label mode: a labelled pixel whose map is unmapped / out of range gets 0, else is left
+  int uncovered_zero;  // a pixel no image writes gets 0, else is left
+};
+
+// Level-0 blend input i = image i remapped at its position inside its box,
+// and the cleared buffer's zero outside.  DM = false: the default pixel of
+// BatchedRemapKernelExOffset(ROI) (cudaRemap.cu:95-122 / :129-198: alpha 0
+// for the sentinel, 255 for other out-of-range entries); DM = true: zero for
+// both (FusedRemapToFullKernel3, cudaFusedKernels3.cu:25-119, writes only
+// mapped in-range pixels into the cleared buffer).
+template <int C, int N, bool DM>
+__global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
+  const int t = (int)blockIdx.x;
+  const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
+  const int b = blockIdx.z;
+  if (Y >= a.y1) return;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int ry = Y - a.py[i];
+    const bool row_ok = ry >= 0 && ry < a.mh[i];
+    const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
+    uint32_t m[kPx];
+    bool in[kPx];
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) {
+      const int X = X0 + j * (int)blockDim.x;
+      const int rx = X - a.px[i];
+      in[j] = row_ok && rx >= 0 && rx < a.mw[i] && X < a.x1;
+      m[j] = in[j] ? mrow[rx] : 0u;
+    }
+    uint32_t r[kPx];
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) r[j] = in[j] ? (DM ? remap_px_dm<C>(a.src[i], b, m[j]) : remap_px<C>(a.src[i], b, m[j])) : 0u;
+    uint8_t* grow = a.g0[i] + ((size_t)b * a.bh + (Y - a.y0)) * a.bw * C;
+#pragma unroll
+    for (int j = 0; j < kPx; ++j) {
+      const int X = X0 + j * (int)blockDim.x;
+      if (X < a.x1) store_px<C>(grow + (size_t)(X - a.x0) * C, r[j]);
+    }
+  }
+}
+
+// Canvas writers (pixels inside the skip rectangle are never written):
+//  kCanvasLabel: the seam label selects the image (batched_remap_hard_seam_kernel_n,
+//    cudaRemap.cu:252-294; BatchedRemapKernelExOffsetWithDestMap, :204-246;
+//    FusedRemapHardSeam3, cudaFusedKernels3.cu:124-217): its source pixel
+//    where the map is in range, else zero or nothing (invalid_zero);
+//  kCanvasLast: remaps of every image in order onto the canvas
+//    (BatchedRemapKernelExOffset): the last image whose box covers the pixel,
+//    with its default pixel (alpha 0 / 255) where unmapped / out of range;
+//  kCanvasClear: no image.
+// Pixels no image writes get zero (the reference's canvas clear) or are left
+// (uncovered_zero).
+enum CanvasMode { kCanvasLabel = 0, kCanvasLast = 1, kCanvasClear = 2 };
+
+template <int C, int N>
+__global__ void __launch_bounds__(256) multi_canvas_kernel(MultiArgs<N> a) {
+  const int t = (int)blockIdx.x;
+  const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
+  const int b = blockIdx.z;
+  if (Y >= a.y1) return;
+  const bool skip_row = Y >= a.skip_y0 && Y < a.skip_y1;
+  const uint8_t* srow = a.seam ? a.seam + (size_t)Y * a.cw : nullptr;
+  int lab[kPx];
+  bool act[kPx];
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const int X = X0 + j * (int)blockDim.x;
+    act[j] = X < a.x1 && !(skip_row && X >= a.skip_x0 && X < a.skip_x1);
+    lab[j] = (act[j] && a.mode == kCanvasLabel) ? (int)srow[X] : -1;
+  }
+  uint32_t v[kPx] = {};
+  bool wr[kPx];
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) wr[j] = act[j] && a.uncovered_zero;
+  if (a.mode != kCanvasClear) {
+    // One branch per image with compile-time kernel-argument indices; a
+    // wavefront runs only the branches its lanes take.
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      uint32_t m[kPx];
+      bool hit[kPx];
+#pragma unroll
+      for (int j = 0; j < kPx; ++j) {
+        const int rx = X0 + j * (int)blockDim.x - a.px[i], ry = Y - a.py[i];
+        const bool in_box = act[j] && rx >= 0 && ry >= 0 && rx < a.mw[i] && ry < a.mh[i];
+        hit[j] = in_box && (a.mode == kCanvasLast || lab[j] == i);
+        m[j] = hit[j] ? a.map[i][(size_t)ry * a.mw[i] + rx] : 0u;
+      }
+#pragma unroll
+      for (int j = 0; j < kPx; ++j) {
+        if (!hit[j]) continue;
+        if (a.mode == kCanvasLast) {
+          v[j] = remap_px<C>(a.src[i], b, m[j]);
+          wr[j] = true;
+        } else {
+          const uint32_t sx = m[j] & 0xffffu, sy = m[j] >> 16;
+          const bool ok = sx < a.src[i].w && sy < a.src[i].h;
+          v[j] = ok ? load_src_px<C>(a.src[i], b, (int)sx, (int)sy) : 0u;
+          wr[j] = ok || a.invalid_zero;
+        }
+      }
+    }
+  }
+  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+#pragma unroll
+  for (int j = 0; j < kPx; ++j)
+    if (wr[j]) store_px<C>(crow + (size_t)(X0 + j * (int)blockDim.x) * C, v[j]);
+}
+
+// ---------------------------------------------------------------------------
+// Host side of the multi-image kernels
+// ---------------------------------------------------------------------------
+// Per-pano geometry (constant after set-up).
+struct MultiGeom {
+  const uint32_t* map[kMaxImages] = {};
+  int mw[kMaxImages] = {}, mh[kMaxImages] = {}, px[kMaxImages] = {}, py[kMaxImages] = {};
+  uint8_t* g0[kMaxImages] = {};
+  int bw = 0, bh = 0;
+  const uint8_t* seam = nullptr;  // canvas-sized labels
+  int cw = 0, ch = 0;
+};
+enum MultiKernel { kMultiInputs = 0, kMultiInputsDM = 1, kMultiCanvas = 2 };
+// One launch over the canvas region [x0, x1) x [y0, y1).
+struct MultiRegion {
+  int kernel = kMultiInputs;
+  int x0 = 0, x1 = 0, y0 = 0, y1 = 0;
+  int skip_x = 0, skip_y = 0, skip_w = 0, skip_h = 0;
+  int mode = kCanvasLabel, invalid_zero = 1, uncovered_zero = 1;
+};
+
+template <int N>
+hipError_t launch_multi(const MultiGeom& g, const MultiRegion& r, int channels, const hmSurface* inputs,
+                        const hmSurface& canvas, int batch, hipStream_t s) {
+  if (r.x1 <= r.x0 || r.y1 <= r.y0) return hipSuccess;
+  MultiArgs<N> a{};
+  for (int i = 0; i < N; ++i) {
+    a.src[i] = to_surf(inputs[i]);
+    a.map[i] = g.map[i];
+    a.mw[i] = g.mw[i];
+    a.mh[i] = g.mh[i];
+    a.px[i] = g.px[i];
+    a.py[i] = g.py[i];
+    a.g0[i] = g.g0[i];
+  }
+  a.x0 = r.x0;
+  a.x1 = r.x1;
+  a.y0 = r.y0;
+  a.y1 = r.y1;
+  const int tile_w = (int)kBlock.x * kPx;
+  a.tiles_x = (r.x1 - r.x0 + tile_w - 1) / tile_w;
+  const int tiles_y = (r.y1 - r.y0 + (int)kBlock.y - 1) / (int)kBlock.y;
+  a.bw = g.bw;
+  a.bh = g.bh;
+  a.canvas = static_cast<uint8_t*>(canvas.d_ptr);
+  a.cpitch = canvas.pitch;
+  a.cw = g.cw;
+  a.ch = g.ch;
+  a.seam = g.seam;
+  a.skip_x0 = r.skip_x;
+  a.skip_x1 = r.skip_x + r.skip_w;
+  a.skip_y0 = r.skip_y;
+  a.skip_y1 = r.skip_y + r.skip_h;
+  a.mode = r.mode;
+  a.invalid_zero = r.invalid_zero;
+  a.uncovered_zero = r.uncovered_zero;
+  const dim3 grid((unsigned)(a.tiles_x * tiles_y), 1u, (unsigned)batch);
+  const bool c4 = channels == 4;
+  switch (r.kernel) {
+    case kMultiInputs:
+      if (c4)
+        hipLaunchKernelGGL((multi_inputs_kernel<4, N, false>), grid, kBlock, 0, s, a);
+      else
+        hipLaunchKernelGGL((multi_inputs_kernel<3, N, false>), grid, kBlock, 0, s, a);
+      break;
+    case kMultiInputsDM:
+      if (c4)
+        hipLaunchKernelGGL((multi_inputs_kernel<4, N, true>), grid, kBlock, 0, s, a);
+      else
+        hipLaunchKernelGGL((multi_inputs_kernel<3, N, true>), grid, kBlock, 0, s, a);
+      break;
+    default:
+      if (c4)
+        hipLaunchKernelGGL((multi_canvas_kernel<4, N>), grid, kBlock, 0, s, a);
+      else
+        hipLaunchKernelGGL((multi_canvas_kernel<3, N>), grid, kBlock, 0, s, a);
+      break;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace hmdev

@@ -57,6 +57,15 @@ class hmControlMasksN(ctypes.Structure):
     ]
 
 
+class hmControlMasks3(ctypes.Structure):
+    _fields_ = [
+        ("img_col", ctypes.c_void_p * 3), ("img_row", ctypes.c_void_p * 3),
+        ("img_w", ctypes.c_int * 3), ("img_h", ctypes.c_int * 3),
+        ("whole_seam_mask_image", ctypes.c_void_p), ("seam_w", ctypes.c_int), ("seam_h", ctypes.c_int),
+        ("xpos", ctypes.c_float * 3), ("ypos", ctypes.c_float * 3),
+    ]
+
+
 _lib = None
 
 
@@ -87,6 +96,8 @@ def lib() -> ctypes.CDLL:
         "hmStitchPanoProcess": (i, [vp, S, S, S, vp]),
         "hmStitchPanoNCreate": (i, [i, i, ctypes.POINTER(hmControlMasksN), i, i, i, ctypes.POINTER(vp)]),
         "hmStitchPanoNProcess": (i, [vp, ctypes.POINTER(S), i, S, vp]),
+        "hmStitchPano3Create": (i, [i, i, ctypes.POINTER(hmControlMasks3), i, i, ctypes.POINTER(vp)]),
+        "hmStitchPano3Process": (i, [vp, S, S, S, S, i, vp]),
         "hmStitchPanoStatus": (i, [vp]),
         "hmStitchPanoCanvasSize": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i)]),
         "hmStitchPanoBatchSize": (i, [vp]),
@@ -282,6 +293,45 @@ class StitchPanoN(StitchPano):
 
     def _mask_channels(self) -> int:
         return self.n_images
+
+
+class StitchPano3(StitchPano):
+    """CudaStitchPano3<uchar4|uchar3, float4|float3> (src/pano/cudaPano3.h:64-120).
+
+    `rig` carries three maps, integer positions and the seam (labels 0..2)."""
+
+    def __init__(self, batch_size: int, num_levels: int, rig, channels: int = 4, debug: bool = False):
+        L = lib()
+        m = hmControlMasks3()
+        self._keep = []
+        for k in range(3):
+            mx, my = _u16(rig.map_x[k]), _u16(rig.map_y[k])
+            self._keep += [mx, my]
+            m.img_col[k], m.img_row[k] = mx.ctypes.data, my.ctypes.data
+            m.img_h[k], m.img_w[k] = mx.shape
+            m.xpos[k], m.ypos[k] = float(rig.pos_x[k]), float(rig.pos_y[k])
+        seam = np.ascontiguousarray(rig.seam, dtype=np.uint8)
+        self._keep.append(seam)
+        m.whole_seam_mask_image = seam.ctypes.data
+        m.seam_h, m.seam_w = seam.shape
+        self._masks = m
+        h = ctypes.c_void_p()
+        rc = L.hmStitchPano3Create(batch_size, num_levels, ctypes.byref(m), 1, channels, ctypes.byref(h))
+        if not h.value:
+            raise HmError(rc, "hmStitchPano3Create")
+        self._h = h
+        self.batch_size = batch_size
+        self.channels = channels
+        self.num_levels = num_levels
+        if debug:
+            check(L.hmStitchPanoSetDebug(h, 1), "hmStitchPanoSetDebug")
+
+    def process(self, in0, in1, in2, canvas, fused: bool = True, stream=None) -> int:
+        return lib().hmStitchPano3Process(self._h, surface(in0), surface(in1), surface(in2), surface(canvas),
+                                          int(fused), _stream(stream))
+
+    def _mask_channels(self) -> int:
+        return 3
 
 
 # ---------------------------------------------------------------------------

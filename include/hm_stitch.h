@@ -200,6 +200,29 @@ int hmStitchPanoNCreate(int batch_size, int num_levels, const hmControlMasksN* m
 int hmStitchPanoNProcess(hmStitchPano* pano, const hmSurface* inputs, int num_inputs, hmSurface canvas,
                          hipStream_t stream);
 
+/* In-memory ControlMasks3 (src/pano/controlMasks3.h:12-43): img0..img2
+ * col/row CV_16U tables as arrays, the single-channel seam with labels 0..2,
+ * and the three normalised positions. */
+typedef struct hmControlMasks3 {
+  const uint16_t* img_col[3];
+  const uint16_t* img_row[3];
+  int img_w[3], img_h[3];
+  const uint8_t* whole_seam_mask_image;
+  int seam_w, seam_h;
+  float xpos[3], ypos[3];
+} hmControlMasks3;
+
+/* CudaStitchPano3<uchar4|uchar3, float4|float3> ctor (src/pano/cudaPano3.h:70,
+ * cudaPano3.inl:26-110).  Constructor errors are sticky. */
+int hmStitchPano3Create(int batch_size, int num_levels, const hmControlMasks3* masks, int quiet, int channels,
+                        hmStitchPano** out);
+/* CudaStitchPano3::process (cudaPano3.h:101-107, cudaPano3.inl:404-422):
+ * fused != 0 is the default process_optimized path
+ * (cudaPano3_optimized.inl:19-163).  Errors are sticky on both paths.  A
+ * 3-channel canvas is not cleared: pixels no image writes keep their value. */
+int hmStitchPano3Process(hmStitchPano* pano, hmSurface input0, hmSurface input1, hmSurface input2, hmSurface canvas,
+                         int fused, hipStream_t stream);
+
 /* ------------------------------------------------------------------------- */
 /* Pyramid inspection and launch timing (all pano classes)                   */
 /* ------------------------------------------------------------------------- */
