@@ -30,15 +30,18 @@ def log(msg):
 
 
 def algorithmic_bytes(rig, levels, pano_levels):
-    """Minimum HBM bytes per launch of each kernel (what it must read + write).
+    """Minimum HBM bytes per frame of each kernel label (what it must read + write).
 
-    Maps are read as one packed 32-bit word per pixel; a source pixel is read
-    once per valid map entry; uint8 BGRA pixels are 4 B, float pyramid pixels
-    16 B, the 2-image mask 4 B.
+    Maps are read as one packed 32-bit word per pixel; each source image is
+    read once in full (W_in x H_in x 4 B), shared between the two remap
+    launches in proportion to their valid map entries; uint8 BGRA pixels are
+    4 B, float pyramid pixels 16 B, masks 4 B per float (the level-0 2-image
+    mask 1 B).
     """
     cw, ch = rig.canvas_w, rig.canvas_h
     map_px = [m.size for m in rig.map_x]
     valid_px = [int((m != 65535).sum()) for m in rig.map_x]
+    src_bytes = rig.src_w * rig.src_h * 4
     out = {}
     if levels == 0:
         # each canvas pixel: seam byte, the selected image's map word + source pixel, canvas write
@@ -50,7 +53,8 @@ def algorithmic_bytes(rig, levels, pano_levels):
             sub = seam[rig.pos_y[i]: rig.pos_y[i] + h, rig.pos_x[i]: rig.pos_x[i] + w] == lab
             sel += int(sub.sum())
             sel_valid += int((sub & (rig.map_x[i] != 65535)).sum())
-        out["hard_seam"] = cw * ch * 1 + sel * 4 + sel_valid * 4 + cw * ch * 4
+        # source bytes: the part of each image its selected map entries read, in full-image units
+        out["hard_seam"] = cw * ch * 1 + sel * 4 + 2 * src_bytes + cw * ch * 4
         return out
     x2 = rig.pos_x[1]
     w1 = rig.map_x[0].shape[1]
@@ -62,35 +66,48 @@ def algorithmic_bytes(rig, levels, pano_levels):
         a.append(((a[-1][0] + 1) // 2, (a[-1][1] + 1) // 2))
     A = [w * h for w, h in a]
     coarse = sum(A[1:])  # pixels of levels >= 1
-    # map words and valid source pixels inside / outside the blend columns
-    roi_map = roi_valid = 0
+    # map words inside the blend columns, and the source bytes that go with them
+    roi_map = 0
+    roi_src = 0.0
     for i in range(2):
         c0 = max(0, rx0 - rig.pos_x[i])
         c1 = min(rig.map_x[i].shape[1], rx1 - rig.pos_x[i])
         if c1 > c0:
             sub = rig.map_x[i][:, c0:c1]
             roi_map += sub.size
-            roi_valid += int((sub != 65535).sum())
-    # remap of the blend columns: maps, valid source pixels, both level-0 images (uint8)
-    out["remap_roi"] = roi_map * 4 + roi_valid * 4 + 2 * A[0] * 4
-    # remap of the rest of the canvas: maps, valid source pixels, canvas writes
-    out["remap_canvas"] = ((sum(map_px) - roi_map) * 4 + (sum(valid_px) - roi_valid) * 4
-                           + (cw - (rx1 - rx0)) * ch * 4)
+            roi_src += src_bytes * int((sub != 65535).sum()) / max(1, valid_px[i])
+    # remap of the blend columns: maps, source, both level-0 images (uint8)
+    out["remap_roi"] = int(roi_map * 4 + roi_src + 2 * A[0] * 4)
+    # remap of the rest of the canvas: maps, source, canvas writes
+    out["remap_canvas"] = int((sum(map_px) - roi_map) * 4 + (2 * src_bytes - roi_src) + (cw - (rx1 - rx0)) * ch * 4)
     # Gaussian levels >= 1 of both images from level 0
     out["down_fused"] = 2 * A[0] * 4 + 2 * coarse * 16
-    if pano_levels > 3:
-        # levels >= 2: both images' levels and masks in, R_2 out
-        c2 = sum(A[2:])
-        out["up_coarse"] = 2 * c2 * 16 + c2 * 4 + A[2] * 16
-    if pano_levels > 2:
-        # level 1: both images' level-1 pixels, mask, level-2 taps, R_2 in; R_1 out
-        out["up_quad"] = 2 * A[1] * 16 + A[1] * 4 + 2 * A[2] * 16 + A[2] * 16 + A[1] * 16
+    # reconstruction, one launch per level in 2x2 quads (per frame, all levels 1..L-1):
+    # the level's images and mask, the next level's images and R (taps), R out
+    quad = 0
+    for k in range(1, pano_levels):
+        quad += 2 * A[k] * 16 + A[k] * 4 + A[k] * 16
+        if k + 1 < pano_levels:
+            quad += 2 * A[k + 1] * 16 + A[k + 1] * 16
+    if quad:
+        out["up_quad"] = quad
+    # level 0: both level-0 images, the uint8 mask, level-1 taps and R_1 in, canvas out
+    out["up_level0"] = 2 * A[0] * 4 + A[0] * 1 + A[0] * 4
     if pano_levels > 1:
-        # level 0: both level-0 images, the uint8 mask, level-1 taps and R_1 in, canvas out
-        out["up_level0"] = 2 * A[0] * 4 + A[0] * 1 + 2 * A[1] * 16 + A[1] * 16 + A[0] * 4
-    else:
-        out["up_level0"] = 2 * A[0] * 4 + A[0] * 1 + A[0] * 4
+        out["up_level0"] += 2 * A[1] * 16 + A[1] * 16
     return out
+
+
+TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_8500x3300.json")
+
+
+def measured_traffic():
+    """HBM bytes per frame of each kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes over this workload, scaled by the same counters on a copy kernel of known size."""
+    if not os.path.exists(TRAFFIC):
+        return None
+    with open(TRAFFIC) as f:
+        return json.load(f)["kernels"]
 
 
 def cpu_baseline(rig, f1, f2, levels, budget_s=12.0):
@@ -110,6 +127,10 @@ def cpu_baseline(rig, f1, f2, levels, budget_s=12.0):
     return {"value": round(mpix, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
             "sample": f"{n} frame(s) of the same 8500x3300 L={levels} stitch on the oracle in {el:.1f} s",
             "fps": round(n / el, 4)}
+
+
+def kt_of(r):
+    return r["kt"]
 
 
 def main():
@@ -135,15 +156,17 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.calibrate_copy:
-        n = 1 << 30
-        a = torch.empty(n, dtype=torch.uint8, device="cuda")
-        b = torch.empty(n, dtype=torch.uint8, device="cuda")
-        a.fill_(1)
+        # A plain elementwise copy kernel of known size: b = a + 0 over 1 GiB of int32
+        # reads 1 GiB and writes 1 GiB per launch.
+        n = 1 << 28
+        a = torch.ones(n, dtype=torch.int32, device="cuda")
+        b = torch.empty(n, dtype=torch.int32, device="cuda")
         torch.cuda.synchronize()
         for _ in range(5):
-            b.copy_(a)
+            torch.add(a, 0, out=b)
         torch.cuda.synchronize()
-        print(json.dumps({"calibration": "device copy", "bytes_read": n, "bytes_written": n, "launches": 5}))
+        print(json.dumps({"calibration": "elementwise copy (torch.add, int32)", "bytes_read": 4 * n,
+                          "bytes_written": 4 * n, "launches": 5}))
         return
     hm_stitch.lib()
     log("generating the 8500x3300 rig and frames")
@@ -188,9 +211,10 @@ def main():
         alg = algorithmic_bytes(rig, levels, len(pano.levels()) if levels else 0)
         for k, b in alg.items():
             if k in kernels:
-                kernels[k]["alg_bytes"] = b
-                kernels[k]["achieved_GBps"] = b / (kernels[k]["avg_ms"] * 1e-3) / 1e9
-        results[levels] = dict(ms=ms, lat=np.array(lat), kernels=kernels, canvas=(cw, ch))
+                per_frame = kernels[k]["launches"] / kt
+                kernels[k]["alg_bytes"] = int(b / per_frame)  # per launch
+                kernels[k]["achieved_GBps"] = kernels[k]["alg_bytes"] / (kernels[k]["avg_ms"] * 1e-3) / 1e9
+        results[levels] = dict(ms=ms, lat=np.array(lat), kernels=kernels, canvas=(cw, ch), kt=kt)
         del pano, canvas
         torch.cuda.synchronize()
 
@@ -202,9 +226,24 @@ def main():
     cands = {k: v for k, v in r["kernels"].items() if "alg_bytes" in v}
     dom = max(cands, key=lambda k: cands[k]["per_frame_ms"])
     dk = cands[dom]
+    mt = measured_traffic()
+    traffic = None
+    if mt and dom in mt:
+        traffic = int(mt[dom]["traffic_bytes_per_frame"] / mt[dom]["launches_per_frame"])
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(dk["achieved_GBps"], 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(dk["achieved_GBps"] / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(dk["achieved_GBps"] / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": round(dk["avg_ms"], 5)}
+    if traffic:
+        roofline["traffic_GBps"] = round(traffic / (dk["avg_ms"] * 1e-3) / 1e9, 1)
+        roofline["traffic_frac"] = round(traffic / (dk["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+    # the whole frame: every kernel's bytes over the frame time
+    frame_alg = sum(v["alg_bytes"] * v["launches"] / kt_of(r) for v in cands.values())
+    frame = {"alg_bytes": int(frame_alg), "alg_GBps": round(frame_alg / (r["ms"] * 1e-3) / 1e9, 1)}
+    frame["alg_frac"] = round(frame["alg_GBps"] / HBM_PEAK_GBPS, 4)
+    if mt and all(k in mt for k in cands):
+        ft = sum(mt[k]["traffic_bytes_per_frame"] for k in cands)
+        frame.update({"traffic_bytes": int(ft), "traffic_GBps": round(ft / (r["ms"] * 1e-3) / 1e9, 1),
+                      "traffic_frac": round(ft / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)})
     line = {
         "metric": "pano Mpix/s + fps @8500x3300 L=6; p50/p99 frame latency, served vs plain",
         "value": round(mpix, 1),
@@ -224,6 +263,9 @@ def main():
         "latency_ms_p50": round(float(np.percentile(r["lat"], 50)), 4),
         "latency_ms_p99": round(float(np.percentile(r["lat"], 99)), 4),
         "roofline": roofline,
+        "hbm_frame": frame,
+        "reference_published": {"fps": 340, "hardware": "RTX 4090", "levels": 6,
+                                "note": "README.md:7; same canvas size, different GPU"},
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in r["kernels"].items()},
     }

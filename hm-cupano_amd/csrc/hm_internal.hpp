@@ -96,21 +96,6 @@ hipError_t launch_up(int c, int n, int kind, bool hi_u8, const UpLaunch& a, int 
 // lv[j] describes the j-th level produced (down) or level j+1 (up).
 hipError_t launch_down_fused(int c, int n, bool in_u8, const void* const* g0, int w0, int h0, int n_out,
                              const hmdev::LevelDesc* lv, int batch, hipStream_t s);
-struct UpFusedLaunch {
-  const void* g0[hmdev::kMaxImages] = {};
-  const void* mask0 = nullptr;
-  int mask0_mode = hmdev::kMask0F32;
-  int w0 = 0, h0 = 0, levels = 0;
-  const hmdev::LevelDesc* lv = nullptr;
-  float* f32_out = nullptr;
-  uint8_t* u8_out = nullptr;
-  size_t u8_pitch = 0;
-  int u8_w = 0, u8_h = 0, ox = 0, oy = 0;
-  int win_x = 0, win_y = 0, win_w = 0, win_h = 0;
-};
-// Levels u.levels-1 .. 0 of a pyramid whose base (float) is u.g0, written as
-// float to u.f32_out.
-hipError_t launch_up_coarse(int c, int n, int kind, const UpFusedLaunch& u, int batch, hipStream_t s);
 struct UpQuadLaunch {
   const void* g0[hmdev::kMaxImages] = {};
   const float* g1[hmdev::kMaxImages] = {};
@@ -125,11 +110,6 @@ struct UpQuadLaunch {
   int win_x = 0, win_y = 0, win_w = 0, win_h = 0;
 };
 hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch& u, int batch, hipStream_t s);
-
-// The levels below this one run one launch per level in 2x2 quads; the
-// levels from it on (when there are more than one) run in one tile-fused
-// launch.
-constexpr int kQuadLevels = 16;
 
 // Where the level-0 reconstruction goes.
 struct Level0Out {

@@ -233,34 +233,6 @@ static hipError_t dispatch_cnk(int c, int n, int kind, F&& f) {
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_up_coarse(int c, int n, int kind, const UpFusedLaunch& u, int batch, hipStream_t s) {
-  if (u.levels < 1 || u.levels > kMaxLevels) return hipErrorInvalidValue;
-  if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
-  return dispatch_cnk(c, n, kind, [&](auto C, auto N, auto K) -> hipError_t {
-    constexpr int kC = decltype(C)::value, kN = decltype(N)::value, kK = decltype(K)::value;
-    UpFusedArgs<float, kN> a;
-    for (int i = 0; i < kN; ++i) a.g0[i] = static_cast<const float*>(u.g0[i]);
-    a.mask0 = u.mask0;
-    a.w0 = u.w0;
-    a.h0 = u.h0;
-    a.levels = u.levels;
-    a.lv = u.lv;
-    a.f32_out = u.f32_out;
-    a.u8_out = nullptr;
-    a.u8_pitch = 0;
-    a.u8_w = a.u8_h = a.ox = a.oy = 0;
-    a.win_x = u.win_x;
-    a.win_y = u.win_y;
-    a.win_w = u.win_w;
-    a.win_h = u.win_h;
-    a.tiles_x = (u.win_w + kUpTile - 1) / kUpTile;
-    a.tiles_y = (u.win_h + kUpTile - 1) / kUpTile;
-    hipLaunchKernelGGL((up_fused_kernel<float, kC, kN, kK, kMask0F32>),
-                       dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch), dim3(256), 0, s, a);
-    return hipGetLastError();
-  });
-}
-
 hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch& u, int batch,
                             hipStream_t s) {
   if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
@@ -475,33 +447,9 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
     fx1[k] = std::min((fx1[k - 1] >> 1) + 1, W[k] - 1);
     fy1[k] = std::min((fy1[k - 1] >> 1) + 1, H[k] - 1);
   }
-  // 1. The small levels from kQuadLevels on, per base-level tile in one
-  //    launch, giving R at that level; their chain of dependent steps stays
-  //    inside a workgroup.
-  int top = levels - 1;  // first level run in quads
-  if (levels - 1 > kQuadLevels) {
-    const int base = kQuadLevels;
-    UpFusedLaunch u;
-    for (int i = 0; i < n; ++i) u.g0[i] = g[i][base].p;
-    u.mask0 = mask[base].p;
-    u.mask0_mode = kMask0F32;
-    u.w0 = W[base];
-    u.h0 = H[base];
-    u.levels = levels - base;
-    u.lv = level_table.as<LevelDesc>() + base;
-    u.f32_out = r[base].as<float>();
-    u.win_x = fx0[base];
-    u.win_y = fy0[base];
-    u.win_w = fx1[base] - fx0[base] + 1;
-    u.win_h = fy1[base] - fy0[base] + 1;
-    if (timer) timer->begin(s);
-    HM_RETURN_IF_ERROR(launch_up_coarse(c, n, kind, u, batch, s));
-    if (timer) timer->end(s, "up_coarse");
-    top = base - 1;
-  }
-  // 2. The remaining levels one launch each, one thread per 2x2 quad; level 0
-  //    goes straight to the output.
-  for (int k = top; k >= 0; --k) {
+  // One launch per level, coarsest first, one thread per 2x2 quad; level 0
+  // goes straight to the output.
+  for (int k = levels - 1; k >= 0; --k) {
     const bool has_next = k + 1 < levels;
     UpQuadLaunch u;
     for (int i = 0; i < n; ++i) {

@@ -4,19 +4,13 @@
 //   matching 16x16 .. 1x1 tiles of up to five coarser levels in LDS and writes
 //   them once.  The 2x2 mean never looks outside its own block, so aligned
 //   tiles need no halo and the result is the per-level kernel's exactly.
-// up_fused_kernel: one workgroup reconstructs a 32x32 tile of its base level.
-//   Going from the coarsest level down, it evaluates the Laplacian of every
-//   image, the blend and the reconstruction on the tile's footprint at each
-//   level (plus the one-pixel halo the top-left aligned upsample needs),
-//   keeping R_k in LDS; only the base level is written.  Every level is
-//   processed in 2x2 quads, whose four pixels share their taps and have fixed
-//   upsample positions (dx, dy) in {0, 1/2}^2.  The pyramid runs it on the
-//   small levels from 2 on, producing R_2.
 // up_quad_kernel: one level, one thread per 2x2 quad: the Laplacian of every
 //   image from the level's pixels and the shared taps of the next level, the
 //   blend, and the reconstruction from the shared R taps of the next level.
-//   It keeps no state, so it runs at full occupancy.  Levels 1 and 0 run it;
-//   level 0 writes straight to the output.
+//   Every level of the reconstruction runs it, coarsest first; it keeps no
+//   state, so it runs at full occupancy, and level 0 writes straight to the
+//   output.  The four pixels of a quad share their taps and sit at fixed
+//   upsample positions (dx, dy) in {0, 1/2}^2.
 // Per-pixel arithmetic is the same as the per-level kernels
 // (hm_pyramid_kernels.hpp), in the same order.
 #pragma once
@@ -214,9 +208,6 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
 // ---------------------------------------------------------------------------
 // up pass
 // ---------------------------------------------------------------------------
-constexpr int kUpTile = 32;                        // base-level tile edge
-constexpr int kUpRegion = kUpTile / 2 + 2;         // max footprint edge at levels >= 1
-
 template <int C, int N, int KIND, int MODE>
 __device__ __forceinline__ float4 blend_level0(const float4* L, const void* mask, size_t mpix) {
   if constexpr (MODE == kMask0F32) {
@@ -234,146 +225,6 @@ __device__ __forceinline__ float4 blend_level0(const float4* L, const void* mask
       return blend3_px<C>(L, mp);
     else
       return blendN_px<C, N>(L, mp);
-  }
-}
-
-template <typename T0, int N>
-struct UpFusedArgs {
-  const T0* g0[N];            // level-0 images
-  const void* mask0;          // level-0 mask (see Mask0Mode)
-  int w0, h0;
-  int levels;                 // total levels (>= 1)
-  const LevelDesc* lv;        // lv[k - 1] describes level k
-  float* f32_out;             // level-0 float output (tight), or nullptr
-  uint8_t* u8_out;            // level-0 uint8 canvas, or nullptr
-  size_t u8_pitch;
-  int u8_w, u8_h, ox, oy;
-  int win_x, win_y, win_w, win_h;  // level-0 pixels produced
-  int tiles_x, tiles_y;
-};
-
-template <typename T0, int C, int N, int KIND, int MODE>
-__global__ void __launch_bounds__(256) up_fused_kernel(UpFusedArgs<T0, N> a) {
-  __shared__ float4 buf[2][kUpRegion * kUpRegion];
-  __shared__ int reg[kMaxLevels][4];  // footprint [x0, x1] x [y0, y1] per level
-  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
-  const int tx = t % a.tiles_x, ty = t / a.tiles_x;
-  const int b = blockIdx.z;
-  const int tid = threadIdx.x;
-  const int L = a.levels;
-  const int ax0 = a.win_x + tx * kUpTile, ay0 = a.win_y + ty * kUpTile;
-  const int bx0 = min(ax0 + kUpTile, a.win_x + a.win_w) - 1;
-  const int by0 = min(ay0 + kUpTile, a.win_y + a.win_h) - 1;
-  if (tid == 0) {
-    int x0 = ax0, x1 = bx0, y0 = ay0, y1 = by0;
-    reg[0][0] = x0; reg[0][1] = x1; reg[0][2] = y0; reg[0][3] = y1;
-    for (int k = 1; k < L; ++k) {
-      const LevelDesc& d = a.lv[k - 1];
-      x0 >>= 1;
-      y0 >>= 1;
-      x1 = min((x1 >> 1) + 1, d.w - 1);
-      y1 = min((y1 >> 1) + 1, d.h - 1);
-      reg[k][0] = x0; reg[k][1] = x1; reg[k][2] = y0; reg[k][3] = y1;
-    }
-  }
-  __syncthreads();
-
-  int cur = 0;
-  // ---- coarsest level L-1 >= 1: R = blend of the Gaussians, per pixel
-  if (L > 1) {
-    const int k = L - 1;
-    const LevelDesc& d = a.lv[k - 1];
-    const int rx0 = reg[k][0], ry0 = reg[k][2];
-    const int rw = reg[k][1] - rx0 + 1, rh = reg[k][3] - ry0 + 1;
-    for (int idx = tid; idx < rw * rh; idx += blockDim.x) {
-      const int x = rx0 + idx % rw, y = ry0 + idx / rw;
-      const size_t pix = ((size_t)b * d.h + y) * d.w + x;
-      float4 Lv[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) Lv[i] = ld_px<C>(d.g[i], pix);
-      buf[cur][idx] = blend_any<C, N, KIND>(Lv, d.mask, (size_t)y * d.w + x);
-    }
-    __syncthreads();
-    cur ^= 1;
-  }
-
-  // ---- levels L-2 .. 0 in 2x2 quads; level k reads level k+1's Gaussian
-  // taps from global memory and its R taps from LDS.
-  for (int level = L > 1 ? L - 2 : 0; level >= 0; --level) {
-    const bool has_next = level + 1 < L;  // a coarser level exists
-    const LevelDesc* d = level > 0 ? &a.lv[level - 1] : nullptr;
-    const LevelDesc* dn = has_next ? &a.lv[level] : nullptr;
-    const int lw = level > 0 ? d->w : a.w0, lh = level > 0 ? d->h : a.h0;
-    const int rx0 = reg[level][0], rx1 = reg[level][1], ry0 = reg[level][2], ry1 = reg[level][3];
-    const int rw = rx1 - rx0 + 1;
-    const int qx0 = rx0 >> 1, qy0 = ry0 >> 1;
-    const int nqx = (rx1 >> 1) - qx0 + 1, nqy = (ry1 >> 1) - qy0 + 1;
-    int nx0 = 0, ny0 = 0, nrw = 1;
-    if (has_next) {
-      nx0 = reg[level + 1][0];
-      ny0 = reg[level + 1][2];
-      nrw = reg[level + 1][1] - nx0 + 1;
-    }
-    const float4* prev = buf[cur ^ 1];
-    for (int q = tid; q < nqx * nqy; q += blockDim.x) {
-      const int qx = qx0 + q % nqx, qy = qy0 + q / nqx;
-      float4 gt[N][4];
-      float4 rt[4];
-      if (has_next) {
-        const int qx1 = min(qx + 1, dn->w - 1), qy1 = min(qy + 1, dn->h - 1);
-        const size_t l0 = ((size_t)b * dn->h + qy) * dn->w, l1 = ((size_t)b * dn->h + qy1) * dn->w;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          gt[i][0] = ld_px<C>(dn->g[i], l0 + qx);
-          gt[i][1] = ld_px<C>(dn->g[i], l0 + qx1);
-          gt[i][2] = ld_px<C>(dn->g[i], l1 + qx);
-          gt[i][3] = ld_px<C>(dn->g[i], l1 + qx1);
-        }
-        const int lx = qx - nx0, lx1 = qx1 - nx0, ly = qy - ny0, ly1 = qy1 - ny0;
-        rt[0] = prev[ly * nrw + lx];
-        rt[1] = prev[ly * nrw + lx1];
-        rt[2] = prev[ly1 * nrw + lx];
-        rt[3] = prev[ly1 * nrw + lx1];
-      }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
-        if (x < rx0 || x > rx1 || y < ry0 || y > ry1) continue;
-        const size_t pix = ((size_t)b * lh + y) * lw + x;
-        float4 Lv[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          const float4 g = level > 0 ? ld_px<C>(d->g[i], pix) : ld_px<C>(a.g0[i], pix);
-          if (has_next) {
-            const float4 up = up_quad_lap<C>(e, gt[i]);
-            Lv[i] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
-          } else {
-            Lv[i] = g;
-          }
-        }
-        const size_t mpix = (size_t)y * lw + x;
-        const float4 B = level > 0 ? blend_any<C, N, KIND>(Lv, d->mask, mpix)
-                                   : blend_level0<C, N, KIND, MODE>(Lv, a.mask0, mpix);
-        float4 R = B;
-        if (has_next) {
-          const float4 up = up_quad_rec<C>(e, rt);
-          R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
-        }
-        if (level > 0) {
-          buf[cur][(y - ry0) * rw + (x - rx0)] = R;
-        } else {
-          if (a.f32_out) st_px<C>(a.f32_out, pix, R);
-          if (a.u8_out) {
-            const int X = x + a.ox, Y = y + a.oy;
-            if (X < a.u8_w && Y < a.u8_h)
-              st_px_u8<C>(a.u8_out + ((size_t)b * a.u8_h + Y) * a.u8_pitch + (size_t)X * C, R);
-          }
-        }
-      }
-    }
-    if (level == 0) break;
-    __syncthreads();
-    cur ^= 1;
   }
 }
 

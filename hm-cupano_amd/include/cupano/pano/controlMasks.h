@@ -51,5 +51,52 @@ class ControlMasks {
   }
 };
 
+// ControlMasksN (src/pano/controlMasksN.h:13-38): N remap tables, the
+// indexed seam with labels 0..N-1, N positions.
+class ControlMasksN {
+ public:
+  std::vector<HostMat<uint16_t>> img_col, img_row;
+  HostMat<uint8_t> whole_seam_mask_indexed;
+  std::vector<SpatialTiff> positions;
+
+  bool is_valid() const {  // controlMasksN.cpp:195-203
+    if (img_col.empty() || img_row.empty() || img_col.size() != img_row.size()) return false;
+    if (positions.size() != img_col.size()) return false;
+    return !whole_seam_mask_indexed.empty();
+  }
+  size_t canvas_width() const {  // controlMasksN.cpp:205-211
+    float w = 0.0f;
+    for (size_t i = 0; i < img_col.size(); ++i) w = std::max(w, positions[i].xpos + img_col[i].cols);
+    return (size_t)w;
+  }
+  size_t canvas_height() const {  // controlMasksN.cpp:213-219
+    float h = 0.0f;
+    for (size_t i = 0; i < img_row.size(); ++i) h = std::max(h, positions[i].ypos + img_row[i].rows);
+    return (size_t)h;
+  }
+};
+
+// ControlMasks3 (src/pano/controlMasks3.h:12-43): three remap tables, the
+// single-channel seam with labels 0..2, three positions.
+class ControlMasks3 {
+ public:
+  HostMat<uint16_t> img0_col, img0_row, img1_col, img1_row, img2_col, img2_row;
+  HostMat<uint8_t> whole_seam_mask_image;
+  std::vector<SpatialTiff> positions;
+
+  bool is_valid() const {  // controlMasks3.cpp:297-301
+    return !img0_col.empty() && !img0_row.empty() && !img1_col.empty() && !img1_row.empty() && !img2_col.empty() &&
+           !img2_row.empty() && !whole_seam_mask_image.empty() && positions.size() == 3;
+  }
+  size_t canvas_width() const {  // controlMasks3.cpp:303-309
+    return (size_t)std::max({positions.at(0).xpos + img0_col.cols, positions.at(1).xpos + img1_col.cols,
+                             positions.at(2).xpos + img2_col.cols});
+  }
+  size_t canvas_height() const {  // controlMasks3.cpp:311-317
+    return (size_t)std::max({positions.at(0).ypos + img0_col.rows, positions.at(1).ypos + img1_col.rows,
+                             positions.at(2).ypos + img2_col.rows});
+  }
+};
+
 }  // namespace pano
 }  // namespace hm

@@ -1,16 +1,24 @@
-// C++ API test: drives hm::pano::cuda::CudaStitchPano through the reference's
-// class interface (ControlMasks -> ctor -> process(in1, in2, stream, canvas)),
-// restating src/pano/cudaPano_test.cpp and tests/python/test_pytorch_pano.py
-// cases with a uint8 pipeline.  Prints one line per case; exit code 0 = pass.
+// C++ API test: drives hm::pano::cuda::CudaStitchPano, CudaStitchPanoN and
+// CudaStitchPano3 through the reference's class interfaces (ControlMasks* ->
+// ctor -> process(..., stream, canvas)), restating cases of
+// src/pano/cudaPano_test.cpp, src/pano/cudaPano3_test.cpp and
+// tests/python/test_pytorch_pano.py with a uint8 pipeline.  Prints one line
+// per case; exit code 0 = pass.
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
 #include <vector>
 
 #include "cupano/pano/cudaPano.h"
+#include "cupano/pano/cudaPano3.h"
+#include "cupano/pano/cudaPanoN.h"
 
 using hm::CudaMat;
 using hm::pano::ControlMasks;
+using hm::pano::ControlMasks3;
+using hm::pano::ControlMasksN;
+using hm::pano::cuda::CudaStitchPano3;
+using hm::pano::cuda::CudaStitchPanoN;
 using hm::pano::HostMat;
 using hm::pano::SpatialTiff;
 using hm::pano::cuda::CudaStitchPano;
@@ -78,6 +86,48 @@ std::vector<uchar4> stitch(const ControlMasks& masks, int levels, bool minimize,
 
 bool same(uchar4 a, uchar4 b) { return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w; }
 
+void identity(int w, int h, HostMat<uint16_t>* mx, HostMat<uint16_t>* my) {
+  *mx = HostMat<uint16_t>(h, w);
+  *my = HostMat<uint16_t>(h, w);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      mx->at(y, x) = (uint16_t)x;
+      my->at(y, x) = (uint16_t)y;
+    }
+}
+
+// Three bands: label 0 in [0, w/3), 1 in [w/3, 2w/3), 2 beyond.
+HostMat<uint8_t> bands3(int w, int h) {
+  HostMat<uint8_t> seam(h, w, 0);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) seam.at(y, x) = (uint8_t)(x < w / 3 ? 0 : (x < 2 * w / 3 ? 1 : 2));
+  return seam;
+}
+
+const uchar4 kColors[3] = {make_uchar4(200, 0, 0, 255), make_uchar4(0, 150, 0, 255), make_uchar4(0, 0, 100, 255)};
+
+// Constant images per label: every level's blend selects the labelled colour,
+// so away from the band edges the output is exactly that colour.
+bool bands_ok(const std::vector<uchar4>& out, int w, int h, int margin) {
+  if (out.size() != (size_t)w * h) return false;
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const int lbl = x < w / 3 ? 0 : (x < 2 * w / 3 ? 1 : 2);
+      const int e0 = lbl == 0 ? 0 : (lbl == 1 ? w / 3 : 2 * w / 3);
+      const int e1 = lbl == 0 ? w / 3 : (lbl == 1 ? 2 * w / 3 : w);
+      if (x - e0 < margin && e0 > 0) continue;
+      if (e1 - x <= margin && e1 < w) continue;
+      if (!same(out[(size_t)y * w + x], kColors[lbl])) return false;
+    }
+  return true;
+}
+
+std::vector<uchar4> download(CudaMat<uchar4>& m, int w, int h) {
+  std::vector<uchar4> host((size_t)w * h);
+  if (m.download(host.data()) != hipSuccess) return {};
+  return host;
+}
+
 }  // namespace
 
 int main() {
@@ -132,6 +182,63 @@ int main() {
     CudaMat<uchar4> a(1, 4, 4), b(1, 4, 4);
     auto r = pano.process(a, b, nullptr, std::make_unique<CudaMat<uchar4>>(1, 4, 4));
     check(!r.ok() && r.status().code() == hipErrorFileNotFound, "process returns the constructor's status");
+  }
+  // CudaStitchPanoN: three identity-mapped images, three label bands.
+  {
+    const int w = 96, h = 40;
+    ControlMasksN masks;
+    masks.img_col.resize(3);
+    masks.img_row.resize(3);
+    for (int i = 0; i < 3; ++i) identity(w, h, &masks.img_col[i], &masks.img_row[i]);
+    masks.whole_seam_mask_indexed = bands3(w, h);
+    masks.positions = {SpatialTiff{0, 0}, SpatialTiff{0, 0}, SpatialTiff{0, 0}};
+    std::vector<std::vector<uchar4>> imgs;
+    for (int i = 0; i < 3; ++i) imgs.push_back(constant(w, h, kColors[i]));
+    std::vector<std::unique_ptr<CudaMat<uchar4>>> ins;
+    for (int i = 0; i < 3; ++i) ins.push_back(std::make_unique<CudaMat<uchar4>>(imgs[i].data(), w, h));
+    std::vector<const CudaMat<uchar4>*> in_ptrs = {ins[0].get(), ins[1].get(), ins[2].get()};
+    for (int levels : {0, 4}) {
+      CudaStitchPanoN<uchar4, float4> pano(1, levels, masks, /*minimize_blend=*/true, /*quiet=*/true);
+      bool ok = pano.status().ok() && pano.canvas_width() == w && pano.canvas_height() == h;
+      if (ok) {
+        auto r = pano.process(in_ptrs, nullptr, std::make_unique<CudaMat<uchar4>>(1, w, h));
+        ok = r.ok();
+        if (ok) {
+          auto out = r.ConsumeValueOrDie();
+          ok = bands_ok(download(*out, w, h), w, h, levels == 0 ? 0 : 16);
+        }
+      }
+      check(ok, levels == 0 ? "CudaStitchPanoN hard seam selects each labelled image"
+                            : "CudaStitchPanoN soft seam of constant images keeps each colour away from the edges");
+    }
+    CudaStitchPanoN<uchar4, float4> pano(1, 2, masks, true, true);
+    std::vector<const CudaMat<uchar4>*> two = {ins[0].get(), ins[1].get()};
+    auto bad = pano.process(two, nullptr, std::make_unique<CudaMat<uchar4>>(1, w, h));
+    check(!bad.ok() && pano.status().ok(), "CudaStitchPanoN input-count errors are not sticky");
+  }
+  // CudaStitchPano3: src/pano/cudaPano3_test.cpp:133-196 (1x1, one-hot label 1 -> image 1), both paths.
+  {
+    ControlMasks3 masks;
+    for (auto* m : {&masks.img0_col, &masks.img0_row, &masks.img1_col, &masks.img1_row, &masks.img2_col,
+                    &masks.img2_row})
+      *m = HostMat<uint16_t>(1, 1, 0);
+    masks.whole_seam_mask_image = HostMat<uint8_t>(1, 1, 1);
+    masks.positions = {SpatialTiff{0, 0}, SpatialTiff{0, 0}, SpatialTiff{0, 0}};
+    const uchar4 v[3] = {make_uchar4(30, 30, 30, 255), make_uchar4(60, 60, 60, 255), make_uchar4(90, 90, 90, 255)};
+    CudaMat<uchar4> i0(&v[0], 1, 1), i1(&v[1], 1, 1), i2(&v[2], 1, 1);
+    bool ok = true;
+    for (bool fused : {true, false}) {
+      CudaStitchPano3<uchar4, float4> pano(1, 1, masks, true);
+      auto r = pano.process(i0, i1, i2, nullptr, std::make_unique<CudaMat<uchar4>>(1, 1, 1), fused);
+      if (!r.ok()) {
+        ok = false;
+        continue;
+      }
+      auto out = r.ConsumeValueOrDie();
+      const auto host = download(*out, 1, 1);
+      ok = ok && host.size() == 1 && same(host[0], v[1]);
+    }
+    check(ok, "CudaStitchPano3 one-hot label 1 selects image 1 (fused and unfused)");
   }
   std::printf("%d failure(s)\n", g_failures);
   return g_failures ? 1 : 0;

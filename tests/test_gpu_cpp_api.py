@@ -1,6 +1,6 @@
 """Runs the C++ API test program (tests/cpp/pano_api_test.cpp): the reference's
-class interface (ControlMasks, CudaStitchPano, CudaMat, CudaStatusOr) driving
-the gfx950 library."""
+class interfaces (ControlMasks*, CudaStitchPano, CudaStitchPanoN,
+CudaStitchPano3, CudaMat, CudaStatusOr) driving the gfx950 library."""
 import os
 import subprocess
 
@@ -18,4 +18,4 @@ def test_cpp_pano_api(torch_gpu):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
-    assert r.stdout.count("PASS") == 5
+    assert r.stdout.count("PASS") == 9
