@@ -39,6 +39,21 @@ struct hmStitchPano {
   // Allocates the pyramid and the resident mask pyramid for a bw x bh blend
   // region; mask_host is bw x bh x M float.
   hipError_t init_pyramid(int kind, int bw, int bh, const std::vector<float>& mask_host);
+
+  // Allocates everything a soft-seam process needs (done by the first process
+  // otherwise), so that later calls allocate nothing.
+  virtual hipError_t prepare() = 0;
+  // Device memory held by the object.
+  virtual size_t device_bytes() const = 0;
+  // One frame with the class's own error rules (sticky or not).  `fused`
+  // selects the 3-image class's path and is ignored by the others.
+  virtual hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int fused,
+                                 hipStream_t s) = 0;
+  size_t base_device_bytes() const {
+    size_t b = mask0.bytes + pyr.allocation_size + pyr.level_table.bytes;
+    for (const auto& g : g0) b += g.bytes;
+    return b;
+  }
 };
 
 namespace hm {

@@ -180,6 +180,15 @@ struct Pano2 : hmStitchPano {
 
   hipError_t setup(const hmControlMasks2& m, int minimize_blend);
   hipError_t ensure_pyramid();
+  hipError_t prepare() override { return ensure_pyramid(); }
+  hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int,
+                         hipStream_t s) override {
+    if (status != hipSuccess) return (hipError_t)status;
+    const hipError_t e = n_inputs == 2 && inputs ? process(inputs[0], inputs[1], canvas, s) : hipErrorInvalidValue;
+    if (e != hipSuccess) status = e;  // CudaStitchPano::process makes errors sticky (cudaPano.inl:468-470)
+    return e;
+  }
+  size_t device_bytes() const override { return base_device_bytes() + map[0].bytes + map[1].bytes + seam.bytes + mask_u8.bytes; }
   hipError_t process(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s);
   hipError_t run_blend(const hmSurface& canvas, hipStream_t s);
 };
@@ -414,10 +423,8 @@ extern "C" int hmStitchPanoCreate(int batch_size, int num_levels, const hmContro
 extern "C" int hmStitchPanoProcess(hmStitchPano* pano, hmSurface input1, hmSurface input2, hmSurface canvas,
                                    hipStream_t stream) {
   if (!pano) return hipErrorInvalidValue;
-  if (pano->status != hipSuccess) return pano->status;
   auto* p = dynamic_cast<Pano2*>(pano);
   if (!p) return hipErrorInvalidValue;
-  const hipError_t e = p->process(input1, input2, canvas, stream);
-  if (e != hipSuccess) pano->status = e;  // CudaStitchPano::process makes errors sticky (cudaPano.inl:468-470)
-  return e;
+  const hmSurface in[2] = {input1, input2};
+  return p->process_any(in, 2, canvas, 1, stream);
 }

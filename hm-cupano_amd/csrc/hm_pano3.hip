@@ -36,6 +36,15 @@ struct Pano3 : hmStitchPano {
 
   hipError_t setup(const hmControlMasks3& m);
   hipError_t ensure_pyramid();
+  hipError_t prepare() override { return ensure_pyramid(); }
+  hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int fused,
+                         hipStream_t s) override {
+    if (status != hipSuccess) return (hipError_t)status;
+    const hipError_t e = n_inputs == 3 && inputs ? process(inputs, canvas, fused != 0, s) : hipErrorInvalidValue;
+    if (e != hipSuccess) status = e;  // both process paths make errors sticky (cudaPano3.inl:415-421)
+    return e;
+  }
+  size_t device_bytes() const override { return base_device_bytes() + map[0].bytes + map[1].bytes + map[2].bytes + seam.bytes + labels.bytes; }
   hipError_t process(const hmSurface* inputs, const hmSurface& canvas, bool fused, hipStream_t s);
   MultiGeom geom() const;
 };
@@ -202,11 +211,8 @@ extern "C" int hmStitchPano3Create(int batch_size, int num_levels, const hmContr
 extern "C" int hmStitchPano3Process(hmStitchPano* pano, hmSurface input0, hmSurface input1, hmSurface input2,
                                     hmSurface canvas, int fused, hipStream_t stream) {
   if (!pano) return hipErrorInvalidValue;
-  if (pano->status != hipSuccess) return pano->status;
   auto* p = dynamic_cast<Pano3*>(pano);
   if (!p) return hipErrorInvalidValue;
   const hmSurface in[3] = {input0, input1, input2};
-  const hipError_t e = p->process(in, canvas, fused != 0, stream);
-  if (e != hipSuccess) pano->status = e;  // both process paths make errors sticky (cudaPano3.inl:415-421)
-  return e;
+  return p->process_any(in, 3, canvas, fused, stream);
 }

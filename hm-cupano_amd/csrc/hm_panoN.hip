@@ -88,6 +88,17 @@ struct PanoN : hmStitchPano {
 
   hipError_t setup(const hmControlMasksN& m, int minimize_blend);
   hipError_t ensure_pyramid();
+  hipError_t prepare() override { return ensure_pyramid(); }
+  hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int,
+                         hipStream_t s) override {
+    if (status != hipSuccess) return (hipError_t)status;
+    return process(inputs, n_inputs, canvas, s);  // CudaStitchPanoN::process: errors are not sticky
+  }
+  size_t device_bytes() const override {
+    size_t b = base_device_bytes() + seam.bytes + labels.bytes;
+    for (const auto& m : map) b += m.bytes;
+    return b;
+  }
   hipError_t process(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, hipStream_t s);
   template <int N>
   hipError_t launch_region(const MultiRegion& r, const hmSurface* inputs, const hmSurface& canvas, hipStream_t s,
@@ -281,9 +292,7 @@ extern "C" int hmStitchPanoNCreate(int batch_size, int num_levels, const hmContr
 extern "C" int hmStitchPanoNProcess(hmStitchPano* pano, const hmSurface* inputs, int num_inputs, hmSurface canvas,
                                     hipStream_t stream) {
   if (!pano) return hipErrorInvalidValue;
-  if (pano->status != hipSuccess) return pano->status;
   auto* p = dynamic_cast<PanoN*>(pano);
   if (!p) return hipErrorInvalidValue;
-  // CudaStitchPanoN::process returns errors without making them sticky.
-  return p->process(inputs, num_inputs, canvas, stream);
+  return p->process_any(inputs, num_inputs, canvas, 1, stream);
 }

@@ -90,6 +90,20 @@ hipError_t hmStitchPano::join(hipStream_t s) {
 
 extern "C" int hmStitchPanoStatus(const hmStitchPano* pano) { return pano ? pano->status : hipErrorInvalidValue; }
 
+extern "C" int hmStitchPanoPrepare(hmStitchPano* pano) {
+  if (!pano) return hipErrorInvalidValue;
+  if (pano->status != hipSuccess) return pano->status;
+  const hipError_t e = pano->prepare();
+  if (e != hipSuccess) pano->status = e;
+  return e;
+}
+
+extern "C" int hmStitchPanoDeviceBytes(const hmStitchPano* pano, size_t* bytes) {
+  if (!pano || !bytes) return hipErrorInvalidValue;
+  *bytes = pano->device_bytes();
+  return hipSuccess;
+}
+
 extern "C" int hmStitchPanoCanvasSize(const hmStitchPano* pano, int* width, int* height) {
   if (!pano || !width || !height) return hipErrorInvalidValue;
   *width = pano->canvas_w;

@@ -66,6 +66,18 @@ class hmControlMasks3(ctypes.Structure):
     ]
 
 
+class hmServerResult(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("rig_id", ctypes.c_int), ("status", ctypes.c_int),
+                ("t_submit_ns", ctypes.c_int64), ("t_start_ns", ctypes.c_int64), ("t_done_ns", ctypes.c_int64)]
+
+
+class hmServerStats(ctypes.Structure):
+    _fields_ = [("n_rigs", ctypes.c_int), ("device_budget_bytes", ctypes.c_size_t),
+                ("device_used_bytes", ctypes.c_size_t), ("queued", ctypes.c_int), ("in_flight", ctypes.c_int),
+                ("graphs_captured", ctypes.c_long), ("graph_launches", ctypes.c_long),
+                ("eager_launches", ctypes.c_long)]
+
+
 _lib = None
 
 
@@ -98,6 +110,16 @@ def lib() -> ctypes.CDLL:
         "hmStitchPanoNProcess": (i, [vp, ctypes.POINTER(S), i, S, vp]),
         "hmStitchPano3Create": (i, [i, i, ctypes.POINTER(hmControlMasks3), i, i, ctypes.POINTER(vp)]),
         "hmStitchPano3Process": (i, [vp, S, S, S, S, i, vp]),
+        "hmStitchPanoPrepare": (i, [vp]),
+        "hmStitchPanoDeviceBytes": (i, [vp, ctypes.POINTER(ctypes.c_size_t)]),
+        "hmServerCreate": (i, [i, ctypes.c_size_t, i, ctypes.POINTER(vp)]),
+        "hmServerAddRig": (i, [vp, vp, ctypes.POINTER(i)]),
+        "hmServerRemoveRig": (i, [vp, i]),
+        "hmServerSubmit": (i, [vp, i, ctypes.POINTER(S), i, S, i, ctypes.c_uint64]),
+        "hmServerPoll": (i, [vp, ctypes.POINTER(hmServerResult), i, ctypes.POINTER(i)]),
+        "hmServerWaitIdle": (i, [vp]),
+        "hmServerGetStats": (i, [vp, ctypes.POINTER(hmServerStats)]),
+        "hmServerDestroy": (None, [vp]),
         "hmStitchPanoStatus": (i, [vp]),
         "hmStitchPanoCanvasSize": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i)]),
         "hmStitchPanoBatchSize": (i, [vp]),
@@ -214,6 +236,14 @@ class StitchPano:
 
     def process(self, in1, in2, canvas, stream=None) -> int:
         return lib().hmStitchPanoProcess(self._h, surface(in1), surface(in2), surface(canvas), _stream(stream))
+
+    def prepare(self) -> int:
+        return lib().hmStitchPanoPrepare(self._h)
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_size_t()
+        check(lib().hmStitchPanoDeviceBytes(self._h, ctypes.byref(b)), "hmStitchPanoDeviceBytes")
+        return b.value
 
     def levels(self):
         n = ctypes.c_int()
@@ -335,6 +365,60 @@ class StitchPano3(StitchPano):
 
 
 # ---------------------------------------------------------------------------
+# served form
+# ---------------------------------------------------------------------------
+class Server:
+    """Many rigs on one GPU (hmServer*): admission by device bytes, a scheduler
+    thread over `n_streams` streams, hipGraph replay per buffer set."""
+
+    def __init__(self, n_streams: int = 4, device_budget_bytes: int = 64 << 30, use_graphs: bool = True):
+        h = ctypes.c_void_p()
+        check(lib().hmServerCreate(n_streams, device_budget_bytes, int(use_graphs), ctypes.byref(h)), "hmServerCreate")
+        self._h = h
+        self._keep = {}  # rig id -> pano (the caller's objects must outlive their requests)
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hmServerDestroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_rig(self, pano) -> int:
+        rid = ctypes.c_int()
+        rc = lib().hmServerAddRig(self._h, pano._h, ctypes.byref(rid))
+        if rc != 0:
+            raise HmError(rc, "hmServerAddRig")
+        self._keep[rid.value] = pano
+        return rid.value
+
+    def remove_rig(self, rig_id: int) -> None:
+        check(lib().hmServerRemoveRig(self._h, rig_id), "hmServerRemoveRig")
+        self._keep.pop(rig_id, None)
+
+    def submit(self, rig_id: int, inputs, canvas, tag: int = 0, fused: bool = True) -> int:
+        arr = (hmSurface * len(inputs))(*[surface(t) for t in inputs])
+        return lib().hmServerSubmit(self._h, rig_id, arr, len(inputs), surface(canvas), int(fused), tag)
+
+    def poll(self, max_results: int = 4096):
+        buf = (hmServerResult * max_results)()
+        n = ctypes.c_int()
+        check(lib().hmServerPoll(self._h, buf, max_results, ctypes.byref(n)), "hmServerPoll")
+        return [dict(tag=r.tag, rig=r.rig_id, status=r.status, t_submit_ns=r.t_submit_ns, t_start_ns=r.t_start_ns,
+                     t_done_ns=r.t_done_ns) for r in buf[: n.value]]
+
+    def wait_idle(self) -> int:
+        return lib().hmServerWaitIdle(self._h)
+
+    def stats(self) -> dict:
+        st = hmServerStats()
+        check(lib().hmServerGetStats(self._h, ctypes.byref(st)), "hmServerGetStats")
+        return {k: getattr(st, k) for k, _ in hmServerStats._fields_}
+
+
+# ---------------------------------------------------------------------------
 # kernel tier
 # ---------------------------------------------------------------------------
 class BlendContext:
@@ -354,6 +438,14 @@ class BlendContext:
         if h is not None and h.value and _lib is not None:
             _lib.hmBlendContextDestroy(h)
             self._h = None
+
+    def prepare(self) -> int:
+        return lib().hmStitchPanoPrepare(self._h)
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_size_t()
+        check(lib().hmStitchPanoDeviceBytes(self._h, ctypes.byref(b)), "hmStitchPanoDeviceBytes")
+        return b.value
 
     def levels(self):
         n = ctypes.c_int()

@@ -173,6 +173,16 @@ int hmStitchPanoCanvasSize(const hmStitchPano* pano, int* width, int* height);
 int hmStitchPanoBatchSize(const hmStitchPano* pano);
 void hmStitchPanoDestroy(hmStitchPano* pano);
 
+/* No counterpart in the reference (it allocates lazily in the first
+ * process): allocate everything a process call needs now, for callers that
+ * must not allocate later (a serving loop, graph capture).  Errors are
+ * sticky. */
+int hmStitchPanoPrepare(hmStitchPano* pano);
+/* No counterpart in the reference: device memory held by the object (the
+ * allocation_size accounting of src/cuda/cudaBlend.h:107, for the whole
+ * pano), used for admission against a memory budget. */
+int hmStitchPanoDeviceBytes(const hmStitchPano* pano, size_t* bytes);
+
 #define HM_MAX_IMAGES 8
 
 /* In-memory ControlMasksN (src/pano/controlMasksN.h:13-38): per-image CV_16U
@@ -222,6 +232,58 @@ int hmStitchPano3Create(int batch_size, int num_levels, const hmControlMasks3* m
  * 3-channel canvas is not cleared: pixels no image writes keep their value. */
 int hmStitchPano3Process(hmStitchPano* pano, hmSurface input0, hmSurface input1, hmSurface input2, hmSurface canvas,
                          int fused, hipStream_t stream);
+
+/* ------------------------------------------------------------------------- */
+/* Served form: many rigs, one GPU                                           */
+/* ------------------------------------------------------------------------- */
+/* No counterpart in the reference, whose apps call process() once per frame
+ * and synchronise (tests/stitch_two_videos.cpp:552-585).  For a frame loop
+ * that serves many camera rigs: rigs (pano objects of any class) are
+ * admitted against a device-memory budget and keep their state resident;
+ * frames are submitted as requests and started by a scheduler thread on one
+ * of n_streams streams as soon as a stream and the rig are free, so work of
+ * different rigs overlaps on the device; each rig's pipeline is captured as a
+ * hipGraph per (inputs, canvas) buffer set and replayed.  Output is
+ * bit-identical to the pano's own process().  The caller keeps ownership of
+ * the pano objects and of every buffer until its request has completed. */
+typedef struct hmServer hmServer;
+
+typedef struct hmServerResult {
+  uint64_t tag;        /* the caller's tag of the request */
+  int rig_id;
+  int status;          /* hipError_t of the request */
+  int64_t t_submit_ns; /* steady clock: hmServerSubmit */
+  int64_t t_start_ns;  /* launched on a stream */
+  int64_t t_done_ns;   /* completion observed */
+} hmServerResult;
+
+typedef struct hmServerStats {
+  int n_rigs;
+  size_t device_budget_bytes;
+  size_t device_used_bytes;
+  int queued;
+  int in_flight;
+  long graphs_captured;
+  long graph_launches;
+  long eager_launches;
+} hmServerStats;
+
+int hmServerCreate(int n_streams, size_t device_budget_bytes, int use_graphs, hmServer** out);
+/* Prepares the pano (hmStitchPanoPrepare) and admits it if its device bytes
+ * fit the remaining budget (else hipErrorOutOfMemory, not admitted). */
+int hmServerAddRig(hmServer* server, hmStitchPano* pano, int* rig_id);
+/* Waits for the rig's requests, then stops serving it (the pano survives). */
+int hmServerRemoveRig(hmServer* server, int rig_id);
+/* Queues one frame; returns at once.  fused: the 3-image class's path. */
+int hmServerSubmit(hmServer* server, int rig_id, const hmSurface* inputs, int num_inputs, hmSurface canvas, int fused,
+                   uint64_t tag);
+/* Moves up to max_results completed requests out (oldest first). */
+int hmServerPoll(hmServer* server, hmServerResult* results, int max_results, int* num_results);
+/* Blocks until nothing is queued or running; returns the first error seen. */
+int hmServerWaitIdle(hmServer* server);
+int hmServerGetStats(hmServer* server, hmServerStats* stats);
+/* Drains the queue, then releases streams and graphs. */
+void hmServerDestroy(hmServer* server);
 
 /* ------------------------------------------------------------------------- */
 /* Pyramid inspection and launch timing (all pano classes)                   */

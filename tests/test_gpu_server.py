@@ -1,0 +1,159 @@
+"""The served form (hmServer*): many rigs of every pano class in flight on one
+GPU, hipGraph replay, admission by device memory.  Every served frame must be
+bit-identical to the same frame stitched alone (the oracle)."""
+import numpy as np
+import pytest
+
+from conftest import frames_for
+
+pytestmark = pytest.mark.gpu
+
+HIP_ERROR_OUT_OF_MEMORY = 2
+
+
+def to_dev(torch, frames):
+    return [torch.from_numpy(np.ascontiguousarray(f)).cuda() for f in frames]
+
+
+def make_rigs(hm):
+    """(name, pano, rig, levels, kind, fused) for a mix of classes and levels."""
+    import hm_synth
+
+    out = []
+    r2 = hm_synth.small_rig(seed=11)
+    out.append(("pano2_L4", hm.StitchPano(1, 4, r2), r2, 4, "2", True))
+    r2b = hm_synth.small_rig(seed=12)
+    out.append(("pano2_L0", hm.StitchPano(1, 0, r2b), r2b, 0, "2", True))
+    rn = hm_synth.small_rig(n=3, seed=13, src=(120, 56), canvas=(330, 64))
+    out.append(("panoN3_L3", hm.StitchPanoN(1, 3, rn), rn, 3, "N", True))
+    rn5 = hm_synth.small_rig(n=5, seed=14, src=(120, 56), canvas=(510, 64))
+    out.append(("panoN5_L0", hm.StitchPanoN(1, 0, rn5), rn5, 0, "N", True))
+    r3 = hm_synth.small_rig(n=3, seed=15, src=(120, 56), canvas=(330, 64))
+    out.append(("pano3_L2_fused", hm.StitchPano3(1, 2, r3), r3, 2, "3", True))
+    r3b = hm_synth.small_rig(n=3, seed=16, src=(120, 56), canvas=(330, 64))
+    out.append(("pano3_L3_unfused", hm.StitchPano3(1, 3, r3b), r3b, 3, "3", False))
+    return out
+
+
+def oracle_out(oracle, kind, rig, frames, levels, fused, prior):
+    if kind == "2":
+        return oracle.pano2(rig, frames[0], frames[1], levels)
+    if kind == "N":
+        return oracle.panoN(rig, frames, levels)
+    return oracle.pano3(rig, frames, levels, fused=fused, canvas=prior)
+
+
+@pytest.mark.parametrize("use_graphs", [True, False])
+def test_server_mixed_rigs_match_alone(hm, torch_gpu, oracle, use_graphs):
+    torch = torch_gpu
+    rigs = make_rigs(hm)
+    server = hm.Server(n_streams=4, use_graphs=use_graphs)
+    ids = [server.add_rig(p) for (_, p, *_rest) in rigs]
+    inputs = {}  # (rig index, input set) -> (host frames, device tensors)
+    for ri, (_, _, rig, *_r) in enumerate(rigs):
+        for s in range(2):
+            fr = frames_for(rig, 1, 100 * ri + s)
+            inputs[ri, s] = (fr, to_dev(torch, fr))
+    rounds = 4
+    jobs = []
+    for rnd in range(rounds):
+        for ri, (name, pano, rig, levels, kind, fused) in enumerate(rigs):
+            w, h = pano.canvas_size()
+            prior = np.full((1, h, w, 4), 9 + rnd, np.uint8)
+            jobs.append((ri, rnd % 2, torch.from_numpy(prior).cuda(), prior))
+    # the server's streams do not wait for torch's: every buffer is ready first
+    torch.cuda.synchronize()
+    for tag, (ri, s, canvas, prior) in enumerate(jobs):
+        assert server.submit(ids[ri], inputs[ri, s][1], canvas, tag=tag, fused=rigs[ri][5]) == 0
+    assert server.wait_idle() == 0
+    res = server.poll()
+    assert sorted(r["tag"] for r in res) == list(range(len(jobs)))
+    for r in res:
+        assert r["status"] == 0
+        assert r["t_submit_ns"] <= r["t_start_ns"] <= r["t_done_ns"]
+    for tag, (ri, s, canvas, prior) in enumerate(jobs):
+        name, pano, rig, levels, kind, fused = rigs[ri]
+        want = oracle_out(oracle, kind, rig, inputs[ri, s][0], levels, fused, prior)
+        np.testing.assert_array_equal(canvas.cpu().numpy(), want, err_msg=f"{name} round {tag}")
+    st = server.stats()
+    if use_graphs:
+        assert st["graphs_captured"] == len(jobs) and st["graph_launches"] == len(jobs)
+    else:
+        assert st["eager_launches"] == len(jobs)
+    server.close()
+
+
+def test_server_graph_replays_with_eager_work_between(hm, torch_gpu, oracle):
+    """One buffer set, replayed several times with new input content written in between."""
+    import hm_synth
+
+    torch = torch_gpu
+    rig = hm_synth.small_rig(seed=21)
+    pano = hm.StitchPano(1, 5, rig)
+    server = hm.Server(n_streams=2, use_graphs=True)
+    rid = server.add_rig(pano)
+    dev = to_dev(torch, frames_for(rig, 1, 0))
+    canvas = pano.alloc_canvas()
+    torch.cuda.synchronize()
+    for k in range(5):
+        fr = frames_for(rig, 1, 50 + k)
+        for d, f in zip(dev, fr):
+            d.copy_(torch.from_numpy(f).cuda())  # eager work on torch's stream
+        torch.cuda.synchronize()
+        assert server.submit(rid, dev, canvas, tag=k) == 0
+        assert server.wait_idle() == 0
+        scratch = torch.ones(1 << 20, device="cuda").sum()  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        assert float(scratch) == float(1 << 20)
+        np.testing.assert_array_equal(canvas.cpu().numpy(), oracle.pano2(rig, fr[0], fr[1], 5), err_msg=f"replay {k}")
+    st = server.stats()
+    assert st["graphs_captured"] == 1 and st["graph_launches"] == 5
+    server.close()
+
+
+def test_server_admission_by_device_budget(hm, torch_gpu):
+    import hm_synth
+
+    rig = hm_synth.small_rig(seed=31)
+    a = hm.StitchPano(1, 4, rig)
+    b = hm.StitchPano(1, 4, rig)
+    assert a.prepare() == 0
+    need = a.device_bytes()
+    assert need > 0
+    server = hm.Server(n_streams=2, device_budget_bytes=need + need // 2)
+    ra = server.add_rig(a)
+    with pytest.raises(hm.HmError) as ei:
+        server.add_rig(b)
+    assert ei.value.code == HIP_ERROR_OUT_OF_MEMORY
+    assert server.stats()["n_rigs"] == 1
+    server.remove_rig(ra)
+    rb = server.add_rig(b)
+    assert server.stats()["device_used_bytes"] == b.device_bytes()
+    server.remove_rig(rb)
+    server.close()
+
+
+def test_server_request_errors(hm, torch_gpu, oracle):
+    """Bad submissions are refused; a request whose process fails reports its status and the
+    server goes on with the next one (the N-image class's errors are not sticky)."""
+    import hm_synth
+
+    torch = torch_gpu
+    rig = hm_synth.small_rig(n=3, seed=41, src=(120, 56), canvas=(330, 64))
+    pano = hm.StitchPanoN(1, 2, rig)
+    server = hm.Server(n_streams=2)
+    rid = server.add_rig(pano)
+    fr = frames_for(rig, 1, 3)
+    dev = to_dev(torch, fr)
+    good = pano.alloc_canvas()
+    bad = torch.zeros((1, 8, 8, 4), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    assert server.submit(rid, dev[:2], good) != 0  # wrong input count
+    assert server.submit(rid + 100, dev, good) != 0  # unknown rig
+    assert server.submit(rid, dev, bad, tag=1) == 0
+    assert server.submit(rid, dev, good, tag=2) == 0
+    server.wait_idle()
+    res = {r["tag"]: r for r in server.poll()}
+    assert res[1]["status"] != 0 and res[2]["status"] == 0
+    np.testing.assert_array_equal(good.cpu().numpy(), oracle.panoN(rig, fr, 2))
+    server.close()
