@@ -133,6 +133,142 @@ def kt_of(r):
     return r["kt"]
 
 
+# The mixed served load: 32 rigs drawn from 5 seeded configs (2 to 8 images,
+# 1920x1080 to 5312x2988 inputs, canvases 2000 to 14000 wide, levels 0/4/6).
+SERVED_CONFIGS = [
+    # name, class, (n, src_w, src_h, canvas_w, canvas_h, overlap, focal, seed), levels, rigs, input sets
+    ("2img_5312x2988_8500x3300_L6", "2", None, 6, 6, 2),
+    ("2img_1920x1080_2000x700_L4", "2", (2, 1920, 1080, 2000, 700, 0.25, 1300.0, 5), 4, 8, 2),
+    ("3img_2704x1520_6000x2000_L4", "3", (3, 2704, 1520, 6000, 2000, 0.25, 1500.0, 77), 4, 6, 2),
+    ("4img_1920x1080_4000x1500_L0", "N", (4, 1920, 1080, 4000, 1500, 0.25, 1100.0, 6), 0, 8, 2),
+    ("8img_3840x2160_14000x2600_L6", "N", (8, 3840, 2160, 14000, 2600, 0.25, 2000.0, 1234), 6, 4, 1),
+]
+
+
+def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_closed, seed=7):
+    """Plain (process + synchronize per frame, one frame per launch sequence) against served
+    (hmServer: rigs resident, concurrent streams, graph replay) on the mixed 32-rig load:
+    a closed loop (as fast as possible) for the throughput ratio, then Poisson arrivals at
+    30 fps per rig for `seconds` for p50/p99 frame latency."""
+    rng = np.random.default_rng(seed)
+    rigs = []  # dict(name, kind, pano, inputs=[[tensors]], canvases=[t, t], mpix)
+    for name, kind, geo, levels, count, n_sets in SERVED_CONFIGS:
+        log(f"served load: generating {name} ({count} rigs)")
+        rig = hm_synth.config_rig("cfg1") if geo is None else hm_synth.cylinder_rig(
+            geo[0], geo[1], geo[2], geo[3], geo[4], overlap=geo[5], focal=geo[6], seed=geo[7])
+        sets = []
+        for k in range(n_sets):
+            sets.append([torch.from_numpy(hm_synth.make_frame(rig.src_h, rig.src_w, 1000 * k + 17 * i)[None]).cuda()
+                         for i in range(rig.n)])
+        for _ in range(count):
+            if kind == "2":
+                pano = hm_stitch.StitchPano(1, levels, rig)
+            elif kind == "3":
+                pano = hm_stitch.StitchPano3(1, levels, rig)
+            else:
+                pano = hm_stitch.StitchPanoN(1, levels, rig)
+            assert pano.status == 0, (name, pano.status)
+            assert pano.prepare() == 0
+            w, h = pano.canvas_size()
+            rigs.append(dict(name=name, kind=kind, pano=pano, inputs=sets,
+                             canvases=[pano.alloc_canvas(), pano.alloc_canvas()], mpix=w * h / 1e6))
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+
+    def plain_one(r, k):
+        if r["kind"] == "2":
+            return r["pano"].process(r["inputs"][k % len(r["inputs"])][0], r["inputs"][k % len(r["inputs"])][1],
+                                     r["canvases"][k % 2], stream)
+        if r["kind"] == "3":
+            ins = r["inputs"][k % len(r["inputs"])]
+            return r["pano"].process(ins[0], ins[1], ins[2], r["canvases"][k % 2], stream=stream)
+        return r["pano"].process(r["inputs"][k % len(r["inputs"])], r["canvases"][k % 2], stream)
+
+    order = [(i % len(rigs), i // len(rigs)) for i in range(frames_closed)]
+    mpix_closed = sum(rigs[ri]["mpix"] for ri, _ in order)
+    # plain: warm-up, then one frame at a time
+    for ri, k in order[: len(rigs) * 2]:
+        hm_stitch.check(plain_one(rigs[ri], k), "plain process")
+    torch.cuda.synchronize()
+    per_cfg = {}
+    t0 = time.perf_counter()
+    for ri, k in order:
+        t1 = time.perf_counter()
+        hm_stitch.check(plain_one(rigs[ri], k), "plain process")
+        torch.cuda.synchronize()
+        per_cfg.setdefault(rigs[ri]["name"], []).append(time.perf_counter() - t1)
+    t_plain = time.perf_counter() - t0
+    plain_ref = [r["canvases"][0].clone() for r in rigs]  # each rig's canvas 0 after its last even frame
+    torch.cuda.synchronize()
+
+    server = hm_stitch.Server(n_streams=n_streams, device_budget_bytes=200 << 30, use_graphs=True)
+    ids = [server.add_rig(r["pano"]) for r in rigs]
+    used = server.stats()["device_used_bytes"]
+
+    def submit(ri, k, tag):
+        r = rigs[ri]
+        return server.submit(ids[ri], r["inputs"][k % len(r["inputs"])], r["canvases"][k % 2], tag=tag)
+
+    # served: warm-up (captures the graphs), then everything at once
+    for i, (ri, k) in enumerate(order[: len(rigs) * 4]):
+        hm_stitch.check(submit(ri, k, i), "submit")
+    hm_stitch.check(server.wait_idle(), "served warm-up")
+    server.poll(1 << 16)
+    t0 = time.perf_counter()
+    for i, (ri, k) in enumerate(order):
+        hm_stitch.check(submit(ri, k, i), "submit")
+    hm_stitch.check(server.wait_idle(), "served run")
+    t_served = time.perf_counter() - t0
+    server.poll(1 << 16)
+    torch.cuda.synchronize()
+    equal = all(bool(torch.equal(r["canvases"][0], ref)) for r, ref in zip(rigs, plain_ref))
+
+    # open loop: Poisson arrivals, 30 fps per rig
+    arrivals = []
+    for ri in range(len(rigs)):
+        t = float(rng.uniform(0, 1 / 30))
+        k = 0
+        while t < seconds:
+            arrivals.append((t, ri, k))
+            t += float(rng.exponential(1 / 30))
+            k += 1
+    arrivals.sort()
+    lat = []
+    t_start = time.perf_counter()
+    for tag, (t_arr, ri, k) in enumerate(arrivals):
+        while True:  # sleep (yielding the CPU to the scheduler thread) until the arrival time
+            dt = t_arr - (time.perf_counter() - t_start)
+            if dt <= 0:
+                break
+            time.sleep(dt if dt < 2e-4 else dt - 1e-4)
+        hm_stitch.check(submit(ri, k, tag), "submit")
+    hm_stitch.check(server.wait_idle(), "open loop")
+    t_open = time.perf_counter() - t_start
+    res = server.poll(len(arrivals) + 16)
+    lat = np.array([(r["t_done_ns"] - r["t_submit_ns"]) / 1e6 for r in res])
+    mpix_open = sum(rigs[ri]["mpix"] for _, ri, _ in arrivals)
+    st = server.stats()
+    server.close()
+    plain_fps = frames_closed / t_plain
+    served_fps = frames_closed / t_served
+    return {
+        "rigs": len(rigs), "configs": [c[0] + f" x{c[4]}" for c in SERVED_CONFIGS], "streams": n_streams,
+        "device_bytes_resident": int(used),
+        "closed_loop_frames": frames_closed,
+        "plain_fps": round(plain_fps, 1), "plain_Mpix_s": round(mpix_closed / t_plain, 1),
+        "served_fps": round(served_fps, 1), "served_Mpix_s": round(mpix_closed / t_served, 1),
+        "served_vs_plain": round(served_fps / plain_fps, 3),
+        "served_equals_plain_output": equal,
+        "plain_ms_per_frame": {k: round(1e3 * float(np.median(v)), 3) for k, v in per_cfg.items()},
+        "open_loop": {"seconds": seconds, "offered_fps": round(len(arrivals) / seconds, 1),
+                      "achieved_fps": round(len(res) / t_open, 1), "Mpix_s": round(mpix_open / t_open, 1),
+                      "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                      "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                      "latency_ms_max": round(float(lat.max()), 3)},
+        "graphs_captured": st["graphs_captured"], "graph_launches": st["graph_launches"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +277,11 @@ def main():
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-frames", type=int, default=200)
+    ap.add_argument("--served-seconds", type=float, default=5.0,
+                    help="open-loop duration of the 32-rig served load (0: skip the served measurement)")
+    ap.add_argument("--served-streams", type=int, default=4)
+    ap.add_argument("--served-frames", type=int, default=640, help="frames of the closed-loop served/plain runs")
+    ap.add_argument("--served-only", action="store_true", help="measure only the served load")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -169,6 +310,10 @@ def main():
                           "bytes_written": 4 * n, "launches": 5}))
         return
     hm_stitch.lib()
+    if args.served_only:
+        print(json.dumps(served_benchmark(torch, hm_stitch, hm_synth, args.served_seconds, args.served_streams,
+                                          args.served_frames)), flush=True)
+        return
     log("generating the 8500x3300 rig and frames")
     rig = hm_synth.config_rig("cfg1")
     f1 = hm_synth.make_frame(rig.src_h, rig.src_w, 101)[None]
@@ -277,6 +422,9 @@ def main():
                            "latency_ms_p99": round(float(np.percentile(r0["lat"], 99)), 4),
                            "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                                        for k, v in r0["kernels"].items()}}
+    if args.served_seconds > 0:
+        line["served"] = served_benchmark(torch, hm_stitch, hm_synth, args.served_seconds, args.served_streams,
+                                          args.served_frames)
     if not args.no_cpu:
         log("timing the CPU oracle on the same workload")
         line["cpu_baseline"] = cpu_baseline(rig, f1, f2, main_l)
