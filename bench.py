@@ -141,7 +141,7 @@ SERVED_CONFIGS = [
     ("2img_1920x1080_2000x700_L4", "2", (2, 1920, 1080, 2000, 700, 0.25, 1300.0, 5), 4, 8, 2),
     ("3img_2704x1520_6000x2000_L4", "3", (3, 2704, 1520, 6000, 2000, 0.25, 1500.0, 77), 4, 6, 2),
     ("4img_1920x1080_4000x1500_L0", "N", (4, 1920, 1080, 4000, 1500, 0.25, 1100.0, 6), 0, 8, 2),
-    ("8img_3840x2160_14000x2600_L6", "N", (8, 3840, 2160, 14000, 2600, 0.25, 2000.0, 1234), 6, 4, 1),
+    ("8img_3840x2160_14000x2600_L6", "N", (8, 3840, 2160, 14000, 2600, 0.25, 3500.0, 1234), 6, 4, 1),
 ]
 
 

@@ -108,6 +108,7 @@ struct UpQuadLaunch {
   size_t u8_pitch = 0;
   int u8_w = 0, u8_h = 0, ox = 0, oy = 0;
   int win_x = 0, win_y = 0, win_w = 0, win_h = 0;
+  hmdev::ImgRect z[hmdev::kMaxImages] = {};  // valid region of each image at this level
 };
 hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch& u, int batch, hipStream_t s);
 
@@ -134,7 +135,13 @@ class Pyramid {
   // whole reconstruction.
   hipError_t run_down(const void* const* g0, bool g0_u8, int from, hipStream_t s, LaunchTimer* timer);
   hipError_t run_up(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s, LaunchTimer* timer);
-  const hmdev::LevelDesc* levels_table() const { return level_table.as<hmdev::LevelDesc>(); }
+  // Sparse images: s0[i] is the level-0 region outside which image i's input
+  // is zero (an empty rectangle: zero everywhere).  Derives every level's
+  // support and valid region (hmdev::LevelDesc).  Without it (and always in
+  // debug mode, and for one or two images) the pyramid is dense.
+  hipError_t set_supports(const hmdev::ImgRect* s0);
+  hmdev::ImgRect S[hmdev::kMaxLevels][hmdev::kMaxImages] = {};  // supports
+  hmdev::ImgRect Z[hmdev::kMaxLevels][hmdev::kMaxImages] = {};  // valid regions
 
   int kind = 0, n = 0, c = 0, levels = 0, batch = 0;
   bool debug = false;
@@ -148,9 +155,12 @@ class Pyramid {
   DevBuf lap[hmdev::kMaxImages][hmdev::kMaxLevels]; // debug
   DevBuf blend[hmdev::kMaxLevels];                  // debug
   DevBuf r0_debug;                                  // debug copy of level-0 reconstruction
-  DevBuf level_table;                               // LevelDesc of levels 1..levels-1 (device)
+  std::vector<hmdev::LevelDesc> level_table;        // levels 1..levels-1 (host; launches copy what they need)
   size_t allocation_size = 0;
   int mask_channels() const { return kind == hmdev::kBlend2 ? 1 : n; }
+
+ private:
+  void build_level_table();
 };
 
 // Level sizes: the 2-image rule (src/cuda/cudaBlend.cu:860-866) or the

@@ -50,7 +50,7 @@ struct hmStitchPano {
   virtual hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int fused,
                                  hipStream_t s) = 0;
   size_t base_device_bytes() const {
-    size_t b = mask0.bytes + pyr.allocation_size + pyr.level_table.bytes;
+    size_t b = mask0.bytes + pyr.allocation_size;
     for (const auto& g : g0) b += g.bytes;
     return b;
   }

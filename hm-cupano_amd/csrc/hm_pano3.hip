@@ -102,6 +102,10 @@ hipError_t Pano3::ensure_pyramid() {
   HM_RETURN_IF_ERROR(hipMemcpy(labels.p, label_host.data(), label_host.size(), hipMemcpyHostToDevice));
   pyr.mask0_fast = labels.p;  // one-hot of the label: the float mask's values
   pyr.mask0_mode = kMask0Label;
+  // Each image's input is zero outside its box: its pyramid is sparse.
+  ImgRect s0[3];
+  for (int i = 0; i < 3; ++i) s0[i] = ImgRect{pos_x[i], pos_y[i], pos_x[i] + mw[i] - 1, pos_y[i] + mh[i] - 1};
+  HM_RETURN_IF_ERROR(pyr.set_supports(s0));
   if (bw < canvas_w) HM_RETURN_IF_ERROR(ensure_side());
   return hipSuccess;
 }
@@ -115,6 +119,7 @@ MultiGeom Pano3::geom() const {
     g.px[i] = pos_x[i];
     g.py[i] = pos_y[i];
     g.g0[i] = const_cast<uint8_t*>(g0[i].as<uint8_t>());
+    g.z0[i] = pyr.Z[0][i];
   }
   g.bw = bw;
   g.bh = bh;
@@ -129,9 +134,9 @@ hipError_t Pano3::process(const hmSurface* inputs, const hmSurface& canvas, bool
     if (!hm::surface_ok(inputs[i], channels, true)) return hipErrorInvalidValue;
   if (!hm::surface_ok(canvas, channels, true) || (int)canvas.width != canvas_w || (int)canvas.height != canvas_h)
     return hipErrorInvalidValue;
-  const MultiGeom g = geom();
   const int clear = channels == 4;  // RGBA canvases are cleared first (cudaPano3_optimized.inl:35-39)
   if (hard_seam) {
+    const MultiGeom g = geom();
     MultiRegion r;
     r.kernel = kMultiCanvas;
     r.x1 = canvas_w;
@@ -147,6 +152,7 @@ hipError_t Pano3::process(const hmSurface* inputs, const hmSurface& canvas, bool
     return e;
   }
   HM_RETURN_IF_ERROR(ensure_pyramid());
+  const MultiGeom g = geom();  // after ensure_pyramid: it carries the blend inputs' valid regions
   MultiRegion ir;
   ir.kernel = fused ? kMultiInputsDM : kMultiInputs;
   ir.x1 = bw;

@@ -171,6 +171,11 @@ hipError_t PanoN::ensure_pyramid() {
   HM_RETURN_IF_ERROR(hipMemcpy(labels.p, label_host.data(), label_host.size(), hipMemcpyHostToDevice));
   pyr.mask0_fast = labels.p;  // one-hot of the label: the float mask's values
   pyr.mask0_mode = kMask0Label;
+  // Each image's input is zero outside its box: its pyramid is sparse.
+  ImgRect s0[kMaxImages];
+  for (int i = 0; i < n_images; ++i)
+    s0[i] = ImgRect{pos_x[i] - bx0, pos_y[i] - by0, pos_x[i] + mw[i] - 1 - bx0, pos_y[i] + mh[i] - 1 - by0};
+  HM_RETURN_IF_ERROR(pyr.set_supports(s0));
   if (use_min) HM_RETURN_IF_ERROR(ensure_side());
   return hipSuccess;
 }
@@ -186,6 +191,7 @@ hipError_t PanoN::launch_region(const MultiRegion& r, const hmSurface* inputs, c
     g.px[i] = pos_x[i];
     g.py[i] = pos_y[i];
     g.g0[i] = g0[i].as<uint8_t>();
+    g.z0[i] = pyr.Z[0][i];
   }
   g.bw = bw;
   g.bh = bh;

@@ -64,9 +64,11 @@ struct MultiArgs {
   int mw[N], mh[N], px[N], py[N];
   int x0, x1, y0, y1;      // canvas region of this launch, [x0, x1) x [y0, y1)
   int tiles_x;
-  // blend inputs: uint8 bw x bh images whose origin is canvas (x0, y0)
+  // blend inputs: uint8 bw x bh images whose origin is canvas (x0, y0),
+  // each written only on its valid region z0[i] (blend-input coordinates)
   uint8_t* g0[N];
   int bw, bh;
+  ImgRect z0[N];
   // canvas
   uint8_t* canvas;
   uint32_t cpitch;
@@ -91,28 +93,32 @@ __global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
   const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
   const int b = blockIdx.z;
   if (Y >= a.y1) return;
+  const int v = Y - a.y0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
+    if (v < a.z0[i].y0 || v > a.z0[i].y1) continue;  // this image's input is not kept on this row
     const int ry = Y - a.py[i];
     const bool row_ok = ry >= 0 && ry < a.mh[i];
     const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
     uint32_t m[kPx];
-    bool in[kPx];
+    bool in[kPx], keep[kPx];
 #pragma unroll
     for (int j = 0; j < kPx; ++j) {
       const int X = X0 + j * (int)blockDim.x;
+      const int u = X - a.x0;
+      keep[j] = X < a.x1 && u >= a.z0[i].x0 && u <= a.z0[i].x1;
       const int rx = X - a.px[i];
-      in[j] = row_ok && rx >= 0 && rx < a.mw[i] && X < a.x1;
+      in[j] = keep[j] && row_ok && rx >= 0 && rx < a.mw[i];
       m[j] = in[j] ? mrow[rx] : 0u;
     }
     uint32_t r[kPx];
 #pragma unroll
     for (int j = 0; j < kPx; ++j) r[j] = in[j] ? (DM ? remap_px_dm<C>(a.src[i], b, m[j]) : remap_px<C>(a.src[i], b, m[j])) : 0u;
-    uint8_t* grow = a.g0[i] + ((size_t)b * a.bh + (Y - a.y0)) * a.bw * C;
+    uint8_t* grow = a.g0[i] + ((size_t)b * a.bh + v) * a.bw * C;
 #pragma unroll
     for (int j = 0; j < kPx; ++j) {
       const int X = X0 + j * (int)blockDim.x;
-      if (X < a.x1) store_px<C>(grow + (size_t)(X - a.x0) * C, r[j]);
+      if (keep[j]) store_px<C>(grow + (size_t)(X - a.x0) * C, r[j]);
     }
   }
 }
@@ -195,6 +201,7 @@ struct MultiGeom {
   int mw[kMaxImages] = {}, mh[kMaxImages] = {}, px[kMaxImages] = {}, py[kMaxImages] = {};
   uint8_t* g0[kMaxImages] = {};
   int bw = 0, bh = 0;
+  ImgRect z0[kMaxImages] = {};    // valid region of each blend input (blend-input coordinates)
   const uint8_t* seam = nullptr;  // canvas-sized labels
   int cw = 0, ch = 0;
 };
@@ -220,6 +227,7 @@ hipError_t launch_multi(const MultiGeom& g, const MultiRegion& r, int channels, 
     a.px[i] = g.px[i];
     a.py[i] = g.py[i];
     a.g0[i] = g.g0[i];
+    a.z0[i] = g.z0[i];
   }
   a.x0 = r.x0;
   a.x1 = r.x1;

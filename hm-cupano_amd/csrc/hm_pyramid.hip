@@ -172,11 +172,23 @@ static hipError_t down_fused_t(const void* const* g0, int w0, int h0, int n_out,
   a.w0 = w0;
   a.h0 = h0;
   a.n_out = n_out;
-  a.lv = lv;
+  for (int k = 0; k < 5; ++k) {
+    const LevelDesc& d = lv[k < n_out ? k : 0];
+    for (int i = 0; i < N; ++i) {
+      a.lv[k].g_out[i] = d.g_out[i];
+      a.lv[k].s[i] = d.s[i];
+      a.lv[k].z[i] = d.z[i];
+    }
+    a.lv[k].w = d.w;
+    a.lv[k].h = d.h;
+  }
   a.tiles_x = (w0 + kDownTile - 1) / kDownTile;
   a.tiles_y = (h0 + kDownTile - 1) / kDownTile;
-  hipLaunchKernelGGL((down_fused_kernel<T0, C, N>), dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch),
-                     dim3(256), 0, s, a);
+  const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+  if constexpr (N <= 2)
+    hipLaunchKernelGGL((down_fused_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((down_fused_each_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -271,6 +283,7 @@ hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch
       a.nqy = (a.win_y1 >> 1) - a.qy0 + 1;
       a.tiles_x = (a.nqx + 63) / 64;
       a.tiles_y = (a.nqy + 3) / 4;
+      for (int i = 0; i < kN; ++i) a.z[i] = u.z[i];
       hipLaunchKernelGGL((up_quad_kernel<T0, kC, kN, kK, decltype(M)::value>),
                          dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch), dim3(64, 4, 1), 0, s, a);
       return hipGetLastError();
@@ -383,22 +396,49 @@ hipError_t Pyramid::init(int kind_, int n_, int channels, int width, int height,
     HM_RETURN_IF_ERROR(r0_debug.alloc((size_t)W[0] * H[0] * c * batch * sizeof(float)));
     allocation_size += r0_debug.bytes;
   }
-  if (levels > 1) {
-    std::vector<LevelDesc> table(levels - 1);
-    for (int l = 1; l < levels; ++l) {
-      LevelDesc& d = table[l - 1];
-      for (int i = 0; i < kMaxImages; ++i) {
-        d.g[i] = i < n ? g[i][l].as<float>() : nullptr;
-        d.g_out[i] = i < n ? g[i][l].as<float>() : nullptr;
-      }
-      d.mask = mask[l].as<float>();
-      d.w = W[l];
-      d.h = H[l];
+  // dense until set_supports
+  for (int l = 0; l < levels; ++l)
+    for (int i = 0; i < kMaxImages; ++i) S[l][i] = Z[l][i] = ImgRect{0, 0, W[l] - 1, H[l] - 1};
+  build_level_table();
+  return hipSuccess;
+}
+
+void Pyramid::build_level_table() {
+  level_table.assign(levels > 1 ? levels - 1 : 0, LevelDesc{});
+  for (int l = 1; l < levels; ++l) {
+    LevelDesc& d = level_table[l - 1];
+    for (int i = 0; i < kMaxImages; ++i) {
+      d.g[i] = i < n ? g[i][l].as<float>() : nullptr;
+      d.g_out[i] = i < n ? g[i][l].as<float>() : nullptr;
+      d.s[i] = S[l][i];
+      d.z[i] = Z[l][i];
     }
-    HM_RETURN_IF_ERROR(level_table.alloc(table.size() * sizeof(LevelDesc)));
-    HM_RETURN_IF_ERROR(
-        hipMemcpy(level_table.p, table.data(), table.size() * sizeof(LevelDesc), hipMemcpyHostToDevice));
+    d.mask = mask[l].as<float>();
+    d.w = W[l];
+    d.h = H[l];
   }
+}
+
+hipError_t Pyramid::set_supports(const ImgRect* s0) {
+  // The debug path keeps every level dense, and so do one or two images (the
+  // 2-image quad path reads every image everywhere).
+  if (debug || n <= 2) return hipSuccess;
+  for (int i = 0; i < n; ++i) {
+    ImgRect s = s0[i];
+    for (int l = 0; l < levels; ++l) {
+      if (l > 0 && s.x1 >= s.x0 && s.y1 >= s.y0) s = ImgRect{s.x0 >> 1, s.y0 >> 1, s.x1 >> 1, s.y1 >> 1};
+      // clip to the level, then grow by 2 for the valid region
+      ImgRect c{std::max(s.x0, 0), std::max(s.y0, 0), std::min(s.x1, W[l] - 1), std::min(s.y1, H[l] - 1)};
+      const bool empty = c.x1 < c.x0 || c.y1 < c.y0;
+      S[l][i] = empty ? ImgRect{0, 0, -1, -1} : c;
+      Z[l][i] = empty ? ImgRect{0, 0, -1, -1}
+                      : ImgRect{std::max(c.x0 - 2, 0), std::max(c.y0 - 2, 0), std::min(c.x1 + 2, W[l] - 1),
+                                std::min(c.y1 + 2, H[l] - 1)};
+      s = c;
+      if (empty) s = ImgRect{0, 0, -1, -1};
+    }
+  }
+  build_level_table();
   return hipSuccess;
 }
 
@@ -415,7 +455,7 @@ hipError_t Pyramid::build_mask_pyramid(const float* mask0_, hipStream_t s) {
 
 hipError_t Pyramid::run_down(const void* const* g0, bool g0_u8, int from, hipStream_t s, LaunchTimer* timer) {
   // Up to five levels per launch from aligned tiles.
-  const LevelDesc* table = level_table.as<LevelDesc>();
+  const LevelDesc* table = level_table.data();
   while (from + 1 < levels) {
     const int n_out = std::min(5, levels - 1 - from);
     const void* in[kMaxImages];
@@ -483,6 +523,7 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
     u.win_y = fy0[k];
     u.win_w = fx1[k] - fx0[k] + 1;
     u.win_h = fy1[k] - fy0[k] + 1;
+    for (int i = 0; i < n; ++i) u.z[i] = Z[k][i];
     if (timer) timer->begin(s);
     HM_RETURN_IF_ERROR(launch_up_quad(c, n, kind, k == 0 && g0_u8, u, batch, s));
     if (timer) timer->end(s, k == 0 ? "up_level0" : "up_quad");

@@ -19,12 +19,31 @@
 
 namespace hmdev {
 
-// Per-level buffers of levels >= 1 (device-resident table).
+// Inclusive pixel rectangle; empty when x1 < x0 or y1 < y0.
+struct ImgRect {
+  int x0, y0, x1, y1;
+};
+__host__ __device__ __forceinline__ bool in_rect(const ImgRect& r, int x, int y) {
+  return x >= r.x0 && x <= r.x1 && y >= r.y0 && y <= r.y1;
+}
+
+// Per-level buffers of levels >= 1.
+//
+// Sparse images: outside its support s[i] an image's Gaussian level is
+// exactly zero (alpha 0), because its level-0 input is zero outside the
+// image's box and a 2x2 mean of zeros is zero.  The buffers hold valid values
+// only on z[i] (the support grown by 2 pixels): there the down pass writes
+// them (zero outside s[i]), and only there does the up pass read them; at a
+// pixel outside z[i] the image's Laplacian is exactly zero (its level value
+// and every upsample tap are zero), which the up pass uses without loading.
+// Dense pyramids have s[i] = z[i] = the whole level.
 struct LevelDesc {
   const float* g[kMaxImages];  // Gaussian level of each image
   float* g_out[kMaxImages];    // same buffers, writable (down pass)
   const float* mask;           // mask level, M interleaved channels
   int w, h;
+  ImgRect s[kMaxImages];       // support of each image
+  ImgRect z[kMaxImages];       // valid region of each image's buffer
 };
 
 // ---------------------------------------------------------------------------
@@ -117,12 +136,22 @@ __device__ __forceinline__ float4 blend_any(const float4* L, const float* mask, 
 // ---------------------------------------------------------------------------
 constexpr int kDownTile = 32;  // level-0 tile edge; up to 5 coarser levels per launch
 
+// One produced level, carried by value in the kernel arguments (constant
+// memory: uniform scalar loads, instead of vector loads of a global table the
+// kernel also writes through).
+template <int N>
+struct DownLevel {
+  float* g_out[N];
+  ImgRect s[N], z[N];
+  int w, h;
+};
+
 template <typename T0, int N>
 struct DownFusedArgs {
-  const T0* g0[N];          // level 0 (tight, batch-major)
+  const T0* g0[N];          // input level (tight, batch-major)
   int w0, h0;
   int n_out;                // levels produced (1..5)
-  const LevelDesc* lv;      // lv[0] describes the first produced level, lv[1] the next ...
+  DownLevel<N> lv[5];       // lv[0] describes the first produced level, lv[1] the next ...
   int tiles_x, tiles_y;
 };
 
@@ -136,7 +165,7 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
   const int tid = threadIdx.x;
   // ---- first produced level: one pixel per thread from level 0
   {
-    const LevelDesc& d = a.lv[0];
+    const DownLevel<N>& d = a.lv[0];
     const int lx = tid & 15, ly = tid >> 4;
     const int x = tx * 16 + lx, y = ty * 16 + ly;
     if (x < d.w && y < d.h) {
@@ -145,8 +174,12 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
       const size_t r0 = ((size_t)b * a.h0 + iy) * a.w0;
       const size_t r1 = r0 + a.w0;
       float4 p[N][4];
+      bool in_s[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
+        // outside the image's support the result is zero: no loads
+        in_s[i] = in_rect(d.s[i], x, y);
+        if (!in_s[i]) continue;
         p[i][0] = ld_px<C>(a.g0[i], r0 + ix);
         if (has_x1) p[i][1] = ld_px<C>(a.g0[i], r0 + ix + 1);
         if (has_y1) p[i][2] = ld_px<C>(a.g0[i], r1 + ix);
@@ -156,9 +189,9 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
       const size_t o = ((size_t)b * d.h + y) * d.w + x;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const float4 r = down_px<C>(p[i], in_range);
+        const float4 r = in_s[i] ? down_px<C>(p[i], in_range) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         s1[i][ly * 16 + lx] = r;
-        st_px<C>(d.g_out[i], o, r);
+        if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], o, r);
       }
     }
   }
@@ -167,8 +200,8 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
   float4(*src)[16 * 16] = s1;
   for (int k = 1; k < a.n_out; ++k) {
     __syncthreads();
-    const LevelDesc& din = a.lv[k - 1];
-    const LevelDesc& d = a.lv[k];
+    const DownLevel<N>& din = a.lv[k - 1];
+    const DownLevel<N>& d = a.lv[k];
     const int oedge = edge >> 1;
     const int x0 = tx * oedge, y0 = ty * oedge;
     float4 r[N];
@@ -191,8 +224,8 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
           if (has_x1) p[1] = (k == 1 ? src[i] : s2[i])[sy * edge + sx + 1];
           if (has_y1) p[2] = (k == 1 ? src[i] : s2[i])[(sy + 1) * edge + sx];
           if (has_x1 && has_y1) p[3] = (k == 1 ? src[i] : s2[i])[(sy + 1) * edge + sx + 1];
-          r[i] = down_px<C>(p, in_range);
-          st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r[i]);
+          r[i] = down_px<C>(p, in_range);  // zero outside the support: its LDS inputs are zero
+          if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r[i]);
         }
       }
     }
@@ -202,6 +235,89 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
       for (int i = 0; i < N; ++i) s2[i][ly * oedge + lx] = r[i];
     }
     edge = oedge;
+  }
+}
+
+// The same pyramid one image at a time, for three or more images: each image
+// runs the whole level chain through one set of LDS tiles, and a workgroup
+// skips the images whose valid region misses its tile at every produced
+// level (its buffers there are never read).  Fewer registers and less LDS
+// per workgroup than carrying every image at once.
+template <typename T0, int C, int N>
+__global__ void __launch_bounds__(256) down_fused_each_kernel(DownFusedArgs<T0, N> a) {
+  __shared__ float4 s1[16 * 16];
+  __shared__ float4 s2[8 * 8];
+  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
+  const int tx = t % a.tiles_x, ty = t / a.tiles_x;
+  const int b = blockIdx.z;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    bool touch = false;
+    for (int k = 0; k < a.n_out; ++k) {
+      const int e = 16 >> k;
+      const ImgRect& z = a.lv[k].z[i];
+      const int x0 = tx * e, y0 = ty * e;
+      if (x0 <= z.x1 && x0 + e - 1 >= z.x0 && y0 <= z.y1 && y0 + e - 1 >= z.y0) touch = true;
+    }
+    if (!touch) continue;  // uniform across the workgroup
+    __syncthreads();       // the previous image's LDS reads are done
+    {
+      const DownLevel<N>& d = a.lv[0];
+      const int lx = tid & 15, ly = tid >> 4;
+      const int x = tx * 16 + lx, y = ty * 16 + ly;
+      if (x < d.w && y < d.h) {
+        float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (in_rect(d.s[i], x, y)) {  // outside the support the result is zero: no loads
+          const int ix = 2 * x, iy = 2 * y;
+          const bool has_x1 = ix + 1 < a.w0, has_y1 = iy + 1 < a.h0;
+          const size_t r0 = ((size_t)b * a.h0 + iy) * a.w0;
+          const size_t r1 = r0 + a.w0;
+          float4 p[4];
+          p[0] = ld_px<C>(a.g0[i], r0 + ix);
+          if (has_x1) p[1] = ld_px<C>(a.g0[i], r0 + ix + 1);
+          if (has_y1) p[2] = ld_px<C>(a.g0[i], r1 + ix);
+          if (has_x1 && has_y1) p[3] = ld_px<C>(a.g0[i], r1 + ix + 1);
+          const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
+          r = down_px<C>(p, in_range);
+        }
+        s1[ly * 16 + lx] = r;
+        if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r);
+      }
+    }
+    int edge = 16;
+    for (int k = 1; k < a.n_out; ++k) {
+      __syncthreads();
+      const DownLevel<N>& din = a.lv[k - 1];
+      const DownLevel<N>& d = a.lv[k];
+      const int oedge = edge >> 1;
+      float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      bool active = false;
+      int lx = 0, ly = 0;
+      if (tid < oedge * oedge) {
+        lx = tid % oedge;
+        ly = tid / oedge;
+        const int x = tx * oedge + lx, y = ty * oedge + ly;
+        if (x < d.w && y < d.h) {
+          active = true;
+          const int gx = 2 * x, gy = 2 * y;
+          const bool has_x1 = gx + 1 < din.w, has_y1 = gy + 1 < din.h;
+          const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
+          const float4* src = k == 1 ? s1 : s2;
+          const int sx = 2 * lx, sy = 2 * ly;
+          float4 p[4];
+          p[0] = src[sy * edge + sx];
+          if (has_x1) p[1] = src[sy * edge + sx + 1];
+          if (has_y1) p[2] = src[(sy + 1) * edge + sx];
+          if (has_x1 && has_y1) p[3] = src[(sy + 1) * edge + sx + 1];
+          r = down_px<C>(p, in_range);  // zero outside the support: its LDS inputs are zero
+          if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r);
+        }
+      }
+      __syncthreads();
+      if (active) s2[ly * oedge + lx] = r;
+      edge = oedge;
+    }
   }
 }
 
@@ -244,6 +360,7 @@ struct UpQuadArgs {
   int u8_w, u8_h, ox, oy;
   int win_x0, win_y0, win_x1, win_y1;  // pixels produced: [x0, x1] x [y0, y1]
   int qx0, qy0, nqx, nqy;              // quads covering the window
+  ImgRect z[N];                        // valid region of each image at this level (see LevelDesc)
   int tiles_x, tiles_y;                // 64 x 4 quads per workgroup
 };
 
@@ -255,6 +372,134 @@ __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
   const int qy = a.qy0 + (t / a.tiles_x) * 4 + (int)threadIdx.y;
   const int b = blockIdx.z;
   if (qx >= a.qx0 + a.nqx || qy >= a.qy0 + a.nqy) return;
+  if constexpr (MODE == kMask0Label) {
+    // Level 0 of the N- and 3-image blends.  The mask is one-hot, so the
+    // blend (blendN_px / blend3_px with one weight 1 and the others 0, whose
+    // terms are all +-0) is the labelled image's Laplacian where that image
+    // is opaque, and elsewhere the alpha fallback: the first image of largest
+    // alpha (3 channels: the labelled image, or zero).  Only the images the
+    // result needs are loaded.
+    float4 rt[4];
+    const int qx1 = has_next ? min(qx + 1, a.w1 - 1) : 0, qy1 = has_next ? min(qy + 1, a.h1 - 1) : 0;
+    const size_t l0 = ((size_t)b * a.h1 + qy) * a.w1, l1 = ((size_t)b * a.h1 + qy1) * a.w1;
+    if (has_next) {
+      rt[0] = ld_px<C>(a.r1, l0 + qx);
+      rt[1] = ld_px<C>(a.r1, l0 + qx1);
+      rt[2] = ld_px<C>(a.r1, l1 + qx);
+      rt[3] = ld_px<C>(a.r1, l1 + qx1);
+    }
+    const uint8_t* labels = static_cast<const uint8_t*>(a.mask0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+      if (x < a.win_x0 || x > a.win_x1 || y < a.win_y0 || y > a.win_y1) continue;
+      const size_t pix = ((size_t)b * a.h0 + y) * a.w0 + x;
+      const int lab = labels[(size_t)y * a.w0 + x];
+      // Laplacian of image i at this pixel (zero outside its valid region)
+      auto lap = [&](int i, const T0* g0i, const float* g1i, const ImgRect& zi) -> float4 {
+        (void)i;
+        if (!in_rect(zi, x, y)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 g = ld_px<C>(g0i, pix);
+        if (!has_next) return g;
+        float4 t4[4];
+        t4[0] = ld_px<C>(g1i, l0 + qx);
+        t4[1] = ld_px<C>(g1i, l0 + qx1);
+        t4[2] = ld_px<C>(g1i, l1 + qx);
+        t4[3] = ld_px<C>(g1i, l1 + qx1);
+        const float4 up = up_quad_lap<C>(e, t4);
+        return make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+      };
+      float4 B = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      bool done = false;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        if (i != lab) continue;
+        const float4 Li = lap(i, a.g0[i], a.g1[i], a.z[i]);
+        if (C == 3 || Li.w != 0.0f) {
+          B = C == 4 ? Li : make_float4(Li.x, Li.y, Li.z, 0.0f);
+          done = true;
+        }
+      }
+      if constexpr (C == 4) {
+        if (!done) {
+          float amax = 0.0f;
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            const float4 Li = lap(i, a.g0[i], a.g1[i], a.z[i]);
+            if (Li.w > amax) {
+              amax = Li.w;
+              B = Li;
+            }
+          }
+        }
+      }
+      float4 R = B;
+      if (has_next) {
+        const float4 up = up_quad_rec<C>(e, rt);
+        R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+      }
+      if (a.f32_out) st_px<C>(a.f32_out, pix, R);
+      if (a.u8_out) {
+        const int X = x + a.ox, Y = y + a.oy;
+        if (X < a.u8_w && Y < a.u8_h)
+          st_px_u8<C>(a.u8_out + ((size_t)b * a.u8_h + Y) * a.u8_pitch + (size_t)X * C, R);
+      }
+    }
+    return;
+  }
+  if constexpr (N > 2) {
+    // Three or more images: one pixel of the quad at a time, each image's
+    // taps loaded where needed (cache hits across the quad), so registers
+    // hold one pixel's Laplacians rather than the quad's.
+    float4 rt[4];
+    const int qx1 = has_next ? min(qx + 1, a.w1 - 1) : 0, qy1 = has_next ? min(qy + 1, a.h1 - 1) : 0;
+    const size_t l0 = ((size_t)b * a.h1 + qy) * a.w1, l1 = ((size_t)b * a.h1 + qy1) * a.w1;
+    if (has_next) {
+      rt[0] = ld_px<C>(a.r1, l0 + qx);
+      rt[1] = ld_px<C>(a.r1, l0 + qx1);
+      rt[2] = ld_px<C>(a.r1, l1 + qx);
+      rt[3] = ld_px<C>(a.r1, l1 + qx1);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+      if (x < a.win_x0 || x > a.win_x1 || y < a.win_y0 || y > a.win_y1) continue;
+      const size_t pix = ((size_t)b * a.h0 + y) * a.w0 + x;
+      float4 Lv[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        Lv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (!in_rect(a.z[i], x, y)) continue;
+        const float4 g = ld_px<C>(a.g0[i], pix);
+        if (has_next) {
+          float4 t4[4];
+          t4[0] = ld_px<C>(a.g1[i], l0 + qx);
+          t4[1] = ld_px<C>(a.g1[i], l0 + qx1);
+          t4[2] = ld_px<C>(a.g1[i], l1 + qx);
+          t4[3] = ld_px<C>(a.g1[i], l1 + qx1);
+          const float4 up = up_quad_lap<C>(e, t4);
+          Lv[i] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+        } else {
+          Lv[i] = g;
+        }
+      }
+      const float4 B = blend_level0<C, N, KIND, MODE>(Lv, a.mask0, (size_t)y * a.w0 + x);
+      float4 R = B;
+      if (has_next) {
+        const float4 up = up_quad_rec<C>(e, rt);
+        R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+      }
+      if (a.f32_out) st_px<C>(a.f32_out, pix, R);
+      if (a.u8_out) {
+        const int X = x + a.ox, Y = y + a.oy;
+        if (X < a.u8_w && Y < a.u8_h)
+          st_px_u8<C>(a.u8_out + ((size_t)b * a.u8_h + Y) * a.u8_pitch + (size_t)X * C, R);
+      }
+    }
+    return;
+  }
+  // One or two images (dense): the quad's loads all issued before any
+  // arithmetic, the four level-(k+1) taps shared by the quad's pixels.
   float4 gt[N][4];
   float4 rt[4];
   if (has_next) {
@@ -272,7 +517,7 @@ __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
     rt[2] = ld_px<C>(a.r1, l1 + qx);
     rt[3] = ld_px<C>(a.r1, l1 + qx1);
   }
-  // level-0 pixels of the quad, all loaded before any arithmetic
+  // the level's pixels of the quad, all loaded before any arithmetic
   float4 g[4][N];
   bool in[4];
 #pragma unroll

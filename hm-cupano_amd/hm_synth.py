@@ -229,8 +229,8 @@ def config_rig(name: str, seed: int = 1234) -> Rig:
         return cylinder_rig(2, 5312, 2988, 8500, 3300, overlap=0.20, focal=2700.0, seed=seed)
     if name == "cfg2_n3":  # 3 x 3840x2160 -> 6200x2300
         return cylinder_rig(3, 3840, 2160, 6200, 2300, overlap=0.25, focal=2000.0, seed=seed)
-    if name == "cfg2_n8":  # 8 x 3840x2160 -> 14000x2600
-        return cylinder_rig(8, 3840, 2160, 14000, 2600, overlap=0.25, focal=2000.0, seed=seed)
+    if name == "cfg2_n8":  # 8 x 3840x2160 -> 14000x2600; focal such that 8 cameras at 25% overlap span < 360 deg
+        return cylinder_rig(8, 3840, 2160, 14000, 2600, overlap=0.25, focal=3500.0, seed=seed)
     raise KeyError(name)
 
 

@@ -293,7 +293,9 @@ int hmStitchPanoSetDebug(hmStitchPano* pano, int enable);
 int hmStitchPanoLevels(const hmStitchPano* pano, int* num_levels, int* widths, int* heights);
 /* Copy one level to host memory (synchronises the stream), the data that
  * dump_soft_blend_pyramid writes (cudaPano.inl:420-458).  Level-0 Gaussians
- * are converted from the resident uint8 form to float. */
+ * are converted from the resident uint8 form to float.  Complete in debug
+ * mode; otherwise the N- and 3-image Gaussian levels are kept only around
+ * each image's box (elsewhere they are zero and not stored). */
 int hmStitchPanoReadLevel(hmStitchPano* pano, int what, int image, int level, void* host_dst, size_t bytes,
                           hipStream_t stream);
 /* Per-launch event timing on the process stream. */
