@@ -82,13 +82,22 @@ def algorithmic_bytes(rig, levels, pano_levels):
     out["remap_canvas"] = int((sum(map_px) - roi_map) * 4 + (2 * src_bytes - roi_src) + (cw - (rx1 - rx0)) * ch * 4)
     # Gaussian levels >= 1 of both images from level 0
     out["down_fused"] = 2 * A[0] * 4 + 2 * coarse * 16
-    # reconstruction, one launch per level in 2x2 quads (per frame, all levels 1..L-1):
-    # the level's images and mask, the next level's images and R (taps), R out
-    quad = 0
-    for k in range(1, pano_levels):
-        quad += 2 * A[k] * 16 + A[k] * 4 + A[k] * 16
+    # reconstruction of levels 1..L-1: per level, the level's images and mask in, the next
+    # level's images and R (taps) in, R out.  Levels 2..L-1 run as one tile-fused launch
+    # ("up_tail": only R_2 leaves it) when there are more than three levels, level 1 (and
+    # the small levels otherwise) one launch each ("up_quad").
+    def level_bytes(k, r_out=True):
+        b = 2 * A[k] * 16 + A[k] * 4 + (A[k] * 16 if r_out else 0)
         if k + 1 < pano_levels:
-            quad += 2 * A[k + 1] * 16 + A[k + 1] * 16
+            b += 2 * A[k + 1] * 16 + A[k + 1] * 16
+        return b
+    tail_base = 2
+    if pano_levels - 1 > tail_base:
+        out["up_tail"] = sum(2 * A[k] * 16 + A[k] * 4 for k in range(tail_base, pano_levels)) + A[tail_base] * 16
+        quad_levels = range(1, tail_base)
+    else:
+        quad_levels = range(1, pano_levels)
+    quad = sum(level_bytes(k) for k in quad_levels)
     if quad:
         out["up_quad"] = quad
     # level 0: both level-0 images, the uint8 mask, level-1 taps and R_1 in, canvas out

@@ -112,6 +112,19 @@ struct UpQuadLaunch {
 };
 hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch& u, int batch, hipStream_t s);
 
+// The tail of the up pass for two images: levels kTailBase .. levels-1 in
+// one launch (hmdev::up_tail_kernel), giving R at kTailBase.
+constexpr int kTailBase = 2;
+struct UpTailLaunch {
+  int n = 0;  // tail levels; [0] = kTailBase
+  const float* g[hmdev::kMaxTail][2] = {};
+  const float* mask[hmdev::kMaxTail] = {};
+  int w[hmdev::kMaxTail] = {}, h[hmdev::kMaxTail] = {};
+  float* r_out = nullptr;
+  int wx0 = 0, wy0 = 0, wx1 = -1, wy1 = -1;  // base-level pixels produced
+};
+hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int batch, hipStream_t s);
+
 // Where the level-0 reconstruction goes.
 struct Level0Out {
   float* f32 = nullptr;   // tight float output (blend-context API)

@@ -245,6 +245,42 @@ static hipError_t dispatch_cnk(int c, int n, int kind, F&& f) {
   return hipErrorInvalidValue;
 }
 
+hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int batch, hipStream_t s) {
+  if (n != 2 || u.n < 2 || u.n > kMaxTail) return hipErrorInvalidValue;
+  if (u.wx1 < u.wx0 || u.wy1 < u.wy0) return hipSuccess;
+  UpTailArgs<2> a;
+  a.n = u.n;
+  for (int j = 0; j < kMaxTail; ++j) {
+    const int jj = j < u.n ? j : 0;
+    for (int i = 0; i < 2; ++i) a.lv[j].g[i] = u.g[jj][i];
+    a.lv[j].mask = u.mask[jj];
+    a.lv[j].w = u.w[jj];
+    a.lv[j].h = u.h[jj];
+  }
+  a.r_out = u.r_out;
+  a.wx0 = u.wx0;
+  a.wy0 = u.wy0;
+  a.wx1 = u.wx1;
+  a.wy1 = u.wy1;
+  a.tiles_x = (u.wx1 - u.wx0 + kTailTile) / kTailTile;
+  a.tiles_y = (u.wy1 - u.wy0 + kTailTile) / kTailTile;
+  const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+  if (c == 4) {
+    if (kind == kBlend2)
+      hipLaunchKernelGGL((up_tail_kernel<4, 2, kBlend2>), grid, dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((up_tail_kernel<4, 2, kBlendN>), grid, dim3(256), 0, s, a);
+  } else if (c == 3) {
+    if (kind == kBlend2)
+      hipLaunchKernelGGL((up_tail_kernel<3, 2, kBlend2>), grid, dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((up_tail_kernel<3, 2, kBlendN>), grid, dim3(256), 0, s, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch& u, int batch,
                             hipStream_t s) {
   if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
@@ -487,9 +523,32 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
     fx1[k] = std::min((fx1[k - 1] >> 1) + 1, W[k] - 1);
     fy1[k] = std::min((fy1[k - 1] >> 1) + 1, H[k] - 1);
   }
-  // One launch per level, coarsest first, one thread per 2x2 quad; level 0
-  // goes straight to the output.
-  for (int k = levels - 1; k >= 0; --k) {
+  // Two images: levels kTailBase .. L-1 in one tile-fused launch, giving R at
+  // kTailBase on the footprint the finer levels read.
+  int top = levels - 1;  // first level run in quads
+  if (n == 2 && levels - 1 > kTailBase && levels - kTailBase <= kMaxTail) {
+    UpTailLaunch t;
+    t.n = levels - kTailBase;
+    for (int j = 0; j < t.n; ++j) {
+      const int l = kTailBase + j;
+      for (int i = 0; i < n; ++i) t.g[j][i] = g[i][l].as<float>();
+      t.mask[j] = mask[l].as<float>();
+      t.w[j] = W[l];
+      t.h[j] = H[l];
+    }
+    t.r_out = r[kTailBase].as<float>();
+    t.wx0 = fx0[kTailBase];
+    t.wy0 = fy0[kTailBase];
+    t.wx1 = fx1[kTailBase];
+    t.wy1 = fy1[kTailBase];
+    if (timer) timer->begin(s);
+    HM_RETURN_IF_ERROR(launch_up_tail(c, n, kind, t, batch, s));
+    if (timer) timer->end(s, "up_tail");
+    top = kTailBase - 1;
+  }
+  // The other levels one launch each, coarsest first, one thread per 2x2
+  // quad; level 0 goes straight to the output.
+  for (int k = top; k >= 0; --k) {
     const bool has_next = k + 1 < levels;
     UpQuadLaunch u;
     for (int i = 0; i < n; ++i) {

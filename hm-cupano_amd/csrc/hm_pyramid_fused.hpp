@@ -345,6 +345,127 @@ __device__ __forceinline__ float4 blend_level0(const float4* L, const void* mask
 }
 
 // ---------------------------------------------------------------------------
+// the small tail of the up pass, one launch (one or two images)
+// ---------------------------------------------------------------------------
+// Levels base .. L-1 are small (1/16 of level 0 and less).  One per-level
+// launch each would be a chain of latency-bound launches; instead one
+// workgroup reconstructs a kTailTile x kTailTile tile of the base level,
+// going from the coarsest level down over the tile's footprint at each level
+// (plus the one-pixel halo of the top-left aligned upsample), with R of the
+// level below the current one in LDS.  Only R_base is written.  Every level's
+// pixels are evaluated in 2x2 quads sharing their taps.
+constexpr int kTailTile = 32;
+constexpr int kTailRegion = kTailTile / 2 + 2;  // footprint edge at the levels above the base
+constexpr int kMaxTail = 15;
+
+template <int N>
+struct TailLevel {
+  const float* g[N];  // Gaussian level of each image
+  const float* mask;  // mask level, M interleaved channels
+  int w, h;
+};
+
+template <int N>
+struct UpTailArgs {
+  int n;                        // levels in the tail (>= 2): lv[0] = base ... lv[n-1] = coarsest
+  TailLevel<N> lv[kMaxTail];
+  float* r_out;                 // R of the base level (tight)
+  int wx0, wy0, wx1, wy1;       // base-level pixels produced
+  int tiles_x, tiles_y;
+};
+
+template <int C, int N, int KIND>
+__global__ void __launch_bounds__(256) up_tail_kernel(UpTailArgs<N> a) {
+  __shared__ float4 buf[2][kTailRegion * kTailRegion];
+  __shared__ int reg[kMaxTail][4];  // footprint [x0, x1] x [y0, y1] per tail level
+  const int t = (int)blockIdx.x;
+  const int tx = t % a.tiles_x, ty = t / a.tiles_x;
+  const int b = blockIdx.z;
+  const int tid = threadIdx.x;
+  const int n = a.n;
+  if (tid == 0) {
+    int x0 = a.wx0 + tx * kTailTile, y0 = a.wy0 + ty * kTailTile;
+    int x1 = min(x0 + kTailTile - 1, a.wx1), y1 = min(y0 + kTailTile - 1, a.wy1);
+    reg[0][0] = x0; reg[0][1] = x1; reg[0][2] = y0; reg[0][3] = y1;
+    for (int j = 1; j < n; ++j) {
+      x0 >>= 1;
+      y0 >>= 1;
+      x1 = min((x1 >> 1) + 1, a.lv[j].w - 1);
+      y1 = min((y1 >> 1) + 1, a.lv[j].h - 1);
+      reg[j][0] = x0; reg[j][1] = x1; reg[j][2] = y0; reg[j][3] = y1;
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  // coarsest level: R = blend of the Gaussians
+  {
+    const TailLevel<N>& d = a.lv[n - 1];
+    const int rx0 = reg[n - 1][0], ry0 = reg[n - 1][2];
+    const int rw = reg[n - 1][1] - rx0 + 1, rh = reg[n - 1][3] - ry0 + 1;
+    for (int idx = tid; idx < rw * rh; idx += blockDim.x) {
+      const int x = rx0 + idx % rw, y = ry0 + idx / rw;
+      const size_t pix = ((size_t)b * d.h + y) * d.w + x;
+      float4 Lv[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) Lv[i] = ld_px<C>(d.g[i], pix);
+      buf[cur][idx] = blend_any<C, N, KIND>(Lv, d.mask, (size_t)y * d.w + x);
+    }
+  }
+  // levels n-2 .. 0 in quads: taps of G from global memory, of R from LDS
+  for (int j = n - 2; j >= 0; --j) {
+    __syncthreads();
+    cur ^= 1;
+    const TailLevel<N>& d = a.lv[j];
+    const TailLevel<N>& dn = a.lv[j + 1];
+    const int rx0 = reg[j][0], rx1 = reg[j][1], ry0 = reg[j][2], ry1 = reg[j][3];
+    const int rw = rx1 - rx0 + 1;
+    const int qx0 = rx0 >> 1, qy0 = ry0 >> 1;
+    const int nqx = (rx1 >> 1) - qx0 + 1, nqy = (ry1 >> 1) - qy0 + 1;
+    const int nx0 = reg[j + 1][0], ny0 = reg[j + 1][2], nrw = reg[j + 1][1] - nx0 + 1;
+    const float4* prev = buf[cur ^ 1];
+    for (int q = tid; q < nqx * nqy; q += blockDim.x) {
+      const int qx = qx0 + q % nqx, qy = qy0 + q / nqx;
+      const int qx1 = min(qx + 1, dn.w - 1), qy1 = min(qy + 1, dn.h - 1);
+      const size_t l0 = ((size_t)b * dn.h + qy) * dn.w, l1 = ((size_t)b * dn.h + qy1) * dn.w;
+      float4 gt[N][4];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        gt[i][0] = ld_px<C>(dn.g[i], l0 + qx);
+        gt[i][1] = ld_px<C>(dn.g[i], l0 + qx1);
+        gt[i][2] = ld_px<C>(dn.g[i], l1 + qx);
+        gt[i][3] = ld_px<C>(dn.g[i], l1 + qx1);
+      }
+      float4 rt[4];
+      const int lx = qx - nx0, lx1 = qx1 - nx0, ly = qy - ny0, ly1 = qy1 - ny0;
+      rt[0] = prev[ly * nrw + lx];
+      rt[1] = prev[ly * nrw + lx1];
+      rt[2] = prev[ly1 * nrw + lx];
+      rt[3] = prev[ly1 * nrw + lx1];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+        if (x < rx0 || x > rx1 || y < ry0 || y > ry1) continue;
+        const size_t pix = ((size_t)b * d.h + y) * d.w + x;
+        float4 Lv[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const float4 g = ld_px<C>(d.g[i], pix);
+          const float4 up = up_quad_lap<C>(e, gt[i]);
+          Lv[i] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+        }
+        const float4 B = blend_any<C, N, KIND>(Lv, d.mask, (size_t)y * d.w + x);
+        const float4 up = up_quad_rec<C>(e, rt);
+        const float4 R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+        if (j == 0)
+          st_px<C>(a.r_out, pix, R);
+        else
+          buf[cur][(y - ry0) * rw + (x - rx0)] = R;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // one level in 2x2 quads (fields named as level 0 and its next level 1)
 // ---------------------------------------------------------------------------
 template <typename T0, int N>
