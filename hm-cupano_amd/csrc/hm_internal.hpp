@@ -153,6 +153,7 @@ class Pyramid {
   // support and valid region (hmdev::LevelDesc).  Without it (and always in
   // debug mode, and for one or two images) the pyramid is dense.
   hipError_t set_supports(const hmdev::ImgRect* s0);
+  int tail_base = kTailBase;  // two images: levels tail_base.. in one launch (0: never)
   hmdev::ImgRect S[hmdev::kMaxLevels][hmdev::kMaxImages] = {};  // supports
   hmdev::ImgRect Z[hmdev::kMaxLevels][hmdev::kMaxImages] = {};  // valid regions
 

@@ -27,6 +27,14 @@ struct hmStitchPano {
 
   hm::LaunchTimer* timer_or_null() { return timer.enabled() ? &timer : nullptr; }
 
+  // Launch-shape knobs (hmTuning); every setting computes the same output.
+  hmTuning tuning{4, 2, 1};
+  // Which knobs matter for this object (the autotuner sweeps only those).
+  virtual bool tunes_remap_px() const { return false; }
+  bool tunes_tail() const { return !hard_seam && n_images == 2 && num_levels >= 4; }
+  bool tunes_overlap() const { return side != nullptr; }
+  void apply_tuning() { pyr.tail_base = tuning.tail_base; }
+
   // Work that does not feed the blend (the canvas outside the written-back
   // region) runs on a second stream, forked from and joined back into the
   // caller's stream, so it overlaps the pyramid's latency-bound launches.

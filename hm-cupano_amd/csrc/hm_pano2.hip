@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "hm_pano.hpp"
@@ -27,6 +28,25 @@ using namespace hmdev;
 namespace {
 
 constexpr int kOverlapPad = 128;  // CanvasManager overlap_pad default (canvasManager.h:65)
+// Calls f(C, PX) with std::integral_constant arguments for the channel count
+// and the remap's pixels per thread (a tuning knob: 2, 4 or 8).
+template <typename F>
+void dispatch_c_px(int channels, int px, F&& f) {
+  using std::integral_constant;
+  auto per_c = [&](auto CV) {
+    if (px == 2)
+      f(CV, integral_constant<int, 2>{});
+    else if (px == 8)
+      f(CV, integral_constant<int, 8>{});
+    else
+      f(CV, integral_constant<int, 4>{});
+  };
+  if (channels == 4)
+    per_c(integral_constant<int, 4>{});
+  else
+    per_c(integral_constant<int, 3>{});
+}
+
 struct Pano2L0 {
   Surf src[2];
   const uint32_t* map[2];
@@ -40,7 +60,7 @@ struct Pano2L0 {
   int d[2], ulo[2], uhi[2];  // G_i[u] <- canvas column u + d[i] for u in [ulo[i], uhi[i])
   int g2_canvas;             // 1: G2 takes the canvas after both remaps; 0: image 2 alone
   // Canvas column spans of one launch, [span_x0[k], span_x1[k]), cut into
-  // span_tiles[k] tiles of kBlock.x * kPx columns.
+  // span_tiles[k] tiles of kBlock.x * PX columns.
   int span_x0[2], span_x1[2], span_tiles[2];
   int tiles_x, tiles_y;  // tiles_x = span_tiles[0] + span_tiles[1]
 };
@@ -50,41 +70,41 @@ struct Pano2L0 {
 // G2 = canvas after remap 2, which overwrites image 2's whole box (the
 // make_full copies, cudaPano.inl:225-236, 306-317), or image 2 alone when the
 // blend region is the whole canvas.
-template <int C>
+template <int C, int PX>
 __global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
   const int t = (int)blockIdx.x;
   const int txa = t % a.tiles_x;
   const int k = txa < a.span_tiles[0] ? 0 : 1;
   const int tx = txa - (k ? a.span_tiles[0] : 0);
-  const int X0 = a.span_x0[k] + tx * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int X0 = a.span_x0[k] + tx * (int)blockDim.x * PX + (int)threadIdx.x;
   const int Xend = a.span_x1[k];
   const int Y = (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
   const int b = blockIdx.z;
   if (Y >= a.ch) return;
-  uint32_t m[2][kPx];
-  bool valid[2][kPx];
+  uint32_t m[2][PX];
+  bool valid[2][PX];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int ry = Y - a.py[i];
     const bool row_ok = ry >= 0 && ry < a.mh[i];
     const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
 #pragma unroll
-    for (int j = 0; j < kPx; ++j) {
+    for (int j = 0; j < PX; ++j) {
       const int rx = X0 + j * (int)blockDim.x - a.px[i];
       valid[i][j] = row_ok && rx >= 0 && rx < a.mw[i] && X0 + j * (int)blockDim.x < Xend;
       m[i][j] = valid[i][j] ? mrow[rx] : 0u;
     }
   }
-  uint32_t r[2][kPx];
+  uint32_t r[2][PX];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < kPx; ++j) r[i][j] = valid[i][j] ? remap_px<C>(a.src[i], b, m[i][j]) : 0u;
+    for (int j = 0; j < PX; ++j) r[i][j] = valid[i][j] ? remap_px<C>(a.src[i], b, m[i][j]) : 0u;
   uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
   uint8_t* g0row = a.g0[0] + ((size_t)b * a.bh + Y) * a.bw * C;
   uint8_t* g1row = a.g0[1] + ((size_t)b * a.bh + Y) * a.bw * C;
 #pragma unroll
-  for (int j = 0; j < kPx; ++j) {
+  for (int j = 0; j < PX; ++j) {
     const int X = X0 + j * (int)blockDim.x;
     if (X >= Xend) continue;
     const uint32_t c1 = valid[0][j] ? r[0][j] : 0u;
@@ -113,19 +133,19 @@ struct Pano2Hard {
 
 // Hard seam: the two dest-map remaps of cudaPano.inl:262-276 and :345-359
 // (image 1 where seam == 1, image 2 where seam == 0) plus the canvas memset,
-// as one pass; kPx pixels per thread as above.
-template <int C>
+// as one pass; PX pixels per thread as above.
+template <int C, int PX>
 __global__ void __launch_bounds__(256) pano2_hard_kernel(Pano2Hard a) {
   const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
-  const int X0 = (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int X0 = (t % a.tiles_x) * (int)blockDim.x * PX + (int)threadIdx.x;
   const int Y = (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
   const int b = blockIdx.z;
   if (Y >= a.ch) return;
   const uint8_t* srow = a.seam + (size_t)Y * a.cw;
-  uint32_t m[kPx];
-  int sel[kPx];  // 0: image 1, 1: image 2, -1: nothing
+  uint32_t m[PX];
+  int sel[PX];  // 0: image 1, 1: image 2, -1: nothing
 #pragma unroll
-  for (int j = 0; j < kPx; ++j) {
+  for (int j = 0; j < PX; ++j) {
     const int X = X0 + j * (int)blockDim.x;
     sel[j] = -1;
     m[j] = 0u;
@@ -144,15 +164,15 @@ __global__ void __launch_bounds__(256) pano2_hard_kernel(Pano2Hard a) {
       m[j] = (first ? a.map[0] : a.map[1])[(size_t)ry * mw + rx];
     }
   }
-  uint32_t v[kPx];
+  uint32_t v[PX];
 #pragma unroll
-  for (int j = 0; j < kPx; ++j) {
+  for (int j = 0; j < PX; ++j) {
     v[j] = 0u;
     if (sel[j] >= 0) v[j] = remap_px_dm<C>(sel[j] == 0 ? a.src[0] : a.src[1], b, m[j]);
   }
   uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
 #pragma unroll
-  for (int j = 0; j < kPx; ++j) {
+  for (int j = 0; j < PX; ++j) {
     const int X = X0 + j * (int)blockDim.x;
     if (X < a.cw) store_px<C>(crow + (size_t)X * C, v[j]);
   }
@@ -181,6 +201,7 @@ struct Pano2 : hmStitchPano {
   hipError_t setup(const hmControlMasks2& m, int minimize_blend);
   hipError_t ensure_pyramid();
   hipError_t prepare() override { return ensure_pyramid(); }
+  bool tunes_remap_px() const override { return true; }
   hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int,
                          hipStream_t s) override {
     if (status != hipSuccess) return (hipError_t)status;
@@ -298,7 +319,8 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   if ((int)canvas.width != canvas_w || (int)canvas.height != canvas_h) return hipErrorInvalidValue;
   const Surf s1 = to_surf(in1), s2 = to_surf(in2);
   if (hard_seam) {
-    const hm::TileGrid tg = hm::tile_grid(canvas_w, canvas_h, batch, dim3(kBlock.x * kPx, kBlock.y, 1));
+    const int px = tuning.remap_px;
+    const hm::TileGrid tg = hm::tile_grid(canvas_w, canvas_h, batch, dim3(kBlock.x * px, kBlock.y, 1));
     const dim3 grid = tg.grid;
     Pano2Hard a{};
     a.src[0] = s1;
@@ -320,10 +342,10 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     a.tiles_x = tg.tiles_x;
     a.tiles_y = tg.tiles_y;
     timer.begin(s);
-    if (channels == 4)
-      hipLaunchKernelGGL(pano2_hard_kernel<4>, grid, kBlock, 0, s, a);
-    else
-      hipLaunchKernelGGL(pano2_hard_kernel<3>, grid, kBlock, 0, s, a);
+    auto go = [&](auto CV, auto PV) {
+      hipLaunchKernelGGL((pano2_hard_kernel<decltype(CV)::value, decltype(PV)::value>), grid, kBlock, 0, s, a);
+    };
+    dispatch_c_px(channels, px, go);
     timer.end(s, "hard_seam");
     return hipGetLastError();
   }
@@ -347,7 +369,8 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   a.ch = canvas_h;
   a.bw = bw;
   a.bh = bh;
-  const int tile_w = (int)kBlock.x * kPx;
+  const int px = tuning.remap_px;
+  const int tile_w = (int)kBlock.x * px;
   a.tiles_y = (canvas_h + (int)kBlock.y - 1) / (int)kBlock.y;
   auto launch_spans = [&](int x0a, int x1a, int x0b, int x1b, hipStream_t st, const char* name) -> hipError_t {
     Pano2L0 k = a;
@@ -361,10 +384,10 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     if (k.tiles_x == 0) return hipSuccess;
     const dim3 grid((unsigned)(k.tiles_x * k.tiles_y), 1u, (unsigned)batch);
     timer.begin(st);
-    if (channels == 4)
-      hipLaunchKernelGGL(pano2_level0_kernel<4>, grid, kBlock, 0, st, k);
-    else
-      hipLaunchKernelGGL(pano2_level0_kernel<3>, grid, kBlock, 0, st, k);
+    auto go = [&](auto CV, auto PV) {
+      hipLaunchKernelGGL((pano2_level0_kernel<decltype(CV)::value, decltype(PV)::value>), grid, kBlock, 0, st, k);
+    };
+    dispatch_c_px(channels, px, go);
     timer.end(st, name);
     return hipGetLastError();
   };
@@ -372,6 +395,10 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   // them.  The canvas remap forks after them, so it does not delay them.
   HM_RETURN_IF_ERROR(launch_spans(roi_x0, roi_x1, 0, 0, s, "remap_roi"));
   if (!side) return run_blend(canvas, s);
+  if (!tuning.overlap) {  // everything on the caller's stream
+    HM_RETURN_IF_ERROR(launch_spans(0, roi_x0, roi_x1, canvas_w, s, "remap_canvas"));
+    return run_blend(canvas, s);
+  }
   HM_RETURN_IF_ERROR(fork(s));
   const hipError_t e1 = launch_spans(0, roi_x0, roi_x1, canvas_w, side, "remap_canvas");
   const hipError_t e2 = e1 == hipSuccess ? run_blend(canvas, s) : e1;

@@ -161,6 +161,7 @@ hipError_t Pano3::process(const hmSurface* inputs, const hmSurface& canvas, bool
   HM_RETURN_IF_ERROR(launch_multi<3>(g, ir, channels, inputs, canvas, batch, s));
   timer.end(s, "remap_inputs");
   const bool extra = bw < canvas_w;
+  const bool forked = extra && tuning.overlap;
   hipError_t e_side = hipSuccess;
   if (extra) {
     // Columns the blend does not write back: cleared (RGBA), and on the
@@ -172,10 +173,14 @@ hipError_t Pano3::process(const hmSurface* inputs, const hmSurface& canvas, bool
     cr.y1 = canvas_h;
     cr.mode = fused ? kCanvasClear : kCanvasLast;
     cr.uncovered_zero = clear;
-    HM_RETURN_IF_ERROR(fork(s));
-    timer.begin(side);
-    e_side = launch_multi<3>(g, cr, channels, inputs, canvas, batch, side);
-    timer.end(side, "remap_canvas");
+    hipStream_t st = s;
+    if (forked) {
+      HM_RETURN_IF_ERROR(fork(s));
+      st = side;
+    }
+    timer.begin(st);
+    e_side = launch_multi<3>(g, cr, channels, inputs, canvas, batch, st);
+    timer.end(st, "remap_canvas");
   }
   const void* g0p[3] = {g0[0].p, g0[1].p, g0[2].p};
   hm::Level0Out out;
@@ -184,7 +189,7 @@ hipError_t Pano3::process(const hmSurface* inputs, const hmSurface& canvas, bool
   out.u8_w = canvas_w;
   out.u8_h = canvas_h;
   const hipError_t e = e_side == hipSuccess ? pyr.run(g0p, true, out, s, timer_or_null()) : e_side;
-  if (!extra) return e;
+  if (!forked) return e;
   // Join after the pyramid's launches (and after an error too).
   const hipError_t j = join(s);
   return e != hipSuccess ? e : j;

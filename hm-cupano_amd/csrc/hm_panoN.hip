@@ -229,13 +229,18 @@ hipError_t PanoN::process(const hmSurface* inputs, int n_inputs, const hmSurface
     ir.y1 = by0 + bh;
     HM_RETURN_IF_ERROR(launch_region<N>(ir, inputs, canvas, s, "remap_inputs"));
     hipError_t e_side = hipSuccess;
+    const bool forked = use_min && tuning.overlap;
     if (use_min) {  // outside the write ROI, which the blend writes back
       cr.skip_x = write_roi.x;
       cr.skip_y = write_roi.y;
       cr.skip_w = write_roi.w;
       cr.skip_h = write_roi.h;
-      HM_RETURN_IF_ERROR(fork(s));
-      e_side = launch_region<N>(cr, inputs, canvas, side, "remap_canvas");
+      if (forked) {
+        HM_RETURN_IF_ERROR(fork(s));
+        e_side = launch_region<N>(cr, inputs, canvas, side, "remap_canvas");
+      } else {
+        HM_RETURN_IF_ERROR(launch_region<N>(cr, inputs, canvas, s, "remap_canvas"));
+      }
     }
     const void* g[kMaxImages];
     for (int i = 0; i < n_images; ++i) g[i] = g0[i].p;
@@ -253,7 +258,7 @@ hipError_t PanoN::process(const hmSurface* inputs, int n_inputs, const hmSurface
       out.win_h = write_roi.h;
     }
     const hipError_t e = e_side == hipSuccess ? pyr.run(g, true, out, s, timer_or_null()) : e_side;
-    if (!use_min) return e;
+    if (!forked) return e;
     // Join after the pyramid's launches (and after an error too).
     const hipError_t j = join(s);
     return e != hipSuccess ? e : j;

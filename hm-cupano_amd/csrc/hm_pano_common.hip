@@ -104,6 +104,68 @@ extern "C" int hmStitchPanoDeviceBytes(const hmStitchPano* pano, size_t* bytes) 
   return hipSuccess;
 }
 
+extern "C" int hmStitchPanoGetTuning(const hmStitchPano* pano, hmTuning* tuning) {
+  if (!pano || !tuning) return hipErrorInvalidValue;
+  *tuning = pano->tuning;
+  return hipSuccess;
+}
+
+static bool tuning_ok(const hmTuning& t) {
+  return (t.remap_px == 2 || t.remap_px == 4 || t.remap_px == 8) && t.tail_base >= 0 && t.tail_base <= 3 &&
+         (t.overlap == 0 || t.overlap == 1);
+}
+
+extern "C" int hmStitchPanoSetTuning(hmStitchPano* pano, const hmTuning* tuning) {
+  if (!pano || !tuning || !tuning_ok(*tuning)) return hipErrorInvalidValue;
+  pano->tuning = *tuning;
+  pano->apply_tuning();
+  return hipSuccess;
+}
+
+extern "C" int hmStitchPanoAutotune(hmStitchPano* pano, const hmSurface* inputs, int num_inputs, hmSurface canvas,
+                                    int fused, int frames, hipStream_t stream, hmTuning* best,
+                                    float* best_ms_per_frame) {
+  if (!pano || !inputs || frames < 1) return hipErrorInvalidValue;
+  if (pano->status != hipSuccess) return pano->status;
+  HM_RETURN_IF_ERROR(pano->prepare());  // allocations and streams first: they decide the knobs
+  const hmTuning start = pano->tuning;
+  std::vector<int> pxs = pano->tunes_remap_px() ? std::vector<int>{2, 4, 8} : std::vector<int>{start.remap_px};
+  std::vector<int> tbs = pano->tunes_tail() ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{start.tail_base};
+  std::vector<int> ovs = pano->tunes_overlap() ? std::vector<int>{0, 1} : std::vector<int>{start.overlap};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HM_RETURN_IF_ERROR(hipEventCreate(&e0));
+  hipError_t err = hipEventCreate(&e1);
+  hmTuning chosen = start;
+  float chosen_ms = -1.0f;
+  for (int px : pxs)
+    for (int tb : tbs)
+      for (int ov : ovs) {
+        if (err != hipSuccess) break;
+        pano->tuning = hmTuning{px, tb, ov};
+        pano->apply_tuning();
+        err = pano->process_any(inputs, num_inputs, canvas, fused, stream);  // warm-up
+        if (err == hipSuccess) err = hipEventRecord(e0, stream);
+        for (int f = 0; f < frames && err == hipSuccess; ++f)
+          err = pano->process_any(inputs, num_inputs, canvas, fused, stream);
+        if (err == hipSuccess) err = hipEventRecord(e1, stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (err == hipSuccess && (chosen_ms < 0.0f || ms / frames < chosen_ms)) {
+          chosen_ms = ms / frames;
+          chosen = pano->tuning;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  pano->tuning = err == hipSuccess ? chosen : start;
+  pano->apply_tuning();
+  if (err != hipSuccess) return err;
+  if (best) *best = chosen;
+  if (best_ms_per_frame) *best_ms_per_frame = chosen_ms;
+  return hipSuccess;
+}
+
 extern "C" int hmStitchPanoCanvasSize(const hmStitchPano* pano, int* width, int* height) {
   if (!pano || !width || !height) return hipErrorInvalidValue;
   *width = pano->canvas_w;

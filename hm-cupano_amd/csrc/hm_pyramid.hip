@@ -523,28 +523,29 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
     fx1[k] = std::min((fx1[k - 1] >> 1) + 1, W[k] - 1);
     fy1[k] = std::min((fy1[k - 1] >> 1) + 1, H[k] - 1);
   }
-  // Two images: levels kTailBase .. L-1 in one tile-fused launch, giving R at
-  // kTailBase on the footprint the finer levels read.
+  // Two images: levels tail_base .. L-1 in one tile-fused launch, giving R at
+  // tail_base on the footprint the finer levels read.
   int top = levels - 1;  // first level run in quads
-  if (n == 2 && levels - 1 > kTailBase && levels - kTailBase <= kMaxTail) {
+  const int tb = tail_base;
+  if (n == 2 && tb > 0 && levels - 1 > tb && levels - tb <= kMaxTail) {
     UpTailLaunch t;
-    t.n = levels - kTailBase;
+    t.n = levels - tb;
     for (int j = 0; j < t.n; ++j) {
-      const int l = kTailBase + j;
+      const int l = tb + j;
       for (int i = 0; i < n; ++i) t.g[j][i] = g[i][l].as<float>();
       t.mask[j] = mask[l].as<float>();
       t.w[j] = W[l];
       t.h[j] = H[l];
     }
-    t.r_out = r[kTailBase].as<float>();
-    t.wx0 = fx0[kTailBase];
-    t.wy0 = fy0[kTailBase];
-    t.wx1 = fx1[kTailBase];
-    t.wy1 = fy1[kTailBase];
+    t.r_out = r[tb].as<float>();
+    t.wx0 = fx0[tb];
+    t.wy0 = fy0[tb];
+    t.wx1 = fx1[tb];
+    t.wy1 = fy1[tb];
     if (timer) timer->begin(s);
     HM_RETURN_IF_ERROR(launch_up_tail(c, n, kind, t, batch, s));
     if (timer) timer->end(s, "up_tail");
-    top = kTailBase - 1;
+    top = tb - 1;
   }
   // The other levels one launch each, coarsest first, one thread per 2x2
   // quad; level 0 goes straight to the output.

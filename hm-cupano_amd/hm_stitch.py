@@ -78,6 +78,10 @@ class hmServerStats(ctypes.Structure):
                 ("eager_launches", ctypes.c_long)]
 
 
+class hmTuning(ctypes.Structure):
+    _fields_ = [("remap_px", ctypes.c_int), ("tail_base", ctypes.c_int), ("overlap", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -120,6 +124,10 @@ def lib() -> ctypes.CDLL:
         "hmServerWaitIdle": (i, [vp]),
         "hmServerGetStats": (i, [vp, ctypes.POINTER(hmServerStats)]),
         "hmServerDestroy": (None, [vp]),
+        "hmStitchPanoGetTuning": (i, [vp, ctypes.POINTER(hmTuning)]),
+        "hmStitchPanoSetTuning": (i, [vp, ctypes.POINTER(hmTuning)]),
+        "hmStitchPanoAutotune": (i, [vp, ctypes.POINTER(S), i, S, i, i, vp, ctypes.POINTER(hmTuning),
+                                     ctypes.POINTER(ctypes.c_float)]),
         "hmStitchPanoStatus": (i, [vp]),
         "hmStitchPanoCanvasSize": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i)]),
         "hmStitchPanoBatchSize": (i, [vp]),
@@ -244,6 +252,25 @@ class StitchPano:
         b = ctypes.c_size_t()
         check(lib().hmStitchPanoDeviceBytes(self._h, ctypes.byref(b)), "hmStitchPanoDeviceBytes")
         return b.value
+
+    def get_tuning(self) -> dict:
+        t = hmTuning()
+        check(lib().hmStitchPanoGetTuning(self._h, ctypes.byref(t)), "hmStitchPanoGetTuning")
+        return {k: getattr(t, k) for k, _ in hmTuning._fields_}
+
+    def set_tuning(self, remap_px: int = 4, tail_base: int = 2, overlap: int = 1) -> None:
+        t = hmTuning(remap_px, tail_base, overlap)
+        check(lib().hmStitchPanoSetTuning(self._h, ctypes.byref(t)), "hmStitchPanoSetTuning")
+
+    def autotune(self, inputs, canvas, frames: int = 10, fused: bool = True, stream=None):
+        """Times every setting of the knobs that matter for this object on this frame and keeps
+        the fastest; returns (tuning dict, ms per frame)."""
+        arr = (hmSurface * len(inputs))(*[surface(t) for t in inputs])
+        t = hmTuning()
+        ms = ctypes.c_float()
+        check(lib().hmStitchPanoAutotune(self._h, arr, len(inputs), surface(canvas), int(fused), frames,
+                                         _stream(stream), ctypes.byref(t), ctypes.byref(ms)), "hmStitchPanoAutotune")
+        return {k: getattr(t, k) for k, _ in hmTuning._fields_}, ms.value
 
     def levels(self):
         n = ctypes.c_int()
@@ -446,6 +473,25 @@ class BlendContext:
         b = ctypes.c_size_t()
         check(lib().hmStitchPanoDeviceBytes(self._h, ctypes.byref(b)), "hmStitchPanoDeviceBytes")
         return b.value
+
+    def get_tuning(self) -> dict:
+        t = hmTuning()
+        check(lib().hmStitchPanoGetTuning(self._h, ctypes.byref(t)), "hmStitchPanoGetTuning")
+        return {k: getattr(t, k) for k, _ in hmTuning._fields_}
+
+    def set_tuning(self, remap_px: int = 4, tail_base: int = 2, overlap: int = 1) -> None:
+        t = hmTuning(remap_px, tail_base, overlap)
+        check(lib().hmStitchPanoSetTuning(self._h, ctypes.byref(t)), "hmStitchPanoSetTuning")
+
+    def autotune(self, inputs, canvas, frames: int = 10, fused: bool = True, stream=None):
+        """Times every setting of the knobs that matter for this object on this frame and keeps
+        the fastest; returns (tuning dict, ms per frame)."""
+        arr = (hmSurface * len(inputs))(*[surface(t) for t in inputs])
+        t = hmTuning()
+        ms = ctypes.c_float()
+        check(lib().hmStitchPanoAutotune(self._h, arr, len(inputs), surface(canvas), int(fused), frames,
+                                         _stream(stream), ctypes.byref(t), ctypes.byref(ms)), "hmStitchPanoAutotune")
+        return {k: getattr(t, k) for k, _ in hmTuning._fields_}, ms.value
 
     def levels(self):
         n = ctypes.c_int()

@@ -183,6 +183,28 @@ int hmStitchPanoPrepare(hmStitchPano* pano);
  * pano), used for admission against a memory budget. */
 int hmStitchPanoDeviceBytes(const hmStitchPano* pano, size_t* bytes);
 
+/* Launch-shape knobs of a pano object.  No counterpart in the reference.
+ * Every setting computes the same output; only the speed differs.
+ *   remap_px   pixels per thread of the 2-image remap kernels: 2, 4 or 8;
+ *   tail_base  2-image soft seam: levels tail_base..L-1 of the up pass run as
+ *              one tile-fused launch (1..3), or 0: one launch per level;
+ *   overlap    1: the canvas outside the blend region is remapped on a second
+ *              stream overlapping the pyramid; 0: everything on one stream. */
+typedef struct hmTuning {
+  int remap_px;
+  int tail_base;
+  int overlap;
+} hmTuning;
+int hmStitchPanoGetTuning(const hmStitchPano* pano, hmTuning* tuning);
+int hmStitchPanoSetTuning(hmStitchPano* pano, const hmTuning* tuning);
+/* The project's autotuner: times every setting of the knobs that matter for
+ * this object on the caller's frame (inputs as for the class's process;
+ * fused: the 3-image path), `frames` frames each after one warm-up frame,
+ * keeps the fastest, and returns it with its ms/frame.  The canvas is
+ * written. */
+int hmStitchPanoAutotune(hmStitchPano* pano, const hmSurface* inputs, int num_inputs, hmSurface canvas, int fused,
+                         int frames, hipStream_t stream, hmTuning* best, float* best_ms_per_frame);
+
 #define HM_MAX_IMAGES 8
 
 /* In-memory ControlMasksN (src/pano/controlMasksN.h:13-38): per-image CV_16U
