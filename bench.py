@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--served-streams", type=int, default=4)
     ap.add_argument("--served-frames", type=int, default=640, help="frames of the closed-loop served/plain runs")
     ap.add_argument("--served-only", action="store_true", help="measure only the served load")
+    ap.add_argument("--tuning", action="append", default=[], metavar="LEVELS:PX,TAIL,OVERLAP",
+                    help="fix the launch-shape knobs for a level count instead of autotuning (e.g. 6:4,2,1)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -341,9 +343,16 @@ def main():
         assert pano.status == 0, pano.status
         canvas = pano.alloc_canvas()
         cw, ch = pano.canvas_size()
-        # the project's autotuner: launch-shape knobs timed on this frame (output unchanged)
-        tuning, tuned_ms = pano.autotune([d1, d2], canvas, frames=10, stream=stream)
-        log(f"levels={levels}: autotuned {tuning} ({tuned_ms:.4f} ms/frame while tuning)")
+        fixed = [t.split(":", 1)[1] for t in args.tuning if int(t.split(":", 1)[0]) == levels]
+        if fixed:
+            px, tb, ov = (int(v) for v in fixed[0].split(","))
+            pano.set_tuning(px, tb, ov)
+            tuning = pano.get_tuning()
+            log(f"levels={levels}: fixed tuning {tuning}")
+        else:
+            # the project's autotuner: launch-shape knobs timed on this frame (output unchanged)
+            tuning, tuned_ms = pano.autotune([d1, d2], canvas, frames=10, stream=stream)
+            log(f"levels={levels}: autotuned {tuning} ({tuned_ms:.4f} ms/frame while tuning)")
         for _ in range(args.warmup):
             hm_stitch.check(pano.process(d1, d2, canvas, stream), "process")
         torch.cuda.synchronize()
