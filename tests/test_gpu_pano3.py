@@ -133,3 +133,15 @@ def test_pano3_2704x1520_to_6000x2000_bit_exact(hm, torch_gpu, oracle, levels):
     want = oracle.pano3(rig, frames, levels, fused=True, canvas=prior)
     diff = got != want
     assert not diff.any(), f"{diff.any(-1).sum()} pixels differ"
+
+
+def test_pano3_sparse_equals_dense_per_level_path(hm, torch_gpu):
+    """3 x 2704x1520 -> 6000x2000 at levels 6: fast path against the dense debug path."""
+    import hm_synth
+
+    rig = hm_synth.cylinder_rig(3, 2704, 1520, 6000, 2000, overlap=0.25, focal=1500.0, seed=78)
+    frames = frames_for(rig, 1, 44)
+    for fused in (True, False):
+        _, fast, prior = run_pano3(hm, torch_gpu, rig, frames, 6, fused)
+        _, dense, _ = run_pano3(hm, torch_gpu, rig, frames, 6, fused, debug=True, prior=prior)
+        assert not (fast != dense).any(), f"fused={fused}"

@@ -160,3 +160,18 @@ def test_panoN_cfg2_n8_hard_seam_bit_exact(hm, torch_gpu, oracle):
     want = oracle.panoN(rig, frames, 0)
     diff = got != want
     assert not diff.any(), f"{diff.any(-1).sum()} pixels differ"
+
+
+def test_panoN_cfg2_n8_sparse_equals_dense_per_level_path(hm, torch_gpu):
+    """8 x 3840x2160 -> 14000x2600 at levels 6: the fast path (sparse supports, label-selecting
+    level 0, tile-fused down pass) against the debug path (dense per-level kernels, every image
+    everywhere), bit for bit -- a full-size check that does not need the CPU oracle's minutes."""
+    import hm_synth
+
+    torch = torch_gpu
+    rig = hm_synth.config_rig("cfg2_n8")
+    frames = frames_for(rig, 1, 43)
+    _, fast = run_panoN(hm, torch, rig, frames, 6)
+    _, dense = run_panoN(hm, torch, rig, frames, 6, debug=True)
+    diff = fast != dense
+    assert not diff.any(), f"{diff.any(-1).sum()} pixels differ"
