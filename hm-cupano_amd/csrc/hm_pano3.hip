@@ -91,7 +91,12 @@ hipError_t Pano3::setup(const hmControlMasks3& m) {
       label_host[(size_t)y * bw + x] = l;
       if (l < 3) mask_host[((size_t)y * bw + x) * 3 + l] = 1.0f;  // split_to_channels (controlMasks3.cpp:187-224)
     }
-  for (int i = 0; i < 3; ++i) HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+  // Cleared once: the sparse fast path writes only each image's valid region,
+  // so the rest stays zero (what the dense pyramid sees there).
+  for (int i = 0; i < 3; ++i) {
+    HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+    HM_RETURN_IF_ERROR(hipMemset(g0[i].p, 0, g0[i].bytes));
+  }
   return hipSuccess;
 }
 

@@ -160,7 +160,12 @@ hipError_t PanoN::setup(const hmControlMasksN& m, int minimize_blend) {
       label_host[(size_t)y * bw + x] = l;
       if (l < n) mask_host[((size_t)y * bw + x) * n + l] = 1.0f;
     }
-  for (int i = 0; i < n; ++i) HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+  // Cleared once: the sparse fast path writes only each image's valid region,
+  // so the rest stays zero (what the dense pyramid sees there).
+  for (int i = 0; i < n; ++i) {
+    HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+    HM_RETURN_IF_ERROR(hipMemset(g0[i].p, 0, g0[i].bytes));
+  }
   return hipSuccess;
 }
 

@@ -1,10 +1,14 @@
 // hm_pano_common.hip -- C-ABI shared by every pano object: status, sizes,
-// pyramid inspection and launch timing.
+// pyramid inspection and dumps, and launch timing.
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <filesystem>
+#include <string>
 #include <vector>
 
+#include "hm_files.h"
 #include "hm_pano.hpp"
 
 namespace {
@@ -250,5 +254,98 @@ extern "C" int hmStitchPanoTimingReport(hmStitchPano* pano, char* buf, size_t le
   memcpy(buf, r.data(), n);
   buf[n] = '\0';
   if (reset) pano->timer.reset_totals();
+  return hipSuccess;
+}
+
+// ===========================================================================
+// pyramid dump (CudaStitchPano::dump_soft_blend_pyramid, cudaPano.inl:420-458)
+// ===========================================================================
+namespace {
+
+hipError_t write_level(const std::string& dir, const std::string& label, const std::string& name, const void* dev,
+                       int w, int h, int c) {
+  std::error_code ec;
+  const std::filesystem::path sub = std::filesystem::path(dir) / label;
+  std::filesystem::create_directories(sub, ec);
+  if (ec) return hipErrorUnknown;
+  std::vector<float> host((size_t)w * h * c);
+  HM_RETURN_IF_ERROR(hipMemcpy(host.data(), dev, host.size() * sizeof(float), hipMemcpyDeviceToHost));
+  hmRaster r;
+  r.data = host.data();
+  r.width = (uint32_t)w;
+  r.height = (uint32_t)h;
+  r.channels = (uint32_t)c;
+  r.bits = 32;
+  r.sample_format = HM_SAMPLE_FLOAT;
+  r.bytes = host.size() * sizeof(float);
+  // cv::imwrite's channel convention, so OpenCV-based readers see memory order
+  const std::string path = (sub / name).string();
+  return hmTiffWrite(path.c_str(), &r, nullptr, 1) == HM_FILE_OK ? hipSuccess : hipErrorUnknown;
+}
+
+}  // namespace
+
+extern "C" int hmStitchPanoDumpPyramid(hmStitchPano* pano, const char* directory, hipStream_t stream) {
+  if (!pano || !directory) return hipErrorInvalidValue;
+  if (pano->status != hipSuccess) return pano->status;
+  if (pano->hard_seam || pano->num_levels <= 0 || !pano->initialized_pyramid) return hipErrorInvalidValue;
+  HM_RETURN_IF_ERROR(hipStreamSynchronize(stream));
+  const hm::Pyramid& p = pano->pyr;
+  // Re-evaluate the last frame's pyramid densely from the resident level-0
+  // inputs and mask, keeping every level (the fused path keeps none of the
+  // Laplacian or blended levels, and the N/3-image Gaussian levels only
+  // around each image).  The values are those of the frame's own pyramid.
+  hm::Pyramid t;
+  HM_RETURN_IF_ERROR(t.init(p.kind, p.n, p.c, p.W[0], p.H[0], pano->num_levels, p.batch, true));
+  HM_RETURN_IF_ERROR(t.build_mask_pyramid(pano->mask0.as<float>(), stream));
+  const void* g0[hmdev::kMaxImages] = {};
+  for (int i = 0; i < p.n; ++i) g0[i] = pano->g0[i].p;
+  hm::Level0Out out;
+  out.f32 = t.r0_debug.as<float>();
+  HM_RETURN_IF_ERROR(t.run(g0, true, out, stream, nullptr));
+  // level-0 Gaussians as float
+  hm::DevBuf g0f[hmdev::kMaxImages];
+  const size_t n0 = (size_t)t.W[0] * t.H[0] * t.c * t.batch;
+  for (int i = 0; i < p.n; ++i) {
+    HM_RETURN_IF_ERROR(g0f[i].alloc(n0 * sizeof(float)));
+    hipLaunchKernelGGL(u8_to_f32_kernel, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, stream,
+                       pano->g0[i].as<uint8_t>(), g0f[i].as<float>(), n0);
+  }
+  HM_RETURN_IF_ERROR(hipGetLastError());
+  HM_RETURN_IF_ERROR(hipStreamSynchronize(stream));
+
+  const std::string dir = directory;
+  std::error_code ec;
+  std::filesystem::create_directories(dir, ec);
+  if (ec) return hipErrorUnknown;
+  {
+    FILE* f = fopen((std::filesystem::path(dir) / "metadata.txt").string().c_str(), "w");
+    if (!f) return hipErrorUnknown;
+    fprintf(f, "num_levels=%d\nbatch_size=%d\nchannels=%d\n", t.levels, t.batch, t.c);
+    for (int l = 0; l < t.levels; ++l) fprintf(f, "level_%d=%dx%d\n", l, t.W[l], t.H[l]);
+    if (fclose(f) != 0) return hipErrorUnknown;
+  }
+  const int M = t.mask_channels();
+  for (int l = 0; l < t.levels; ++l) {
+    const int w = t.W[l], h = t.H[l];
+    const size_t img = (size_t)w * h * t.c;  // floats per batch item
+    char name[64];
+    auto images = [&](const std::string& label, const float* base) -> hipError_t {
+      for (int b = 0; b < t.batch; ++b) {
+        snprintf(name, sizeof(name), "level_%d_batch_%d.tiff", l, b);
+        HM_RETURN_IF_ERROR(write_level(dir, label, name, base + b * img, w, h, t.c));
+      }
+      return hipSuccess;
+    };
+    // The reference's order and names (two images: gauss1 gauss2 mask lap1
+    // lap2 blend reconstruct); N images number gauss1..gaussN, lap1..lapN.
+    for (int i = 0; i < t.n; ++i)
+      HM_RETURN_IF_ERROR(images("gauss" + std::to_string(i + 1), l ? t.g[i][l].as<float>() : g0f[i].as<float>()));
+    snprintf(name, sizeof(name), "level_%d.tiff", l);
+    HM_RETURN_IF_ERROR(write_level(dir, "mask", name, l ? t.mask[l].as<float>() : pano->mask0.as<float>(), w, h, M));
+    for (int i = 0; i < t.n; ++i) HM_RETURN_IF_ERROR(images("lap" + std::to_string(i + 1), t.lap[i][l].as<float>()));
+    HM_RETURN_IF_ERROR(images("blend", t.blend[l].as<float>()));
+    HM_RETURN_IF_ERROR(images("reconstruct", l ? t.r[l].as<float>() : t.r0_debug.as<float>()));
+  }
   return hipSuccess;
 }

@@ -82,6 +82,33 @@ class hmTuning(ctypes.Structure):
     _fields_ = [("remap_px", ctypes.c_int), ("tail_base", ctypes.c_int), ("overlap", ctypes.c_int)]
 
 
+class hmControlFilesInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_images", ctypes.c_int),
+        ("map_w", ctypes.c_uint32 * HM_MAX_IMAGES), ("map_h", ctypes.c_uint32 * HM_MAX_IMAGES),
+        ("map_x", ctypes.c_void_p * HM_MAX_IMAGES), ("map_y", ctypes.c_void_p * HM_MAX_IMAGES),
+        ("xpos", ctypes.c_float * HM_MAX_IMAGES), ("ypos", ctypes.c_float * HM_MAX_IMAGES),
+        ("position_files_found", ctypes.c_uint32),
+        ("seam_w", ctypes.c_uint32), ("seam_h", ctypes.c_uint32), ("seam", ctypes.c_void_p),
+        ("seam_paletted", ctypes.c_int),
+    ]
+
+
+class hmRaster(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("channels", ctypes.c_uint32), ("bits", ctypes.c_uint32), ("sample_format", ctypes.c_uint32),
+                ("bytes", ctypes.c_size_t)]
+
+
+class hmTiffTags(ctypes.Structure):
+    _fields_ = [("has_resolution", ctypes.c_int), ("has_xres", ctypes.c_int), ("has_yres", ctypes.c_int),
+                ("has_xpos", ctypes.c_int), ("has_ypos", ctypes.c_int),
+                ("xres", ctypes.c_float), ("yres", ctypes.c_float), ("xpos", ctypes.c_float), ("ypos", ctypes.c_float),
+                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("channels", ctypes.c_uint32),
+                ("bits", ctypes.c_uint32), ("sample_format", ctypes.c_uint32), ("compression", ctypes.c_uint32),
+                ("predictor", ctypes.c_uint32), ("photometric", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -137,6 +164,16 @@ def lib() -> ctypes.CDLL:
         "hmStitchPanoReadLevel": (i, [vp, i, i, i, vp, ctypes.c_size_t, vp]),
         "hmStitchPanoSetTiming": (i, [vp, i]),
         "hmStitchPanoTimingReport": (i, [vp, ctypes.c_char_p, ctypes.c_size_t, i]),
+        "hmStitchPanoDumpPyramid": (i, [vp, ctypes.c_char_p, vp]),
+        "hmFilesLastError": (ctypes.c_char_p, []),
+        "hmControlFilesLoad": (i, [ctypes.c_char_p, i, i, ctypes.POINTER(vp)]),
+        "hmControlFilesGetInfo": (i, [vp, ctypes.POINTER(hmControlFilesInfo)]),
+        "hmControlFilesDestroy": (None, [vp]),
+        "hmTiffRead": (i, [ctypes.c_char_p, i, ctypes.POINTER(hmRaster)]),
+        "hmTiffReadTags": (i, [ctypes.c_char_p, ctypes.POINTER(hmTiffTags)]),
+        "hmTiffWrite": (i, [ctypes.c_char_p, ctypes.POINTER(hmRaster), ctypes.POINTER(hmTiffTags), i]),
+        "hmPngReadSeam": (i, [ctypes.c_char_p, ctypes.POINTER(hmRaster), ctypes.POINTER(i)]),
+        "hmRasterFree": (None, [ctypes.POINTER(hmRaster)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -293,6 +330,12 @@ class StitchPano:
 
     def _mask_channels(self) -> int:
         return 1
+
+    def dump_soft_blend_pyramid(self, directory: str, stream=None) -> None:
+        """dump_soft_blend_pyramid (src/pano/cudaPano.inl:420-458): metadata.txt and
+        per-level float TIFFs of the last processed frame (hmStitchPanoDumpPyramid)."""
+        check(lib().hmStitchPanoDumpPyramid(self._h, os.fsencode(str(directory)), _stream(stream)),
+              "hmStitchPanoDumpPyramid")
 
     def set_timing(self, on: bool) -> None:
         check(lib().hmStitchPanoSetTiming(self._h, int(on)), "hmStitchPanoSetTiming")
@@ -550,9 +593,128 @@ def make_full(src, src_type, region_w, region_h, sx, sy, ox, oy, batch, dest, de
 
 
 def exported_symbols() -> List[str]:
-    """Names declared in include/hm_stitch.h (for the export test)."""
+    """Names declared in include/*.h (for the export test)."""
+    import glob
     import re
 
-    hdr = os.path.join(os.path.dirname(_HERE), "include", "hm_stitch.h")
-    text = open(hdr).read()
-    return sorted(set(re.findall(r"^(?:int|void)\s+(hm\w+)\s*\(", text, flags=re.M)))
+    names = set()
+    for hdr in glob.glob(os.path.join(os.path.dirname(_HERE), "include", "*.h")):
+        text = open(hdr).read()
+        names.update(re.findall(r"^(?:int|void|const char\*)\s+(hm\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+# ---------------------------------------------------------------------------
+# control directories and raster files (include/hm_files.h)
+# ---------------------------------------------------------------------------
+HM_FILE_OK, HM_FILE_NOT_FOUND, HM_FILE_BAD_FORMAT, HM_FILE_UNSUPPORTED, HM_FILE_WRONG_TYPE = range(5)
+HM_FILE_SEAM_CLASSES, HM_FILE_MISMATCH, HM_FILE_WRITE_FAILED, HM_FILE_INVALID_ARGUMENT = range(5, 9)
+HM_CONTROL_TWO, HM_CONTROL_THREE, HM_CONTROL_N = 2, 3, 0
+HM_SAMPLE_UINT, HM_SAMPLE_INT, HM_SAMPLE_FLOAT = 1, 2, 3
+
+
+class FileError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+def _fcheck(code: int) -> None:
+    if code != HM_FILE_OK:
+        raise FileError(code, lib().hmFilesLastError().decode(errors="replace"))
+
+
+class ControlRig:
+    """A rig loaded from a control directory, in the form the pano classes
+    take: map_x/map_y (uint16 [h, w] per image), pos_x/pos_y (the float
+    placements, truncated by the pano classes), seam (uint8 [h, w])."""
+
+    def __init__(self, info: hmControlFilesInfo):
+        n = info.n_images
+        self.n_images = n
+        self.map_x, self.map_y = [], []
+        for k in range(n):
+            h, w = info.map_h[k], info.map_w[k]
+            for dst, ptr in ((self.map_x, info.map_x[k]), (self.map_y, info.map_y[k])):
+                buf = (ctypes.c_uint16 * (w * h)).from_address(ptr)
+                dst.append(np.frombuffer(buf, dtype=np.uint16).reshape(h, w).copy())
+        self.xpos = [float(info.xpos[k]) for k in range(n)]
+        self.ypos = [float(info.ypos[k]) for k in range(n)]
+        self.pos_x, self.pos_y = list(self.xpos), list(self.ypos)
+        self.position_files_found = int(info.position_files_found)
+        sh, sw = info.seam_h, info.seam_w
+        buf = (ctypes.c_uint8 * (sw * sh)).from_address(info.seam)
+        self.seam = np.frombuffer(buf, dtype=np.uint8).reshape(sh, sw).copy()
+        self.seam_paletted = bool(info.seam_paletted)
+
+
+def load_control_dir(directory: str, kind: int, n_images: int = 0) -> ControlRig:
+    """hmControlFilesLoad: the rig in `directory` under the 2-image (HM_CONTROL_TWO),
+    3-image (HM_CONTROL_THREE) or N-image (HM_CONTROL_N, n_images) rules."""
+    L = lib()
+    h = ctypes.c_void_p()
+    _fcheck(L.hmControlFilesLoad(os.fsencode(str(directory)), kind, n_images, ctypes.byref(h)))
+    try:
+        info = hmControlFilesInfo()
+        _fcheck(L.hmControlFilesGetInfo(h, ctypes.byref(info)))
+        return ControlRig(info)
+    finally:
+        L.hmControlFilesDestroy(h)
+
+
+_NP_OF = {(HM_SAMPLE_UINT, 8): np.uint8, (HM_SAMPLE_UINT, 16): np.uint16, (HM_SAMPLE_UINT, 32): np.uint32,
+          (HM_SAMPLE_INT, 8): np.int8, (HM_SAMPLE_INT, 16): np.int16, (HM_SAMPLE_INT, 32): np.int32,
+          (HM_SAMPLE_FLOAT, 32): np.float32, (HM_SAMPLE_FLOAT, 64): np.float64}
+
+
+def _raster_to_numpy(r: hmRaster) -> np.ndarray:
+    shape = (r.height, r.width) if r.channels == 1 else (r.height, r.width, r.channels)
+    try:
+        dt = np.dtype(_NP_OF[(r.sample_format, r.bits)])
+        buf = (ctypes.c_uint8 * r.bytes).from_address(r.data)
+        a = np.frombuffer(buf, dtype=dt).copy()
+    finally:
+        lib().hmRasterFree(ctypes.byref(r))
+    return a.reshape(shape)
+
+
+def tiff_read(path: str, opencv_order: bool = False) -> np.ndarray:
+    """The first image of a TIFF as [h, w] or [h, w, c] (hmTiffRead)."""
+    r = hmRaster()
+    _fcheck(lib().hmTiffRead(os.fsencode(str(path)), int(opencv_order), ctypes.byref(r)))
+    return _raster_to_numpy(r)
+
+
+def tiff_read_tags(path: str) -> dict:
+    t = hmTiffTags()
+    _fcheck(lib().hmTiffReadTags(os.fsencode(str(path)), ctypes.byref(t)))
+    return {name: getattr(t, name) for name, _ in hmTiffTags._fields_}
+
+
+def tiff_write(path: str, array: np.ndarray, opencv_order: bool = False, xres=None, yres=None, xpos=None,
+               ypos=None) -> None:
+    """Writes [h, w] or [h, w, c] (uint8/16/32, int8/16/32, float32/64) as an uncompressed TIFF."""
+    a = np.ascontiguousarray(array)
+    fmt = {"u": HM_SAMPLE_UINT, "i": HM_SAMPLE_INT, "f": HM_SAMPLE_FLOAT}[a.dtype.kind]
+    r = hmRaster()
+    r.data = a.ctypes.data
+    r.height, r.width = a.shape[:2]
+    r.channels = 1 if a.ndim == 2 else a.shape[2]
+    r.bits = a.dtype.itemsize * 8
+    r.sample_format = fmt
+    r.bytes = a.nbytes
+    t = hmTiffTags()
+    for name, v in (("xres", xres), ("yres", yres), ("xpos", xpos), ("ypos", ypos)):
+        if v is not None:
+            setattr(t, "has_" + name, 1)
+            setattr(t, name, float(v))
+    _fcheck(lib().hmTiffWrite(os.fsencode(str(path)), ctypes.byref(r), ctypes.byref(t), int(opencv_order)))
+
+
+def png_read_seam(path: str):
+    """(uint8 [h, w], paletted) of a seam PNG (hmPngReadSeam)."""
+    r = hmRaster()
+    pal = ctypes.c_int()
+    _fcheck(lib().hmPngReadSeam(os.fsencode(str(path)), ctypes.byref(r), ctypes.byref(pal)))
+    return _raster_to_numpy(r), bool(pal.value)

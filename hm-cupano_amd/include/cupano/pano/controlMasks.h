@@ -1,16 +1,23 @@
-// controlMasks.h -- in-memory control data of a rig (no file I/O).
+// controlMasks.h -- the control data of a rig, in memory or loaded from a
+// control directory.
 //
 // The fields of hm-cupano's ControlMasks (src/pano/controlMasks.h:30-66):
 // img1_col/img1_row/img2_col/img2_row are the CV_16U remap tables,
 // whole_seam_mask_image is the seam AFTER load_seam_mask's inversion
 // (controlMasks.cpp:225-251: 1 selects image 1), positions are the normalised
 // SpatialTiff placements.  cv::Mat is replaced by a minimal host image.
+// load() reads the directory through hmControlFilesLoad (include/hm_files.h)
+// under the same rules as the reference's loaders.
 #pragma once
 
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
+#include <string>
 #include <vector>
+
+#include "hm_files.h"
 
 namespace hm {
 namespace pano {
@@ -32,11 +39,54 @@ struct HostMat {
   const T& at(int y, int x) const { return data[(size_t)y * cols + x]; }
 };
 
+namespace detail {
+
+// Loads `dir` under `kind`'s rules; fills maps, seam and placements.
+inline bool load_control_dir(const std::string& dir, int kind, int n_images, std::vector<HostMat<uint16_t>>& cols,
+                             std::vector<HostMat<uint16_t>>& rows, HostMat<uint8_t>& seam,
+                             std::vector<SpatialTiff>& positions) {
+  hmControlFiles* f = nullptr;
+  if (hmControlFilesLoad(dir.c_str(), kind, n_images, &f) != HM_FILE_OK) return false;
+  hmControlFilesInfo info;
+  hmControlFilesGetInfo(f, &info);
+  cols.assign(info.n_images, HostMat<uint16_t>());
+  rows.assign(info.n_images, HostMat<uint16_t>());
+  positions.clear();
+  for (int i = 0; i < info.n_images; ++i) {
+    const int w = (int)info.map_w[i], h = (int)info.map_h[i];
+    cols[i] = HostMat<uint16_t>(h, w);
+    rows[i] = HostMat<uint16_t>(h, w);
+    memcpy(cols[i].data.data(), info.map_x[i], (size_t)w * h * sizeof(uint16_t));
+    memcpy(rows[i].data.data(), info.map_y[i], (size_t)w * h * sizeof(uint16_t));
+    positions.push_back(SpatialTiff{info.xpos[i], info.ypos[i]});
+  }
+  seam = HostMat<uint8_t>((int)info.seam_h, (int)info.seam_w);
+  memcpy(seam.data.data(), info.seam, (size_t)info.seam_w * info.seam_h);
+  hmControlFilesDestroy(f);
+  return true;
+}
+
+}  // namespace detail
+
 class ControlMasks {
  public:
   HostMat<uint16_t> img1_col, img1_row, img2_col, img2_row;
   HostMat<uint8_t> whole_seam_mask_image;
   std::vector<SpatialTiff> positions;
+
+  ControlMasks() = default;
+  // controlMasks.cpp:253-256: the caller checks is_valid().
+  explicit ControlMasks(std::string game_dir) { (void)load(std::move(game_dir)); }
+  // controlMasks.cpp:273-326; hmFilesLastError() says why a load failed.
+  bool load(std::string game_dir) {
+    std::vector<HostMat<uint16_t>> c, r;
+    if (!detail::load_control_dir(game_dir, HM_CONTROL_TWO, 2, c, r, whole_seam_mask_image, positions)) return false;
+    img1_col = std::move(c[0]);
+    img1_row = std::move(r[0]);
+    img2_col = std::move(c[1]);
+    img2_row = std::move(r[1]);
+    return true;
+  }
 
   bool is_valid() const {
     return !img1_col.empty() && !img1_row.empty() && !img2_col.empty() && !img2_row.empty() &&
@@ -58,6 +108,14 @@ class ControlMasksN {
   std::vector<HostMat<uint16_t>> img_col, img_row;
   HostMat<uint8_t> whole_seam_mask_indexed;
   std::vector<SpatialTiff> positions;
+
+  ControlMasksN() = default;
+  ControlMasksN(const std::string& dir, int n_images) { (void)load(dir, n_images); }
+  // controlMasksN.cpp:125-193
+  bool load(const std::string& dir, int n_images) {
+    return detail::load_control_dir(dir, HM_CONTROL_N, n_images, img_col, img_row, whole_seam_mask_indexed,
+                                    positions);
+  }
 
   bool is_valid() const {  // controlMasksN.cpp:195-203
     if (img_col.empty() || img_row.empty() || img_col.size() != img_row.size()) return false;
@@ -83,6 +141,22 @@ class ControlMasks3 {
   HostMat<uint16_t> img0_col, img0_row, img1_col, img1_row, img2_col, img2_row;
   HostMat<uint8_t> whole_seam_mask_image;
   std::vector<SpatialTiff> positions;
+
+  ControlMasks3() = default;
+  explicit ControlMasks3(const std::string& game_dir) { (void)load(game_dir); }
+  // controlMasks3.cpp:228-295
+  bool load(const std::string& game_dir) {
+    std::vector<HostMat<uint16_t>> c, r;
+    if (!detail::load_control_dir(game_dir, HM_CONTROL_THREE, 3, c, r, whole_seam_mask_image, positions))
+      return false;
+    img0_col = std::move(c[0]);
+    img0_row = std::move(r[0]);
+    img1_col = std::move(c[1]);
+    img1_row = std::move(r[1]);
+    img2_col = std::move(c[2]);
+    img2_row = std::move(r[2]);
+    return true;
+  }
 
   bool is_valid() const {  // controlMasks3.cpp:297-301
     return !img0_col.empty() && !img0_row.empty() && !img1_col.empty() && !img1_row.empty() && !img2_col.empty() &&

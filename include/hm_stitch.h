@@ -320,6 +320,18 @@ int hmStitchPanoLevels(const hmStitchPano* pano, int* num_levels, int* widths, i
  * each image's box (elsewhere they are zero and not stored). */
 int hmStitchPanoReadLevel(hmStitchPano* pano, int what, int image, int level, void* host_dst, size_t bytes,
                           hipStream_t stream);
+/* dump_soft_blend_pyramid (src/pano/cudaPano.inl:420-458): synchronises the
+ * stream, then writes `directory`/metadata.txt ("num_levels=", "batch_size=",
+ * "channels=", "level_k=WxH" lines) and one float32 TIFF per level:
+ *   gauss<i>/level_k_batch_b.tiff  (i = 1..N)   mask/level_k.tiff
+ *   lap<i>/level_k_batch_b.tiff    blend/level_k_batch_b.tiff
+ *   reconstruct/level_k_batch_b.tiff
+ * (two images: gauss1 gauss2 lap1 lap2, the reference's names), in
+ * cv::imwrite's channel order.  The levels are those of the last processed
+ * frame, re-evaluated densely from its resident level-0 inputs (works with
+ * or without hmStitchPanoSetDebug).  hipErrorInvalidValue for a hard seam
+ * (levels 0) or before the first process. */
+int hmStitchPanoDumpPyramid(hmStitchPano* pano, const char* directory, hipStream_t stream);
 /* Per-launch event timing on the process stream. */
 int hmStitchPanoSetTiming(hmStitchPano* pano, int enable);
 /* Writes "name total_ms count\n" lines accumulated since the last reset. */
