@@ -1,13 +1,14 @@
 // cudaPano.h -- hm::pano::cuda::CudaStitchPano<T_pipeline, T_compute>.
 //
-// The constructor, process(), status(), canvas_width/height() and
-// batch_size() of hm-cupano's src/pano/cudaPano.h:68-115, over the gfx950
+// The constructor, process(), status(), canvas_width/height(), batch_size()
+// and dump_soft_blend_pyramid() of hm-cupano's src/pano/cudaPano.h:68-115, over the gfx950
 // library's C ABI (include/hm_stitch.h).  T_pipeline is uchar4 or uchar3
 // with T_compute float4 or float3.  Like the reference, constructor errors
 // are sticky in status() and process() moves the canvas in and back out.
 #pragma once
 
 #include <memory>
+#include <string>
 #include <type_traits>
 
 #include "cupano/cuda/cudaStatus.h"
@@ -84,6 +85,17 @@ class CudaStitchPano {
       return s;
     }
     return std::move(canvas);
+  }
+
+  // src/pano/cudaPano.h:101 / cudaPano.inl:420-458: metadata.txt and one
+  // float TIFF per level of the last processed frame.
+  CudaStatus dump_soft_blend_pyramid(const std::string& directory, hipStream_t stream) const {
+    CUDA_RETURN_IF_ERROR(status_);
+    const int rc = hmStitchPanoDumpPyramid(handle_.get(), directory.c_str(), stream);
+    if (rc != hipSuccess)
+      return CudaStatus(static_cast<hipError_t>(rc), "Soft-blend pyramid is only available when num_levels > 0 "
+                                                     "and after the first process");
+    return CudaStatus::OkStatus();
   }
 
   // The library handle, for pyramid inspection (hmStitchPanoReadLevel ...).

@@ -6,7 +6,9 @@
 // per case; exit code 0 = pass.
 #include <cstdio>
 #include <cstdlib>
+#include <filesystem>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "cupano/pano/cudaPano.h"
@@ -172,6 +174,35 @@ int main() {
       maxd = std::max(maxd, std::abs((int)full[i].w - (int)mini[i].w));
     }
     check(maxd <= 1, "minimize_blend matches the full-canvas blend within 1 LSB");
+  }
+  // src/pano/cudaPano_test.cpp:185-201: pyramid dumps of the full and
+  // minimized blends; level 0 spans the canvas width only for the full one.
+  {
+    const int w = 384, h = 64, x2 = 192, cwid = w + x2;
+    HostMat<uint8_t> seam(h, cwid, 0);
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < cwid / 2; ++x) seam.at(y, x) = 1;
+    const ControlMasks masks = make_masks(w, h, x2, seam);
+    const auto i1 = patterned(w, h, 0), i2 = patterned(w, h, 1);
+    CudaMat<uchar4> a(i1.data(), w, h), b(i2.data(), w, h);
+    char tmpl[] = "/tmp/pano_dump_XXXXXX";
+    const char* root = mkdtemp(tmpl);
+    bool ok = root != nullptr;
+    int lw[2] = {0, 0}, lh[2] = {0, 0};
+    for (int mini = 0; ok && mini < 2; ++mini) {
+      CudaStitchPano<uchar4, float4> pano(1, 4, masks, true, mini == 1);
+      auto r = pano.process(a, b, nullptr,
+                            std::make_unique<CudaMat<uchar4>>(1, pano.canvas_width(), pano.canvas_height()));
+      const std::string dir = std::string(root) + (mini ? "/mini" : "/full");
+      ok = r.ok() && pano.dump_soft_blend_pyramid(dir, nullptr).ok();
+      FILE* f = ok ? std::fopen((dir + "/metadata.txt").c_str(), "r") : nullptr;
+      char line[128];
+      while (f && std::fgets(line, sizeof(line), f)) std::sscanf(line, "level_0=%dx%d", &lw[mini], &lh[mini]);
+      if (f) std::fclose(f);
+    }
+    if (root) std::filesystem::remove_all(root);
+    check(ok && lw[0] == cwid && lh[0] == h && lh[1] == h && lw[1] < lw[0],
+          "dump_soft_blend_pyramid: full blend level 0 spans the canvas, the minimized one is narrower");
   }
   // Constructor errors are sticky (cudaPano.inl:28-31, 460-472).
   {

@@ -18,4 +18,4 @@ def test_cpp_pano_api(torch_gpu):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
-    assert r.stdout.count("PASS") == 9
+    assert r.stdout.count("PASS") == 10
