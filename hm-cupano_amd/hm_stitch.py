@@ -109,6 +109,18 @@ class hmTiffTags(ctypes.Structure):
                 ("predictor", ctypes.c_uint32), ("photometric", ctypes.c_uint32)]
 
 
+class hmFrameLoopStats(ctypes.Structure):
+    _fields_ = [("frames", ctypes.c_int64), ("seconds", ctypes.c_double), ("upload_ms", ctypes.c_double),
+                ("stitch_ms", ctypes.c_double), ("download_ms", ctypes.c_double),
+                ("bytes_up_per_frame", ctypes.c_uint64), ("bytes_down_per_frame", ctypes.c_uint64),
+                ("callback_error", ctypes.c_int)]
+
+
+FRAME_SOURCE = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p),
+                                ctypes.POINTER(ctypes.c_uint32))
+FRAME_SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_uint32,
+                              ctypes.c_uint32, ctypes.c_uint32)
+
 _lib = None
 
 
@@ -119,6 +131,13 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    # torch ships its own HIP runtime (same soname as the system one).  Loading
+    # it first makes the library bind to it; loaded the other way round, one
+    # process would hold two HIP runtimes and torch's GPU initialisation fails.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i, u32, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_float
     S = hmSurface
@@ -165,6 +184,8 @@ def lib() -> ctypes.CDLL:
         "hmStitchPanoSetTiming": (i, [vp, i]),
         "hmStitchPanoTimingReport": (i, [vp, ctypes.c_char_p, ctypes.c_size_t, i]),
         "hmStitchPanoDumpPyramid": (i, [vp, ctypes.c_char_p, vp]),
+        "hmFrameLoopRun": (i, [vp, i, ctypes.POINTER(u32), ctypes.POINTER(u32), i, ctypes.c_int64, FRAME_SOURCE,
+                               FRAME_SINK, vp, vp, ctypes.POINTER(hmFrameLoopStats)]),
         "hmFilesLastError": (ctypes.c_char_p, []),
         "hmControlFilesLoad": (i, [ctypes.c_char_p, i, i, ctypes.POINTER(vp)]),
         "hmControlFilesGetInfo": (i, [vp, ctypes.POINTER(hmControlFilesInfo)]),
@@ -718,3 +739,59 @@ def png_read_seam(path: str):
     pal = ctypes.c_int()
     _fcheck(lib().hmPngReadSeam(os.fsencode(str(path)), ctypes.byref(r), ctypes.byref(pal)))
     return _raster_to_numpy(r), bool(pal.value)
+
+
+# ---------------------------------------------------------------------------
+# frame loop (hmFrameLoopRun)
+# ---------------------------------------------------------------------------
+def frame_loop(pano, input_sizes, source, sink, slots: int = 3, max_frames: int = -1, stream=None) -> dict:
+    """Streams host frames through `pano` (hmFrameLoopRun).
+
+    input_sizes: [(w, h)] per input.  source(frame) returns one uint8 array
+    [B, h, w, 3] (or [h, w, 3]) of BGR pixels per input, or None at the end.
+    sink(frame, bgr) receives the stitched [B * canvas_h, canvas_w, 3] BGR
+    frame (a view of pinned memory, valid during the call).  Returns the
+    loop's statistics; exceptions raised by the callbacks are re-raised."""
+    L = lib()
+    n = len(input_sizes)
+    ws = (ctypes.c_uint32 * n)(*[int(w) for w, _ in input_sizes])
+    hs = (ctypes.c_uint32 * n)(*[int(h) for _, h in input_sizes])
+    b = pano.batch_size
+    errors = []
+
+    def _src(_user, frame, ptrs, pitches):
+        try:
+            frames = source(int(frame))
+            if frames is None:
+                return 0
+            if len(frames) != n:
+                raise ValueError(f"source returned {len(frames)} frames for {n} inputs")
+            for i, f in enumerate(frames):
+                a = np.ascontiguousarray(f, dtype=np.uint8)
+                want = b * hs[i] * ws[i] * 3
+                if a.size != want:
+                    raise ValueError(f"input {i}: {a.shape} is not {b} x {hs[i]} x {ws[i]} x 3")
+                ctypes.memmove(ptrs[i], a.ctypes.data, want)
+            return 1
+        except BaseException as e:  # noqa: BLE001 - reported after the loop
+            errors.append(e)
+            return -1
+
+    def _sink(_user, frame, ptr, width, rows, pitch):
+        try:
+            buf = (ctypes.c_uint8 * (rows * pitch)).from_address(ptr)
+            view = np.frombuffer(buf, dtype=np.uint8).reshape(rows, pitch)[:, : width * 3].reshape(rows, width, 3)
+            return 1 if sink(int(frame), view) is False else 0
+        except BaseException as e:  # noqa: BLE001
+            errors.append(e)
+            return -1
+
+    src_cb, sink_cb = FRAME_SOURCE(_src), FRAME_SINK(_sink)
+    st = hmFrameLoopStats()
+    rc = L.hmFrameLoopRun(pano._h, n, ws, hs, slots, max_frames, src_cb, sink_cb, None, _stream(stream),
+                          ctypes.byref(st))
+    if errors:
+        raise errors[0]
+    if rc != 0 and not (st.callback_error == 1 and rc == 999):  # the sink asked to stop
+        raise HmError(rc, "hmFrameLoopRun")
+    return {name: getattr(st, name) for name, _ in hmFrameLoopStats._fields_}

@@ -308,6 +308,39 @@ int hmServerGetStats(hmServer* server, hmServerStats* stats);
 void hmServerDestroy(hmServer* server);
 
 /* ------------------------------------------------------------------------- */
+/* Frame loop (host frames in, host frames out)                              */
+/* ------------------------------------------------------------------------- */
+/* tests/stitch_two_videos.cpp:552-585 (read -> upload -> process -> download
+ * -> BGRA2BGR -> write) as a pipeline: `slots` pinned host slots, uploads on
+ * one stream, the stitch on `stream`, downloads on another, so frame k+1's
+ * upload, frame k's stitch and frame k-1's download overlap.  Frames cross
+ * PCIe as packed BGR (3 bytes per pixel); a 4-channel pano gets them
+ * expanded to BGRA (alpha 255) and its canvas packed back to BGR on the
+ * device.  One canvas serves every frame, as in the reference's loop.
+ *
+ * source(user, frame, host_frames, pitches) fills, for each input i, batch x
+ * in_h[i] rows of in_w[i] BGR pixels at host_frames[i] (row pitch
+ * pitches[i]); it returns 1 for a frame, 0 at the end of the stream, < 0 on
+ * error.  sink(user, frame, bgr, width, rows, pitch) receives each stitched
+ * frame in order (batch x canvas-height rows; valid during the call only);
+ * nonzero stops the loop.  max_frames < 0: until the source ends.  A
+ * callback's error ends the loop with hipErrorUnknown and its value in
+ * stats->callback_error. */
+typedef int (*hmFrameSource)(void* user, int64_t frame, void* const* host_frames, const uint32_t* pitches);
+typedef int (*hmFrameSink)(void* user, int64_t frame, const void* bgr, uint32_t width, uint32_t rows,
+                           uint32_t pitch);
+typedef struct hmFrameLoopStats {
+  int64_t frames;                /* frames delivered to the sink */
+  double seconds;                /* wall time of the loop */
+  double upload_ms, stitch_ms, download_ms; /* mean per-frame stream times */
+  uint64_t bytes_up_per_frame, bytes_down_per_frame;
+  int callback_error;
+} hmFrameLoopStats;
+int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* in_w, const uint32_t* in_h, int slots,
+                   int64_t max_frames, hmFrameSource source, hmFrameSink sink, void* user, hipStream_t stream,
+                   hmFrameLoopStats* stats);
+
+/* ------------------------------------------------------------------------- */
 /* Pyramid inspection and launch timing (all pano classes)                   */
 /* ------------------------------------------------------------------------- */
 /* Keep the Laplacian/blend pyramids (call before the first process). */
