@@ -283,6 +283,43 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
     }
 
 
+def frame_loop_benchmark(hm_stitch, rig, f1, f2, levels, frames):
+    """Host frames in, host frames out (hmFrameLoopRun): BGR over PCIe, the stitch
+    on the GPU, 3 pinned slots in flight.  The source resends the slot's pinned
+    contents (no host-side copy), so the rate is the transfer + stitch pipeline's."""
+    pano = hm_stitch.StitchPano(1, levels, rig)
+    sizes = [(rig.src_w, rig.src_h)] * 2
+    first = [f1[0, ..., :3], f2[0, ..., :3]]
+    seen = [0]
+
+    def source(k):
+        return first if k < 3 else True  # fill each slot once
+
+    def sink(k, bgr):
+        seen[0] += 1
+
+    names = {hm_stitch.HM_TRANSFER_DMA: "dma", hm_stitch.HM_TRANSFER_KERNEL: "kernel"}
+    runs = {}
+    for up_mode in names:
+        for down_mode in names:
+            kw = dict(slots=3, upload=up_mode, download=down_mode)
+            hm_stitch.frame_loop(pano, sizes, source, sink, max_frames=12, **kw)  # warm-up
+            st = hm_stitch.frame_loop(pano, sizes, source, sink, max_frames=frames, **kw)
+            fps = st["frames"] / st["seconds"]
+            up, down = st["bytes_up_per_frame"], st["bytes_down_per_frame"]
+            key = f"up_{names[up_mode]}_down_{names[down_mode]}"
+            log(f"frame loop {key}: {fps:.1f} fps; upload {st['upload_ms']:.3f} ms, stitch {st['stitch_ms']:.3f} ms, "
+                f"download {st['download_ms']:.3f} ms per frame")
+            runs[key] = {"fps": round(fps, 1), "upload_ms": round(st["upload_ms"], 4),
+                         "stitch_ms": round(st["stitch_ms"], 4), "download_ms": round(st["download_ms"], 4),
+                         "upload_GBps": round(up / (st["upload_ms"] * 1e-3) / 1e9, 1) if st["upload_ms"] else None,
+                         "download_GBps": round(down / (st["download_ms"] * 1e-3) / 1e9, 1) if st["download_ms"]
+                         else None}
+    best = max(runs, key=lambda k: runs[k]["fps"])
+    return {"fps": runs[best]["fps"], "best": best, "frames": frames, "slots": 3,
+            "bytes_up_per_frame": up, "bytes_down_per_frame": down, "transfers": runs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -298,6 +335,8 @@ def main():
     ap.add_argument("--served-only", action="store_true", help="measure only the served load")
     ap.add_argument("--tuning", action="append", default=[], metavar="LEVELS:PX,TAIL,OVERLAP",
                     help="fix the launch-shape knobs for a level count instead of autotuning (e.g. 6:4,2,1)")
+    ap.add_argument("--loop-frames", type=int, default=120,
+                    help="frames of the PCIe-inclusive frame-loop measurement (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -449,6 +488,8 @@ def main():
                            "latency_ms_p99": round(float(np.percentile(r0["lat"], 99)), 4),
                            "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                                        for k, v in r0["kernels"].items()}}
+    if args.loop_frames > 0:
+        line["pcie_frame_loop"] = frame_loop_benchmark(hm_stitch, rig, f1, f2, main_l, args.loop_frames)
     if args.served_seconds > 0:
         line["served"] = served_benchmark(torch, hm_stitch, hm_synth, args.served_seconds, args.served_streams,
                                           args.served_frames)

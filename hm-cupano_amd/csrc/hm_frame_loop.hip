@@ -8,6 +8,9 @@
 //   caller's stream  [BGR -> BGRA, alpha 255]  process  [BGRA -> BGR] into dev_out[slot]
 //   download stream  dev_out[slot] -> host_out[slot]
 //
+// Each transfer is either a DMA copy (hipMemcpyAsync) or a copy kernel that
+// loads from / stores to the mapped pinned buffer directly.
+//
 // so the upload of frame k+1, the stitch of frame k and the download of
 // frame k-1 overlap, and only 3 bytes per pixel cross PCIe each way (the
 // reference converts to BGRA on the host and moves 4).  A 3-channel pano
@@ -55,8 +58,32 @@ __global__ void bgra_to_bgr_kernel(const uint8_t* __restrict__ src, size_t src_p
   d[2] = (uint8_t)(v >> 16);
 }
 
+// A plain byte copy in 16-byte words (both buffers 16-byte aligned), used to
+// move frames across PCIe with loads or stores of mapped pinned memory
+// instead of the DMA engines.
+__global__ void __launch_bounds__(256) stream_copy_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                                          size_t n) {
+  const size_t n16 = n / 16;
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  uint4* d = reinterpret_cast<uint4*>(dst);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+  if (blockIdx.x == 0)
+    for (size_t i = n16 * 16 + threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+constexpr int kCopyBlocks = 512;
+
+hipError_t kernel_copy(const void* src, void* dst, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(kCopyBlocks), dim3(256), 0, s, static_cast<const uint8_t*>(src),
+                     static_cast<uint8_t*>(dst), n);
+  return hipGetLastError();
+}
+
+// Pinned, mapped host memory; `dev` is its address for kernels.
 struct PinnedBuf {
   void* p = nullptr;
+  void* dev = nullptr;
   size_t bytes = 0;
   PinnedBuf() = default;
   PinnedBuf(const PinnedBuf&) = delete;
@@ -66,7 +93,8 @@ struct PinnedBuf {
   }
   hipError_t alloc(size_t n) {
     bytes = n;
-    return hipHostMalloc(&p, n ? n : 1, hipHostMallocDefault);
+    HM_RETURN_IF_ERROR(hipHostMalloc(&p, n ? n : 1, hipHostMallocMapped));
+    return hipHostGetDevicePointer(&dev, p, 0);
   }
 };
 
@@ -100,11 +128,18 @@ struct Slot {
 
 }  // namespace
 
-extern "C" int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* in_w, const uint32_t* in_h, int slots,
-                              int64_t max_frames, hmFrameSource source, hmFrameSink sink, void* user,
+extern "C" int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* in_w, const uint32_t* in_h,
+                              const hmFrameLoopOptions* options, hmFrameSource source, hmFrameSink sink, void* user,
                               hipStream_t stream, hmFrameLoopStats* stats) {
+  hmFrameLoopOptions opt{3, -1, HM_TRANSFER_DEFAULT, HM_TRANSFER_DEFAULT};
+  if (options) opt = *options;
+  const int slots = opt.slots ? opt.slots : 3;
+  const int64_t max_frames = opt.max_frames;
+  const bool kernel_up = opt.upload == HM_TRANSFER_KERNEL;
+  const bool kernel_down = opt.download == HM_TRANSFER_KERNEL;
   if (!pano || !in_w || !in_h || !source || !sink || n_inputs < 1 || n_inputs > HM_MAX_IMAGES || slots < 1 ||
-      slots > 8)
+      slots > 8 || opt.upload < 0 || opt.upload > HM_TRANSFER_KERNEL || opt.download < 0 ||
+      opt.download > HM_TRANSFER_KERNEL)
     return hipErrorInvalidValue;
   if (pano->status != hipSuccess) return pano->status;
   if (n_inputs != pano->n_images) return hipErrorInvalidValue;
@@ -195,7 +230,8 @@ extern "C" int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* 
     // upload
     if ((err = hipEventRecord(E(s, 0), up.s)) != hipSuccess) break;
     for (int i = 0; i < n_inputs && err == hipSuccess; ++i)
-      err = hipMemcpyAsync(sl.raw[i].p, sl.host_in[i].p, sl.raw[i].bytes, hipMemcpyHostToDevice, up.s);
+      err = kernel_up ? kernel_copy(sl.host_in[i].dev, sl.raw[i].p, sl.raw[i].bytes, up.s)
+                      : hipMemcpyAsync(sl.raw[i].p, sl.host_in[i].p, sl.raw[i].bytes, hipMemcpyHostToDevice, up.s);
     if (err != hipSuccess || (err = hipEventRecord(E(s, 1), up.s)) != hipSuccess) break;
     // stitch on the caller's stream
     if ((err = hipStreamWaitEvent(stream, E(s, 1), 0)) != hipSuccess) break;
@@ -224,7 +260,9 @@ extern "C" int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* 
     // download
     if ((err = hipStreamWaitEvent(down.s, E(s, 3), 0)) != hipSuccess) break;
     if ((err = hipEventRecord(E(s, 4), down.s)) != hipSuccess) break;
-    if ((err = hipMemcpyAsync(sl.host_out.p, sl.out.p, out_bytes, hipMemcpyDeviceToHost, down.s)) != hipSuccess) break;
+    err = kernel_down ? kernel_copy(sl.out.p, sl.host_out.dev, out_bytes, down.s)
+                      : hipMemcpyAsync(sl.host_out.p, sl.out.p, out_bytes, hipMemcpyDeviceToHost, down.s);
+    if (err != hipSuccess) break;
     if ((err = hipEventRecord(E(s, 5), down.s)) != hipSuccess) break;
     sl.frame = k;
   }

@@ -109,6 +109,15 @@ class hmTiffTags(ctypes.Structure):
                 ("predictor", ctypes.c_uint32), ("photometric", ctypes.c_uint32)]
 
 
+class hmFrameLoopOptions(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("max_frames", ctypes.c_int64), ("upload", ctypes.c_int),
+                ("download", ctypes.c_int)]
+
+
+HM_TRANSFER_DMA, HM_TRANSFER_KERNEL = 0, 1
+HM_TRANSFER_DEFAULT = HM_TRANSFER_DMA
+
+
 class hmFrameLoopStats(ctypes.Structure):
     _fields_ = [("frames", ctypes.c_int64), ("seconds", ctypes.c_double), ("upload_ms", ctypes.c_double),
                 ("stitch_ms", ctypes.c_double), ("download_ms", ctypes.c_double),
@@ -184,8 +193,8 @@ def lib() -> ctypes.CDLL:
         "hmStitchPanoSetTiming": (i, [vp, i]),
         "hmStitchPanoTimingReport": (i, [vp, ctypes.c_char_p, ctypes.c_size_t, i]),
         "hmStitchPanoDumpPyramid": (i, [vp, ctypes.c_char_p, vp]),
-        "hmFrameLoopRun": (i, [vp, i, ctypes.POINTER(u32), ctypes.POINTER(u32), i, ctypes.c_int64, FRAME_SOURCE,
-                               FRAME_SINK, vp, vp, ctypes.POINTER(hmFrameLoopStats)]),
+        "hmFrameLoopRun": (i, [vp, i, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(hmFrameLoopOptions),
+                               FRAME_SOURCE, FRAME_SINK, vp, vp, ctypes.POINTER(hmFrameLoopStats)]),
         "hmFilesLastError": (ctypes.c_char_p, []),
         "hmControlFilesLoad": (i, [ctypes.c_char_p, i, i, ctypes.POINTER(vp)]),
         "hmControlFilesGetInfo": (i, [vp, ctypes.POINTER(hmControlFilesInfo)]),
@@ -744,11 +753,13 @@ def png_read_seam(path: str):
 # ---------------------------------------------------------------------------
 # frame loop (hmFrameLoopRun)
 # ---------------------------------------------------------------------------
-def frame_loop(pano, input_sizes, source, sink, slots: int = 3, max_frames: int = -1, stream=None) -> dict:
+def frame_loop(pano, input_sizes, source, sink, slots: int = 3, max_frames: int = -1, stream=None,
+               upload: int = HM_TRANSFER_DEFAULT, download: int = HM_TRANSFER_DEFAULT) -> dict:
     """Streams host frames through `pano` (hmFrameLoopRun).
 
     input_sizes: [(w, h)] per input.  source(frame) returns one uint8 array
-    [B, h, w, 3] (or [h, w, 3]) of BGR pixels per input, or None at the end.
+    [B, h, w, 3] (or [h, w, 3]) of BGR pixels per input, None at the end, or
+    True to resend the slot's previous contents (no host copy: timing runs).
     sink(frame, bgr) receives the stitched [B * canvas_h, canvas_w, 3] BGR
     frame (a view of pinned memory, valid during the call).  Returns the
     loop's statistics; exceptions raised by the callbacks are re-raised."""
@@ -764,6 +775,8 @@ def frame_loop(pano, input_sizes, source, sink, slots: int = 3, max_frames: int 
             frames = source(int(frame))
             if frames is None:
                 return 0
+            if frames is True:  # the pinned buffers keep their previous contents
+                return 1
             if len(frames) != n:
                 raise ValueError(f"source returned {len(frames)} frames for {n} inputs")
             for i, f in enumerate(frames):
@@ -788,7 +801,8 @@ def frame_loop(pano, input_sizes, source, sink, slots: int = 3, max_frames: int 
 
     src_cb, sink_cb = FRAME_SOURCE(_src), FRAME_SINK(_sink)
     st = hmFrameLoopStats()
-    rc = L.hmFrameLoopRun(pano._h, n, ws, hs, slots, max_frames, src_cb, sink_cb, None, _stream(stream),
+    opt = hmFrameLoopOptions(slots, max_frames, upload, download)
+    rc = L.hmFrameLoopRun(pano._h, n, ws, hs, ctypes.byref(opt), src_cb, sink_cb, None, _stream(stream),
                           ctypes.byref(st))
     if errors:
         raise errors[0]

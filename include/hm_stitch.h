@@ -323,12 +323,23 @@ void hmServerDestroy(hmServer* server);
  * pitches[i]); it returns 1 for a frame, 0 at the end of the stream, < 0 on
  * error.  sink(user, frame, bgr, width, rows, pitch) receives each stitched
  * frame in order (batch x canvas-height rows; valid during the call only);
- * nonzero stops the loop.  max_frames < 0: until the source ends.  A
+ * nonzero stops the loop.  A
  * callback's error ends the loop with hipErrorUnknown and its value in
  * stats->callback_error. */
 typedef int (*hmFrameSource)(void* user, int64_t frame, void* const* host_frames, const uint32_t* pitches);
 typedef int (*hmFrameSink)(void* user, int64_t frame, const void* bgr, uint32_t width, uint32_t rows,
                            uint32_t pitch);
+enum {
+  HM_TRANSFER_DMA = 0,    /* hipMemcpyAsync on a copy stream */
+  HM_TRANSFER_KERNEL = 1, /* a copy kernel reading / writing the mapped pinned buffer */
+  HM_TRANSFER_DEFAULT = HM_TRANSFER_DMA
+};
+typedef struct hmFrameLoopOptions {
+  int slots;          /* pinned host slots in flight, 1..8 (0: 3) */
+  int64_t max_frames; /* < 0: until the source ends */
+  int upload;         /* HM_TRANSFER_* for host -> device */
+  int download;       /* HM_TRANSFER_* for device -> host */
+} hmFrameLoopOptions;
 typedef struct hmFrameLoopStats {
   int64_t frames;                /* frames delivered to the sink */
   double seconds;                /* wall time of the loop */
@@ -336,9 +347,10 @@ typedef struct hmFrameLoopStats {
   uint64_t bytes_up_per_frame, bytes_down_per_frame;
   int callback_error;
 } hmFrameLoopStats;
-int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* in_w, const uint32_t* in_h, int slots,
-                   int64_t max_frames, hmFrameSource source, hmFrameSink sink, void* user, hipStream_t stream,
-                   hmFrameLoopStats* stats);
+/* options may be NULL (3 slots, until the source ends, default transfers). */
+int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* in_w, const uint32_t* in_h,
+                   const hmFrameLoopOptions* options, hmFrameSource source, hmFrameSink sink, void* user,
+                   hipStream_t stream, hmFrameLoopStats* stats);
 
 /* ------------------------------------------------------------------------- */
 /* Pyramid inspection and launch timing (all pano classes)                   */

@@ -19,7 +19,7 @@ def bgra(f):
     return np.concatenate([f, np.full(f.shape[:-1] + (1,), 255, np.uint8)], -1)
 
 
-def run_loop(hm, pano, rig, n_frames, slots, batch=1, max_frames=-1):
+def run_loop(hm, pano, rig, n_frames, slots, batch=1, max_frames=-1, upload=0, download=0):
     frames = {k: bgr_frames(rig, batch, 1000 + 17 * k) for k in range(n_frames)}
     got = {}
 
@@ -29,7 +29,8 @@ def run_loop(hm, pano, rig, n_frames, slots, batch=1, max_frames=-1):
     def sink(k, bgr):
         got[k] = bgr.copy()
 
-    st = hm.frame_loop(pano, [(rig.src_w, rig.src_h)] * rig.n, source, sink, slots=slots, max_frames=max_frames)
+    st = hm.frame_loop(pano, [(rig.src_w, rig.src_h)] * rig.n, source, sink, slots=slots, max_frames=max_frames,
+                       upload=upload, download=download)
     return frames, got, st
 
 
@@ -48,6 +49,19 @@ def test_two_image_loop_matches_oracle(hm, torch_gpu, oracle, slots, levels):
     w, h = pano.canvas_size()
     assert st["bytes_down_per_frame"] == w * h * 3
     assert st["bytes_up_per_frame"] == 2 * rig.src_w * rig.src_h * 3
+
+
+@pytest.mark.parametrize("upload,download", [(1, 0), (0, 1), (1, 1)])
+def test_kernel_transfers(hm, torch_gpu, oracle, upload, download):
+    """Copy kernels over mapped pinned memory instead of DMA: the same frames."""
+    import hm_synth
+
+    rig = hm_synth.small_rig(seed=155)
+    pano = hm.StitchPano(1, 3, rig)
+    frames, got, _ = run_loop(hm, pano, rig, 5, 3, upload=upload, download=download)
+    for k, f in frames.items():
+        want = oracle.pano2(rig, bgra(f[0]), bgra(f[1]), 3)[0, ..., :3]
+        np.testing.assert_array_equal(got[k], want, err_msg=f"frame {k}")
 
 
 def test_three_channel_pano_loop(hm, torch_gpu, oracle):
