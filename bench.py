@@ -333,8 +333,8 @@ def main():
     ap.add_argument("--served-streams", type=int, default=4)
     ap.add_argument("--served-frames", type=int, default=640, help="frames of the closed-loop served/plain runs")
     ap.add_argument("--served-only", action="store_true", help="measure only the served load")
-    ap.add_argument("--tuning", action="append", default=[], metavar="LEVELS:PX,TAIL,OVERLAP",
-                    help="fix the launch-shape knobs for a level count instead of autotuning (e.g. 6:4,2,1)")
+    ap.add_argument("--tuning", action="append", default=[], metavar="LEVELS:PX,TAIL,OVERLAP[,XCD]",
+                    help="fix the launch-shape knobs for a level count instead of autotuning (e.g. 6:4,2,1,1)")
     ap.add_argument("--loop-frames", type=int, default=120,
                     help="frames of the PCIe-inclusive frame-loop measurement (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -384,8 +384,8 @@ def main():
         cw, ch = pano.canvas_size()
         fixed = [t.split(":", 1)[1] for t in args.tuning if int(t.split(":", 1)[0]) == levels]
         if fixed:
-            px, tb, ov = (int(v) for v in fixed[0].split(","))
-            pano.set_tuning(px, tb, ov)
+            vals = [int(v) for v in fixed[0].split(",")]
+            pano.set_tuning(*vals)
             tuning = pano.get_tuning()
             log(f"levels={levels}: fixed tuning {tuning}")
         else:

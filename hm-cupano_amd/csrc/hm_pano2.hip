@@ -31,15 +31,23 @@ constexpr int kOverlapPad = 128;  // CanvasManager overlap_pad default (canvasMa
 // Calls f(C, PX) with std::integral_constant arguments for the channel count
 // and the remap's pixels per thread (a tuning knob: 2, 4 or 8).
 template <typename F>
-void dispatch_c_px(int channels, int px, F&& f) {
+void dispatch_c_px(int channels, int px, int ry, F&& f) {
   using std::integral_constant;
+  auto per_ry = [&](auto CV, auto PV) {
+    if (ry == 2)
+      f(CV, PV, integral_constant<int, 2>{});
+    else if (ry == 4)
+      f(CV, PV, integral_constant<int, 4>{});
+    else
+      f(CV, PV, integral_constant<int, 1>{});
+  };
   auto per_c = [&](auto CV) {
     if (px == 2)
-      f(CV, integral_constant<int, 2>{});
+      per_ry(CV, integral_constant<int, 2>{});
     else if (px == 8)
-      f(CV, integral_constant<int, 8>{});
+      per_ry(CV, integral_constant<int, 8>{});
     else
-      f(CV, integral_constant<int, 4>{});
+      per_ry(CV, integral_constant<int, 4>{});
   };
   if (channels == 4)
     per_c(integral_constant<int, 4>{});
@@ -63,6 +71,7 @@ struct Pano2L0 {
   // span_tiles[k] tiles of kBlock.x * PX columns.
   int span_x0[2], span_x1[2], span_tiles[2];
   int tiles_x, tiles_y;  // tiles_x = span_tiles[0] + span_tiles[1]
+  int xcd;               // 1: tiles in horizontal bands, one per XCD (its L2 keeps the band's source lines)
 };
 
 // The canvas after both remaps (cudaPano.inl:210-303) outside the blend
@@ -70,52 +79,65 @@ struct Pano2L0 {
 // G2 = canvas after remap 2, which overwrites image 2's whole box (the
 // make_full copies, cudaPano.inl:225-236, 306-317), or image 2 alone when the
 // blend region is the whole canvas.
-template <int C, int PX>
+template <int C, int PX, int RY>
 __global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
-  const int t = (int)blockIdx.x;
+  const int t = a.xcd ? xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y) : (int)blockIdx.x;
   const int txa = t % a.tiles_x;
   const int k = txa < a.span_tiles[0] ? 0 : 1;
   const int tx = txa - (k ? a.span_tiles[0] : 0);
   const int X0 = a.span_x0[k] + tx * (int)blockDim.x * PX + (int)threadIdx.x;
   const int Xend = a.span_x1[k];
-  const int Y = (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
+  // RY rows per thread, blockDim.y apart: a tile is blockDim.x * PX wide and
+  // blockDim.y * RY tall, so the source lines that neighbouring canvas rows
+  // share are read while the tile is resident.
+  const int Y0 = (t / a.tiles_x) * (int)blockDim.y * RY + (int)threadIdx.y;
   const int b = blockIdx.z;
-  if (Y >= a.ch) return;
-  uint32_t m[2][PX];
-  bool valid[2][PX];
+  uint32_t m[RY][2][PX];
+  bool valid[RY][2][PX];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int ry = Y - a.py[i];
-    const bool row_ok = ry >= 0 && ry < a.mh[i];
-    const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
+  for (int r = 0; r < RY; ++r) {
+    const int Y = Y0 + r * (int)blockDim.y;
 #pragma unroll
-    for (int j = 0; j < PX; ++j) {
-      const int rx = X0 + j * (int)blockDim.x - a.px[i];
-      valid[i][j] = row_ok && rx >= 0 && rx < a.mw[i] && X0 + j * (int)blockDim.x < Xend;
-      m[i][j] = valid[i][j] ? mrow[rx] : 0u;
+    for (int i = 0; i < 2; ++i) {
+      const int ry = Y - a.py[i];
+      const bool row_ok = Y < a.ch && ry >= 0 && ry < a.mh[i];
+      const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
+#pragma unroll
+      for (int j = 0; j < PX; ++j) {
+        const int rx = X0 + j * (int)blockDim.x - a.px[i];
+        valid[r][i][j] = row_ok && rx >= 0 && rx < a.mw[i] && X0 + j * (int)blockDim.x < Xend;
+        m[r][i][j] = valid[r][i][j] ? mrow[rx] : 0u;
+      }
     }
   }
-  uint32_t r[2][PX];
+  uint32_t v[RY][2][PX];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int r = 0; r < RY; ++r)
 #pragma unroll
-    for (int j = 0; j < PX; ++j) r[i][j] = valid[i][j] ? remap_px<C>(a.src[i], b, m[i][j]) : 0u;
-  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
-  uint8_t* g0row = a.g0[0] + ((size_t)b * a.bh + Y) * a.bw * C;
-  uint8_t* g1row = a.g0[1] + ((size_t)b * a.bh + Y) * a.bw * C;
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-  for (int j = 0; j < PX; ++j) {
-    const int X = X0 + j * (int)blockDim.x;
-    if (X >= Xend) continue;
-    const uint32_t c1 = valid[0][j] ? r[0][j] : 0u;
-    const uint32_t c12 = valid[1][j] ? r[1][j] : c1;
-    if (a.write_canvas && !(X >= a.skip_x0 && X < a.skip_x1)) store_px<C>(crow + (size_t)X * C, c12);
-    if (Y < a.bh) {
-      const int u1 = X - a.d[0];
-      if (u1 >= a.ulo[0] && u1 < a.uhi[0]) store_px<C>(g0row + (size_t)u1 * C, c1);
-      const int u2 = X - a.d[1];
-      if (u2 >= a.ulo[1] && u2 < a.uhi[1])
-        store_px<C>(g1row + (size_t)u2 * C, a.g2_canvas ? c12 : (valid[1][j] ? r[1][j] : 0u));
+      for (int j = 0; j < PX; ++j) v[r][i][j] = remap_px_sel<C>(a.src[i], b, m[r][i][j], valid[r][i][j]);
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int Y = Y0 + r * (int)blockDim.y;
+    if (Y >= a.ch) break;
+    uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+    uint8_t* g0row = a.g0[0] + ((size_t)b * a.bh + Y) * a.bw * C;
+    uint8_t* g1row = a.g0[1] + ((size_t)b * a.bh + Y) * a.bw * C;
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+      const int X = X0 + j * (int)blockDim.x;
+      if (X >= Xend) continue;
+      const uint32_t c1 = valid[r][0][j] ? v[r][0][j] : 0u;
+      const uint32_t c12 = valid[r][1][j] ? v[r][1][j] : c1;
+      if (a.write_canvas && !(X >= a.skip_x0 && X < a.skip_x1)) store_px<C>(crow + (size_t)X * C, c12);
+      if (Y < a.bh) {
+        const int u1 = X - a.d[0];
+        if (u1 >= a.ulo[0] && u1 < a.uhi[0]) store_px<C>(g0row + (size_t)u1 * C, c1);
+        const int u2 = X - a.d[1];
+        if (u2 >= a.ulo[1] && u2 < a.uhi[1])
+          store_px<C>(g1row + (size_t)u2 * C, a.g2_canvas ? c12 : (valid[r][1][j] ? v[r][1][j] : 0u));
+      }
     }
   }
 }
@@ -134,47 +156,62 @@ struct Pano2Hard {
 // Hard seam: the two dest-map remaps of cudaPano.inl:262-276 and :345-359
 // (image 1 where seam == 1, image 2 where seam == 0) plus the canvas memset,
 // as one pass; PX pixels per thread as above.
-template <int C, int PX>
+template <int C, int PX, int RY>
 __global__ void __launch_bounds__(256) pano2_hard_kernel(Pano2Hard a) {
   const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
   const int X0 = (t % a.tiles_x) * (int)blockDim.x * PX + (int)threadIdx.x;
-  const int Y = (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
+  const int Y0 = (t / a.tiles_x) * (int)blockDim.y * RY + (int)threadIdx.y;
   const int b = blockIdx.z;
-  if (Y >= a.ch) return;
-  const uint8_t* srow = a.seam + (size_t)Y * a.cw;
-  uint32_t m[PX];
-  int sel[PX];  // 0: image 1, 1: image 2, -1: nothing
+  // Three rounds of loads, each issued in full before the next: the seam
+  // bytes, the selected maps, the source pixels.  Loads outside the canvas or
+  // a map go to element 0 (a valid address) and are discarded.
+  uint32_t sv[RY][PX];
 #pragma unroll
-  for (int j = 0; j < PX; ++j) {
-    const int X = X0 + j * (int)blockDim.x;
-    sel[j] = -1;
-    m[j] = 0u;
-    if (X >= a.cw) continue;
-    const uint8_t sv = srow[X];
-    if (sv > 1) continue;
-    // Scalar selects: a dynamic index into the kernel-argument arrays would
-    // force them into scratch memory.
-    const bool first = sv == 1;
-    const int rx = X - (first ? a.px[0] : a.px[1]);
-    const int ry = Y - (first ? a.py[0] : a.py[1]);
-    const int mw = first ? a.mw[0] : a.mw[1];
-    const int mh = first ? a.mh[0] : a.mh[1];
-    if (rx >= 0 && ry >= 0 && rx < mw && ry < mh) {
-      sel[j] = first ? 0 : 1;
-      m[j] = (first ? a.map[0] : a.map[1])[(size_t)ry * mw + rx];
+  for (int r = 0; r < RY; ++r) {
+    const int Y = Y0 + r * (int)blockDim.y;
+    const uint8_t* srow = a.seam + (size_t)(Y < a.ch ? Y : 0) * a.cw;
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+      const int X = X0 + j * (int)blockDim.x;
+      sv[r][j] = srow[X < a.cw ? X : 0];
     }
   }
-  uint32_t v[PX];
+  uint32_t m[RY][PX];
+  int sel[RY][PX];  // 0: image 1, 1: image 2, -1: nothing
 #pragma unroll
-  for (int j = 0; j < PX; ++j) {
-    v[j] = 0u;
-    if (sel[j] >= 0) v[j] = remap_px_dm<C>(sel[j] == 0 ? a.src[0] : a.src[1], b, m[j]);
+  for (int r = 0; r < RY; ++r) {
+    const int Y = Y0 + r * (int)blockDim.y;
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+      const int X = X0 + j * (int)blockDim.x;
+      // Scalar selects: a dynamic index into the kernel-argument arrays would
+      // force them into scratch memory.
+      const bool first = sv[r][j] == 1;
+      const int rx = X - (first ? a.px[0] : a.px[1]);
+      const int ry = Y - (first ? a.py[0] : a.py[1]);
+      const int mw = first ? a.mw[0] : a.mw[1];
+      const int mh = first ? a.mh[0] : a.mh[1];
+      const bool hit = X < a.cw && Y < a.ch && sv[r][j] <= 1 && rx >= 0 && ry >= 0 && rx < mw && ry < mh;
+      sel[r][j] = hit ? (first ? 0 : 1) : -1;
+      m[r][j] = (first ? a.map[0] : a.map[1])[hit ? (size_t)ry * mw + rx : 0];
+    }
   }
-  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+  uint32_t v[RY][PX];
 #pragma unroll
-  for (int j = 0; j < PX; ++j) {
-    const int X = X0 + j * (int)blockDim.x;
-    if (X < a.cw) store_px<C>(crow + (size_t)X * C, v[j]);
+  for (int r = 0; r < RY; ++r)
+#pragma unroll
+    for (int j = 0; j < PX; ++j)
+      v[r][j] = remap_px_dm_sel<C>(sel[r][j] == 0 ? a.src[0] : a.src[1], b, m[r][j], sel[r][j] >= 0);
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int Y = Y0 + r * (int)blockDim.y;
+    if (Y >= a.ch) break;
+    uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+      const int X = X0 + j * (int)blockDim.x;
+      if (X < a.cw) store_px<C>(crow + (size_t)X * C, v[r][j]);
+    }
   }
 }
 
@@ -319,8 +356,8 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   if ((int)canvas.width != canvas_w || (int)canvas.height != canvas_h) return hipErrorInvalidValue;
   const Surf s1 = to_surf(in1), s2 = to_surf(in2);
   if (hard_seam) {
-    const int px = tuning.remap_px;
-    const hm::TileGrid tg = hm::tile_grid(canvas_w, canvas_h, batch, dim3(kBlock.x * px, kBlock.y, 1));
+    const int px = tuning.remap_px, ry = tuning.remap_rows;
+    const hm::TileGrid tg = hm::tile_grid(canvas_w, canvas_h, batch, dim3(kBlock.x * px, kBlock.y * ry, 1));
     const dim3 grid = tg.grid;
     Pano2Hard a{};
     a.src[0] = s1;
@@ -342,10 +379,11 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     a.tiles_x = tg.tiles_x;
     a.tiles_y = tg.tiles_y;
     timer.begin(s);
-    auto go = [&](auto CV, auto PV) {
-      hipLaunchKernelGGL((pano2_hard_kernel<decltype(CV)::value, decltype(PV)::value>), grid, kBlock, 0, s, a);
+    auto go = [&](auto CV, auto PV, auto RV) {
+      hipLaunchKernelGGL((pano2_hard_kernel<decltype(CV)::value, decltype(PV)::value, decltype(RV)::value>), grid,
+                         kBlock, 0, s, a);
     };
-    dispatch_c_px(channels, px, go);
+    dispatch_c_px(channels, px, ry, go);
     timer.end(s, "hard_seam");
     return hipGetLastError();
   }
@@ -371,7 +409,9 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   a.bh = bh;
   const int px = tuning.remap_px;
   const int tile_w = (int)kBlock.x * px;
-  a.tiles_y = (canvas_h + (int)kBlock.y - 1) / (int)kBlock.y;
+  const int ry = tuning.remap_rows;
+  a.tiles_y = (canvas_h + (int)kBlock.y * ry - 1) / ((int)kBlock.y * ry);
+  a.xcd = tuning.remap_xcd;
   auto launch_spans = [&](int x0a, int x1a, int x0b, int x1b, hipStream_t st, const char* name) -> hipError_t {
     Pano2L0 k = a;
     k.span_x0[0] = x0a;
@@ -384,10 +424,11 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     if (k.tiles_x == 0) return hipSuccess;
     const dim3 grid((unsigned)(k.tiles_x * k.tiles_y), 1u, (unsigned)batch);
     timer.begin(st);
-    auto go = [&](auto CV, auto PV) {
-      hipLaunchKernelGGL((pano2_level0_kernel<decltype(CV)::value, decltype(PV)::value>), grid, kBlock, 0, st, k);
+    auto go = [&](auto CV, auto PV, auto RV) {
+      hipLaunchKernelGGL((pano2_level0_kernel<decltype(CV)::value, decltype(PV)::value, decltype(RV)::value>), grid,
+                         kBlock, 0, st, k);
     };
-    dispatch_c_px(channels, px, go);
+    dispatch_c_px(channels, px, ry, go);
     timer.end(st, name);
     return hipGetLastError();
   };

@@ -116,7 +116,8 @@ extern "C" int hmStitchPanoGetTuning(const hmStitchPano* pano, hmTuning* tuning)
 
 static bool tuning_ok(const hmTuning& t) {
   return (t.remap_px == 2 || t.remap_px == 4 || t.remap_px == 8) && t.tail_base >= 0 && t.tail_base <= 3 &&
-         (t.overlap == 0 || t.overlap == 1);
+         (t.overlap == 0 || t.overlap == 1) && (t.remap_xcd == 0 || t.remap_xcd == 1) &&
+         (t.remap_rows == 1 || t.remap_rows == 2 || t.remap_rows == 4);
 }
 
 extern "C" int hmStitchPanoSetTuning(hmStitchPano* pano, const hmTuning* tuning) {
@@ -136,6 +137,8 @@ extern "C" int hmStitchPanoAutotune(hmStitchPano* pano, const hmSurface* inputs,
   std::vector<int> pxs = pano->tunes_remap_px() ? std::vector<int>{2, 4, 8} : std::vector<int>{start.remap_px};
   std::vector<int> tbs = pano->tunes_tail() ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{start.tail_base};
   std::vector<int> ovs = pano->tunes_overlap() ? std::vector<int>{0, 1} : std::vector<int>{start.overlap};
+  std::vector<int> xcs = pano->tunes_remap_px() ? std::vector<int>{0, 1} : std::vector<int>{start.remap_xcd};
+  std::vector<int> rys = pano->tunes_remap_px() ? std::vector<int>{1, 2, 4} : std::vector<int>{start.remap_rows};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HM_RETURN_IF_ERROR(hipEventCreate(&e0));
   hipError_t err = hipEventCreate(&e1);
@@ -143,9 +146,11 @@ extern "C" int hmStitchPanoAutotune(hmStitchPano* pano, const hmSurface* inputs,
   float chosen_ms = -1.0f;
   for (int px : pxs)
     for (int tb : tbs)
-      for (int ov : ovs) {
+      for (int ov : ovs)
+        for (int xc : xcs)
+          for (int ry : rys) {
         if (err != hipSuccess) break;
-        pano->tuning = hmTuning{px, tb, ov};
+        pano->tuning = hmTuning{px, tb, ov, xc, ry};
         pano->apply_tuning();
         err = pano->process_any(inputs, num_inputs, canvas, fused, stream);  // warm-up
         if (err == hipSuccess) err = hipEventRecord(e0, stream);

@@ -50,6 +50,29 @@ __device__ __forceinline__ uint32_t remap_px_dm(const Surf& s, int b, uint32_t m
   return 0u;
 }
 
+// The same two pixels without a branch around the gather: a pixel that is
+// not `take`n, or lies outside the source, loads source pixel (0, 0) (always
+// a valid address) and the result is selected afterwards.  A branch per
+// gather makes the compiler wait for each load before the next one; this form
+// keeps every gather of a thread in flight at once.
+template <int C>
+__device__ __forceinline__ uint32_t remap_px_sel(const Surf& s, int b, uint32_t m, bool take) {
+  const uint32_t sx = m & 0xffffu, sy = m >> 16;
+  const bool inr = take && sx < s.w && sy < s.h;
+  const uint32_t v = load_src_px<C>(s, b, inr ? (int)sx : 0, inr ? (int)sy : 0);
+  uint32_t fb = 0u;
+  if constexpr (C == 4) fb = (take && sx != (uint32_t)kUnmapped) ? 0xff000000u : 0u;
+  return inr ? v : fb;
+}
+
+template <int C>
+__device__ __forceinline__ uint32_t remap_px_dm_sel(const Surf& s, int b, uint32_t m, bool take) {
+  const uint32_t sx = m & 0xffffu, sy = m >> 16;
+  const bool inr = take && sx < s.w && sy < s.h;
+  const uint32_t v = load_src_px<C>(s, b, inr ? (int)sx : 0, inr ? (int)sy : 0);
+  return inr ? v : 0u;
+}
+
 inline Surf to_surf(const hmSurface& s) {
   return Surf{static_cast<uint8_t*>(s.d_ptr), s.width, s.height, s.pitch};
 }
@@ -113,7 +136,8 @@ __global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
     }
     uint32_t r[kPx];
 #pragma unroll
-    for (int j = 0; j < kPx; ++j) r[j] = in[j] ? (DM ? remap_px_dm<C>(a.src[i], b, m[j]) : remap_px<C>(a.src[i], b, m[j])) : 0u;
+    for (int j = 0; j < kPx; ++j)
+      r[j] = DM ? remap_px_dm_sel<C>(a.src[i], b, m[j], in[j]) : remap_px_sel<C>(a.src[i], b, m[j], in[j]);
     uint8_t* grow = a.g0[i] + ((size_t)b * a.bh + v) * a.bw * C;
 #pragma unroll
     for (int j = 0; j < kPx; ++j) {
@@ -171,16 +195,27 @@ __global__ void __launch_bounds__(256) multi_canvas_kernel(MultiArgs<N> a) {
         hit[j] = in_box && (a.mode == kCanvasLast || lab[j] == i);
         m[j] = hit[j] ? a.map[i][(size_t)ry * a.mw[i] + rx] : 0u;
       }
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < kPx; ++j) any = any || hit[j];
+      if (!any) continue;
+      // every gather first (branch-free, pixel (0, 0) for a miss), then the selects
+      uint32_t g[kPx];
+#pragma unroll
+      for (int j = 0; j < kPx; ++j) g[j] = remap_px_dm_sel<C>(a.src[i], b, m[j], hit[j]);
 #pragma unroll
       for (int j = 0; j < kPx; ++j) {
         if (!hit[j]) continue;
+        const uint32_t sx = m[j] & 0xffffu, sy = m[j] >> 16;
+        const bool ok = sx < a.src[i].w && sy < a.src[i].h;
         if (a.mode == kCanvasLast) {
-          v[j] = remap_px<C>(a.src[i], b, m[j]);
+          // remap_px: out of range -> alpha 0 for the sentinel, 255 otherwise (4 channels)
+          uint32_t fb = 0u;
+          if constexpr (C == 4) fb = sx == (uint32_t)kUnmapped ? 0u : 0xff000000u;
+          v[j] = ok ? g[j] : fb;
           wr[j] = true;
         } else {
-          const uint32_t sx = m[j] & 0xffffu, sy = m[j] >> 16;
-          const bool ok = sx < a.src[i].w && sy < a.src[i].h;
-          v[j] = ok ? load_src_px<C>(a.src[i], b, (int)sx, (int)sy) : 0u;
+          v[j] = ok ? g[j] : 0u;
           wr[j] = ok || a.invalid_zero;
         }
       }

@@ -262,6 +262,27 @@ hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int bat
   a.wy0 = u.wy0;
   a.wx1 = u.wx1;
   a.wy1 = u.wy1;
+  if (u.n - 1 <= kTailLds) {
+    // every level's footprint in LDS, loaded in one burst
+    constexpr int T = kTailLdsTile;
+    a.tiles_x = (u.wx1 - u.wx0 + T) / T;
+    a.tiles_y = (u.wy1 - u.wy0 + T) / T;
+    const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+    if (c == 4) {
+      if (kind == kBlend2)
+        hipLaunchKernelGGL((up_tail_lds_kernel<4, 2, kBlend2, T>), grid, dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((up_tail_lds_kernel<4, 2, kBlendN, T>), grid, dim3(256), 0, s, a);
+    } else if (c == 3) {
+      if (kind == kBlend2)
+        hipLaunchKernelGGL((up_tail_lds_kernel<3, 2, kBlend2, T>), grid, dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((up_tail_lds_kernel<3, 2, kBlendN, T>), grid, dim3(256), 0, s, a);
+    } else {
+      return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   a.tiles_x = (u.wx1 - u.wx0 + kTailTile) / kTailTile;
   a.tiles_y = (u.wy1 - u.wy0 + kTailTile) / kTailTile;
   const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);

@@ -465,6 +465,159 @@ __global__ void __launch_bounds__(256) up_tail_kernel(UpTailArgs<N> a) {
   }
 }
 
+// The same tail with every level's footprint (Gaussians and mask) read into
+// LDS first, in one burst: each thread issues all of its loads (for all
+// levels) before the first LDS write, so the workgroup pays one memory round
+// trip instead of one per level, and the level-by-level reconstruction then
+// runs from LDS.  T x T base tiles; up to kTailLds levels above the base.
+// Same helpers and operation order as up_tail_kernel: the same bits.
+constexpr int kTailLds = 4;
+constexpr int kTailLdsTile = 16;
+
+template <int C, int N, int KIND, int T>
+__global__ void __launch_bounds__(256) up_tail_lds_kernel(UpTailArgs<N> a) {
+  constexpr int M = KIND == kBlend2 ? 1 : N;
+  constexpr int RU = T / 2 + 2;                      // footprint edge above the base
+  constexpr int CAP = T * T + kTailLds * RU * RU;    // pixels of all footprints
+  constexpr int SB = (T * T + 255) / 256;            // base-level pixels per thread
+  constexpr int SU = (RU * RU + 255) / 256;          // pixels per thread at each level above
+  __shared__ float4 gl[N * CAP];                     // [level][image][footprint pixel]
+  __shared__ float ml[M * CAP];                      // [level][footprint pixel][M]
+  __shared__ float4 rb[2][RU * RU];
+  __shared__ int reg[kTailLds + 1][4];               // footprint [x0, x1] x [y0, y1]
+  __shared__ int cum[kTailLds + 2];                  // pixels of the footprints before level j
+  const int t = (int)blockIdx.x;
+  const int tx = t % a.tiles_x, ty = t / a.tiles_x;
+  const int b = blockIdx.z;
+  const int tid = threadIdx.x;
+  const int n = a.n;
+  if (tid == 0) {
+    int x0 = a.wx0 + tx * T, y0 = a.wy0 + ty * T;
+    int x1 = min(x0 + T - 1, a.wx1), y1 = min(y0 + T - 1, a.wy1);
+    reg[0][0] = x0; reg[0][1] = x1; reg[0][2] = y0; reg[0][3] = y1;
+    cum[0] = 0;
+    cum[1] = (x1 - x0 + 1) * (y1 - y0 + 1);
+    for (int j = 1; j < n; ++j) {
+      x0 >>= 1;
+      y0 >>= 1;
+      x1 = min((x1 >> 1) + 1, a.lv[j].w - 1);
+      y1 = min((y1 >> 1) + 1, a.lv[j].h - 1);
+      reg[j][0] = x0; reg[j][1] = x1; reg[j][2] = y0; reg[j][3] = y1;
+      cum[j + 1] = cum[j] + (x1 - x0 + 1) * (y1 - y0 + 1);
+    }
+  }
+  __syncthreads();
+  // 1. the burst: every footprint pixel of every level.  The level loop is
+  // unrolled (uniform level: descriptors by scalar loads); each thread holds
+  // its pixels of all levels in registers before the first LDS write.
+  float4 gv[SB + kTailLds * SU][N];
+  float mv[SB + kTailLds * SU][M];
+#pragma unroll
+  for (int j = 0; j <= kTailLds; ++j) {
+    if (j < n) {
+      const TailLevel<N>& d = a.lv[j];
+      const int rw = reg[j][1] - reg[j][0] + 1, area = cum[j + 1] - cum[j];
+#pragma unroll
+      for (int s = 0; s < (j == 0 ? SB : SU); ++s) {
+        const int idx = s * 256 + tid;
+        if (idx < area) {
+          const int x = reg[j][0] + idx % rw, y = reg[j][2] + idx / rw;
+          const size_t pix = ((size_t)b * d.h + y) * d.w + x;
+          const int slot = (j == 0 ? 0 : SB + (j - 1) * SU) + s;
+#pragma unroll
+          for (int i = 0; i < N; ++i) gv[slot][i] = ld_px<C>(d.g[i], pix);
+#pragma unroll
+          for (int c = 0; c < M; ++c) mv[slot][c] = d.mask[((size_t)y * d.w + x) * M + c];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j <= kTailLds; ++j) {
+    if (j < n) {
+      const int area = cum[j + 1] - cum[j];
+#pragma unroll
+      for (int s = 0; s < (j == 0 ? SB : SU); ++s) {
+        const int idx = s * 256 + tid;
+        if (idx < area) {
+          const int slot = (j == 0 ? 0 : SB + (j - 1) * SU) + s;
+#pragma unroll
+          for (int i = 0; i < N; ++i) gl[N * cum[j] + i * area + idx] = gv[slot][i];
+#pragma unroll
+          for (int c = 0; c < M; ++c) ml[(cum[j] + idx) * M + c] = mv[slot][c];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  auto gpx = [&](int j, int i, int lidx) -> float4 { return gl[N * cum[j] + i * (cum[j + 1] - cum[j]) + lidx]; };
+  int cur = 0;
+  // 2. coarsest level: R = blend of the Gaussians
+  {
+    const int j = n - 1;
+    const int rw = reg[j][1] - reg[j][0] + 1, area = cum[j + 1] - cum[j];
+    (void)rw;
+    for (int idx = tid; idx < area; idx += blockDim.x) {
+      float4 Lv[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) Lv[i] = gpx(j, i, idx);
+      rb[cur][idx] = blend_any<C, N, KIND>(Lv, ml + cum[j] * M, (size_t)idx);
+    }
+  }
+  // 3. levels n-2 .. 0 in quads, everything from LDS; the base writes R
+  for (int j = n - 2; j >= 0; --j) {
+    __syncthreads();
+    cur ^= 1;
+    const TailLevel<N>& d = a.lv[j];
+    const TailLevel<N>& dn = a.lv[j + 1];
+    const int rx0 = reg[j][0], rx1 = reg[j][1], ry0 = reg[j][2], ry1 = reg[j][3];
+    const int rw = rx1 - rx0 + 1;
+    const int qx0 = rx0 >> 1, qy0 = ry0 >> 1;
+    const int nqx = (rx1 >> 1) - qx0 + 1, nqy = (ry1 >> 1) - qy0 + 1;
+    const int nx0 = reg[j + 1][0], ny0 = reg[j + 1][2], nrw = reg[j + 1][1] - nx0 + 1;
+    const float4* prev = rb[cur ^ 1];
+    const float* mk = ml + cum[j] * M;
+    for (int q = tid; q < nqx * nqy; q += blockDim.x) {
+      const int qx = qx0 + q % nqx, qy = qy0 + q / nqx;
+      const int qx1 = min(qx + 1, dn.w - 1), qy1 = min(qy + 1, dn.h - 1);
+      const int lx = qx - nx0, lx1 = qx1 - nx0, ly = qy - ny0, ly1 = qy1 - ny0;
+      float4 gt[N][4];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        gt[i][0] = gpx(j + 1, i, ly * nrw + lx);
+        gt[i][1] = gpx(j + 1, i, ly * nrw + lx1);
+        gt[i][2] = gpx(j + 1, i, ly1 * nrw + lx);
+        gt[i][3] = gpx(j + 1, i, ly1 * nrw + lx1);
+      }
+      float4 rt[4];
+      rt[0] = prev[ly * nrw + lx];
+      rt[1] = prev[ly * nrw + lx1];
+      rt[2] = prev[ly1 * nrw + lx];
+      rt[3] = prev[ly1 * nrw + lx1];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+        if (x < rx0 || x > rx1 || y < ry0 || y > ry1) continue;
+        const int lidx = (y - ry0) * rw + (x - rx0);
+        float4 Lv[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const float4 g = gpx(j, i, lidx);
+          const float4 up = up_quad_lap<C>(e, gt[i]);
+          Lv[i] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+        }
+        const float4 B = blend_any<C, N, KIND>(Lv, mk, (size_t)lidx);
+        const float4 up = up_quad_rec<C>(e, rt);
+        const float4 R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+        if (j == 0)
+          st_px<C>(a.r_out, ((size_t)b * d.h + y) * d.w + x, R);
+        else
+          rb[cur][lidx] = R;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // one level in 2x2 quads (fields named as level 0 and its next level 1)
 // ---------------------------------------------------------------------------

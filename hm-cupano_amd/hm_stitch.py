@@ -79,7 +79,8 @@ class hmServerStats(ctypes.Structure):
 
 
 class hmTuning(ctypes.Structure):
-    _fields_ = [("remap_px", ctypes.c_int), ("tail_base", ctypes.c_int), ("overlap", ctypes.c_int)]
+    _fields_ = [("remap_px", ctypes.c_int), ("tail_base", ctypes.c_int), ("overlap", ctypes.c_int),
+                ("remap_xcd", ctypes.c_int), ("remap_rows", ctypes.c_int)]
 
 
 class hmControlFilesInfo(ctypes.Structure):
@@ -325,8 +326,9 @@ class StitchPano:
         check(lib().hmStitchPanoGetTuning(self._h, ctypes.byref(t)), "hmStitchPanoGetTuning")
         return {k: getattr(t, k) for k, _ in hmTuning._fields_}
 
-    def set_tuning(self, remap_px: int = 4, tail_base: int = 2, overlap: int = 1) -> None:
-        t = hmTuning(remap_px, tail_base, overlap)
+    def set_tuning(self, remap_px: int = 4, tail_base: int = 2, overlap: int = 1, remap_xcd: int = 0,
+                   remap_rows: int = 1) -> None:
+        t = hmTuning(remap_px, tail_base, overlap, remap_xcd, remap_rows)
         check(lib().hmStitchPanoSetTuning(self._h, ctypes.byref(t)), "hmStitchPanoSetTuning")
 
     def autotune(self, inputs, canvas, frames: int = 10, fused: bool = True, stream=None):
@@ -552,8 +554,9 @@ class BlendContext:
         check(lib().hmStitchPanoGetTuning(self._h, ctypes.byref(t)), "hmStitchPanoGetTuning")
         return {k: getattr(t, k) for k, _ in hmTuning._fields_}
 
-    def set_tuning(self, remap_px: int = 4, tail_base: int = 2, overlap: int = 1) -> None:
-        t = hmTuning(remap_px, tail_base, overlap)
+    def set_tuning(self, remap_px: int = 4, tail_base: int = 2, overlap: int = 1, remap_xcd: int = 0,
+                   remap_rows: int = 1) -> None:
+        t = hmTuning(remap_px, tail_base, overlap, remap_xcd, remap_rows)
         check(lib().hmStitchPanoSetTuning(self._h, ctypes.byref(t)), "hmStitchPanoSetTuning")
 
     def autotune(self, inputs, canvas, frames: int = 10, fused: bool = True, stream=None):

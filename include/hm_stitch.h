@@ -194,6 +194,8 @@ typedef struct hmTuning {
   int remap_px;
   int tail_base;
   int overlap;
+  int remap_xcd;  /* 1: 2-image remap tiles ordered as horizontal canvas bands, one per XCD */
+  int remap_rows; /* canvas rows per thread of the 2-image remaps: 1, 2 or 4 (taller tiles) */
 } hmTuning;
 int hmStitchPanoGetTuning(const hmStitchPano* pano, hmTuning* tuning);
 int hmStitchPanoSetTuning(hmStitchPano* pano, const hmTuning* tuning);
