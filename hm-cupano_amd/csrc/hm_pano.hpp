@@ -28,7 +28,7 @@ struct hmStitchPano {
   hm::LaunchTimer* timer_or_null() { return timer.enabled() ? &timer : nullptr; }
 
   // Launch-shape knobs (hmTuning); every setting computes the same output.
-  hmTuning tuning{4, 2, 1, 0, 1};
+  hmTuning tuning{4, 2, 1, 0, 1, 0, 0};
   // Which knobs matter for this object (the autotuner sweeps only those).
   virtual bool tunes_remap_px() const { return false; }
   bool tunes_tail() const { return !hard_seam && n_images == 2 && num_levels >= 4; }

@@ -80,8 +80,7 @@ struct Pano2L0 {
 // make_full copies, cudaPano.inl:225-236, 306-317), or image 2 alone when the
 // blend region is the whole canvas.
 template <int C, int PX, int RY>
-__global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
-  const int t = a.xcd ? xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y) : (int)blockIdx.x;
+__device__ __forceinline__ void level0_tile(const Pano2L0& a, int t) {
   const int txa = t % a.tiles_x;
   const int k = txa < a.span_tiles[0] ? 0 : 1;
   const int tx = txa - (k ? a.span_tiles[0] : 0);
@@ -140,6 +139,115 @@ __global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
       }
     }
   }
+}
+
+// One tile per workgroup, or (a.grid_tiles set) a persistent grid that walks
+// the tiles with a stride: a smaller grid holds fewer CU slots and memory
+// requests, leaving room to latency-bound work on another stream.
+template <int C, int PX, int RY>
+__global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
+  const int total = a.tiles_x * a.tiles_y;
+  for (int tt = (int)blockIdx.x; tt < total; tt += (int)gridDim.x)
+    level0_tile<C, PX, RY>(a, a.xcd ? xcd_tile(tt, total) : tt);
+}
+
+// The blend-input remap fused into the first down launch.  A workgroup owns
+// a 32x32 tile of the level-0 blend inputs: each thread remaps its 2x2 block
+// for both inputs (G1 = the canvas after remap 1, G2 = after remap 2 or
+// image 2 alone, as level0_tile computes them), stores the two uint8 blocks
+// (the level-0 pass reads them), and takes the 2x2 mean of each straight from
+// registers into the tile of level 1; levels 2.. follow in LDS as in
+// down_fused_kernel.  Used when the remap of the blend columns writes nothing
+// else (no canvas pixels), so this replaces that launch and the first down
+// launch's read of the level-0 inputs.
+struct RemapDownArgs {
+  Pano2L0 r;                     // remap geometry and the G0 outputs (bw x bh)
+  DownFusedArgs<uint8_t, 2> dn;  // the down pass over G0 (w0 = bw, h0 = bh)
+};
+
+template <int C>
+__device__ __forceinline__ uint32_t remap_at(const Pano2L0& r, int i, int b, int X, int Y) {
+  const int rx = X - r.px[i], ry = Y - r.py[i];
+  const bool ok = rx >= 0 && ry >= 0 && rx < r.mw[i] && ry < r.mh[i];
+  const uint32_t m = r.map[i][ok ? (size_t)ry * r.mw[i] + rx : 0];
+  return remap_px_sel<C>(r.src[i], b, m, ok);
+}
+
+template <int C>
+__device__ __forceinline__ bool covers(const Pano2L0& r, int i, int X, int Y) {
+  const int rx = X - r.px[i], ry = Y - r.py[i];
+  return rx >= 0 && ry >= 0 && rx < r.mw[i] && ry < r.mh[i];
+}
+
+template <int C>
+__global__ void __launch_bounds__(256) remap_down_kernel(RemapDownArgs A) {
+  __shared__ float4 s1[2][16 * 16];
+  __shared__ float4 s2[2][8 * 8];
+  const Pano2L0& r = A.r;
+  const DownFusedArgs<uint8_t, 2>& a = A.dn;
+  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
+  const int tx = t % a.tiles_x, ty = t / a.tiles_x;
+  const int b = blockIdx.z;
+  const int tid = threadIdx.x;
+  const DownLevel<2>& d = a.lv[0];
+  const int lx = tid & 15, ly = tid >> 4;
+  const int x = tx * 16 + lx, y = ty * 16 + ly;  // level-1 pixel
+  if (x < d.w && y < d.h) {
+    const int u0 = 2 * x, v0 = 2 * y;  // its 2x2 block of level 0
+    const bool has_x1 = u0 + 1 < a.w0, has_y1 = v0 + 1 < a.h0;
+    // Remaps first, every load of the block in flight at once: image 1 at
+    // the G1 column, image 2 (and image 1, for the canvas after both) at the
+    // G2 column.  Pixels past the edge repeat an in-range one (not stored,
+    // and ignored by down_px).
+    uint32_t g1[4], g2[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int u = (e & 1) && has_x1 ? u0 + 1 : u0, v = (e >> 1) && has_y1 ? v0 + 1 : v0;
+      const int X1 = u + r.d[0], X2 = u + r.d[1];
+      const uint32_t c1 = remap_at<C>(r, 0, b, X1, v);
+      const uint32_t i2 = remap_at<C>(r, 1, b, X2, v);
+      const bool cov1 = covers<C>(r, 0, X1, v), cov2 = covers<C>(r, 1, X2, v);
+      uint32_t c1b = c1;
+      bool cov1b = cov1;
+      if (r.d[1] != r.d[0]) {  // uniform: image 1 under G2's column too
+        c1b = remap_at<C>(r, 0, b, X2, v);
+        cov1b = covers<C>(r, 0, X2, v);
+      }
+      const uint32_t g1v = cov1 ? c1 : 0u;
+      const uint32_t c12 = cov2 ? i2 : (cov1b ? c1b : 0u);
+      const uint32_t g2v = r.g2_canvas ? c12 : (cov2 ? i2 : 0u);
+      g1[e] = (u >= r.ulo[0] && u < r.uhi[0]) ? g1v : 0u;
+      g2[e] = (u >= r.ulo[1] && u < r.uhi[1]) ? g2v : 0u;
+    }
+    // the uint8 blend inputs
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (((e & 1) && !has_x1) || ((e >> 1) && !has_y1)) continue;
+      const int u = u0 + (e & 1), v = v0 + (e >> 1);
+      const size_t o = (((size_t)b * a.h0 + v) * a.w0 + u) * C;
+      store_px<C>(r.g0[0] + o, g1[e]);
+      store_px<C>(r.g0[1] + o, g2[e]);
+    }
+    // level 1: the 2x2 means (ld_px's uint8 -> float conversion)
+    const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
+    float4 p[2][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint32_t q[2] = {g1[e], g2[e]};
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        p[i][e] = make_float4((float)(q[i] & 0xffu), (float)((q[i] >> 8) & 0xffu), (float)((q[i] >> 16) & 0xffu),
+                              C == 4 ? (float)(q[i] >> 24) : 0.0f);
+    }
+    const size_t o1 = ((size_t)b * d.h + y) * d.w + x;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const float4 v = down_px<C>(p[i], in_range);
+      s1[i][ly * 16 + lx] = v;
+      if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], o1, v);
+    }
+  }
+  down_rest<C, 2>(a, tx, ty, b, tid, s1, s2);
 }
 
 struct Pano2Hard {
@@ -234,6 +342,7 @@ struct Pano2 : hmStitchPano {
   hm::DevBuf mask_u8;
   Pano2L0 l0{};
   int roi_x0 = 0, roi_x1 = 0;  // canvas columns whose remap feeds the blend inputs
+  bool fuse_ok = false;        // that remap writes no canvas pixel: it can run inside the first down launch
 
   hipError_t setup(const hmControlMasks2& m, int minimize_blend);
   hipError_t ensure_pyramid();
@@ -248,7 +357,8 @@ struct Pano2 : hmStitchPano {
   }
   size_t device_bytes() const override { return base_device_bytes() + map[0].bytes + map[1].bytes + seam.bytes + mask_u8.bytes; }
   hipError_t process(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s);
-  hipError_t run_blend(const hmSurface& canvas, hipStream_t s);
+  hipError_t run_blend(const hmSurface& canvas, hipStream_t s, int down_from = 0);
+  hipError_t launch_remap_down(const Pano2L0& a, hipStream_t s, int* n_first);
 };
 
 hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
@@ -309,6 +419,7 @@ hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
     l0.skip_x1 = bx0 + bw;
     roi_x0 = std::max(0, std::min(l0.ulo[0] + l0.d[0], l0.ulo[1] + l0.d[1]));
     roi_x1 = std::min(canvas_w, std::max(l0.uhi[0] + l0.d[0], l0.uhi[1] + l0.d[1]));
+    fuse_ok = roi_x0 >= l0.skip_x0 && roi_x1 <= l0.skip_x1;
   } else {
     bw = canvas_w;
     bh = canvas_h;
@@ -319,6 +430,7 @@ hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
     l0.write_canvas = 0;
     roi_x0 = 0;
     roi_x1 = canvas_w;
+    fuse_ok = true;  // no canvas writes at all
   }
   // convertMaskMat crop + convertTo(float) (cudaPano.inl:62-77).
   mask_host.resize((size_t)bw * bh);
@@ -422,7 +534,9 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     k.span_tiles[1] = std::max(0, (x1b - x0b + tile_w - 1) / tile_w);
     k.tiles_x = k.span_tiles[0] + k.span_tiles[1];
     if (k.tiles_x == 0) return hipSuccess;
-    const dim3 grid((unsigned)(k.tiles_x * k.tiles_y), 1u, (unsigned)batch);
+    const int tiles = k.tiles_x * k.tiles_y;
+    const int wgs = st == side && tuning.canvas_grid > 0 ? std::min(tiles, tuning.canvas_grid) : tiles;
+    const dim3 grid((unsigned)wgs, 1u, (unsigned)batch);
     timer.begin(st);
     auto go = [&](auto CV, auto PV, auto RV) {
       hipLaunchKernelGGL((pano2_level0_kernel<decltype(CV)::value, decltype(PV)::value, decltype(RV)::value>), grid,
@@ -434,22 +548,45 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   };
   // Blend-input columns first, on the caller's stream: the pyramid waits on
   // them.  The canvas remap forks after them, so it does not delay them.
-  HM_RETURN_IF_ERROR(launch_spans(roi_x0, roi_x1, 0, 0, s, "remap_roi"));
-  if (!side) return run_blend(canvas, s);
+  // With fuse_down they are remapped inside the first down launch.
+  int down_from = 0;
+  if (tuning.fuse_down && fuse_ok && !debug && pyr.levels >= 2) {
+    HM_RETURN_IF_ERROR(launch_remap_down(a, s, &down_from));
+  } else {
+    HM_RETURN_IF_ERROR(launch_spans(roi_x0, roi_x1, 0, 0, s, "remap_roi"));
+  }
+  if (!side) return run_blend(canvas, s, down_from);
   if (!tuning.overlap) {  // everything on the caller's stream
     HM_RETURN_IF_ERROR(launch_spans(0, roi_x0, roi_x1, canvas_w, s, "remap_canvas"));
-    return run_blend(canvas, s);
+    return run_blend(canvas, s, down_from);
   }
   HM_RETURN_IF_ERROR(fork(s));
   const hipError_t e1 = launch_spans(0, roi_x0, roi_x1, canvas_w, side, "remap_canvas");
-  const hipError_t e2 = e1 == hipSuccess ? run_blend(canvas, s) : e1;
+  const hipError_t e2 = e1 == hipSuccess ? run_blend(canvas, s, down_from) : e1;
   // Join after the pyramid's launches (and after an error too, so the
   // caller's stream still orders the side work).
   const hipError_t j = join(s);
   return e2 != hipSuccess ? e2 : j;
 }
 
-hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s) {
+hipError_t Pano2::launch_remap_down(const Pano2L0& a, hipStream_t s, int* n_first) {
+  RemapDownArgs ra;
+  ra.r = a;
+  const int n_out = std::min(5, pyr.levels - 1);
+  for (int i = 0; i < 2; ++i) ra.dn.g0[i] = g0[i].as<uint8_t>();
+  fill_down_args(ra.dn, pyr.level_table.data(), bw, bh, n_out);
+  const dim3 grid((unsigned)(ra.dn.tiles_x * ra.dn.tiles_y), 1u, (unsigned)batch);
+  timer.begin(s);
+  if (channels == 4)
+    hipLaunchKernelGGL((remap_down_kernel<4>), grid, dim3(256), 0, s, ra);
+  else
+    hipLaunchKernelGGL((remap_down_kernel<3>), grid, dim3(256), 0, s, ra);
+  timer.end(s, "remap_down");
+  *n_first = n_out;
+  return hipGetLastError();
+}
+
+hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s, int down_from) {
   const void* g[2] = {g0[0].p, g0[1].p};
   hm::Level0Out out;
   out.u8 = static_cast<uint8_t*>(canvas.d_ptr);
@@ -458,8 +595,11 @@ hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s) {
   out.u8_h = canvas_h;
   out.ox = minimize ? x2 - kOverlapPad : 0;  // copy-back offset (cudaPano.inl:408)
   out.oy = 0;
-  // Debug: the per-level path keeping every level; else the fused passes.
-  return pyr.run(g, true, out, s, timer_or_null());
+  // Debug: the per-level path keeping every level; else the fused passes
+  // (the first down_from levels already built by the fused remap).
+  if (pyr.debug) return pyr.run(g, true, out, s, timer_or_null());
+  HM_RETURN_IF_ERROR(pyr.run_down(g, true, down_from, s, timer_or_null()));
+  return pyr.run_up(g, true, out, s, timer_or_null());
 }
 
 }  // namespace

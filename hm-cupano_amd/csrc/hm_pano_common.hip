@@ -117,7 +117,10 @@ extern "C" int hmStitchPanoGetTuning(const hmStitchPano* pano, hmTuning* tuning)
 static bool tuning_ok(const hmTuning& t) {
   return (t.remap_px == 2 || t.remap_px == 4 || t.remap_px == 8) && t.tail_base >= 0 && t.tail_base <= 3 &&
          (t.overlap == 0 || t.overlap == 1) && (t.remap_xcd == 0 || t.remap_xcd == 1) &&
-         (t.remap_rows == 1 || t.remap_rows == 2 || t.remap_rows == 4);
+         (t.remap_rows == 1 || t.remap_rows == 2 || t.remap_rows == 4) &&
+         (t.canvas_grid == 0 || t.canvas_grid == 256 || t.canvas_grid == 512 || t.canvas_grid == 1024 ||
+          t.canvas_grid == 2048) &&
+         (t.fuse_down == 0 || t.fuse_down == 1);
 }
 
 extern "C" int hmStitchPanoSetTuning(hmStitchPano* pano, const hmTuning* tuning) {
@@ -134,36 +137,49 @@ extern "C" int hmStitchPanoAutotune(hmStitchPano* pano, const hmSurface* inputs,
   if (pano->status != hipSuccess) return pano->status;
   HM_RETURN_IF_ERROR(pano->prepare());  // allocations and streams first: they decide the knobs
   const hmTuning start = pano->tuning;
-  std::vector<int> pxs = pano->tunes_remap_px() ? std::vector<int>{2, 4, 8} : std::vector<int>{start.remap_px};
+  const bool remap = pano->tunes_remap_px();
+  std::vector<int> pxs = remap ? std::vector<int>{2, 4, 8} : std::vector<int>{start.remap_px};
   std::vector<int> tbs = pano->tunes_tail() ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{start.tail_base};
   std::vector<int> ovs = pano->tunes_overlap() ? std::vector<int>{0, 1} : std::vector<int>{start.overlap};
-  std::vector<int> xcs = pano->tunes_remap_px() ? std::vector<int>{0, 1} : std::vector<int>{start.remap_xcd};
-  std::vector<int> rys = pano->tunes_remap_px() ? std::vector<int>{1, 2, 4} : std::vector<int>{start.remap_rows};
+  std::vector<int> xcs = remap ? std::vector<int>{0, 1} : std::vector<int>{start.remap_xcd};
+  std::vector<int> rys = remap ? std::vector<int>{1, 2, 4} : std::vector<int>{start.remap_rows};
+  std::vector<int> gds = remap && pano->tunes_overlap() ? std::vector<int>{0, 256, 512, 1024, 2048}
+                                                        : std::vector<int>{start.canvas_grid};
+  std::vector<int> fds = remap ? std::vector<int>{0, 1} : std::vector<int>{start.fuse_down};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HM_RETURN_IF_ERROR(hipEventCreate(&e0));
   hipError_t err = hipEventCreate(&e1);
   hmTuning chosen = start;
   float chosen_ms = -1.0f;
+  auto trial = [&](const hmTuning& t) {
+    if (err != hipSuccess) return;
+    pano->tuning = t;
+    pano->apply_tuning();
+    err = pano->process_any(inputs, num_inputs, canvas, fused, stream);  // warm-up
+    if (err == hipSuccess) err = hipEventRecord(e0, stream);
+    for (int f = 0; f < frames && err == hipSuccess; ++f)
+      err = pano->process_any(inputs, num_inputs, canvas, fused, stream);
+    if (err == hipSuccess) err = hipEventRecord(e1, stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    if (err == hipSuccess && (chosen_ms < 0.0f || ms / frames < chosen_ms)) {
+      chosen_ms = ms / frames;
+      chosen = t;
+    }
+  };
+  // Two stages: the remap's shape (pixels and rows per thread, tile order)
+  // first, then the pyramid and stream knobs on top of the best shape.
   for (int px : pxs)
-    for (int tb : tbs)
-      for (int ov : ovs)
-        for (int xc : xcs)
-          for (int ry : rys) {
-        if (err != hipSuccess) break;
-        pano->tuning = hmTuning{px, tb, ov, xc, ry};
-        pano->apply_tuning();
-        err = pano->process_any(inputs, num_inputs, canvas, fused, stream);  // warm-up
-        if (err == hipSuccess) err = hipEventRecord(e0, stream);
-        for (int f = 0; f < frames && err == hipSuccess; ++f)
-          err = pano->process_any(inputs, num_inputs, canvas, fused, stream);
-        if (err == hipSuccess) err = hipEventRecord(e1, stream);
-        if (err == hipSuccess) err = hipEventSynchronize(e1);
-        float ms = 0.0f;
-        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-        if (err == hipSuccess && (chosen_ms < 0.0f || ms / frames < chosen_ms)) {
-          chosen_ms = ms / frames;
-          chosen = pano->tuning;
-        }
+    for (int ry : rys)
+      for (int xc : xcs)
+        for (int fd : fds) trial(hmTuning{px, start.tail_base, start.overlap, xc, ry, start.canvas_grid, fd});
+  const hmTuning shape = chosen;
+  for (int tb : tbs)
+    for (int ov : ovs)
+      for (int gd : gds) {
+        if (!ov && gd) continue;  // the grid knob applies to the second stream only
+        trial(hmTuning{shape.remap_px, tb, ov, shape.remap_xcd, shape.remap_rows, gd, shape.fuse_down});
       }
   (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

@@ -169,21 +169,7 @@ static hipError_t down_fused_t(const void* const* g0, int w0, int h0, int n_out,
                                hipStream_t s) {
   DownFusedArgs<T0, N> a;
   for (int i = 0; i < N; ++i) a.g0[i] = static_cast<const T0*>(g0[i]);
-  a.w0 = w0;
-  a.h0 = h0;
-  a.n_out = n_out;
-  for (int k = 0; k < 5; ++k) {
-    const LevelDesc& d = lv[k < n_out ? k : 0];
-    for (int i = 0; i < N; ++i) {
-      a.lv[k].g_out[i] = d.g_out[i];
-      a.lv[k].s[i] = d.s[i];
-      a.lv[k].z[i] = d.z[i];
-    }
-    a.lv[k].w = d.w;
-    a.lv[k].h = d.h;
-  }
-  a.tiles_x = (w0 + kDownTile - 1) / kDownTile;
-  a.tiles_y = (h0 + kDownTile - 1) / kDownTile;
+  fill_down_args(a, lv, w0, h0, n_out);
   const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
   if constexpr (N <= 2)
     hipLaunchKernelGGL((down_fused_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
@@ -341,8 +327,12 @@ hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch
       a.tiles_x = (a.nqx + 63) / 64;
       a.tiles_y = (a.nqy + 3) / 4;
       for (int i = 0; i < kN; ++i) a.z[i] = u.z[i];
-      hipLaunchKernelGGL((up_quad_kernel<T0, kC, kN, kK, decltype(M)::value>),
-                         dim3((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch), dim3(64, 4, 1), 0, s, a);
+      const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+      if (a.r1)
+        hipLaunchKernelGGL((up_quad_kernel<T0, kC, kN, kK, decltype(M)::value, true>), grid, dim3(64, 4, 1), 0, s, a);
+      else
+        hipLaunchKernelGGL((up_quad_kernel<T0, kC, kN, kK, decltype(M)::value, false>), grid, dim3(64, 4, 1), 0, s,
+                           a);
       return hipGetLastError();
     };
     using std::integral_constant;

@@ -155,6 +155,74 @@ struct DownFusedArgs {
   int tiles_x, tiles_y;
 };
 
+// Levels 2 .. n_out of the down pass: the first produced level's 16x16 tile
+// of every image is in s1; each further level's tile (8x8, 4x4, 2x2, 1x1)
+// comes from the one above it in LDS and is written once.
+template <int C, int N, typename Args>
+__device__ __forceinline__ void down_rest(const Args& a, int tx, int ty, int b, int tid, float4 (*s1)[16 * 16],
+                                          float4 (*s2)[8 * 8]) {
+  int edge = 16;
+  for (int k = 1; k < a.n_out; ++k) {
+    __syncthreads();
+    const DownLevel<N>& din = a.lv[k - 1];
+    const DownLevel<N>& d = a.lv[k];
+    const int oedge = edge >> 1;
+    const int x0 = tx * oedge, y0 = ty * oedge;
+    float4 r[N];
+    bool active = false;
+    int lx = 0, ly = 0;
+    if (tid < oedge * oedge) {
+      lx = tid % oedge;
+      ly = tid / oedge;
+      const int x = x0 + lx, y = y0 + ly;
+      if (x < d.w && y < d.h) {
+        active = true;
+        const int gx = 2 * x, gy = 2 * y;  // level k-1 coordinates
+        const bool has_x1 = gx + 1 < din.w, has_y1 = gy + 1 < din.h;
+        const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
+        const int sx = 2 * lx, sy = 2 * ly;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const float4* src = k == 1 ? s1[i] : s2[i];
+          float4 p[4];
+          p[0] = src[sy * edge + sx];
+          if (has_x1) p[1] = src[sy * edge + sx + 1];
+          if (has_y1) p[2] = src[(sy + 1) * edge + sx];
+          if (has_x1 && has_y1) p[3] = src[(sy + 1) * edge + sx + 1];
+          r[i] = down_px<C>(p, in_range);  // zero outside the support: its LDS inputs are zero
+          if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r[i]);
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) s2[i][ly * oedge + lx] = r[i];
+    }
+    edge = oedge;
+  }
+}
+
+// The host's fill of DownFusedArgs from the pyramid's level table.
+template <typename T0, int N>
+inline void fill_down_args(DownFusedArgs<T0, N>& a, const LevelDesc* lv, int w0, int h0, int n_out) {
+  a.w0 = w0;
+  a.h0 = h0;
+  a.n_out = n_out;
+  for (int k = 0; k < 5; ++k) {
+    const LevelDesc& d = lv[k < n_out ? k : 0];
+    for (int i = 0; i < N; ++i) {
+      a.lv[k].g_out[i] = d.g_out[i];
+      a.lv[k].s[i] = d.s[i];
+      a.lv[k].z[i] = d.z[i];
+    }
+    a.lv[k].w = d.w;
+    a.lv[k].h = d.h;
+  }
+  a.tiles_x = (w0 + kDownTile - 1) / kDownTile;
+  a.tiles_y = (h0 + kDownTile - 1) / kDownTile;
+}
+
 template <typename T0, int C, int N>
 __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a) {
   __shared__ float4 s1[N][16 * 16];
@@ -177,13 +245,17 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
       bool in_s[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        // outside the image's support the result is zero: no loads
+        // Outside the image's support the result is zero (one or two images
+        // are dense, so the support is the whole level).  Taps past the edge
+        // load their in-range neighbour, ignored by down_px: no branch between
+        // the loads, so all of them are in flight together.
         in_s[i] = in_rect(d.s[i], x, y);
-        if (!in_s[i]) continue;
+        const int ix1 = has_x1 ? ix + 1 : ix;
+        const size_t rr1 = has_y1 ? r1 : r0;
         p[i][0] = ld_px<C>(a.g0[i], r0 + ix);
-        if (has_x1) p[i][1] = ld_px<C>(a.g0[i], r0 + ix + 1);
-        if (has_y1) p[i][2] = ld_px<C>(a.g0[i], r1 + ix);
-        if (has_x1 && has_y1) p[i][3] = ld_px<C>(a.g0[i], r1 + ix + 1);
+        p[i][1] = ld_px<C>(a.g0[i], r0 + ix1);
+        p[i][2] = ld_px<C>(a.g0[i], rr1 + ix);
+        p[i][3] = ld_px<C>(a.g0[i], rr1 + ix1);
       }
       const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
       const size_t o = ((size_t)b * d.h + y) * d.w + x;
@@ -195,47 +267,7 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
       }
     }
   }
-  // ---- further levels from LDS: 8x8, 4x4, 2x2, 1x1 per tile
-  int edge = 16;
-  float4(*src)[16 * 16] = s1;
-  for (int k = 1; k < a.n_out; ++k) {
-    __syncthreads();
-    const DownLevel<N>& din = a.lv[k - 1];
-    const DownLevel<N>& d = a.lv[k];
-    const int oedge = edge >> 1;
-    const int x0 = tx * oedge, y0 = ty * oedge;
-    float4 r[N];
-    bool active = false;
-    int lx = 0, ly = 0;
-    if (tid < oedge * oedge) {
-      lx = tid % oedge;
-      ly = tid / oedge;
-      const int x = x0 + lx, y = y0 + ly;
-      if (x < d.w && y < d.h) {
-        active = true;
-        const int gx = 2 * x, gy = 2 * y;  // level k-1 coordinates
-        const bool has_x1 = gx + 1 < din.w, has_y1 = gy + 1 < din.h;
-        const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
-        const int sx = 2 * lx, sy = 2 * ly;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          float4 p[4];
-          p[0] = (k == 1 ? src[i] : s2[i])[sy * edge + sx];
-          if (has_x1) p[1] = (k == 1 ? src[i] : s2[i])[sy * edge + sx + 1];
-          if (has_y1) p[2] = (k == 1 ? src[i] : s2[i])[(sy + 1) * edge + sx];
-          if (has_x1 && has_y1) p[3] = (k == 1 ? src[i] : s2[i])[(sy + 1) * edge + sx + 1];
-          r[i] = down_px<C>(p, in_range);  // zero outside the support: its LDS inputs are zero
-          if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r[i]);
-        }
-      }
-    }
-    __syncthreads();
-    if (active) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) s2[i][ly * oedge + lx] = r[i];
-    }
-    edge = oedge;
-  }
+  down_rest<C, N>(a, tx, ty, b, tid, s1, s2);
 }
 
 // The same pyramid one image at a time, for three or more images: each image
@@ -274,10 +306,12 @@ __global__ void __launch_bounds__(256) down_fused_each_kernel(DownFusedArgs<T0, 
           const size_t r0 = ((size_t)b * a.h0 + iy) * a.w0;
           const size_t r1 = r0 + a.w0;
           float4 p[4];
+          const int ix1 = has_x1 ? ix + 1 : ix;
+          const size_t rr1 = has_y1 ? r1 : r0;
           p[0] = ld_px<C>(a.g0[i], r0 + ix);
-          if (has_x1) p[1] = ld_px<C>(a.g0[i], r0 + ix + 1);
-          if (has_y1) p[2] = ld_px<C>(a.g0[i], r1 + ix);
-          if (has_x1 && has_y1) p[3] = ld_px<C>(a.g0[i], r1 + ix + 1);
+          p[1] = ld_px<C>(a.g0[i], r0 + ix1);
+          p[2] = ld_px<C>(a.g0[i], rr1 + ix);
+          p[3] = ld_px<C>(a.g0[i], rr1 + ix1);
           const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
           r = down_px<C>(p, in_range);
         }
@@ -638,9 +672,12 @@ struct UpQuadArgs {
   int tiles_x, tiles_y;                // 64 x 4 quads per workgroup
 };
 
-template <typename T0, int C, int N, int KIND, int MODE>
+// NEXT: the level has a coarser level (false at the coarsest: blend only).
+// A template parameter rather than a branch, so that the taps of the coarser
+// level and the level's own pixels are loaded in one straight run.
+template <typename T0, int C, int N, int KIND, int MODE, bool NEXT>
 __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
-  const bool has_next = a.r1 != nullptr;  // uniform across the launch
+  constexpr bool has_next = NEXT;
   const int t = (int)blockIdx.x;
   const int qx = a.qx0 + (t % a.tiles_x) * 64 + (int)threadIdx.x;
   const int qy = a.qy0 + (t / a.tiles_x) * 4 + (int)threadIdx.y;
@@ -791,16 +828,27 @@ __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
     rt[2] = ld_px<C>(a.r1, l1 + qx);
     rt[3] = ld_px<C>(a.r1, l1 + qx1);
   }
-  // the level's pixels of the quad, all loaded before any arithmetic
+  // the level's pixels and mask values of the quad, all loaded before any
+  // arithmetic (pixels outside the window load the window's first pixel)
+  constexpr int M = KIND == kBlend2 ? 1 : N;
   float4 g[4][N];
+  float mk[4][M];
   bool in[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
     in[e] = x >= a.win_x0 && x <= a.win_x1 && y >= a.win_y0 && y <= a.win_y1;
-    const size_t pix = ((size_t)b * a.h0 + (in[e] ? y : a.win_y0)) * a.w0 + (in[e] ? x : a.win_x0);
+    const int cx = in[e] ? x : a.win_x0, cy = in[e] ? y : a.win_y0;
+    const size_t pix = ((size_t)b * a.h0 + cy) * a.w0 + cx;
 #pragma unroll
     for (int i = 0; i < N; ++i) g[e][i] = ld_px<C>(a.g0[i], pix);
+    const size_t mp = (size_t)cy * a.w0 + cx;
+    if constexpr (MODE == kMask0U8) {
+      mk[e][0] = (float)static_cast<const uint8_t*>(a.mask0)[mp];
+    } else {
+#pragma unroll
+      for (int c = 0; c < M; ++c) mk[e][c] = static_cast<const float*>(a.mask0)[mp * M + c];
+    }
   }
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -816,7 +864,7 @@ __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
         Lv[i] = g[e][i];
       }
     }
-    const float4 B = blend_level0<C, N, KIND, MODE>(Lv, a.mask0, (size_t)y * a.w0 + x);
+    const float4 B = blend_any<C, N, KIND>(Lv, mk[e], 0);  // the preloaded mask values
     float4 R = B;
     if (has_next) {
       const float4 up = up_quad_rec<C>(e, rt);

@@ -195,7 +195,10 @@ typedef struct hmTuning {
   int tail_base;
   int overlap;
   int remap_xcd;  /* 1: 2-image remap tiles ordered as horizontal canvas bands, one per XCD */
-  int remap_rows; /* canvas rows per thread of the 2-image remaps: 1, 2 or 4 (taller tiles) */
+  int remap_rows;  /* canvas rows per thread of the 2-image remaps: 1, 2 or 4 (taller tiles) */
+  int canvas_grid; /* workgroups of the 2-image canvas remap on the second stream: 0 = one per tile,
+                      else 256, 512, 1024 or 2048 walking the tiles (a smaller share of the GPU) */
+  int fuse_down;   /* 1: the 2-image blend-input remap runs inside the first down-pass launch */
 } hmTuning;
 int hmStitchPanoGetTuning(const hmStitchPano* pano, hmTuning* tuning);
 int hmStitchPanoSetTuning(hmStitchPano* pano, const hmTuning* tuning);
