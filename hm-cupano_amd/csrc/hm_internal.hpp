@@ -75,6 +75,10 @@ hipError_t launch_down(int c, int n, bool in_u8, const void* const* in, float* c
                        int out_w, int out_h, int batch, hipStream_t s);
 hipError_t launch_mask_down(const float* in, int in_w, int in_h, float* out, int out_w, int out_h, int m,
                             hipStream_t s);
+// Planar normalised weights of the N-image blend from an interleaved mask of
+// m channels: plane i = mask_i / (sum of the pixel's values), or mask_i when
+// that sum is 0 (the blend's first normalisation, done once per rig).
+hipError_t launch_mask_planar(const float* in, int w, int h, float* out, int m, hipStream_t s);
 
 struct UpLaunch {
   const void* g_hi[hmdev::kMaxImages] = {};
@@ -94,8 +98,9 @@ hipError_t launch_up(int c, int n, int kind, bool hi_u8, const UpLaunch& a, int 
 
 // Whole-pyramid kernels (hm_pyramid_fused.hpp).  `lv` is a device table:
 // lv[j] describes the j-th level produced (down) or level j+1 (up).
+// touch: per tile, the images meeting it (three or more images), or nullptr.
 hipError_t launch_down_fused(int c, int n, bool in_u8, const void* const* g0, int w0, int h0, int n_out,
-                             const hmdev::LevelDesc* lv, int batch, hipStream_t s);
+                             const hmdev::LevelDesc* lv, const uint32_t* touch, int batch, hipStream_t s);
 struct UpQuadLaunch {
   const void* g0[hmdev::kMaxImages] = {};
   const float* g1[hmdev::kMaxImages] = {};
@@ -109,6 +114,8 @@ struct UpQuadLaunch {
   int u8_w = 0, u8_h = 0, ox = 0, oy = 0;
   int win_x = 0, win_y = 0, win_w = 0, win_h = 0;
   hmdev::ImgRect z[hmdev::kMaxImages] = {};  // valid region of each image at this level
+  const float* wn = nullptr;                 // N-image blend: planar normalised weights (levels >= 1)
+  size_t wn_plane = 0;                       // floats per plane
 };
 hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch& u, int batch, hipStream_t s);
 
@@ -165,6 +172,8 @@ class Pyramid {
   int mask0_mode = hmdev::kMask0F32;
   DevBuf g[hmdev::kMaxImages][hmdev::kMaxLevels];  // level >= 1
   DevBuf mask[hmdev::kMaxLevels];                   // level >= 1
+  DevBuf wn[hmdev::kMaxLevels];                     // N-image, level >= 1: planar normalised weights
+  DevBuf touch[hmdev::kMaxLevels];                  // sparse: per down launch (by first level), tile image sets
   DevBuf r[hmdev::kMaxLevels];                      // level >= 1
   DevBuf lap[hmdev::kMaxImages][hmdev::kMaxLevels]; // debug
   DevBuf blend[hmdev::kMaxLevels];                  // debug

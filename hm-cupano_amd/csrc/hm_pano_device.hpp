@@ -109,40 +109,87 @@ struct MultiArgs {
 // for the sentinel, 255 for other out-of-range entries); DM = true: zero for
 // both (FusedRemapToFullKernel3, cudaFusedKernels3.cu:25-119, writes only
 // mapped in-range pixels into the cleared buffer).
-template <int C, int N, bool DM>
-__global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
-  const int t = (int)blockIdx.x;
-  const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
-  const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
-  const int b = blockIdx.z;
-  if (Y >= a.y1) return;
+//
+// A workgroup covers 64 * kPx x 4 canvas pixels; only the images whose valid
+// region meets that tile are written there.  They are taken up to
+// kRemapSlots at a time, the slot count S a template parameter, so that the
+// map loads of every slot and pixel are issued together, then every gather:
+// two memory round trips per group of images.  Map loads are branch-free
+// (a pixel off the map loads the row's first entry, ignored).
+constexpr int kRemapSlots = 4;
+
+template <int C, int N, bool DM, int S>
+__device__ __forceinline__ void multi_inputs_slots(const MultiArgs<N>& a, const int* idx, int X0, int Y, int b) {
   const int v = Y - a.y0;
+  uint32_t m[S][kPx];
+  bool in[S][kPx], keep[S][kPx];
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    if (v < a.z0[i].y0 || v > a.z0[i].y1) continue;  // this image's input is not kept on this row
-    const int ry = Y - a.py[i];
-    const bool row_ok = ry >= 0 && ry < a.mh[i];
-    const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * a.mw[i];
-    uint32_t m[kPx];
-    bool in[kPx], keep[kPx];
+  for (int s = 0; s < S; ++s) {
+    const int i = idx[s];
+    const ImgRect z = a.z0[i];
+    const int ry = Y - a.py[i], mw = a.mw[i];
+    const bool row_keep = v >= z.y0 && v <= z.y1;
+    const bool row_ok = row_keep && ry >= 0 && ry < a.mh[i];
+    const uint32_t* mrow = a.map[i] + (size_t)(row_ok ? ry : 0) * mw;
 #pragma unroll
     for (int j = 0; j < kPx; ++j) {
       const int X = X0 + j * (int)blockDim.x;
       const int u = X - a.x0;
-      keep[j] = X < a.x1 && u >= a.z0[i].x0 && u <= a.z0[i].x1;
+      keep[s][j] = row_keep && X < a.x1 && u >= z.x0 && u <= z.x1;
       const int rx = X - a.px[i];
-      in[j] = keep[j] && row_ok && rx >= 0 && rx < a.mw[i];
-      m[j] = in[j] ? mrow[rx] : 0u;
+      in[s][j] = keep[s][j] && row_ok && rx >= 0 && rx < mw;
+      m[s][j] = mrow[in[s][j] ? rx : 0];
     }
-    uint32_t r[kPx];
+  }
+  uint32_t r[S][kPx];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const Surf src = a.src[idx[s]];
 #pragma unroll
     for (int j = 0; j < kPx; ++j)
-      r[j] = DM ? remap_px_dm_sel<C>(a.src[i], b, m[j], in[j]) : remap_px_sel<C>(a.src[i], b, m[j], in[j]);
-    uint8_t* grow = a.g0[i] + ((size_t)b * a.bh + v) * a.bw * C;
+      r[s][j] = DM ? remap_px_dm_sel<C>(src, b, m[s][j], in[s][j]) : remap_px_sel<C>(src, b, m[s][j], in[s][j]);
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    uint8_t* grow = a.g0[idx[s]] + ((size_t)b * a.bh + v) * a.bw * C;
 #pragma unroll
     for (int j = 0; j < kPx; ++j) {
       const int X = X0 + j * (int)blockDim.x;
-      if (keep[j]) store_px<C>(grow + (size_t)(X - a.x0) * C, r[j]);
+      if (keep[s][j]) store_px<C>(grow + (size_t)(X - a.x0) * C, r[s][j]);
+    }
+  }
+}
+
+template <int C, int N, bool DM>
+__global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
+  const int t = (int)blockIdx.x;
+  const int tx0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx, ty0 = a.y0 + (t / a.tiles_x) * (int)blockDim.y;
+  const int X0 = tx0 + (int)threadIdx.x;
+  const int Y = ty0 + (int)threadIdx.y;
+  const int b = blockIdx.z;
+  if (Y >= a.y1) return;
+  // the images whose valid region meets the tile (uniform)
+  const int u0 = tx0 - a.x0, u1 = u0 + (int)blockDim.x * kPx - 1;
+  const int v0 = ty0 - a.y0, v1 = v0 + (int)blockDim.y - 1;
+  unsigned act = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const ImgRect& z = a.z0[i];
+    if (z.x0 <= u1 && z.x1 >= u0 && z.y0 <= v1 && z.y1 >= v0) act |= 1u << i;
+  }
+  while (act) {
+    int idx[kRemapSlots];
+    const int n = min(__builtin_popcount(act), kRemapSlots);
+#pragma unroll
+    for (int s = 0; s < kRemapSlots; ++s) {
+      idx[s] = act ? __builtin_ctz(act) : 0;
+      act &= act - 1u;
+    }
+    switch (n) {  // uniform
+      case 1: multi_inputs_slots<C, N, DM, 1>(a, idx, X0, Y, b); break;
+      case 2: multi_inputs_slots<C, N, DM, 2>(a, idx, X0, Y, b); break;
+      case 3: multi_inputs_slots<C, N, DM, 3>(a, idx, X0, Y, b); break;
+      default: multi_inputs_slots<C, N, DM, 4>(a, idx, X0, Y, b); break;
     }
   }
 }

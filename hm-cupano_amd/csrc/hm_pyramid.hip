@@ -51,6 +51,19 @@ static __global__ void __launch_bounds__(256) mask_down_kernel(const float* in, 
   for (int k = 0; k < m; ++k) o[k] = sums[k] / (float)count;
 }
 
+// The N-image blend's first normalisation of every pixel, stored as planes.
+template <int M>
+static __global__ void __launch_bounds__(256) mask_planar_kernel(const float* in, int w, int h, float* out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= w || y >= h) return;
+  const size_t o = (size_t)y * w + x, plane = (size_t)w * h;
+  float v[M];
+  blendN_normalize<M>(in + o * M, v);
+#pragma unroll
+  for (int k = 0; k < M; ++k) out[k * plane + o] = v[k];
+}
+
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
@@ -102,6 +115,20 @@ hipError_t launch_mask_down(const float* in, int in_w, int in_h, float* out, int
   if (m < 1 || m > kMaxImages) return hipErrorInvalidValue;
   hipLaunchKernelGGL(mask_down_kernel, grid_for(out_w, out_h, 1, kBlock), kBlock, 0, s, in, in_w, in_h, out, out_w,
                      out_h, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_mask_planar(const float* in, int w, int h, float* out, int m, hipStream_t s) {
+  const dim3 grid = grid_for(w, h, 1, kBlock);
+  switch (m) {
+    case 3: hipLaunchKernelGGL(mask_planar_kernel<3>, grid, kBlock, 0, s, in, w, h, out); break;
+    case 4: hipLaunchKernelGGL(mask_planar_kernel<4>, grid, kBlock, 0, s, in, w, h, out); break;
+    case 5: hipLaunchKernelGGL(mask_planar_kernel<5>, grid, kBlock, 0, s, in, w, h, out); break;
+    case 6: hipLaunchKernelGGL(mask_planar_kernel<6>, grid, kBlock, 0, s, in, w, h, out); break;
+    case 7: hipLaunchKernelGGL(mask_planar_kernel<7>, grid, kBlock, 0, s, in, w, h, out); break;
+    case 8: hipLaunchKernelGGL(mask_planar_kernel<8>, grid, kBlock, 0, s, in, w, h, out); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -165,41 +192,43 @@ hipError_t launch_up(int c, int n, int kind, bool hi_u8, const UpLaunch& u, int 
 // fused whole-pyramid kernels
 // ---------------------------------------------------------------------------
 template <typename T0, int C, int N>
-static hipError_t down_fused_t(const void* const* g0, int w0, int h0, int n_out, const LevelDesc* lv, int batch,
-                               hipStream_t s) {
+static hipError_t down_fused_t(const void* const* g0, int w0, int h0, int n_out, const LevelDesc* lv,
+                               const uint32_t* touch, int batch, hipStream_t s) {
   DownFusedArgs<T0, N> a;
   for (int i = 0; i < N; ++i) a.g0[i] = static_cast<const T0*>(g0[i]);
   fill_down_args(a, lv, w0, h0, n_out);
+  a.touch = touch;
   const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
   if constexpr (N <= 2)
     hipLaunchKernelGGL((down_fused_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((down_fused_each_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((down_fused_slots_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
 template <typename T0, int C>
 static hipError_t down_fused_c(int n, const void* const* g0, int w0, int h0, int n_out, const LevelDesc* lv,
+                               const uint32_t* touch,
                                int batch, hipStream_t s) {
   switch (n) {
-    case 2: return down_fused_t<T0, C, 2>(g0, w0, h0, n_out, lv, batch, s);
-    case 3: return down_fused_t<T0, C, 3>(g0, w0, h0, n_out, lv, batch, s);
-    case 4: return down_fused_t<T0, C, 4>(g0, w0, h0, n_out, lv, batch, s);
-    case 5: return down_fused_t<T0, C, 5>(g0, w0, h0, n_out, lv, batch, s);
-    case 6: return down_fused_t<T0, C, 6>(g0, w0, h0, n_out, lv, batch, s);
-    case 7: return down_fused_t<T0, C, 7>(g0, w0, h0, n_out, lv, batch, s);
-    case 8: return down_fused_t<T0, C, 8>(g0, w0, h0, n_out, lv, batch, s);
+    case 2: return down_fused_t<T0, C, 2>(g0, w0, h0, n_out, lv, touch, batch, s);
+    case 3: return down_fused_t<T0, C, 3>(g0, w0, h0, n_out, lv, touch, batch, s);
+    case 4: return down_fused_t<T0, C, 4>(g0, w0, h0, n_out, lv, touch, batch, s);
+    case 5: return down_fused_t<T0, C, 5>(g0, w0, h0, n_out, lv, touch, batch, s);
+    case 6: return down_fused_t<T0, C, 6>(g0, w0, h0, n_out, lv, touch, batch, s);
+    case 7: return down_fused_t<T0, C, 7>(g0, w0, h0, n_out, lv, touch, batch, s);
+    case 8: return down_fused_t<T0, C, 8>(g0, w0, h0, n_out, lv, touch, batch, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_down_fused(int c, int n, bool in_u8, const void* const* g0, int w0, int h0, int n_out,
-                             const LevelDesc* lv, int batch, hipStream_t s) {
+                             const LevelDesc* lv, const uint32_t* touch, int batch, hipStream_t s) {
   if (n_out < 1 || n_out > 5) return hipErrorInvalidValue;
-  if (c == 4) return in_u8 ? down_fused_c<uint8_t, 4>(n, g0, w0, h0, n_out, lv, batch, s)
-                           : down_fused_c<float, 4>(n, g0, w0, h0, n_out, lv, batch, s);
-  if (c == 3) return in_u8 ? down_fused_c<uint8_t, 3>(n, g0, w0, h0, n_out, lv, batch, s)
-                           : down_fused_c<float, 3>(n, g0, w0, h0, n_out, lv, batch, s);
+  if (c == 4) return in_u8 ? down_fused_c<uint8_t, 4>(n, g0, w0, h0, n_out, lv, touch, batch, s)
+                           : down_fused_c<float, 4>(n, g0, w0, h0, n_out, lv, touch, batch, s);
+  if (c == 3) return in_u8 ? down_fused_c<uint8_t, 3>(n, g0, w0, h0, n_out, lv, touch, batch, s)
+                           : down_fused_c<float, 3>(n, g0, w0, h0, n_out, lv, touch, batch, s);
   return hipErrorInvalidValue;
 }
 
@@ -327,6 +356,8 @@ hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch
       a.tiles_x = (a.nqx + 63) / 64;
       a.tiles_y = (a.nqy + 3) / 4;
       for (int i = 0; i < kN; ++i) a.z[i] = u.z[i];
+      a.wn = u.wn;
+      a.wn_plane = u.wn_plane;
       const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
       if (a.r1)
         hipLaunchKernelGGL((up_quad_kernel<T0, kC, kN, kK, decltype(M)::value, true>), grid, dim3(64, 4, 1), 0, s, a);
@@ -424,6 +455,10 @@ hipError_t Pyramid::init(int kind_, int n_, int channels, int width, int height,
   levels = pyramid_sizes(width, height, levels_req, kind != kBlend2, W, H);
   if (levels < 1) return hipErrorInvalidValue;
   allocation_size = 0;
+  for (int l = 0; l < kMaxLevels; ++l) {
+    wn[l].reset();
+    touch[l].reset();
+  }
   for (int l = 0; l < levels; ++l) {
     const size_t px = (size_t)W[l] * H[l];
     const size_t img = px * c * batch * sizeof(float);
@@ -432,6 +467,10 @@ hipError_t Pyramid::init(int kind_, int n_, int channels, int width, int height,
       HM_RETURN_IF_ERROR(mask[l].alloc(px * mask_channels() * sizeof(float)));
       HM_RETURN_IF_ERROR(r[l].alloc(img));
       allocation_size += n * img + img + px * mask_channels() * sizeof(float);
+      if (kind == kBlendN && n > 2 && !debug) {  // the slot path of three or more images
+        HM_RETURN_IF_ERROR(wn[l].alloc(px * n * sizeof(float)));
+        allocation_size += px * n * sizeof(float);
+      }
     }
     if (debug) {
       for (int i = 0; i < n; ++i) HM_RETURN_IF_ERROR(lap[i][l].alloc(img));
@@ -486,6 +525,25 @@ hipError_t Pyramid::set_supports(const ImgRect* s0) {
     }
   }
   build_level_table();
+  // Per down launch (levels from+1 .. from+n_out out of one launch), the
+  // images each tile meets: the same test the kernel makes without a table.
+  for (int from = 0; from + 1 < levels;) {
+    const int n_out = std::min(5, levels - 1 - from);
+    const int tiles_x = (W[from] + kDownTile - 1) / kDownTile, tiles_y = (H[from] + kDownTile - 1) / kDownTile;
+    std::vector<uint32_t> tab((size_t)tiles_x * tiles_y, 0u);
+    for (int ty = 0; ty < tiles_y; ++ty)
+      for (int tx = 0; tx < tiles_x; ++tx) {
+        uint32_t act = 0;
+        for (int i = 0; i < n; ++i)
+          for (int k = 0; k < n_out; ++k)
+            if (tile_meets(Z[from + 1 + k][i], tx, ty, 16 >> k)) act |= 1u << i;
+        tab[(size_t)ty * tiles_x + tx] = act;
+      }
+    HM_RETURN_IF_ERROR(touch[from].alloc(tab.size() * sizeof(uint32_t)));
+    HM_RETURN_IF_ERROR(hipMemcpy(touch[from].p, tab.data(), touch[from].bytes, hipMemcpyHostToDevice));
+    allocation_size += touch[from].bytes;
+    from += n_out;
+  }
   return hipSuccess;
 }
 
@@ -496,6 +554,8 @@ hipError_t Pyramid::build_mask_pyramid(const float* mask0_, hipStream_t s) {
     HM_RETURN_IF_ERROR(
         launch_mask_down(prev, W[l], H[l], mask[l + 1].as<float>(), W[l + 1], H[l + 1], mask_channels(), s));
     prev = mask[l + 1].as<float>();
+    if (wn[l + 1].p)
+      HM_RETURN_IF_ERROR(launch_mask_planar(prev, W[l + 1], H[l + 1], wn[l + 1].as<float>(), n, s));
   }
   return hipSuccess;
 }
@@ -508,8 +568,8 @@ hipError_t Pyramid::run_down(const void* const* g0, bool g0_u8, int from, hipStr
     const void* in[kMaxImages];
     for (int i = 0; i < n; ++i) in[i] = from == 0 ? g0[i] : g[i][from].p;
     if (timer) timer->begin(s);
-    HM_RETURN_IF_ERROR(
-        launch_down_fused(c, n, from == 0 && g0_u8, in, W[from], H[from], n_out, table + from, batch, s));
+    HM_RETURN_IF_ERROR(launch_down_fused(c, n, from == 0 && g0_u8, in, W[from], H[from], n_out, table + from,
+                                         touch[from].as<uint32_t>(), batch, s));
     if (timer) timer->end(s, "down_fused");
     from += n_out;
   }
@@ -573,6 +633,8 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
     } else {
       u.mask0 = mask[k].p;
       u.mask0_mode = kMask0F32;
+      u.wn = wn[k].as<float>();
+      u.wn_plane = (size_t)W[k] * H[k];
     }
     u.r1 = has_next ? r[k + 1].as<float>() : nullptr;
     u.w0 = W[k];

@@ -153,6 +153,8 @@ struct DownFusedArgs {
   int n_out;                // levels produced (1..5)
   DownLevel<N> lv[5];       // lv[0] describes the first produced level, lv[1] the next ...
   int tiles_x, tiles_y;
+  const uint32_t* touch;    // three or more images: per tile, the images whose valid
+                            // region meets it at some produced level (bit i), or nullptr
 };
 
 // Levels 2 .. n_out of the down pass: the first produced level's 16x16 tile
@@ -221,6 +223,7 @@ inline void fill_down_args(DownFusedArgs<T0, N>& a, const LevelDesc* lv, int w0,
   }
   a.tiles_x = (w0 + kDownTile - 1) / kDownTile;
   a.tiles_y = (h0 + kDownTile - 1) / kDownTile;
+  a.touch = nullptr;
 }
 
 template <typename T0, int C, int N>
@@ -270,87 +273,138 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
   down_rest<C, N>(a, tx, ty, b, tid, s1, s2);
 }
 
-// The same pyramid one image at a time, for three or more images: each image
-// runs the whole level chain through one set of LDS tiles, and a workgroup
-// skips the images whose valid region misses its tile at every produced
-// level (its buffers there are never read).  Fewer registers and less LDS
-// per workgroup than carrying every image at once.
-template <typename T0, int C, int N>
-__global__ void __launch_bounds__(256) down_fused_each_kernel(DownFusedArgs<T0, N> a) {
-  __shared__ float4 s1[16 * 16];
-  __shared__ float4 s2[8 * 8];
-  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
-  const int tx = t % a.tiles_x, ty = t / a.tiles_x;
-  const int b = blockIdx.z;
-  const int tid = threadIdx.x;
+// Three or more images: the images whose valid region meets the tile at some
+// produced level run the level chain together, up to kDownSlots at a time in
+// one set of LDS tiles; a workgroup skips the other images (their buffers
+// there are never read).  The slot count S is a template parameter (the
+// kernel switches on it, a uniform value), so all loads of the first level
+// are issued without a branch between them.
+constexpr int kDownSlots = 4;
+
+// Whether the e x e tile (tx, ty) of a level meets the rectangle z.
+__host__ __device__ __forceinline__ bool tile_meets(const ImgRect& z, int tx, int ty, int e) {
+  const int x0 = tx * e, y0 = ty * e;
+  return x0 <= z.x1 && x0 + e - 1 >= z.x0 && y0 <= z.y1 && y0 + e - 1 >= z.y0;
+}
+
+template <typename T0, int C, int N, int S>
+__device__ __forceinline__ void down_slots(const DownFusedArgs<T0, N>& a, const int* idx, int tx, int ty, int b,
+                                           int tid, float4 (*s1)[16 * 16], float4 (*s2)[8 * 8]) {
+  {
+    const DownLevel<N>& d = a.lv[0];
+    const int lx = tid & 15, ly = tid >> 4;
+    const int x = tx * 16 + lx, y = ty * 16 + ly;
+    if (x < d.w && y < d.h) {
+      // Taps past the edge load their in-range neighbour (ignored by
+      // down_px), and pixels outside an image's support load too (the
+      // result is zero there).
+      const int ix = 2 * x, iy = 2 * y;
+      const bool has_x1 = ix + 1 < a.w0, has_y1 = iy + 1 < a.h0;
+      const size_t r0 = ((size_t)b * a.h0 + iy) * a.w0;
+      const size_t rr1 = has_y1 ? r0 + a.w0 : r0;
+      const int ix1 = has_x1 ? ix + 1 : ix;
+      float4 p[S][4];
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    bool touch = false;
-    for (int k = 0; k < a.n_out; ++k) {
-      const int e = 16 >> k;
-      const ImgRect& z = a.lv[k].z[i];
-      const int x0 = tx * e, y0 = ty * e;
-      if (x0 <= z.x1 && x0 + e - 1 >= z.x0 && y0 <= z.y1 && y0 + e - 1 >= z.y0) touch = true;
-    }
-    if (!touch) continue;  // uniform across the workgroup
-    __syncthreads();       // the previous image's LDS reads are done
-    {
-      const DownLevel<N>& d = a.lv[0];
-      const int lx = tid & 15, ly = tid >> 4;
-      const int x = tx * 16 + lx, y = ty * 16 + ly;
-      if (x < d.w && y < d.h) {
-        float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (in_rect(d.s[i], x, y)) {  // outside the support the result is zero: no loads
-          const int ix = 2 * x, iy = 2 * y;
-          const bool has_x1 = ix + 1 < a.w0, has_y1 = iy + 1 < a.h0;
-          const size_t r0 = ((size_t)b * a.h0 + iy) * a.w0;
-          const size_t r1 = r0 + a.w0;
-          float4 p[4];
-          const int ix1 = has_x1 ? ix + 1 : ix;
-          const size_t rr1 = has_y1 ? r1 : r0;
-          p[0] = ld_px<C>(a.g0[i], r0 + ix);
-          p[1] = ld_px<C>(a.g0[i], r0 + ix1);
-          p[2] = ld_px<C>(a.g0[i], rr1 + ix);
-          p[3] = ld_px<C>(a.g0[i], rr1 + ix1);
-          const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
-          r = down_px<C>(p, in_range);
-        }
-        s1[ly * 16 + lx] = r;
-        if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r);
+      for (int s = 0; s < S; ++s) {
+        const T0* g = a.g0[idx[s]];
+        p[s][0] = ld_px<C>(g, r0 + ix);
+        p[s][1] = ld_px<C>(g, r0 + ix1);
+        p[s][2] = ld_px<C>(g, rr1 + ix);
+        p[s][3] = ld_px<C>(g, rr1 + ix1);
+      }
+      const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
+      const size_t o = ((size_t)b * d.h + y) * d.w + x;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int i = idx[s];
+        const float4 r = in_rect(d.s[i], x, y) ? down_px<C>(p[s], in_range) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        s1[s][ly * 16 + lx] = r;
+        if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], o, r);
       }
     }
-    int edge = 16;
-    for (int k = 1; k < a.n_out; ++k) {
-      __syncthreads();
-      const DownLevel<N>& din = a.lv[k - 1];
-      const DownLevel<N>& d = a.lv[k];
-      const int oedge = edge >> 1;
-      float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      bool active = false;
-      int lx = 0, ly = 0;
-      if (tid < oedge * oedge) {
-        lx = tid % oedge;
-        ly = tid / oedge;
-        const int x = tx * oedge + lx, y = ty * oedge + ly;
-        if (x < d.w && y < d.h) {
-          active = true;
-          const int gx = 2 * x, gy = 2 * y;
-          const bool has_x1 = gx + 1 < din.w, has_y1 = gy + 1 < din.h;
-          const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
-          const float4* src = k == 1 ? s1 : s2;
-          const int sx = 2 * lx, sy = 2 * ly;
+  }
+  int edge = 16;
+  for (int k = 1; k < a.n_out; ++k) {
+    __syncthreads();
+    const DownLevel<N>& din = a.lv[k - 1];
+    const DownLevel<N>& d = a.lv[k];
+    const int oedge = edge >> 1;
+    float4 r[S];
+    bool active = false;
+    int lx = 0, ly = 0;
+    if (tid < oedge * oedge) {
+      lx = tid % oedge;
+      ly = tid / oedge;
+      const int x = tx * oedge + lx, y = ty * oedge + ly;
+      if (x < d.w && y < d.h) {
+        active = true;
+        const int gx = 2 * x, gy = 2 * y;
+        const bool has_x1 = gx + 1 < din.w, has_y1 = gy + 1 < din.h;
+        const bool in_range[4] = {true, has_x1, has_y1, has_x1 && has_y1};
+        const int sx = 2 * lx, sy = 2 * ly;
+        const size_t o = ((size_t)b * d.h + y) * d.w + x;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const float4* src = k == 1 ? s1[s] : s2[s];
           float4 p[4];
           p[0] = src[sy * edge + sx];
           if (has_x1) p[1] = src[sy * edge + sx + 1];
           if (has_y1) p[2] = src[(sy + 1) * edge + sx];
           if (has_x1 && has_y1) p[3] = src[(sy + 1) * edge + sx + 1];
-          r = down_px<C>(p, in_range);  // zero outside the support: its LDS inputs are zero
-          if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], ((size_t)b * d.h + y) * d.w + x, r);
+          r[s] = down_px<C>(p, in_range);  // zero outside the support: its LDS inputs are zero
+          const int i = idx[s];
+          if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], o, r[s]);
         }
       }
-      __syncthreads();
-      if (active) s2[ly * oedge + lx] = r;
-      edge = oedge;
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) s2[s][ly * oedge + lx] = r[s];
+    }
+    edge = oedge;
+  }
+}
+
+template <typename T0, int C, int N>
+__global__ void __launch_bounds__(256) down_fused_slots_kernel(DownFusedArgs<T0, N> a) {
+  constexpr int K = N < kDownSlots ? N : kDownSlots;
+  __shared__ float4 s1[K][16 * 16];
+  __shared__ float4 s2[K][8 * 8];
+  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
+  const int tx = t % a.tiles_x, ty = t / a.tiles_x;
+  const int b = blockIdx.z;
+  const int tid = threadIdx.x;
+  unsigned act = 0;  // uniform
+  if (a.touch) {
+    act = __builtin_amdgcn_readfirstlane(a.touch[t]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      for (int k = 0; k < a.n_out; ++k) {
+        const ImgRect& z = a.lv[k].z[i];
+        if (tile_meets(z, tx, ty, 16 >> k)) act |= 1u << i;
+      }
+    }
+  }
+  bool first = true;
+  while (act) {
+    int idx[K];
+    const int n = min(__builtin_popcount(act), K);
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      idx[s] = act ? __builtin_ctz(act) : 0;
+      act &= act - 1u;
+    }
+    if (!first) __syncthreads();  // the previous slots' LDS reads are done
+    first = false;
+    switch (n) {  // uniform
+      case 1: down_slots<T0, C, N, 1>(a, idx, tx, ty, b, tid, s1, s2); break;
+      case 2: down_slots<T0, C, N, 2>(a, idx, tx, ty, b, tid, s1, s2); break;
+      case 3: down_slots<T0, C, N, 3>(a, idx, tx, ty, b, tid, s1, s2); break;
+      default:
+        if constexpr (K >= 4) down_slots<T0, C, N, 4>(a, idx, tx, ty, b, tid, s1, s2);
+        break;
     }
   }
 }
@@ -670,7 +724,214 @@ struct UpQuadArgs {
   int qx0, qy0, nqx, nqy;              // quads covering the window
   ImgRect z[N];                        // valid region of each image at this level (see LevelDesc)
   int tiles_x, tiles_y;                // 64 x 4 quads per workgroup
+  const float* wn;                     // N-image blend: planar normalised weights, or nullptr
+  size_t wn_plane;                     // floats per plane (w0 * h0)
 };
+
+// ---------------------------------------------------------------------------
+// Three or more images: the images of a workgroup's tile
+// ---------------------------------------------------------------------------
+// A workgroup covers 64 x 4 quads (128 x 8 pixels).  Outside its valid region
+// an image's Laplacian is zero, so only the images whose region meets the
+// tile can contribute there.  Up to kQuadSlots of them are carried in slots,
+// in index order; the slot count S is a template parameter (the kernel
+// switches on the tile's count, a uniform value), so every slot is in use and
+// all loads are issued without a branch between them.  The blend of the slots
+// gives the bits of the blend of all images (blendN_weighted).  A tile met by
+// more images takes the per-pixel path.
+constexpr int kQuadSlots = 4;
+
+// The images meeting the tile [x0, x1] x [y0, y1] (uniform): their number,
+// their set (bit i: image i), and the first kQuadSlots of them in idx, in
+// index order (0 past the end).
+template <int N>
+__device__ __forceinline__ int tile_images(const ImgRect* z, int x0, int y0, int x1, int y1, int* idx,
+                                           unsigned* set) {
+  unsigned act = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (z[i].x0 <= x1 && z[i].x1 >= x0 && z[i].y0 <= y1 && z[i].y1 >= y0) act |= 1u << i;
+  *set = act;
+  const int n = __builtin_popcount(act);
+#pragma unroll
+  for (int s = 0; s < kQuadSlots; ++s) {
+    idx[s] = act ? __builtin_ctz(act) : 0;
+    act &= act - 1u;
+  }
+  return n;
+}
+
+// The quad's pixels: position, whether in the window, and the clamped pixel
+// loaded (pixels outside the window load the window's first pixel).
+struct QuadPix {
+  int x[4], y[4];
+  bool in[4];
+  size_t pix[4];  // batched image index
+  size_t mpx[4];  // unbatched (mask) index
+};
+template <typename Args>
+__device__ __forceinline__ QuadPix quad_pixels(const Args& a, int qx, int qy, int b) {
+  QuadPix q;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+    q.x[e] = x;
+    q.y[e] = y;
+    q.in[e] = x >= a.win_x0 && x <= a.win_x1 && y >= a.win_y0 && y <= a.win_y1;
+    const int cx = q.in[e] ? x : a.win_x0, cy = q.in[e] ? y : a.win_y0;
+    q.pix[e] = ((size_t)b * a.h0 + cy) * a.w0 + cx;
+    q.mpx[e] = (size_t)cy * a.w0 + cx;
+  }
+  return q;
+}
+
+// R of level pixel (x, y) of batch b to the float output and/or the canvas.
+template <int C, typename Args>
+__device__ __forceinline__ void quad_store(const Args& a, int x, int y, int b, const float4& R) {
+  if (a.f32_out) st_px<C>(a.f32_out, ((size_t)b * a.h0 + y) * a.w0 + x, R);
+  if (a.u8_out) {
+    const int X = x + a.ox, Y = y + a.oy;
+    if (X < a.u8_w && Y < a.u8_h)
+      st_px_u8<C>(a.u8_out + ((size_t)b * a.u8_h + Y) * a.u8_pitch + (size_t)X * C, R);
+  }
+}
+
+// Level 0 of the N- and 3-image blends from the seam labels: the labelled
+// image's Laplacian where it is opaque (3 channels: always), else the first
+// slot of largest alpha (see up_quad_kernel).  Images outside the slots are
+// zero at the pixel and never win the alpha search.
+template <int C, int S>
+__device__ __forceinline__ float4 label_blend(const float4* Lv, const int* idx, int lab) {
+  float4 B = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  bool done = false;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (idx[s] == lab && (C == 3 || Lv[s].w != 0.0f)) {
+      B = C == 4 ? Lv[s] : make_float4(Lv[s].x, Lv[s].y, Lv[s].z, 0.0f);
+      done = true;
+    }
+  }
+  if constexpr (C == 4) {
+    if (!done) {
+      float amax = 0.0f;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        if (Lv[s].w > amax) {
+          amax = Lv[s].w;
+          B = Lv[s];
+        }
+      }
+    }
+  }
+  return B;
+}
+
+// Slot of image j under the image set MASK (its rank among the set's images).
+__host__ __device__ constexpr int slot_of(int mask, int j) { return __builtin_popcount(mask & ((1 << j) - 1)); }
+// Image of slot s under the image set MASK.
+__host__ __device__ constexpr int image_of(int mask, int s) {
+  return (mask & 1) ? (s == 0 ? 0 : 1 + image_of(mask >> 1, s - 1)) : 1 + image_of(mask >> 1, s);
+}
+
+// One quad of a level of the N- or 3-image blend through the tile's S slot
+// images: idx[0..S-1], or, when MASK >= 0, the images of the set MASK (a
+// compile-time set: the 3-image blend places each image's Laplacian at its
+// own position).  Weights: the seam labels (level 0), the planar normalised
+// weights (N-image levels >= 1) or the interleaved 3-image mask.
+template <typename T0, int C, int N, int KIND, int MODE, bool NEXT, int S, int MASK = -1>
+__device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const int* idx_rt, int qx, int qy,
+                                              int b) {
+  constexpr int SA = S > 0 ? S : 1;  // array extent
+  int idx[SA];
+#pragma unroll
+  for (int s = 0; s < S; ++s) idx[s] = MASK >= 0 ? image_of(MASK, s) : idx_rt[s];
+  const QuadPix q = quad_pixels(a, qx, qy, b);
+  // every load first: R's taps, each slot's next-level taps and pixels, the weights
+  const int qx1 = NEXT ? min(qx + 1, a.w1 - 1) : 0, qy1 = NEXT ? min(qy + 1, a.h1 - 1) : 0;
+  const size_t l0 = ((size_t)b * a.h1 + qy) * a.w1, l1 = ((size_t)b * a.h1 + qy1) * a.w1;
+  float4 rt[4], gt[SA][4], gv[SA][4];
+  ImgRect z[SA];
+  if (NEXT) {
+    rt[0] = ld_px<C>(a.r1, l0 + qx);
+    rt[1] = ld_px<C>(a.r1, l0 + qx1);
+    rt[2] = ld_px<C>(a.r1, l1 + qx);
+    rt[3] = ld_px<C>(a.r1, l1 + qx1);
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int i = idx[s];
+    z[s] = a.z[i];
+    if (NEXT) {
+      const float* g1 = a.g1[i];
+      gt[s][0] = ld_px<C>(g1, l0 + qx);
+      gt[s][1] = ld_px<C>(g1, l0 + qx1);
+      gt[s][2] = ld_px<C>(g1, l1 + qx);
+      gt[s][3] = ld_px<C>(g1, l1 + qx1);
+    }
+    const T0* g0 = a.g0[i];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) gv[s][e] = ld_px<C>(g0, q.pix[e]);
+  }
+  int lab[4];
+  float wv[SA][4];
+  float mk[4][3];
+  if constexpr (MODE == kMask0Label) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) lab[e] = static_cast<const uint8_t*>(a.mask0)[q.mpx[e]];
+  } else if constexpr (KIND == kBlend3) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) mk[e][c] = static_cast<const float*>(a.mask0)[q.mpx[e] * 3 + c];
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const float* w = a.wn + (size_t)idx[s] * a.wn_plane;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) wv[s][e] = w[q.mpx[e]];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (!q.in[e]) continue;
+    float4 Lv[SA];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      Lv[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (in_rect(z[s], q.x[e], q.y[e])) {
+        const float4 g = gv[s][e];
+        if (NEXT) {
+          const float4 up = up_quad_lap<C>(e, gt[s]);
+          Lv[s] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+        } else {
+          Lv[s] = g;
+        }
+      }
+    }
+    float4 B;
+    if constexpr (MODE == kMask0Label) {
+      B = label_blend<C, S>(Lv, idx, lab[e]);
+    } else if constexpr (KIND == kBlend3) {
+      static_assert(MASK >= 0, "the 3-image blend takes a compile-time image set");
+      float4 L3[3];  // the three images' Laplacians, zero for those outside the set
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        L3[j] = ((MASK >> j) & 1) ? Lv[slot_of(MASK, j)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      B = blend3_px<C>(L3, mk[e]);
+    } else {
+      float w[SA];
+#pragma unroll
+      for (int s = 0; s < S; ++s) w[s] = wv[s][e];
+      B = blendN_weighted<C, S>(Lv, w);
+    }
+    float4 R = B;
+    if (NEXT) {
+      const float4 up = up_quad_rec<C>(e, rt);
+      R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+    }
+    quad_store<C>(a, q.x[e], q.y[e], b, R);
+  }
+}
 
 // NEXT: the level has a coarser level (false at the coarsest: blend only).
 // A template parameter rather than a branch, so that the taps of the coarser
@@ -683,8 +944,44 @@ __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
   const int qy = a.qy0 + (t / a.tiles_x) * 4 + (int)threadIdx.y;
   const int b = blockIdx.z;
   if (qx >= a.qx0 + a.nqx || qy >= a.qy0 + a.nqy) return;
+  if constexpr (N > 2) {
+    // the tile's images in slots when they fit and the weights allow
+    if (MODE == kMask0Label || KIND == kBlend3 || a.wn != nullptr) {
+      int idx[kQuadSlots];
+      const int tqx = a.qx0 + (t % a.tiles_x) * 64, tqy = a.qy0 + (t / a.tiles_x) * 4;
+      unsigned set = 0;
+      const int n = tile_images<N>(a.z, 2 * tqx, 2 * tqy, 2 * tqx + 127, 2 * tqy + 7, idx, &set);
+      if constexpr (KIND == kBlend3) {
+        switch (set) {  // uniform
+          case 0: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 0, 0>(a, idx, qx, qy, b); return;
+          case 1: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 1>(a, idx, qx, qy, b); return;
+          case 2: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 2>(a, idx, qx, qy, b); return;
+          case 3: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2, 3>(a, idx, qx, qy, b); return;
+          case 4: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 4>(a, idx, qx, qy, b); return;
+          case 5: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2, 5>(a, idx, qx, qy, b); return;
+          case 6: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2, 6>(a, idx, qx, qy, b); return;
+          default: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 3, 7>(a, idx, qx, qy, b); return;
+        }
+      } else {
+        switch (n) {  // uniform
+          case 0: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 0>(a, idx, qx, qy, b); return;
+          case 1: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1>(a, idx, qx, qy, b); return;
+          case 2: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2>(a, idx, qx, qy, b); return;
+          case 3: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 3>(a, idx, qx, qy, b); return;
+          case 4:
+            if constexpr (N >= 4) {
+              up_quad_slots<T0, C, N, KIND, MODE, NEXT, 4>(a, idx, qx, qy, b);
+              return;
+            }
+            break;
+          default: break;  // more images than slots: the per-pixel path below
+        }
+      }
+    }
+  }
   if constexpr (MODE == kMask0Label) {
-    // Level 0 of the N- and 3-image blends.  The mask is one-hot, so the
+    // Level 0 of the N- and 3-image blends on a tile met by more images
+    // than there are slots.  The mask is one-hot, so the
     // blend (blendN_px / blend3_px with one weight 1 and the others 0, whose
     // terms are all +-0) is the labelled image's Laplacian where that image
     // is opaque, and elsewhere the alpha fallback: the first image of largest
@@ -759,9 +1056,10 @@ __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
     return;
   }
   if constexpr (N > 2) {
-    // Three or more images: one pixel of the quad at a time, each image's
-    // taps loaded where needed (cache hits across the quad), so registers
-    // hold one pixel's Laplacians rather than the quad's.
+    // Three or more images on a tile met by more images than there are
+    // slots, or without planar weights: one pixel of the quad at a time, each
+    // image's taps loaded where needed (cache hits across the quad), so
+    // registers hold one pixel's Laplacians rather than the quad's.
     float4 rt[4];
     const int qx1 = has_next ? min(qx + 1, a.w1 - 1) : 0, qy1 = has_next ? min(qy + 1, a.h1 - 1) : 0;
     const size_t l0 = ((size_t)b * a.h1 + qy) * a.w1, l1 = ((size_t)b * a.h1 + qy1) * a.w1;

@@ -100,19 +100,16 @@ __device__ __forceinline__ float4 blend2_px(const float4& l1, const float4& l2, 
   }
 }
 
+// The N-image blend after its first normalisation: w[i] = mask_i / sum of
+// the pixel's mask values (the mask values themselves when that sum is 0).
+// w is modified.  An image whose Laplacian is zero at the pixel (alpha 0, or
+// RGB zeros) adds nothing here beyond what the mask sum already took from it:
+// its weight is dropped (RGBA) or multiplies zeros (RGB), and the running sums
+// start at +0 and never become -0 (rounding to nearest), so adding a +-0 term
+// changes nothing.  A caller may therefore pass only the images that can be
+// non-zero at the pixel, in index order.
 template <int C, int N>
-__device__ __forceinline__ float4 blendN_px(const float4* L, const float* mp) {
-  float w[N];
-  float sum_w = 0.0f;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    w[i] = mp[i];
-    sum_w += w[i];
-  }
-  if (sum_w > 0.0f) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) w[i] /= sum_w;
-  }
+__device__ __forceinline__ float4 blendN_weighted(const float4* L, float* w) {
   if constexpr (C == 4) {
     float sum_w2 = 0.0f, max_alpha = 0.0f;
     int max_idx = -1;
@@ -154,6 +151,29 @@ __device__ __forceinline__ float4 blendN_px(const float4* L, const float* mp) {
     }
   }
   return make_float4(ax, ay, az, aw);
+}
+
+// First normalisation of the N-image blend: the mask values over their sum
+// when that is positive (cudaBlendN.cu:216-223).
+template <int N>
+__device__ __forceinline__ void blendN_normalize(const float* mp, float* w) {
+  float sum_w = 0.0f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    w[i] = mp[i];
+    sum_w += w[i];
+  }
+  if (sum_w > 0.0f) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] /= sum_w;
+  }
+}
+
+template <int C, int N>
+__device__ __forceinline__ float4 blendN_px(const float4* L, const float* mp) {
+  float w[N];
+  blendN_normalize<N>(mp, w);
+  return blendN_weighted<C, N>(L, w);
 }
 
 template <int C>
