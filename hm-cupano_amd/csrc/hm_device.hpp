@@ -24,6 +24,19 @@ __device__ __forceinline__ uint8_t* row_ptr(const Surf& s, int b, int y) {
   return s.ptr + ((size_t)b * s.h + (size_t)y) * (size_t)s.pitch;
 }
 
+// ---- the workgroup a kernel body runs as ----------------------------------
+// Kernel bodies take the workgroup's index (x: tile, z: batch entry) rather
+// than reading blockIdx, so that the same body runs as its own launch or as a
+// work item of a batched pass (hm_batch.hip).  Every body runs on 256 threads;
+// flat_tid() is a thread's index among them for either launch shape (256 x 1
+// or 64 x 4).
+struct Blk {
+  int x, z;
+};
+__device__ __forceinline__ Blk hw_blk() { return Blk{(int)blockIdx.x, (int)blockIdx.z}; }
+constexpr int kWgThreads = 256;
+__device__ __forceinline__ int flat_tid() { return (int)(threadIdx.x + blockDim.x * threadIdx.y); }
+
 // ---- XCD-aware tile order ------------------------------------------------
 // MI355X dispatches consecutive workgroup ids round-robin over its 8 XCDs,
 // each with its own L2.  Renumbering so that XCD k runs the k-th contiguous

@@ -1,6 +1,7 @@
 // hm_pano_device.hpp -- per-pixel remap helpers shared by the pano kernels.
 #pragma once
 
+#include "hm_batch.hpp"
 #include "hm_device.hpp"
 #include "hm_stitch.h"
 
@@ -160,13 +161,14 @@ __device__ __forceinline__ void multi_inputs_slots(const MultiArgs<N>& a, const 
   }
 }
 
+// Runs on 64 x 4 threads (kBlock).
 template <int C, int N, bool DM>
-__global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
-  const int t = (int)blockIdx.x;
+__device__ __forceinline__ void multi_inputs_body(const MultiArgs<N>& a, Blk blk) {
+  const int t = blk.x;
   const int tx0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx, ty0 = a.y0 + (t / a.tiles_x) * (int)blockDim.y;
   const int X0 = tx0 + (int)threadIdx.x;
   const int Y = ty0 + (int)threadIdx.y;
-  const int b = blockIdx.z;
+  const int b = blk.z;
   if (Y >= a.y1) return;
   // the images whose valid region meets the tile (uniform)
   const int u0 = tx0 - a.x0, u1 = u0 + (int)blockDim.x * kPx - 1;
@@ -194,6 +196,11 @@ __global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
   }
 }
 
+template <int C, int N, bool DM>
+__global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
+  multi_inputs_body<C, N, DM>(a, hw_blk());
+}
+
 // Canvas writers (pixels inside the skip rectangle are never written):
 //  kCanvasLabel: the seam label selects the image (batched_remap_hard_seam_kernel_n,
 //    cudaRemap.cu:252-294; BatchedRemapKernelExOffsetWithDestMap, :204-246;
@@ -207,12 +214,13 @@ __global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
 // (uncovered_zero).
 enum CanvasMode { kCanvasLabel = 0, kCanvasLast = 1, kCanvasClear = 2 };
 
+// Runs on 64 x 4 threads (kBlock).
 template <int C, int N>
-__global__ void __launch_bounds__(256) multi_canvas_kernel(MultiArgs<N> a) {
-  const int t = (int)blockIdx.x;
+__device__ __forceinline__ void multi_canvas_body(const MultiArgs<N>& a, Blk blk) {
+  const int t = blk.x;
   const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
   const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
-  const int b = blockIdx.z;
+  const int b = blk.z;
   if (Y >= a.y1) return;
   const bool skip_row = Y >= a.skip_y0 && Y < a.skip_y1;
   const uint8_t* srow = a.seam ? a.seam + (size_t)Y * a.cw : nullptr;
@@ -272,6 +280,11 @@ __global__ void __launch_bounds__(256) multi_canvas_kernel(MultiArgs<N> a) {
 #pragma unroll
   for (int j = 0; j < kPx; ++j)
     if (wr[j]) store_px<C>(crow + (size_t)(X0 + j * (int)blockDim.x) * C, v[j]);
+}
+
+template <int C, int N>
+__global__ void __launch_bounds__(256) multi_canvas_kernel(MultiArgs<N> a) {
+  multi_canvas_body<C, N>(a, hw_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -336,18 +349,27 @@ hipError_t launch_multi(const MultiGeom& g, const MultiRegion& r, int channels, 
   const bool c4 = channels == 4;
   switch (r.kernel) {
     case kMultiInputs:
+      if (hm::g_recorder)
+        return c4 ? hm::g_recorder->add(hm::op_multi_inputs<4, N, false>(), grid, kBlock, s, a, "multi_inputs")
+                  : hm::g_recorder->fail("multi_inputs");
       if (c4)
         hipLaunchKernelGGL((multi_inputs_kernel<4, N, false>), grid, kBlock, 0, s, a);
       else
         hipLaunchKernelGGL((multi_inputs_kernel<3, N, false>), grid, kBlock, 0, s, a);
       break;
     case kMultiInputsDM:
+      if (hm::g_recorder)
+        return c4 ? hm::g_recorder->add(hm::op_multi_inputs<4, N, true>(), grid, kBlock, s, a, "multi_inputs")
+                  : hm::g_recorder->fail("multi_inputs");
       if (c4)
         hipLaunchKernelGGL((multi_inputs_kernel<4, N, true>), grid, kBlock, 0, s, a);
       else
         hipLaunchKernelGGL((multi_inputs_kernel<3, N, true>), grid, kBlock, 0, s, a);
       break;
     default:
+      if (hm::g_recorder)
+        return c4 ? hm::g_recorder->add(hm::op_multi_canvas<4, N>(), grid, kBlock, s, a, "multi_canvas")
+                  : hm::g_recorder->fail("multi_canvas");
       if (c4)
         hipLaunchKernelGGL((multi_canvas_kernel<4, N>), grid, kBlock, 0, s, a);
       else

@@ -7,6 +7,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "hm_batch.hpp"
 #include "hm_internal.hpp"
 #include "hm_stitch.h"
 
@@ -82,6 +83,7 @@ static hipError_t down_t(const void* const* in, float* const* out, int in_w, int
   const TileGrid tg = tile_grid(out_w, out_h, batch, kBlock);
   a.tiles_x = tg.tiles_x;
   a.tiles_y = tg.tiles_y;
+  if (g_recorder) return g_recorder->fail("down");
   hipLaunchKernelGGL((down_kernel<TIn, C, N>), tg.grid, kBlock, 0, s, a);
   return hipGetLastError();
 }
@@ -113,6 +115,7 @@ hipError_t launch_down(int c, int n, bool in_u8, const void* const* in, float* c
 hipError_t launch_mask_down(const float* in, int in_w, int in_h, float* out, int out_w, int out_h, int m,
                             hipStream_t s) {
   if (m < 1 || m > kMaxImages) return hipErrorInvalidValue;
+  if (g_recorder) return g_recorder->fail("mask_down");
   hipLaunchKernelGGL(mask_down_kernel, grid_for(out_w, out_h, 1, kBlock), kBlock, 0, s, in, in_w, in_h, out, out_w,
                      out_h, m);
   return hipGetLastError();
@@ -120,6 +123,7 @@ hipError_t launch_mask_down(const float* in, int in_w, int in_h, float* out, int
 
 hipError_t launch_mask_planar(const float* in, int w, int h, float* out, int m, hipStream_t s) {
   const dim3 grid = grid_for(w, h, 1, kBlock);
+  if (g_recorder) return g_recorder->fail("mask_planar");
   switch (m) {
     case 3: hipLaunchKernelGGL(mask_planar_kernel<3>, grid, kBlock, 0, s, in, w, h, out); break;
     case 4: hipLaunchKernelGGL(mask_planar_kernel<4>, grid, kBlock, 0, s, in, w, h, out); break;
@@ -162,6 +166,7 @@ static hipError_t up_t(const UpLaunch& u, int batch, hipStream_t s) {
   const TileGrid tg = tile_grid(u.win_w, u.win_h, batch, kBlock);
   a.tiles_x = tg.tiles_x;
   a.tiles_y = tg.tiles_y;
+  if (g_recorder) return g_recorder->fail("up");
   hipLaunchKernelGGL((up_kernel<THi, C, N, KIND>), tg.grid, kBlock, 0, s, a);
   return hipGetLastError();
 }
@@ -199,6 +204,7 @@ static hipError_t down_fused_t(const void* const* g0, int w0, int h0, int n_out,
   fill_down_args(a, lv, w0, h0, n_out);
   a.touch = touch;
   const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+  if (g_recorder) return g_recorder->add(op_down<T0, C, N>(), grid, dim3(256), s, a, "down_fused");
   if constexpr (N <= 2)
     hipLaunchKernelGGL((down_fused_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
   else
@@ -277,7 +283,7 @@ hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int bat
   a.wy0 = u.wy0;
   a.wx1 = u.wx1;
   a.wy1 = u.wy1;
-  if (u.n - 1 <= kTailLds) {
+  if (u.n - 1 <= kTailLds && !g_recorder) {  // (a batched pass carries the other form)
     // every level's footprint in LDS, loaded in one burst
     constexpr int T = kTailLdsTile;
     a.tiles_x = (u.wx1 - u.wx0 + T) / T;
@@ -301,6 +307,10 @@ hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int bat
   a.tiles_x = (u.wx1 - u.wx0 + kTailTile) / kTailTile;
   a.tiles_y = (u.wy1 - u.wy0 + kTailTile) / kTailTile;
   const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+  if (g_recorder) {
+    const int op = c == 4 && kind == kBlend2 ? kOpTail2 : kOpNone;
+    return g_recorder->add(op, grid, dim3(256), s, a, "up_tail");
+  }
   if (c == 4) {
     if (kind == kBlend2)
       hipLaunchKernelGGL((up_tail_kernel<4, 2, kBlend2>), grid, dim3(256), 0, s, a);
@@ -359,6 +369,11 @@ hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch
       a.wn = u.wn;
       a.wn_plane = u.wn_plane;
       const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+      if (g_recorder) {
+        const int op = a.r1 ? op_up_quad<T0, kC, kN, kK, decltype(M)::value, true>()
+                            : op_up_quad<T0, kC, kN, kK, decltype(M)::value, false>();
+        return g_recorder->add(op, grid, dim3(64, 4, 1), s, a, "up_quad");
+      }
       if (a.r1)
         hipLaunchKernelGGL((up_quad_kernel<T0, kC, kN, kK, decltype(M)::value, true>), grid, dim3(64, 4, 1), 0, s, a);
       else

@@ -226,14 +226,19 @@ inline void fill_down_args(DownFusedArgs<T0, N>& a, const LevelDesc* lv, int w0,
   a.touch = nullptr;
 }
 
+// LDS of the down pass: the first produced level's 16x16 tile of every image
+// (or slot), then the 8x8 tile of the next.
+template <int N>
+constexpr int down_lds_float4() { return N * (16 * 16 + 8 * 8); }
+
 template <typename T0, int C, int N>
-__global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a) {
-  __shared__ float4 s1[N][16 * 16];
-  __shared__ float4 s2[N][8 * 8];
-  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
+__device__ __forceinline__ void down_fused_body(const DownFusedArgs<T0, N>& a, Blk blk, float4* lds) {
+  float4(*s1)[16 * 16] = reinterpret_cast<float4(*)[16 * 16]>(lds);
+  float4(*s2)[8 * 8] = reinterpret_cast<float4(*)[8 * 8]>(lds + N * 16 * 16);
+  const int t = xcd_tile(blk.x, a.tiles_x * a.tiles_y);
   const int tx = t % a.tiles_x, ty = t / a.tiles_x;
-  const int b = blockIdx.z;
-  const int tid = threadIdx.x;
+  const int b = blk.z;
+  const int tid = flat_tid();
   // ---- first produced level: one pixel per thread from level 0
   {
     const DownLevel<N>& d = a.lv[0];
@@ -271,6 +276,12 @@ __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a)
     }
   }
   down_rest<C, N>(a, tx, ty, b, tid, s1, s2);
+}
+
+template <typename T0, int C, int N>
+__global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a) {
+  __shared__ float4 lds[down_lds_float4<N>()];
+  down_fused_body<T0, C, N>(a, hw_blk(), lds);
 }
 
 // Three or more images: the images whose valid region meets the tile at some
@@ -366,15 +377,18 @@ __device__ __forceinline__ void down_slots(const DownFusedArgs<T0, N>& a, const 
   }
 }
 
+template <int N>
+constexpr int down_slots_count() { return N < kDownSlots ? N : kDownSlots; }
+
 template <typename T0, int C, int N>
-__global__ void __launch_bounds__(256) down_fused_slots_kernel(DownFusedArgs<T0, N> a) {
-  constexpr int K = N < kDownSlots ? N : kDownSlots;
-  __shared__ float4 s1[K][16 * 16];
-  __shared__ float4 s2[K][8 * 8];
-  const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
+__device__ __forceinline__ void down_fused_slots_body(const DownFusedArgs<T0, N>& a, Blk blk, float4* lds) {
+  constexpr int K = down_slots_count<N>();
+  float4(*s1)[16 * 16] = reinterpret_cast<float4(*)[16 * 16]>(lds);
+  float4(*s2)[8 * 8] = reinterpret_cast<float4(*)[8 * 8]>(lds + K * 16 * 16);
+  const int t = xcd_tile(blk.x, a.tiles_x * a.tiles_y);
   const int tx = t % a.tiles_x, ty = t / a.tiles_x;
-  const int b = blockIdx.z;
-  const int tid = threadIdx.x;
+  const int b = blk.z;
+  const int tid = flat_tid();
   unsigned act = 0;  // uniform
   if (a.touch) {
     act = __builtin_amdgcn_readfirstlane(a.touch[t]);
@@ -407,6 +421,12 @@ __global__ void __launch_bounds__(256) down_fused_slots_kernel(DownFusedArgs<T0,
         break;
     }
   }
+}
+
+template <typename T0, int C, int N>
+__global__ void __launch_bounds__(256) down_fused_slots_kernel(DownFusedArgs<T0, N> a) {
+  __shared__ float4 lds[down_lds_float4<down_slots_count<N>()>()];
+  down_fused_slots_body<T0, C, N>(a, hw_blk(), lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -462,14 +482,17 @@ struct UpTailArgs {
   int tiles_x, tiles_y;
 };
 
+// LDS of the tail: R of two levels, then the footprints (as ints).
+constexpr int kTailLdsFloat4 = 2 * kTailRegion * kTailRegion + kMaxTail;
+
 template <int C, int N, int KIND>
-__global__ void __launch_bounds__(256) up_tail_kernel(UpTailArgs<N> a) {
-  __shared__ float4 buf[2][kTailRegion * kTailRegion];
-  __shared__ int reg[kMaxTail][4];  // footprint [x0, x1] x [y0, y1] per tail level
-  const int t = (int)blockIdx.x;
+__device__ __forceinline__ void up_tail_body(const UpTailArgs<N>& a, Blk blk, float4* lds) {
+  float4(*buf)[kTailRegion * kTailRegion] = reinterpret_cast<float4(*)[kTailRegion * kTailRegion]>(lds);
+  int(*reg)[4] = reinterpret_cast<int(*)[4]>(lds + 2 * kTailRegion * kTailRegion);  // footprint per tail level
+  const int t = blk.x;
   const int tx = t % a.tiles_x, ty = t / a.tiles_x;
-  const int b = blockIdx.z;
-  const int tid = threadIdx.x;
+  const int b = blk.z;
+  const int tid = flat_tid();
   const int n = a.n;
   if (tid == 0) {
     int x0 = a.wx0 + tx * kTailTile, y0 = a.wy0 + ty * kTailTile;
@@ -490,7 +513,7 @@ __global__ void __launch_bounds__(256) up_tail_kernel(UpTailArgs<N> a) {
     const TailLevel<N>& d = a.lv[n - 1];
     const int rx0 = reg[n - 1][0], ry0 = reg[n - 1][2];
     const int rw = reg[n - 1][1] - rx0 + 1, rh = reg[n - 1][3] - ry0 + 1;
-    for (int idx = tid; idx < rw * rh; idx += blockDim.x) {
+    for (int idx = tid; idx < rw * rh; idx += kWgThreads) {
       const int x = rx0 + idx % rw, y = ry0 + idx / rw;
       const size_t pix = ((size_t)b * d.h + y) * d.w + x;
       float4 Lv[N];
@@ -511,7 +534,7 @@ __global__ void __launch_bounds__(256) up_tail_kernel(UpTailArgs<N> a) {
     const int nqx = (rx1 >> 1) - qx0 + 1, nqy = (ry1 >> 1) - qy0 + 1;
     const int nx0 = reg[j + 1][0], ny0 = reg[j + 1][2], nrw = reg[j + 1][1] - nx0 + 1;
     const float4* prev = buf[cur ^ 1];
-    for (int q = tid; q < nqx * nqy; q += blockDim.x) {
+    for (int q = tid; q < nqx * nqy; q += kWgThreads) {
       const int qx = qx0 + q % nqx, qy = qy0 + q / nqx;
       const int qx1 = min(qx + 1, dn.w - 1), qy1 = min(qy + 1, dn.h - 1);
       const size_t l0 = ((size_t)b * dn.h + qy) * dn.w, l1 = ((size_t)b * dn.h + qy1) * dn.w;
@@ -551,6 +574,12 @@ __global__ void __launch_bounds__(256) up_tail_kernel(UpTailArgs<N> a) {
       }
     }
   }
+}
+
+template <int C, int N, int KIND>
+__global__ void __launch_bounds__(256) up_tail_kernel(UpTailArgs<N> a) {
+  __shared__ float4 lds[kTailLdsFloat4];
+  up_tail_body<C, N, KIND>(a, hw_blk(), lds);
 }
 
 // The same tail with every level's footprint (Gaussians and mask) read into
@@ -936,13 +965,14 @@ __device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const 
 // NEXT: the level has a coarser level (false at the coarsest: blend only).
 // A template parameter rather than a branch, so that the taps of the coarser
 // level and the level's own pixels are loaded in one straight run.
+// Runs on 64 x 4 threads.
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT>
-__global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
+__device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk) {
   constexpr bool has_next = NEXT;
-  const int t = (int)blockIdx.x;
+  const int t = blk.x;
   const int qx = a.qx0 + (t % a.tiles_x) * 64 + (int)threadIdx.x;
   const int qy = a.qy0 + (t / a.tiles_x) * 4 + (int)threadIdx.y;
-  const int b = blockIdx.z;
+  const int b = blk.z;
   if (qx >= a.qx0 + a.nqx || qy >= a.qy0 + a.nqy) return;
   if constexpr (N > 2) {
     // the tile's images in slots when they fit and the weights allow
@@ -1175,6 +1205,11 @@ __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
         st_px_u8<C>(a.u8_out + ((size_t)b * a.u8_h + Y) * a.u8_pitch + (size_t)X * C, R);
     }
   }
+}
+
+template <typename T0, int C, int N, int KIND, int MODE, bool NEXT>
+__global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
+  up_quad_body<T0, C, N, KIND, MODE, NEXT>(a, hw_blk());
 }
 
 }  // namespace hmdev

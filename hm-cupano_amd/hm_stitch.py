@@ -78,6 +78,11 @@ class hmServerStats(ctypes.Structure):
                 ("eager_launches", ctypes.c_long)]
 
 
+class hmBatchCall(ctypes.Structure):
+    _fields_ = [("pano", ctypes.c_void_p), ("inputs", ctypes.POINTER(hmSurface)), ("num_inputs", ctypes.c_int),
+                ("canvas", hmSurface), ("fused", ctypes.c_int)]
+
+
 class hmTuning(ctypes.Structure):
     _fields_ = [("remap_px", ctypes.c_int), ("tail_base", ctypes.c_int), ("overlap", ctypes.c_int),
                 ("remap_xcd", ctypes.c_int), ("remap_rows", ctypes.c_int), ("canvas_grid", ctypes.c_int),
@@ -181,6 +186,12 @@ def lib() -> ctypes.CDLL:
         "hmServerWaitIdle": (i, [vp]),
         "hmServerGetStats": (i, [vp, ctypes.POINTER(hmServerStats)]),
         "hmServerDestroy": (None, [vp]),
+        "hmBatchCreate": (i, [ctypes.POINTER(hmBatchCall), i, ctypes.POINTER(vp)]),
+        "hmBatchSetGrid": (i, [vp, i]),
+        "hmBatchGetInfo": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "hmBatchRun": (i, [vp, vp, i]),
+        "hmBatchDestroy": (i, [vp]),
+        "hmBatchAutotune": (i, [vp, vp, i, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_float)]),
         "hmStitchPanoGetTuning": (i, [vp, ctypes.POINTER(hmTuning)]),
         "hmStitchPanoSetTuning": (i, [vp, ctypes.POINTER(hmTuning)]),
         "hmStitchPanoAutotune": (i, [vp, ctypes.POINTER(S), i, S, i, i, vp, ctypes.POINTER(hmTuning),
@@ -522,6 +533,65 @@ class Server:
 
 
 # ---------------------------------------------------------------------------
+# batched pass over mixed configurations
+# ---------------------------------------------------------------------------
+class Batch:
+    """One pass over several pano calls of mixed classes and sizes (hmBatch*).
+
+    `calls` is a list of (pano, inputs, canvas) or (pano, inputs, canvas,
+    fused); each pano appears once, and a pano with batch size B takes B frames
+    per call.  The pass is built for these tensors: run() stitches whatever they
+    hold when it runs.  Launch k of every call forms stage k; each kernel body
+    of a stage is one persistent launch walking the tile work-list of every
+    call's launch of that body."""
+
+    def __init__(self, calls):
+        self._keep = []
+        arr = (hmBatchCall * len(calls))()
+        for k, c in enumerate(calls):
+            pano, inputs, canvas = c[0], c[1], c[2]
+            fused = c[3] if len(c) > 3 else True
+            surfs = (hmSurface * len(inputs))(*[surface(t) for t in inputs])
+            self._keep.append((pano, inputs, canvas, surfs))
+            arr[k].pano = pano._h
+            arr[k].inputs = surfs
+            arr[k].num_inputs = len(inputs)
+            arr[k].canvas = surface(canvas)
+            arr[k].fused = int(fused)
+        h = ctypes.c_void_p()
+        check(lib().hmBatchCreate(arr, len(calls), ctypes.byref(h)), "hmBatchCreate")
+        self._h = h
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.hmBatchDestroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_grid(self, workgroups: int) -> None:
+        check(lib().hmBatchSetGrid(self._h, workgroups), "hmBatchSetGrid")
+
+    def info(self) -> dict:
+        st, ln, rec, gr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().hmBatchGetInfo(self._h, ctypes.byref(st), ctypes.byref(ln), ctypes.byref(rec), ctypes.byref(gr)),
+              "hmBatchGetInfo")
+        return dict(stages=st.value, launches=ln.value, recorded_launches=rec.value, grid=gr.value)
+
+    def run(self, stream=None, use_graph: bool = False) -> int:
+        return lib().hmBatchRun(self._h, _stream(stream), int(use_graph))
+
+    def autotune(self, passes: int = 5, stream=None):
+        """Keeps the fastest grid; returns (workgroups per launch, ms per pass)."""
+        g, ms = ctypes.c_int(), ctypes.c_float()
+        check(lib().hmBatchAutotune(self._h, _stream(stream), passes, ctypes.byref(g), ctypes.byref(ms)),
+              "hmBatchAutotune")
+        return g.value, ms.value
+
+
+# ---------------------------------------------------------------------------
 # kernel tier
 # ---------------------------------------------------------------------------
 class BlendContext:
@@ -541,34 +611,6 @@ class BlendContext:
         if h is not None and h.value and _lib is not None:
             _lib.hmBlendContextDestroy(h)
             self._h = None
-
-    def prepare(self) -> int:
-        return lib().hmStitchPanoPrepare(self._h)
-
-    def device_bytes(self) -> int:
-        b = ctypes.c_size_t()
-        check(lib().hmStitchPanoDeviceBytes(self._h, ctypes.byref(b)), "hmStitchPanoDeviceBytes")
-        return b.value
-
-    def get_tuning(self) -> dict:
-        t = hmTuning()
-        check(lib().hmStitchPanoGetTuning(self._h, ctypes.byref(t)), "hmStitchPanoGetTuning")
-        return {k: getattr(t, k) for k, _ in hmTuning._fields_}
-
-    def set_tuning(self, remap_px: int = 4, tail_base: int = 2, overlap: int = 1, remap_xcd: int = 0,
-                   remap_rows: int = 1, canvas_grid: int = 0, fuse_down: int = 0) -> None:
-        t = hmTuning(remap_px, tail_base, overlap, remap_xcd, remap_rows, canvas_grid, fuse_down)
-        check(lib().hmStitchPanoSetTuning(self._h, ctypes.byref(t)), "hmStitchPanoSetTuning")
-
-    def autotune(self, inputs, canvas, frames: int = 10, fused: bool = True, stream=None):
-        """Times every setting of the knobs that matter for this object on this frame and keeps
-        the fastest; returns (tuning dict, ms per frame)."""
-        arr = (hmSurface * len(inputs))(*[surface(t) for t in inputs])
-        t = hmTuning()
-        ms = ctypes.c_float()
-        check(lib().hmStitchPanoAutotune(self._h, arr, len(inputs), surface(canvas), int(fused), frames,
-                                         _stream(stream), ctypes.byref(t), ctypes.byref(ms)), "hmStitchPanoAutotune")
-        return {k: getattr(t, k) for k, _ in hmTuning._fields_}, ms.value
 
     def levels(self):
         n = ctypes.c_int()

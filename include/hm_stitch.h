@@ -312,6 +312,49 @@ int hmServerGetStats(hmServer* server, hmServerStats* stats);
 /* Drains the queue, then releases streams and graphs. */
 void hmServerDestroy(hmServer* server);
 
+/* -------------------------------------------------------------------------
+ * Batched pass over mixed pano configurations.  No counterpart in the
+ * reference (its apps call process() once per frame).  Several pano calls --
+ * rigs of the 2-, 3- and N-image classes with different image counts, canvas
+ * sizes and level counts, each call possibly a batch of frames -- run as one
+ * pass: launch k of every call forms stage k, and each kernel body of a stage
+ * is one launch of a persistent kernel whose workgroups walk the tile
+ * work-list of every call's launch of that body (the bodies of a stage run
+ * concurrently).  The
+ * pass is built for the given buffers (inputs and canvases are read and
+ * written where they are at create time, like a captured graph); each frame's
+ * output is bit-identical to its own process().
+ *
+ * Carried: 4-channel pipelines of the 2-image class, the 3-image class and the
+ * N-image class with 3 or 4 images (any levels).  Anything else, a pano in
+ * debug mode, or one pano in two calls: hipErrorNotSupported /
+ * hipErrorInvalidValue at create time.
+ * ------------------------------------------------------------------------- */
+typedef struct hmBatch hmBatch;
+typedef struct hmBatchCall {
+  hmStitchPano* pano;
+  const hmSurface* inputs; /* the pano's inputs (2, 3 or N surfaces) */
+  int num_inputs;
+  hmSurface canvas;
+  int fused; /* the 3-image class's path (see hmStitchPano3Process) */
+} hmBatchCall;
+/* Prepares every pano, records each call's launches and builds the stages. */
+int hmBatchCreate(const hmBatchCall* calls, int num_calls, hmBatch** out);
+/* Workgroups per launch: 0 (the default) one per work unit; fewer makes the
+ * grid persistent, each workgroup walking the list with that stride. */
+int hmBatchSetGrid(hmBatch* batch, int workgroups);
+/* Times the pass (passes runs after a warm-up) for one workgroup per work
+ * unit and for persistent grids of 2, 4, 8 and 16 per compute unit, and keeps
+ * the fastest. */
+int hmBatchAutotune(hmBatch* batch, hipStream_t stream, int passes, int* best_grid, float* best_ms_per_pass);
+/* stages: pipeline stages; launches: kernel launches of one pass; recorded:
+ * the launches the calls make one by one; grid: workgroups per launch. */
+int hmBatchGetInfo(const hmBatch* batch, int* stages, int* launches, int* recorded_launches, int* grid);
+/* Runs the pass on `stream`; use_graph != 0 captures it as a hipGraph on the
+ * first call and replays that graph afterwards. */
+int hmBatchRun(hmBatch* batch, hipStream_t stream, int use_graph);
+int hmBatchDestroy(hmBatch* batch);
+
 /* ------------------------------------------------------------------------- */
 /* Frame loop (host frames in, host frames out)                              */
 /* ------------------------------------------------------------------------- */
