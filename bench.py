@@ -262,6 +262,7 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
     mpix_open = sum(rigs[ri]["mpix"] for _, ri, _ in arrivals)
     st = server.stats()
     server.close()
+    batched = batched_mixed_benchmark(torch, hm_stitch, rigs, stream)
     plain_fps = frames_closed / t_plain
     served_fps = frames_closed / t_served
     return {
@@ -280,7 +281,65 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
                       "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
                       "latency_ms_max": round(float(lat.max()), 3)},
         "graphs_captured": st["graphs_captured"], "graph_launches": st["graph_launches"],
+        "batched_mixed": batched,
     }
+
+
+# The 16-frame mixed batch: 6 frames of 2 x 5312x2988 -> 8500x3300 L=6, 6 of
+# 3 x 2704x1520 -> 6000x2000 L=4 and 4 of 4 x 1920x1080 -> 4000x1500 L=0, one
+# frame per resident rig of the served load.
+BATCH_MIX = [("2img_5312x2988_8500x3300_L6", 6), ("3img_2704x1520_6000x2000_L4", 6), ("4img_1920x1080_4000x1500_L0", 4)]
+
+
+def batched_mixed_benchmark(torch, hm_stitch, rigs, stream, passes=20):
+    """Plain (one frame per process call, synchronised) against the batched pass
+    (hmBatch: the 16 frames' pipelines as one pass of persistent work-list
+    launches, grid autotuned, replayed as a hipGraph).  Every frame of the pass is
+    compared with the same frame stitched alone."""
+    sel = []
+    for name, count in BATCH_MIX:
+        sel += [r for r in rigs if r["name"] == name][:count]
+
+    def plain_one(r):
+        ins, can = r["inputs"][0], r["canvases"][0]
+        if r["kind"] == "2":
+            return r["pano"].process(ins[0], ins[1], can, stream)
+        if r["kind"] == "3":
+            return r["pano"].process(ins[0], ins[1], ins[2], can, stream=stream)
+        return r["pano"].process(ins, can, stream)
+
+    for r in sel:
+        r["canvases"][0].fill_(3)
+    for r in sel:
+        hm_stitch.check(plain_one(r), "plain process")
+    torch.cuda.synchronize()
+    alone = [r["canvases"][0].clone() for r in sel]
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        for r in sel:
+            hm_stitch.check(plain_one(r), "plain process")
+            torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / passes
+    batch = hm_stitch.Batch([(r["pano"], r["inputs"][0], r["canvases"][0]) for r in sel])
+    grid, _ = batch.autotune(passes=3, stream=stream)
+    for r in sel:
+        r["canvases"][0].fill_(3)
+    torch.cuda.synchronize()
+    hm_stitch.check(batch.run(stream, use_graph=True), "batched pass")
+    torch.cuda.synchronize()
+    identical = all(bool(torch.equal(r["canvases"][0], a)) for r, a in zip(sel, alone))
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        hm_stitch.check(batch.run(stream, use_graph=True), "batched pass")
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / passes
+    info = batch.info()
+    batch.close()
+    return {"frames": len(sel), "mix": [f"{n} x{c}" for n, c in BATCH_MIX],
+            "plain_ms_per_16_frames": round(1e3 * t_plain, 3), "batched_ms_per_16_frames": round(1e3 * t_batch, 3),
+            "batched_vs_plain": round(t_plain / t_batch, 3), "plain_launches": info["recorded_launches"],
+            "batched_launches": info["launches"], "stages": info["stages"], "grid": grid,
+            "every_frame_equals_alone": identical}
 
 
 def frame_loop_benchmark(hm_stitch, rig, f1, f2, levels, frames):

@@ -34,6 +34,15 @@ struct Blk {
   int x, z;
 };
 __device__ __forceinline__ Blk hw_blk() { return Blk{(int)blockIdx.x, (int)blockIdx.z}; }
+
+// A kernel's single by-value argument, read in place in the kernel-argument
+// segment (constant memory: scalar loads).  Handing the parameter itself to a
+// body by reference can make the compiler copy all of it to scratch memory
+// once the body indexes its arrays with run-time (uniform) indices.
+template <typename A>
+__device__ __forceinline__ const A& kernel_arg() {
+  return *(const A*)(const __attribute__((address_space(4))) A*)__builtin_amdgcn_kernarg_segment_ptr();
+}
 constexpr int kWgThreads = 256;
 __device__ __forceinline__ int flat_tid() { return (int)(threadIdx.x + blockDim.x * threadIdx.y); }
 

@@ -105,7 +105,8 @@ __device__ __forceinline__ void pano2_level0_body(const Pano2L0& a, Blk blk, int
 
 template <int C, int PX, int RY>
 __global__ void __launch_bounds__(256) pano2_level0_kernel(Pano2L0 a) {
-  pano2_level0_body<C, PX, RY>(a, hw_blk(), (int)gridDim.x);
+  (void)a;
+  pano2_level0_body<C, PX, RY>(kernel_arg<Pano2L0>(), hw_blk(), (int)gridDim.x);
 }
 
 
@@ -185,7 +186,8 @@ __device__ __forceinline__ void pano2_hard_body(const Pano2Hard& a, Blk blk) {
 
 template <int C, int PX, int RY>
 __global__ void __launch_bounds__(256) pano2_hard_kernel(Pano2Hard a) {
-  pano2_hard_body<C, PX, RY>(a, hw_blk());
+  (void)a;
+  pano2_hard_body<C, PX, RY>(kernel_arg<Pano2Hard>(), hw_blk());
 }
 
 }  // namespace hmdev

@@ -198,7 +198,8 @@ __device__ __forceinline__ void multi_inputs_body(const MultiArgs<N>& a, Blk blk
 
 template <int C, int N, bool DM>
 __global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
-  multi_inputs_body<C, N, DM>(a, hw_blk());
+  (void)a;
+  multi_inputs_body<C, N, DM>(kernel_arg<MultiArgs<N>>(), hw_blk());
 }
 
 // Canvas writers (pixels inside the skip rectangle are never written):
@@ -284,7 +285,8 @@ __device__ __forceinline__ void multi_canvas_body(const MultiArgs<N>& a, Blk blk
 
 template <int C, int N>
 __global__ void __launch_bounds__(256) multi_canvas_kernel(MultiArgs<N> a) {
-  multi_canvas_body<C, N>(a, hw_blk());
+  (void)a;
+  multi_canvas_body<C, N>(kernel_arg<MultiArgs<N>>(), hw_blk());
 }
 
 // ---------------------------------------------------------------------------

@@ -281,7 +281,8 @@ __device__ __forceinline__ void down_fused_body(const DownFusedArgs<T0, N>& a, B
 template <typename T0, int C, int N>
 __global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a) {
   __shared__ float4 lds[down_lds_float4<N>()];
-  down_fused_body<T0, C, N>(a, hw_blk(), lds);
+  (void)a;
+  down_fused_body<T0, C, N>(kernel_arg<DownFusedArgs<T0, N>>(), hw_blk(), lds);
 }
 
 // Three or more images: the images whose valid region meets the tile at some
@@ -426,7 +427,8 @@ __device__ __forceinline__ void down_fused_slots_body(const DownFusedArgs<T0, N>
 template <typename T0, int C, int N>
 __global__ void __launch_bounds__(256) down_fused_slots_kernel(DownFusedArgs<T0, N> a) {
   __shared__ float4 lds[down_lds_float4<down_slots_count<N>()>()];
-  down_fused_slots_body<T0, C, N>(a, hw_blk(), lds);
+  (void)a;
+  down_fused_slots_body<T0, C, N>(kernel_arg<DownFusedArgs<T0, N>>(), hw_blk(), lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -579,7 +581,8 @@ __device__ __forceinline__ void up_tail_body(const UpTailArgs<N>& a, Blk blk, fl
 template <int C, int N, int KIND>
 __global__ void __launch_bounds__(256) up_tail_kernel(UpTailArgs<N> a) {
   __shared__ float4 lds[kTailLdsFloat4];
-  up_tail_body<C, N, KIND>(a, hw_blk(), lds);
+  (void)a;
+  up_tail_body<C, N, KIND>(kernel_arg<UpTailArgs<N>>(), hw_blk(), lds);
 }
 
 // The same tail with every level's footprint (Gaussians and mask) read into
@@ -1209,7 +1212,8 @@ __device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk
 
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT>
 __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
-  up_quad_body<T0, C, N, KIND, MODE, NEXT>(a, hw_blk());
+  (void)a;
+  up_quad_body<T0, C, N, KIND, MODE, NEXT>(kernel_arg<UpQuadArgs<T0, N>>(), hw_blk());
 }
 
 }  // namespace hmdev
