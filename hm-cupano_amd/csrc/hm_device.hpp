@@ -46,6 +46,28 @@ __device__ __forceinline__ const A& kernel_arg() {
 constexpr int kWgThreads = 256;
 __device__ __forceinline__ int flat_tid() { return (int)(threadIdx.x + blockDim.x * threadIdx.y); }
 
+// A uniform value pinned in a scalar register.  A per-lane choice between two
+// pinned values stays a select; between two argument fields read in place, the
+// compiler turns it into a per-lane load of the chosen field.  Pointers come
+// back as global-memory pointers (global, not flat, accesses).
+__device__ __forceinline__ int pin_sgpr(int v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+__device__ __forceinline__ uint32_t pin_sgpr(uint32_t v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T* pin_sgpr(T* p) {
+  uint64_t v = (uint64_t)p;
+  asm volatile("" : "+s"(v));
+  return (T*)(__attribute__((address_space(1))) T*)v;
+}
+__device__ __forceinline__ Surf pin_sgpr(const Surf& s) {
+  return Surf{pin_sgpr(s.ptr), pin_sgpr(s.w), pin_sgpr(s.h), pin_sgpr(s.pitch)};
+}
+
 // ---- XCD-aware tile order ------------------------------------------------
 // MI355X dispatches consecutive workgroup ids round-robin over its 8 XCDs,
 // each with its own L2.  Renumbering so that XCD k runs the k-th contiguous

@@ -145,6 +145,15 @@ __device__ __forceinline__ void pano2_hard_body(const Pano2Hard& a, Blk blk) {
       sv[r][j] = srow[X < a.cw ? X : 0];
     }
   }
+  // Both images' geometry pinned in scalar registers: the per-pixel choice
+  // between them is a select, not a per-lane load of an argument field.
+  const int px0 = pin_sgpr(a.px[0]), px1 = pin_sgpr(a.px[1]);
+  const int py0 = pin_sgpr(a.py[0]), py1 = pin_sgpr(a.py[1]);
+  const int mw0 = pin_sgpr(a.mw[0]), mw1 = pin_sgpr(a.mw[1]);
+  const int mh0 = pin_sgpr(a.mh[0]), mh1 = pin_sgpr(a.mh[1]);
+  const uint32_t* map0 = pin_sgpr(a.map[0]);
+  const uint32_t* map1 = pin_sgpr(a.map[1]);
+  const Surf src0 = pin_sgpr(a.src[0]), src1 = pin_sgpr(a.src[1]);
   uint32_t m[RY][PX];
   int sel[RY][PX];  // 0: image 1, 1: image 2, -1: nothing
 #pragma unroll
@@ -153,16 +162,14 @@ __device__ __forceinline__ void pano2_hard_body(const Pano2Hard& a, Blk blk) {
 #pragma unroll
     for (int j = 0; j < PX; ++j) {
       const int X = X0 + j * (int)blockDim.x;
-      // Scalar selects: a dynamic index into the kernel-argument arrays would
-      // force them into scratch memory.
       const bool first = sv[r][j] == 1;
-      const int rx = X - (first ? a.px[0] : a.px[1]);
-      const int ry = Y - (first ? a.py[0] : a.py[1]);
-      const int mw = first ? a.mw[0] : a.mw[1];
-      const int mh = first ? a.mh[0] : a.mh[1];
+      const int rx = X - (first ? px0 : px1);
+      const int ry = Y - (first ? py0 : py1);
+      const int mw = first ? mw0 : mw1;
+      const int mh = first ? mh0 : mh1;
       const bool hit = X < a.cw && Y < a.ch && sv[r][j] <= 1 && rx >= 0 && ry >= 0 && rx < mw && ry < mh;
       sel[r][j] = hit ? (first ? 0 : 1) : -1;
-      m[r][j] = (first ? a.map[0] : a.map[1])[hit ? (size_t)ry * mw + rx : 0];
+      m[r][j] = (first ? map0 : map1)[hit ? (size_t)ry * mw + rx : 0];
     }
   }
   uint32_t v[RY][PX];
@@ -170,7 +177,7 @@ __device__ __forceinline__ void pano2_hard_body(const Pano2Hard& a, Blk blk) {
   for (int r = 0; r < RY; ++r)
 #pragma unroll
     for (int j = 0; j < PX; ++j)
-      v[r][j] = remap_px_dm_sel<C>(sel[r][j] == 0 ? a.src[0] : a.src[1], b, m[r][j], sel[r][j] >= 0);
+      v[r][j] = remap_px_dm_sel<C>(sel[r][j] == 0 ? src0 : src1, b, m[r][j], sel[r][j] >= 0);
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     const int Y = Y0 + r * (int)blockDim.y;
