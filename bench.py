@@ -342,6 +342,81 @@ def batched_mixed_benchmark(torch, hm_stitch, rigs, stream, passes=20):
             "every_frame_equals_alone": identical}
 
 
+# Single workloads for profiling: (class, rig geometry, levels, batch).  n3/n8:
+# 3 or 8 x 3840x2160 -> 6200x2300 / 14000x2600 at L=6; p3: 3 x 2704x1520 ->
+# 6000x2000 at L=4; mix16: the 16-frame mixed batch as one batched pass.
+WORKLOADS = {
+    "n3": ("N", "cfg2_n3", 6),
+    "n8": ("N", "cfg2_n8", 6),
+    "p3": ("3", (3, 2704, 1520, 6000, 2000, 0.25, 1500.0, 77), 4),
+    "mix16": ("mix", None, None),
+}
+
+
+def workload_benchmark(torch, hm_stitch, hm_synth, name, steps, warmup):
+    """Frames (or batched passes) of one workload back to back on one stream,
+    timed over `steps` after `warmup`; prints per-kernel times from HIP events."""
+    kind, geo, levels = WORKLOADS[name]
+    stream = torch.cuda.current_stream()
+
+    def rig_of(g):
+        if isinstance(g, str):
+            return hm_synth.config_rig(g)
+        return hm_synth.cylinder_rig(g[0], g[1], g[2], g[3], g[4], overlap=g[5], focal=g[6], seed=g[7])
+
+    def frames_of(rig, seed):
+        return [torch.from_numpy(hm_synth.make_frame(rig.src_h, rig.src_w, seed + 17 * i)[None]).cuda()
+                for i in range(rig.n)]
+
+    if kind == "mix":
+        calls = []
+        for cname, count in BATCH_MIX:
+            _, ckind, cgeo, clevels, _, _ = next(c for c in SERVED_CONFIGS if c[0] == cname)
+            rig = hm_synth.config_rig("cfg1") if cgeo is None else rig_of(cgeo)
+            for k in range(count):
+                cls = {"2": hm_stitch.StitchPano, "3": hm_stitch.StitchPano3, "N": hm_stitch.StitchPanoN}[ckind]
+                pano = cls(1, clevels, rig)
+                calls.append((pano, frames_of(rig, 100 * k), pano.alloc_canvas()))
+        batch = hm_stitch.Batch(calls)
+        run = lambda: hm_stitch.check(batch.run(stream, use_graph=True), "batched pass")  # noqa: E731
+        mpix = sum(c[0].canvas_size()[0] * c[0].canvas_size()[1] for c in calls) / 1e6
+        timer = None
+    else:
+        rig = rig_of(geo)
+        cls = hm_stitch.StitchPanoN if kind == "N" else hm_stitch.StitchPano3
+        pano = cls(1, levels, rig)
+        fr = frames_of(rig, 0)
+        canvas = pano.alloc_canvas()
+        if kind == "N":
+            run = lambda: hm_stitch.check(pano.process(fr, canvas, stream), "process")  # noqa: E731
+        else:
+            run = lambda: hm_stitch.check(pano.process(fr[0], fr[1], fr[2], canvas, stream=stream), "process")  # noqa: E731
+        w, h = pano.canvas_size()
+        mpix = w * h / 1e6
+        timer = pano
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = {"workload": name, "steps": steps, "ms_per_step": round(ms, 4), "Mpix_s": round(mpix / ms * 1e3, 1)}
+    if timer is not None:
+        timer.set_timing(True)
+        timer.timing_report(True)
+        for _ in range(min(steps, 20)):
+            run()
+        rep = timer.timing_report(True)
+        n = min(steps, 20)
+        out["kernels_ms"] = {k: round(v[0] / n, 4) for k, v in rep.items()}
+        out["levels"] = pano.levels()
+    else:
+        out["batch"] = batch.info()
+    return out
+
+
 def frame_loop_benchmark(hm_stitch, rig, f1, f2, levels, frames):
     """Host frames in, host frames out (hmFrameLoopRun): BGR over PCIe, the stitch
     on the GPU, 3 pinned slots in flight.  The source resends the slot's pinned
@@ -398,6 +473,8 @@ def main():
                     help="frames of the PCIe-inclusive frame-loop measurement (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (scale for the HBM byte counters)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default=None,
+                    help="time only one other workload (--steps frames or passes after --warmup) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the stitch path shards by rig/frame with no collectives, "
@@ -424,6 +501,10 @@ def main():
                           "bytes_written": 4 * n, "launches": 5}))
         return
     hm_stitch.lib()
+    if args.workload:
+        print(json.dumps(workload_benchmark(torch, hm_stitch, hm_synth, args.workload, args.steps, args.warmup)),
+              flush=True)
+        return
     if args.served_only:
         print(json.dumps(served_benchmark(torch, hm_stitch, hm_synth, args.served_seconds, args.served_streams,
                                           args.served_frames)), flush=True)
