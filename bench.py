@@ -342,13 +342,15 @@ def batched_mixed_benchmark(torch, hm_stitch, rigs, stream, passes=20):
             "every_frame_equals_alone": identical}
 
 
-# Single workloads for profiling: (class, rig geometry, levels, batch).  n3/n8:
+# Single workloads for profiling: (class, rig geometry, levels).  n3/n8:
 # 3 or 8 x 3840x2160 -> 6200x2300 / 14000x2600 at L=6; p3: 3 x 2704x1520 ->
-# 6000x2000 at L=4; mix16: the 16-frame mixed batch as one batched pass.
+# 6000x2000 at L=4; h4: 4 x 1920x1080 -> 4000x1500 at L=0 (hard seam); mix16:
+# the 16-frame mixed batch as one batched pass.
 WORKLOADS = {
     "n3": ("N", "cfg2_n3", 6),
     "n8": ("N", "cfg2_n8", 6),
     "p3": ("3", (3, 2704, 1520, 6000, 2000, 0.25, 1500.0, 77), 4),
+    "h4": ("N", (4, 1920, 1080, 4000, 1500, 0.25, 1100.0, 6), 0),
     "mix16": ("mix", None, None),
 }
 

@@ -94,8 +94,8 @@ __device__ __forceinline__ const A& arg_at(kptr<StageArgs> s, uint32_t off) {
   X(kOpMi3, 0, (multi_inputs_body<4, 3, false>(arg_at<MultiArgs<3>>(s, off), blk)))                          \
   X(kOpMi3DM, 0, (multi_inputs_body<4, 3, true>(arg_at<MultiArgs<3>>(s, off), blk)))                         \
   X(kOpMi4, 0, (multi_inputs_body<4, 4, false>(arg_at<MultiArgs<4>>(s, off), blk)))                          \
-  X(kOpMc3, 0, (multi_canvas_body<4, 3>(arg_at<MultiArgs<3>>(s, off), blk)))                                 \
-  X(kOpMc4, 0, (multi_canvas_body<4, 4>(arg_at<MultiArgs<4>>(s, off), blk)))                                 \
+  X(kOpMc3, 0, (multi_canvas_body<4, 3>(arg_at<MultiArgs<3>>(s, off), blk, lds)))                            \
+  X(kOpMc4, 0, (multi_canvas_body<4, 4>(arg_at<MultiArgs<4>>(s, off), blk, lds)))                            \
   X(kOpDown2U8, 1, (down_fused_body<uint8_t, 4, 2>(arg_at<DownFusedArgs<uint8_t, 2>>(s, off), blk, lds)))    \
   X(kOpDown2F, 1, (down_fused_body<float, 4, 2>(arg_at<DownFusedArgs<float, 2>>(s, off), blk, lds)))         \
   X(kOpDown3U8, 1, (down_fused_slots_body<uint8_t, 4, 3>(arg_at<DownFusedArgs<uint8_t, 3>>(s, off), blk, lds))) \
@@ -138,7 +138,9 @@ template <int OP>
 __global__ void __launch_bounds__(256) batch_op_kernel(StageArgs stage) {
   (void)stage;  // read through the segment pointer below
   const kptr<StageArgs> s = (kptr<StageArgs>)__builtin_amdgcn_kernarg_segment_ptr();
-  constexpr int kLds = (OP >= kOpDown2U8 && OP <= kOpDown4F) ? down_lds_float4<4>() : (OP == kOpTail2 ? kTailLdsFloat4 : 1);
+  constexpr int kLds = (OP >= kOpDown2U8 && OP <= kOpDown4F)
+                           ? down_lds_float4<4>()
+                           : (OP == kOpTail2 ? kTailLdsFloat4 : ((OP == kOpMc3 || OP == kOpMc4) ? kCanvasGeoFloat4 : 1));
   __shared__ float4 lds[kLds];
   (void)lds;
   for (uint32_t w = blockIdx.x; w < s->total; w += gridDim.x) {

@@ -236,15 +236,29 @@ hipError_t PanoN::process(const hmSurface* inputs, int n_inputs, const hmSurface
     hipError_t e_side = hipSuccess;
     const bool forked = use_min && tuning.overlap;
     if (use_min) {  // outside the write ROI, which the blend writes back
-      cr.skip_x = write_roi.x;
-      cr.skip_y = write_roi.y;
-      cr.skip_w = write_roi.w;
-      cr.skip_h = write_roi.h;
+      // The bands above and below the ROI and the strips left and right of
+      // it, one launch each: no workgroup is spent on the ROI itself.
+      const int rx0 = write_roi.x, ry0 = write_roi.y, rx1 = write_roi.x + write_roi.w, ry1 = write_roi.y + write_roi.h;
+      const int bands[4][4] = {{0, canvas_w, 0, ry0}, {0, canvas_w, ry1, canvas_h}, {0, rx0, ry0, ry1},
+                               {rx1, canvas_w, ry0, ry1}};  // x0, x1, y0, y1
+      hipStream_t cs = s;
       if (forked) {
         HM_RETURN_IF_ERROR(fork(s));
-        e_side = launch_region<N>(cr, inputs, canvas, side, "remap_canvas");
-      } else {
-        HM_RETURN_IF_ERROR(launch_region<N>(cr, inputs, canvas, s, "remap_canvas"));
+        cs = side;
+      }
+      for (const auto& bd : bands) {
+        MultiRegion br = cr;
+        br.x0 = bd[0];
+        br.x1 = bd[1];
+        br.y0 = bd[2];
+        br.y1 = bd[3];
+        if (br.x1 <= br.x0 || br.y1 <= br.y0) continue;
+        const hipError_t e = launch_region<N>(br, inputs, canvas, cs, "remap_canvas");
+        if (e != hipSuccess) {
+          if (!forked) return e;
+          e_side = e;
+          break;
+        }
       }
     }
     const void* g[kMaxImages];

@@ -215,9 +215,83 @@ __global__ void __launch_bounds__(256) multi_inputs_kernel(MultiArgs<N> a) {
 // (uncovered_zero).
 enum CanvasMode { kCanvasLabel = 0, kCanvasLast = 1, kCanvasClear = 2 };
 
-// Runs on 64 x 4 threads (kBlock).
+// One image's remap geometry, copied to LDS once per workgroup so that each
+// pixel reads its labelled image's with a per-lane LDS load.
+struct CanvasGeo {
+  const uint32_t* map;
+  uint8_t* src;
+  int px, py, mw, mh;
+  uint32_t sw, sh, spitch, pad;
+};
+constexpr int kCanvasGeoFloat4 = kMaxImages * (int)(sizeof(CanvasGeo) / 16);
+static_assert(sizeof(CanvasGeo) % 16 == 0, "CanvasGeo is carved from float4 LDS");
+
+// Label mode: per pixel only the labelled image, in three rounds of loads
+// each issued in full before the next (labels, the labelled images' map
+// entries, their source pixels); a load for a pixel that takes nothing goes
+// to a valid address and is discarded.
 template <int C, int N>
-__device__ __forceinline__ void multi_canvas_body(const MultiArgs<N>& a, Blk blk) {
+__device__ __forceinline__ void multi_canvas_label(const MultiArgs<N>& a, Blk blk, float4* lds) {
+  CanvasGeo* geo = reinterpret_cast<CanvasGeo*>(lds);
+  const int tid = flat_tid();
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (tid == i)
+      geo[i] = CanvasGeo{a.map[i], a.src[i].ptr, a.px[i], a.py[i], a.mw[i], a.mh[i], a.src[i].w, a.src[i].h,
+                         a.src[i].pitch, 0u};
+  __syncthreads();
+  const int t = blk.x;
+  const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
+  const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
+  const int b = blk.z;
+  if (Y >= a.y1) return;
+  const bool skip_row = Y >= a.skip_y0 && Y < a.skip_y1;
+  const uint8_t* srow = a.seam + (size_t)Y * a.cw;
+  int lab[kPx];
+  bool act[kPx];
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const int X = X0 + j * (int)blockDim.x;
+    act[j] = X < a.x1 && !(skip_row && X >= a.skip_x0 && X < a.skip_x1);
+    lab[j] = srow[act[j] ? X : a.x0];
+  }
+  uint32_t m[kPx];
+  bool hit[kPx];
+  int img[kPx];
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const int X = X0 + j * (int)blockDim.x;
+    img[j] = lab[j] < N ? lab[j] : 0;
+    const CanvasGeo& g = geo[img[j]];
+    const int rx = X - g.px, ry = Y - g.py;
+    hit[j] = act[j] && lab[j] < N && rx >= 0 && ry >= 0 && rx < g.mw && ry < g.mh;
+    m[j] = g.map[hit[j] ? (size_t)ry * g.mw + rx : 0];
+  }
+  uint32_t v[kPx];
+  bool ok[kPx];
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const CanvasGeo& g = geo[img[j]];
+    const uint32_t sx = m[j] & 0xffffu, sy = m[j] >> 16;
+    ok[j] = sx < g.sw && sy < g.sh;
+    const Surf src{g.src, g.sw, g.sh, g.spitch};
+    v[j] = remap_px_dm_sel<C>(src, b, m[j], hit[j]);
+  }
+  uint8_t* crow = a.canvas + ((size_t)b * a.ch + Y) * a.cpitch;
+#pragma unroll
+  for (int j = 0; j < kPx; ++j) {
+    const bool wr = hit[j] ? (ok[j] || a.invalid_zero != 0) : (act[j] && a.uncovered_zero != 0);
+    if (wr) store_px<C>(crow + (size_t)(X0 + j * (int)blockDim.x) * C, v[j]);
+  }
+}
+
+// Runs on 64 x 4 threads (kBlock).  lds: kCanvasGeoFloat4 float4.
+template <int C, int N>
+__device__ __forceinline__ void multi_canvas_body(const MultiArgs<N>& a, Blk blk, float4* lds) {
+  if (a.mode == kCanvasLabel && a.seam) {  // uniform
+    multi_canvas_label<C, N>(a, blk, lds);
+    return;
+  }
   const int t = blk.x;
   const int X0 = a.x0 + (t % a.tiles_x) * (int)blockDim.x * kPx + (int)threadIdx.x;
   const int Y = a.y0 + (t / a.tiles_x) * (int)blockDim.y + (int)threadIdx.y;
@@ -286,7 +360,8 @@ __device__ __forceinline__ void multi_canvas_body(const MultiArgs<N>& a, Blk blk
 template <int C, int N>
 __global__ void __launch_bounds__(256) multi_canvas_kernel(MultiArgs<N> a) {
   (void)a;
-  multi_canvas_body<C, N>(kernel_arg<MultiArgs<N>>(), hw_blk());
+  __shared__ float4 lds[kCanvasGeoFloat4];
+  multi_canvas_body<C, N>(kernel_arg<MultiArgs<N>>(), hw_blk(), lds);
 }
 
 // ---------------------------------------------------------------------------
