@@ -394,15 +394,35 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     return hipGetLastError();
   };
   // Blend-input columns first, on the caller's stream: the pyramid waits on
-  // them.  The canvas remap forks after them, so it does not delay them.
-  // With fuse_down they are remapped inside the first down launch.
+  // them.  The canvas remap forks after them (overlap 1), so it does not
+  // delay them, or at the start of the frame (overlap 2): it reads nothing the
+  // blend columns write, and the fork's event then sits next to the previous
+  // frame's join instead of between two launches of the chain.
+  // With fuse_down the columns are remapped inside the first down launch.
+  const bool fork_first = side && tuning.overlap == 2;
+  hipError_t e_first = hipSuccess;
+  if (fork_first) {
+    HM_RETURN_IF_ERROR(fork(s));
+    e_first = launch_spans(0, roi_x0, roi_x1, canvas_w, side, "remap_canvas");
+    if (e_first != hipSuccess) {
+      (void)join(s);
+      return e_first;
+    }
+  }
   int down_from = 0;
-  if (tuning.fuse_down && fuse_ok && !debug && pyr.levels >= 2) {
-    HM_RETURN_IF_ERROR(launch_remap_down(a, s, &down_from));
-  } else {
-    HM_RETURN_IF_ERROR(launch_spans(roi_x0, roi_x1, 0, 0, s, "remap_roi"));
+  const hipError_t e_roi = tuning.fuse_down && fuse_ok && !debug && pyr.levels >= 2
+                               ? launch_remap_down(a, s, &down_from)
+                               : launch_spans(roi_x0, roi_x1, 0, 0, s, "remap_roi");
+  if (e_roi != hipSuccess) {
+    if (fork_first) (void)join(s);  // the caller's stream still orders the side work
+    return e_roi;
   }
   if (!side) return run_blend(canvas, s, down_from);
+  if (fork_first) {
+    const hipError_t e2 = run_blend(canvas, s, down_from);
+    const hipError_t j = join(s);
+    return e2 != hipSuccess ? e2 : j;
+  }
   if (!tuning.overlap) {  // everything on the caller's stream
     HM_RETURN_IF_ERROR(launch_spans(0, roi_x0, roi_x1, canvas_w, s, "remap_canvas"));
     return run_blend(canvas, s, down_from);

@@ -189,7 +189,9 @@ int hmStitchPanoDeviceBytes(const hmStitchPano* pano, size_t* bytes);
  *   tail_base  2-image soft seam: levels tail_base..L-1 of the up pass run as
  *              one tile-fused launch (1..3), or 0: one launch per level;
  *   overlap    1: the canvas outside the blend region is remapped on a second
- *              stream overlapping the pyramid; 0: everything on one stream. */
+ *              stream overlapping the pyramid, forked after the blend inputs;
+ *              2 (2-image class): the same, forked at the start of the frame;
+ *              0: everything on one stream. */
 typedef struct hmTuning {
   int remap_px;
   int tail_base;
