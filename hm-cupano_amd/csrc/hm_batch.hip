@@ -103,34 +103,34 @@ __device__ __forceinline__ const A& arg_at(kptr<StageArgs> s, uint32_t off) {
   X(kOpDown4U8, 1, (down_fused_slots_body<uint8_t, 4, 4>(arg_at<DownFusedArgs<uint8_t, 4>>(s, off), blk, lds))) \
   X(kOpDown4F, 1, (down_fused_slots_body<float, 4, 4>(arg_at<DownFusedArgs<float, 4>>(s, off), blk, lds)))   \
   X(kOpTail2, 2, (up_tail_body<4, 2, kBlend2>(arg_at<UpTailArgs<2>>(s, off), blk, lds)))                     \
-  X(kOpUq2F, 2, (up_quad_body<float, 4, 2, kBlend2, kMask0F32, true>(arg_at<UpQuadArgs<float, 2>>(s, off), blk))) \
+  X(kOpUq2F, 2, (up_quad_body<float, 4, 2, kBlend2, kMask0F32, true>(arg_at<UpQuadArgs<float, 2>>(s, off), blk, lds))) \
   X(kOpUq2FLast, 2,                                                                                           \
-    (up_quad_body<float, 4, 2, kBlend2, kMask0F32, false>(arg_at<UpQuadArgs<float, 2>>(s, off), blk)))       \
+    (up_quad_body<float, 4, 2, kBlend2, kMask0F32, false>(arg_at<UpQuadArgs<float, 2>>(s, off), blk, lds)))       \
   X(kOpUq2U8, 2,                                                                                              \
-    (up_quad_body<uint8_t, 4, 2, kBlend2, kMask0U8, true>(arg_at<UpQuadArgs<uint8_t, 2>>(s, off), blk)))     \
+    (up_quad_body<uint8_t, 4, 2, kBlend2, kMask0U8, true>(arg_at<UpQuadArgs<uint8_t, 2>>(s, off), blk, lds)))     \
   X(kOpUq2U8Last, 2,                                                                                          \
-    (up_quad_body<uint8_t, 4, 2, kBlend2, kMask0U8, false>(arg_at<UpQuadArgs<uint8_t, 2>>(s, off), blk)))    \
-  X(kOpUq3F, 3, (up_quad_body<float, 4, 3, kBlend3, kMask0F32, true>(arg_at<UpQuadArgs<float, 3>>(s, off), blk))) \
+    (up_quad_body<uint8_t, 4, 2, kBlend2, kMask0U8, false>(arg_at<UpQuadArgs<uint8_t, 2>>(s, off), blk, lds)))    \
+  X(kOpUq3F, 3, (up_quad_body<float, 4, 3, kBlend3, kMask0F32, true>(arg_at<UpQuadArgs<float, 3>>(s, off), blk, lds))) \
   X(kOpUq3FLast, 3,                                                                                           \
-    (up_quad_body<float, 4, 3, kBlend3, kMask0F32, false>(arg_at<UpQuadArgs<float, 3>>(s, off), blk)))       \
+    (up_quad_body<float, 4, 3, kBlend3, kMask0F32, false>(arg_at<UpQuadArgs<float, 3>>(s, off), blk, lds)))       \
   X(kOpUq3L, 3,                                                                                               \
-    (up_quad_body<uint8_t, 4, 3, kBlend3, kMask0Label, true>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk)))  \
+    (up_quad_body<uint8_t, 4, 3, kBlend3, kMask0Label, true>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk, lds)))  \
   X(kOpUq3LLast, 3,                                                                                           \
-    (up_quad_body<uint8_t, 4, 3, kBlend3, kMask0Label, false>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk))) \
-  X(kOpUqN3F, 3, (up_quad_body<float, 4, 3, kBlendN, kMask0F32, true>(arg_at<UpQuadArgs<float, 3>>(s, off), blk))) \
+    (up_quad_body<uint8_t, 4, 3, kBlend3, kMask0Label, false>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk, lds))) \
+  X(kOpUqN3F, 3, (up_quad_body<float, 4, 3, kBlendN, kMask0F32, true>(arg_at<UpQuadArgs<float, 3>>(s, off), blk, lds))) \
   X(kOpUqN3FLast, 3,                                                                                          \
-    (up_quad_body<float, 4, 3, kBlendN, kMask0F32, false>(arg_at<UpQuadArgs<float, 3>>(s, off), blk)))       \
+    (up_quad_body<float, 4, 3, kBlendN, kMask0F32, false>(arg_at<UpQuadArgs<float, 3>>(s, off), blk, lds)))       \
   X(kOpUqN3L, 3,                                                                                              \
-    (up_quad_body<uint8_t, 4, 3, kBlendN, kMask0Label, true>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk)))  \
+    (up_quad_body<uint8_t, 4, 3, kBlendN, kMask0Label, true>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk, lds)))  \
   X(kOpUqN3LLast, 3,                                                                                          \
-    (up_quad_body<uint8_t, 4, 3, kBlendN, kMask0Label, false>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk))) \
-  X(kOpUqN4F, 3, (up_quad_body<float, 4, 4, kBlendN, kMask0F32, true>(arg_at<UpQuadArgs<float, 4>>(s, off), blk))) \
+    (up_quad_body<uint8_t, 4, 3, kBlendN, kMask0Label, false>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk, lds))) \
+  X(kOpUqN4F, 3, (up_quad_body<float, 4, 4, kBlendN, kMask0F32, true>(arg_at<UpQuadArgs<float, 4>>(s, off), blk, lds))) \
   X(kOpUqN4FLast, 3,                                                                                          \
-    (up_quad_body<float, 4, 4, kBlendN, kMask0F32, false>(arg_at<UpQuadArgs<float, 4>>(s, off), blk)))       \
+    (up_quad_body<float, 4, 4, kBlendN, kMask0F32, false>(arg_at<UpQuadArgs<float, 4>>(s, off), blk, lds)))       \
   X(kOpUqN4L, 3,                                                                                              \
-    (up_quad_body<uint8_t, 4, 4, kBlendN, kMask0Label, true>(arg_at<UpQuadArgs<uint8_t, 4>>(s, off), blk)))  \
+    (up_quad_body<uint8_t, 4, 4, kBlendN, kMask0Label, true>(arg_at<UpQuadArgs<uint8_t, 4>>(s, off), blk, lds)))  \
   X(kOpUqN4LLast, 3,                                                                                          \
-    (up_quad_body<uint8_t, 4, 4, kBlendN, kMask0Label, false>(arg_at<UpQuadArgs<uint8_t, 4>>(s, off), blk)))
+    (up_quad_body<uint8_t, 4, 4, kBlendN, kMask0Label, false>(arg_at<UpQuadArgs<uint8_t, 4>>(s, off), blk, lds)))
 
 // Runs on 64 x 4 threads; every body uses all 256 (see Blk).  The
 // workgroups walk the launch's work-list with a stride of the grid size.
@@ -138,9 +138,13 @@ template <int OP>
 __global__ void __launch_bounds__(256) batch_op_kernel(StageArgs stage) {
   (void)stage;  // read through the segment pointer below
   const kptr<StageArgs> s = (kptr<StageArgs>)__builtin_amdgcn_kernarg_segment_ptr();
-  constexpr int kLds = (OP >= kOpDown2U8 && OP <= kOpDown4F)
-                           ? down_lds_float4<4>()
-                           : (OP == kOpTail2 ? kTailLdsFloat4 : ((OP == kOpMc3 || OP == kOpMc4) ? kCanvasGeoFloat4 : 1));
+  constexpr int kLds =
+      (OP >= kOpDown2U8 && OP <= kOpDown4F)
+          ? down_lds_float4<4>()
+          : (OP == kOpTail2 ? kTailLdsFloat4
+                            : ((OP == kOpMc3 || OP == kOpMc4)
+                                   ? kCanvasGeoFloat4
+                                   : ((OP >= kOpUq3F && OP <= kOpUqN4LLast) ? quad_lds_float4<4>() : 1)));
   __shared__ float4 lds[kLds];
   (void)lds;
   for (uint32_t w = blockIdx.x; w < s->total; w += gridDim.x) {

@@ -865,42 +865,74 @@ __host__ __device__ constexpr int image_of(int mask, int s) {
   return (mask & 1) ? (s == 0 ? 0 : 1 + image_of(mask >> 1, s - 1)) : 1 + image_of(mask >> 1, s);
 }
 
+// The coarser level's taps of a 64 x 4 quad tile: quad (qx, qy) reads
+// positions qx..qx+1 x qy..qy+1, so the tile reads (64 + 1) x (4 + 1) of
+// them, each clamped to the level like a thread's own taps.  With a coarser
+// level, the tile stages R's and each slot image's taps there in LDS (two
+// loads per thread and image instead of four, and no tap registers held
+// across the loads of the level's own pixels).
+constexpr int kQuadTapW = 64 + 1, kQuadTapH = 4 + 1, kQuadTaps = kQuadTapW * kQuadTapH;
+template <int N>
+__host__ __device__ constexpr int quad_lds_float4() {
+  return ((N < kQuadSlots ? N : kQuadSlots) + 1) * kQuadTaps;
+}
+
 // One quad of a level of the N- or 3-image blend through the tile's S slot
 // images: idx[0..S-1], or, when MASK >= 0, the images of the set MASK (a
 // compile-time set: the 3-image blend places each image's Laplacian at its
 // own position).  Weights: the seam labels (level 0), the planar normalised
-// weights (N-image levels >= 1) or the interleaved 3-image mask.
+// weights (N-image levels >= 1) or the interleaved 3-image mask.  (tqx, tqy):
+// the tile's first quad; every thread of the workgroup calls this (the tap
+// staging has a barrier), those past the window (active false) store nothing.
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT, int S, int MASK = -1>
-__device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const int* idx_rt, int qx, int qy,
-                                              int b) {
+__device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const int* idx_rt, int tqx, int tqy,
+                                              int b, float4* lds) {
   constexpr int SA = S > 0 ? S : 1;  // array extent
   int idx[SA];
 #pragma unroll
   for (int s = 0; s < S; ++s) idx[s] = MASK >= 0 ? image_of(MASK, s) : idx_rt[s];
+  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
+  const int qx = tqx + lx, qy = tqy + ly;
   const QuadPix q = quad_pixels(a, qx, qy, b);
-  // every load first: R's taps, each slot's next-level taps and pixels, the weights
-  const int qx1 = NEXT ? min(qx + 1, a.w1 - 1) : 0, qy1 = NEXT ? min(qy + 1, a.h1 - 1) : 0;
-  const size_t l0 = ((size_t)b * a.h1 + qy) * a.w1, l1 = ((size_t)b * a.h1 + qy1) * a.w1;
-  float4 rt[4], gt[SA][4], gv[SA][4];
-  ImgRect z[SA];
+  // every load first: the tile's taps of R and of each slot image, each
+  // slot's pixels, the weights
+  constexpr int kRounds = (kQuadTaps + kWgThreads - 1) / kWgThreads;
+  const int tid = ly * 64 + lx;
+  size_t tap[kRounds];
+  float4 st[S + 1][kRounds];
   if (NEXT) {
-    rt[0] = ld_px<C>(a.r1, l0 + qx);
-    rt[1] = ld_px<C>(a.r1, l0 + qx1);
-    rt[2] = ld_px<C>(a.r1, l1 + qx);
-    rt[3] = ld_px<C>(a.r1, l1 + qx1);
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      const int p = min(tid + r * kWgThreads, kQuadTaps - 1);
+      const int px = min(tqx + p % kQuadTapW, a.w1 - 1), py = min(tqy + p / kQuadTapW, a.h1 - 1);
+      tap[r] = ((size_t)b * a.h1 + py) * a.w1 + px;
+      st[0][r] = ld_px<C>(a.r1, tap[r]);
+    }
   }
+  ImgRect z[SA];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const int i = idx[s];
-    z[s] = a.z[i];
+    z[s] = a.z[idx[s]];
     if (NEXT) {
-      const float* g1 = a.g1[i];
-      gt[s][0] = ld_px<C>(g1, l0 + qx);
-      gt[s][1] = ld_px<C>(g1, l0 + qx1);
-      gt[s][2] = ld_px<C>(g1, l1 + qx);
-      gt[s][3] = ld_px<C>(g1, l1 + qx1);
+      const float* g1 = a.g1[idx[s]];
+#pragma unroll
+      for (int r = 0; r < kRounds; ++r) st[s + 1][r] = ld_px<C>(g1, tap[r]);
     }
-    const T0* g0 = a.g0[i];
+  }
+  if (NEXT) {
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      const int p = tid + r * kWgThreads;
+      if (p < kQuadTaps) {
+#pragma unroll
+        for (int s = 0; s <= S; ++s) lds[s * kQuadTaps + p] = st[s][r];
+      }
+    }
+  }
+  float4 gv[SA][4];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const T0* g0 = a.g0[idx[s]];
 #pragma unroll
     for (int e = 0; e < 4; ++e) gv[s][e] = ld_px<C>(g0, q.pix[e]);
   }
@@ -923,23 +955,47 @@ __device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const 
       for (int e = 0; e < 4; ++e) wv[s][e] = w[q.mpx[e]];
     }
   }
+  const int t0 = ly * kQuadTapW + lx;  // tap (qx, qy) in the staged footprint
+  if (NEXT) __syncthreads();
+  // each slot's Laplacian at the quad's pixels (zero outside its region)
+  float4 Lq[SA][4];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    float4 gt[4];
+    if (NEXT) {
+      const float4* tp = lds + (s + 1) * kQuadTaps + t0;
+      gt[0] = tp[0];
+      gt[1] = tp[1];
+      gt[2] = tp[kQuadTapW];
+      gt[3] = tp[kQuadTapW + 1];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      Lq[s][e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (in_rect(z[s], q.x[e], q.y[e])) {
+        const float4 g = gv[s][e];
+        if (NEXT) {
+          const float4 up = up_quad_lap<C>(e, gt);
+          Lq[s][e] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+        } else {
+          Lq[s][e] = g;
+        }
+      }
+    }
+  }
+  float4 rt[4];
+  if (NEXT) {
+    rt[0] = lds[t0];
+    rt[1] = lds[t0 + 1];
+    rt[2] = lds[t0 + kQuadTapW];
+    rt[3] = lds[t0 + kQuadTapW + 1];
+  }
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     if (!q.in[e]) continue;
     float4 Lv[SA];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      Lv[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (in_rect(z[s], q.x[e], q.y[e])) {
-        const float4 g = gv[s][e];
-        if (NEXT) {
-          const float4 up = up_quad_lap<C>(e, gt[s]);
-          Lv[s] = make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
-        } else {
-          Lv[s] = g;
-        }
-      }
-    }
+    for (int s = 0; s < S; ++s) Lv[s] = Lq[s][e];
     float4 B;
     if constexpr (MODE == kMask0Label) {
       B = label_blend<C, S>(Lv, idx, lab[e]);
@@ -965,18 +1021,25 @@ __device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const 
   }
 }
 
+// LDS of an up_quad launch: the tap footprint where the slot path runs with a
+// coarser level, else none.
+template <int N, int KIND, int MODE, bool NEXT>
+__host__ __device__ constexpr int up_quad_lds_float4() {
+  return (N > 2 && NEXT && !(KIND == kBlend3 && MODE == kMask0Label)) ? quad_lds_float4<N>() : 1;
+}
+
 // NEXT: the level has a coarser level (false at the coarsest: blend only).
 // A template parameter rather than a branch, so that the taps of the coarser
 // level and the level's own pixels are loaded in one straight run.
 // Runs on 64 x 4 threads.
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT>
-__device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk) {
+__device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk, float4* lds) {
   constexpr bool has_next = NEXT;
   const int t = blk.x;
-  const int qx = a.qx0 + (t % a.tiles_x) * 64 + (int)threadIdx.x;
-  const int qy = a.qy0 + (t / a.tiles_x) * 4 + (int)threadIdx.y;
+  const int tqx = a.qx0 + (t % a.tiles_x) * 64, tqy = a.qy0 + (t / a.tiles_x) * 4;
+  const int qx = tqx + (int)threadIdx.x;
+  const int qy = tqy + (int)threadIdx.y;
   const int b = blk.z;
-  if (qx >= a.qx0 + a.nqx || qy >= a.qy0 + a.nqy) return;
   // Three or more images: the tile's images in slots when they fit and the
   // weights allow.  Not at level 0 of the 3-image class: there the images
   // cover the whole canvas and the labels are coherent, so loading only the
@@ -985,29 +1048,28 @@ __device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk
   if constexpr (N > 2 && !(KIND == kBlend3 && MODE == kMask0Label)) {
     if (MODE == kMask0Label || KIND == kBlend3 || a.wn != nullptr) {
       int idx[kQuadSlots];
-      const int tqx = a.qx0 + (t % a.tiles_x) * 64, tqy = a.qy0 + (t / a.tiles_x) * 4;
       unsigned set = 0;
       const int n = tile_images<N>(a.z, 2 * tqx, 2 * tqy, 2 * tqx + 127, 2 * tqy + 7, idx, &set);
       if constexpr (KIND == kBlend3) {
         switch (set) {  // uniform
-          case 0: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 0, 0>(a, idx, qx, qy, b); return;
-          case 1: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 1>(a, idx, qx, qy, b); return;
-          case 2: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 2>(a, idx, qx, qy, b); return;
-          case 3: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2, 3>(a, idx, qx, qy, b); return;
-          case 4: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 4>(a, idx, qx, qy, b); return;
-          case 5: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2, 5>(a, idx, qx, qy, b); return;
-          case 6: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2, 6>(a, idx, qx, qy, b); return;
-          default: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 3, 7>(a, idx, qx, qy, b); return;
+          case 0: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 0, 0>(a, idx, tqx, tqy, b, lds); return;
+          case 1: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 1>(a, idx, tqx, tqy, b, lds); return;
+          case 2: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 2>(a, idx, tqx, tqy, b, lds); return;
+          case 3: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2, 3>(a, idx, tqx, tqy, b, lds); return;
+          case 4: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 4>(a, idx, tqx, tqy, b, lds); return;
+          case 5: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2, 5>(a, idx, tqx, tqy, b, lds); return;
+          case 6: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2, 6>(a, idx, tqx, tqy, b, lds); return;
+          default: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 3, 7>(a, idx, tqx, tqy, b, lds); return;
         }
       } else {
         switch (n) {  // uniform
-          case 0: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 0>(a, idx, qx, qy, b); return;
-          case 1: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1>(a, idx, qx, qy, b); return;
-          case 2: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2>(a, idx, qx, qy, b); return;
-          case 3: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 3>(a, idx, qx, qy, b); return;
+          case 0: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 0>(a, idx, tqx, tqy, b, lds); return;
+          case 1: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1>(a, idx, tqx, tqy, b, lds); return;
+          case 2: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 2>(a, idx, tqx, tqy, b, lds); return;
+          case 3: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 3>(a, idx, tqx, tqy, b, lds); return;
           case 4:
             if constexpr (N >= 4) {
-              up_quad_slots<T0, C, N, KIND, MODE, NEXT, 4>(a, idx, qx, qy, b);
+              up_quad_slots<T0, C, N, KIND, MODE, NEXT, 4>(a, idx, tqx, tqy, b, lds);
               return;
             }
             break;
@@ -1016,6 +1078,8 @@ __device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk
       }
     }
   }
+  (void)lds;
+  if (qx >= a.qx0 + a.nqx || qy >= a.qy0 + a.nqy) return;
   if constexpr (MODE == kMask0Label) {
     // Level 0 of the N- and 3-image blends on a tile met by more images
     // than there are slots.  The mask is one-hot, so the
@@ -1216,8 +1280,9 @@ __device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk
 
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT>
 __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
+  __shared__ float4 lds[up_quad_lds_float4<N, KIND, MODE, NEXT>()];
   (void)a;
-  up_quad_body<T0, C, N, KIND, MODE, NEXT>(kernel_arg<UpQuadArgs<T0, N>>(), hw_blk());
+  up_quad_body<T0, C, N, KIND, MODE, NEXT>(kernel_arg<UpQuadArgs<T0, N>>(), hw_blk(), lds);
 }
 
 }  // namespace hmdev
