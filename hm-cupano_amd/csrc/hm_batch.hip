@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(256) batch_op_kernel(StageArgs stage) {
           : (OP == kOpTail2 ? kTailLdsFloat4
                             : ((OP == kOpMc3 || OP == kOpMc4)
                                    ? kCanvasGeoFloat4
-                                   : ((OP >= kOpUq3F && OP <= kOpUqN4LLast) ? quad_lds_float4<4>() : 1)));
+                                   : ((OP >= kOpUq2F && OP <= kOpUqN4LLast) ? quad_lds_float4<4>() : 1)));
   __shared__ float4 lds[kLds];
   (void)lds;
   for (uint32_t w = blockIdx.x; w < s->total; w += gridDim.x) {

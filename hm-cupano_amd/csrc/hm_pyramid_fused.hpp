@@ -1025,7 +1025,9 @@ __device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const 
 // coarser level, else none.
 template <int N, int KIND, int MODE, bool NEXT>
 __host__ __device__ constexpr int up_quad_lds_float4() {
-  return (N > 2 && NEXT && !(KIND == kBlend3 && MODE == kMask0Label)) ? quad_lds_float4<N>() : 1;
+  return ((N > 2 && NEXT && !(KIND == kBlend3 && MODE == kMask0Label)) || (N <= 2 && NEXT && MODE != kMask0Label))
+             ? quad_lds_float4<N>()
+             : 1;
 }
 
 // NEXT: the level has a coarser level (false at the coarsest: blend only).
@@ -1078,8 +1080,11 @@ __device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk
       }
     }
   }
-  (void)lds;
-  if (qx >= a.qx0 + a.nqx || qy >= a.qy0 + a.nqy) return;
+  // The dense path with a coarser level stages the tile's taps with every
+  // thread (a barrier); every other path drops the threads past the window.
+  constexpr bool kDenseStaged = N <= 2 && NEXT && MODE != kMask0Label;
+  const bool active = qx < a.qx0 + a.nqx && qy < a.qy0 + a.nqy;
+  if (!kDenseStaged && !active) return;
   if constexpr (MODE == kMask0Label) {
     // Level 0 of the N- and 3-image blends on a tile met by more images
     // than there are slots.  The mask is one-hot, so the
@@ -1209,23 +1214,22 @@ __device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk
     return;
   }
   // One or two images (dense): the quad's loads all issued before any
-  // arithmetic, the four level-(k+1) taps shared by the quad's pixels.
-  float4 gt[N][4];
-  float4 rt[4];
+  // arithmetic.  With a coarser level, the tile's taps of R and of each
+  // image (the four taps each quad's pixels share) are staged in LDS, as in
+  // the slot path.
+  constexpr int kRounds = (kQuadTaps + kWgThreads - 1) / kWgThreads;
+  const int tid = (int)threadIdx.y * 64 + (int)threadIdx.x;
+  float4 st[N + 1][kRounds];
   if (has_next) {
-    const int qx1 = min(qx + 1, a.w1 - 1), qy1 = min(qy + 1, a.h1 - 1);
-    const size_t l0 = ((size_t)b * a.h1 + qy) * a.w1, l1 = ((size_t)b * a.h1 + qy1) * a.w1;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      gt[i][0] = ld_px<C>(a.g1[i], l0 + qx);
-      gt[i][1] = ld_px<C>(a.g1[i], l0 + qx1);
-      gt[i][2] = ld_px<C>(a.g1[i], l1 + qx);
-      gt[i][3] = ld_px<C>(a.g1[i], l1 + qx1);
+    for (int r = 0; r < kRounds; ++r) {
+      const int p = min(tid + r * kWgThreads, kQuadTaps - 1);
+      const int px = min(tqx + p % kQuadTapW, a.w1 - 1), py = min(tqy + p / kQuadTapW, a.h1 - 1);
+      const size_t tap = ((size_t)b * a.h1 + py) * a.w1 + px;
+      st[0][r] = ld_px<C>(a.r1, tap);
+#pragma unroll
+      for (int i = 0; i < N; ++i) st[i + 1][r] = ld_px<C>(a.g1[i], tap);
     }
-    rt[0] = ld_px<C>(a.r1, l0 + qx);
-    rt[1] = ld_px<C>(a.r1, l0 + qx1);
-    rt[2] = ld_px<C>(a.r1, l1 + qx);
-    rt[3] = ld_px<C>(a.r1, l1 + qx1);
   }
   // the level's pixels and mask values of the quad, all loaded before any
   // arithmetic (pixels outside the window load the window's first pixel)
@@ -1247,6 +1251,30 @@ __device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk
     } else {
 #pragma unroll
       for (int c = 0; c < M; ++c) mk[e][c] = static_cast<const float*>(a.mask0)[mp * M + c];
+    }
+  }
+  float4 gt[N][4];
+  float4 rt[4];
+  if (has_next) {
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      const int p = tid + r * kWgThreads;
+      if (p < kQuadTaps) {
+#pragma unroll
+        for (int i = 0; i <= N; ++i) lds[i * kQuadTaps + p] = st[i][r];
+      }
+    }
+    __syncthreads();
+    if (!active) return;  // past the window: nothing to store
+    const int t0 = (int)threadIdx.y * kQuadTapW + (int)threadIdx.x;  // tap (qx, qy) in the staged footprint
+#pragma unroll
+    for (int i = 0; i <= N; ++i) {
+      const float4* tp = lds + i * kQuadTaps + t0;
+      float4* dst = i == 0 ? rt : gt[i - 1];
+      dst[0] = tp[0];
+      dst[1] = tp[1];
+      dst[2] = tp[kQuadTapW];
+      dst[3] = tp[kQuadTapW + 1];
     }
   }
 #pragma unroll
