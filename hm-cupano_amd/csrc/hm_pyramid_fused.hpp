@@ -1033,7 +1033,8 @@ __host__ __device__ constexpr int up_quad_lds_float4() {
 // NEXT: the level has a coarser level (false at the coarsest: blend only).
 // A template parameter rather than a branch, so that the taps of the coarser
 // level and the level's own pixels are loaded in one straight run.
-// Runs on 64 x 4 threads.
+// Runs on 64 x 4 threads; lds: up_quad_lds_float4<N, KIND, MODE, NEXT>()
+// float4 for the staged taps.
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT>
 __device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk, float4* lds) {
   constexpr bool has_next = NEXT;
