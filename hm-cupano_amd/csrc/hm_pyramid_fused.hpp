@@ -159,13 +159,21 @@ struct DownFusedArgs {
 
 // Levels 2 .. n_out of the down pass: the first produced level's 16x16 tile
 // of every image is in s1; each further level's tile (8x8, 4x4, 2x2, 1x1)
-// comes from the one above it in LDS and is written once.
+// comes from the one above it in LDS and is written once.  These levels fit
+// one wavefront (64 pixels at most), so after the barrier that completes s1
+// the other three wavefronts end, freeing their slots for the next tiles'
+// loads, and wavefront 0 runs the chain alone: a wavefront's LDS operations
+// complete in program order, so within it each level's reads of the level
+// above are done before its writes replace them, and its writes before the
+// next level's reads.  Must be the caller's last step (no barrier follows).
 template <int C, int N, typename Args>
 __device__ __forceinline__ void down_rest(const Args& a, int tx, int ty, int b, int tid, float4 (*s1)[16 * 16],
                                           float4 (*s2)[8 * 8]) {
+  if (a.n_out < 2) return;
+  __syncthreads();  // the first produced level's tile is complete
+  if (tid >= 64) return;
   int edge = 16;
   for (int k = 1; k < a.n_out; ++k) {
-    __syncthreads();
     const DownLevel<N>& din = a.lv[k - 1];
     const DownLevel<N>& d = a.lv[k];
     const int oedge = edge >> 1;
@@ -196,11 +204,12 @@ __device__ __forceinline__ void down_rest(const Args& a, int tx, int ty, int b, 
         }
       }
     }
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();  // the level's reads stay before its writes
     if (active) {
 #pragma unroll
       for (int i = 0; i < N; ++i) s2[i][ly * oedge + lx] = r[i];
     }
+    __builtin_amdgcn_wave_barrier();  // and its writes before the next level's reads
     edge = oedge;
   }
 }
