@@ -892,7 +892,8 @@ __host__ __device__ constexpr int quad_lds_float4() {
 // own position).  Weights: the seam labels (level 0), the planar normalised
 // weights (N-image levels >= 1) or the interleaved 3-image mask.  (tqx, tqy):
 // the tile's first quad; every thread of the workgroup calls this (the tap
-// staging has a barrier), those past the window (active false) store nothing.
+// staging has a barrier); those past the window store nothing (none of their
+// pixels is inside it).
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT, int S, int MASK = -1>
 __device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const int* idx_rt, int tqx, int tqy,
                                               int b, float4* lds) {
@@ -903,8 +904,9 @@ __device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const 
   const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
   const int qx = tqx + lx, qy = tqy + ly;
   const QuadPix q = quad_pixels(a, qx, qy, b);
-  // every load first: the tile's taps of R and of each slot image, each
-  // slot's pixels, the weights
+  // the tile's taps of R and of each slot image first, stored in LDS before
+  // the slots' pixels and the weights are loaded (no tap registers live
+  // across those loads)
   constexpr int kRounds = (kQuadTaps + kWgThreads - 1) / kWgThreads;
   const int tid = ly * 64 + lx;
   size_t tap[kRounds];
@@ -1030,8 +1032,8 @@ __device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const 
   }
 }
 
-// LDS of an up_quad launch: the tap footprint where the slot path runs with a
-// coarser level, else none.
+// LDS of an up_quad launch: the tap footprint where the slot path or the
+// dense path (one or two images) runs with a coarser level, else none.
 template <int N, int KIND, int MODE, bool NEXT>
 __host__ __device__ constexpr int up_quad_lds_float4() {
   return ((N > 2 && NEXT && !(KIND == kBlend3 && MODE == kMask0Label)) || (N <= 2 && NEXT && MODE != kMask0Label))
