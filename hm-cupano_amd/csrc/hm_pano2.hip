@@ -87,7 +87,6 @@ __device__ __forceinline__ bool covers(const Pano2L0& r, int i, int X, int Y) {
 template <int C>
 __global__ void __launch_bounds__(256) remap_down_kernel(RemapDownArgs A) {
   __shared__ float4 s1[2][16 * 16];
-  __shared__ float4 s2[2][8 * 8];
   const Pano2L0& r = A.r;
   const DownFusedArgs<uint8_t, 2>& a = A.dn;
   const int t = xcd_tile((int)blockIdx.x, a.tiles_x * a.tiles_y);
@@ -152,7 +151,7 @@ __global__ void __launch_bounds__(256) remap_down_kernel(RemapDownArgs A) {
       if (in_rect(d.z[i], x, y)) st_px<C>(d.g_out[i], o1, v);
     }
   }
-  down_rest<C, 2>(a, tx, ty, b, tid, s1, s2);
+  down_rest<C, 2>(a, tx, ty, b, tid, s1);
 }
 
 }  // namespace

@@ -159,7 +159,8 @@ struct DownFusedArgs {
 
 // Levels 2 .. n_out of the down pass: the first produced level's 16x16 tile
 // of every image is in s1; each further level's tile (8x8, 4x4, 2x2, 1x1)
-// comes from the one above it in LDS and is written once.  These levels fit
+// comes from the one above it and replaces it at the start of s1 (each
+// level's reads are done before its writes), and is written once.  These levels fit
 // one wavefront (64 pixels at most), so after the barrier that completes s1
 // the other three wavefronts end, freeing their slots for the next tiles'
 // loads, and wavefront 0 runs the chain alone: a wavefront's LDS operations
@@ -167,8 +168,7 @@ struct DownFusedArgs {
 // above are done before its writes replace them, and its writes before the
 // next level's reads.  Must be the caller's last step (no barrier follows).
 template <int C, int N, typename Args>
-__device__ __forceinline__ void down_rest(const Args& a, int tx, int ty, int b, int tid, float4 (*s1)[16 * 16],
-                                          float4 (*s2)[8 * 8]) {
+__device__ __forceinline__ void down_rest(const Args& a, int tx, int ty, int b, int tid, float4 (*s1)[16 * 16]) {
   if (a.n_out < 2) return;
   __syncthreads();  // the first produced level's tile is complete
   if (tid >= 64) return;
@@ -193,7 +193,7 @@ __device__ __forceinline__ void down_rest(const Args& a, int tx, int ty, int b, 
         const int sx = 2 * lx, sy = 2 * ly;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-          const float4* src = k == 1 ? s1[i] : s2[i];
+          const float4* src = s1[i];
           float4 p[4];
           p[0] = src[sy * edge + sx];
           if (has_x1) p[1] = src[sy * edge + sx + 1];
@@ -207,7 +207,7 @@ __device__ __forceinline__ void down_rest(const Args& a, int tx, int ty, int b, 
     __builtin_amdgcn_wave_barrier();  // the level's reads stay before its writes
     if (active) {
 #pragma unroll
-      for (int i = 0; i < N; ++i) s2[i][ly * oedge + lx] = r[i];
+      for (int i = 0; i < N; ++i) s1[i][ly * oedge + lx] = r[i];
     }
     __builtin_amdgcn_wave_barrier();  // and its writes before the next level's reads
     edge = oedge;
@@ -236,14 +236,13 @@ inline void fill_down_args(DownFusedArgs<T0, N>& a, const LevelDesc* lv, int w0,
 }
 
 // LDS of the down pass: the first produced level's 16x16 tile of every image
-// (or slot), then the 8x8 tile of the next.
+// (or slot); the further levels' tiles take its place.
 template <int N>
-constexpr int down_lds_float4() { return N * (16 * 16 + 8 * 8); }
+constexpr int down_lds_float4() { return N * 16 * 16; }
 
 template <typename T0, int C, int N>
 __device__ __forceinline__ void down_fused_body(const DownFusedArgs<T0, N>& a, Blk blk, float4* lds) {
   float4(*s1)[16 * 16] = reinterpret_cast<float4(*)[16 * 16]>(lds);
-  float4(*s2)[8 * 8] = reinterpret_cast<float4(*)[8 * 8]>(lds + N * 16 * 16);
   const int t = xcd_tile(blk.x, a.tiles_x * a.tiles_y);
   const int tx = t % a.tiles_x, ty = t / a.tiles_x;
   const int b = blk.z;
@@ -284,7 +283,7 @@ __device__ __forceinline__ void down_fused_body(const DownFusedArgs<T0, N>& a, B
       }
     }
   }
-  down_rest<C, N>(a, tx, ty, b, tid, s1, s2);
+  down_rest<C, N>(a, tx, ty, b, tid, s1);
 }
 
 template <typename T0, int C, int N>
@@ -308,9 +307,12 @@ __host__ __device__ __forceinline__ bool tile_meets(const ImgRect& z, int tx, in
   return x0 <= z.x1 && x0 + e - 1 >= z.x0 && y0 <= z.y1 && y0 + e - 1 >= z.y0;
 }
 
+// last: no chunk of the tile follows, so the levels past the first run in
+// wavefront 0 alone as in down_rest (the other wavefronts end); otherwise
+// the whole workgroup runs them.
 template <typename T0, int C, int N, int S>
 __device__ __forceinline__ void down_slots(const DownFusedArgs<T0, N>& a, const int* idx, int tx, int ty, int b,
-                                           int tid, float4 (*s1)[16 * 16], float4 (*s2)[8 * 8]) {
+                                           int tid, float4 (*s1)[16 * 16], bool last) {
   {
     const DownLevel<N>& d = a.lv[0];
     const int lx = tid & 15, ly = tid >> 4;
@@ -344,9 +346,11 @@ __device__ __forceinline__ void down_slots(const DownFusedArgs<T0, N>& a, const 
       }
     }
   }
+  if (a.n_out < 2) return;
+  __syncthreads();  // the first produced level's tiles are complete
+  if (last && tid >= 64) return;
   int edge = 16;
   for (int k = 1; k < a.n_out; ++k) {
-    __syncthreads();
     const DownLevel<N>& din = a.lv[k - 1];
     const DownLevel<N>& d = a.lv[k];
     const int oedge = edge >> 1;
@@ -366,7 +370,7 @@ __device__ __forceinline__ void down_slots(const DownFusedArgs<T0, N>& a, const 
         const size_t o = ((size_t)b * d.h + y) * d.w + x;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-          const float4* src = k == 1 ? s1[s] : s2[s];
+          const float4* src = s1[s];
           float4 p[4];
           p[0] = src[sy * edge + sx];
           if (has_x1) p[1] = src[sy * edge + sx + 1];
@@ -378,11 +382,18 @@ __device__ __forceinline__ void down_slots(const DownFusedArgs<T0, N>& a, const 
         }
       }
     }
-    __syncthreads();
+    if (last)
+      __builtin_amdgcn_wave_barrier();  // the level's reads stay before its writes
+    else
+      __syncthreads();
     if (active) {
 #pragma unroll
-      for (int s = 0; s < S; ++s) s2[s][ly * oedge + lx] = r[s];
+      for (int s = 0; s < S; ++s) s1[s][ly * oedge + lx] = r[s];
     }
+    if (last)
+      __builtin_amdgcn_wave_barrier();  // and its writes before the next level's reads
+    else
+      __syncthreads();
     edge = oedge;
   }
 }
@@ -394,7 +405,6 @@ template <typename T0, int C, int N>
 __device__ __forceinline__ void down_fused_slots_body(const DownFusedArgs<T0, N>& a, Blk blk, float4* lds) {
   constexpr int K = down_slots_count<N>();
   float4(*s1)[16 * 16] = reinterpret_cast<float4(*)[16 * 16]>(lds);
-  float4(*s2)[8 * 8] = reinterpret_cast<float4(*)[8 * 8]>(lds + K * 16 * 16);
   const int t = xcd_tile(blk.x, a.tiles_x * a.tiles_y);
   const int tx = t % a.tiles_x, ty = t / a.tiles_x;
   const int b = blk.z;
@@ -422,12 +432,13 @@ __device__ __forceinline__ void down_fused_slots_body(const DownFusedArgs<T0, N>
     }
     if (!first) __syncthreads();  // the previous slots' LDS reads are done
     first = false;
+    const bool last = act == 0;  // uniform
     switch (n) {  // uniform
-      case 1: down_slots<T0, C, N, 1>(a, idx, tx, ty, b, tid, s1, s2); break;
-      case 2: down_slots<T0, C, N, 2>(a, idx, tx, ty, b, tid, s1, s2); break;
-      case 3: down_slots<T0, C, N, 3>(a, idx, tx, ty, b, tid, s1, s2); break;
+      case 1: down_slots<T0, C, N, 1>(a, idx, tx, ty, b, tid, s1, last); break;
+      case 2: down_slots<T0, C, N, 2>(a, idx, tx, ty, b, tid, s1, last); break;
+      case 3: down_slots<T0, C, N, 3>(a, idx, tx, ty, b, tid, s1, last); break;
       default:
-        if constexpr (K >= 4) down_slots<T0, C, N, 4>(a, idx, tx, ty, b, tid, s1, s2);
+        if constexpr (K >= 4) down_slots<T0, C, N, 4>(a, idx, tx, ty, b, tid, s1, last);
         break;
     }
   }
