@@ -10,10 +10,34 @@
 
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Every per-pixel stage writes each output pixel from its own inputs only, so
+ * its rows (and batch entries) run on several threads with identical results;
+ * hmo_set_threads() picks how many (1 = the sequential restatement). */
+#ifdef _OPENMP
+#define HMO_PARALLEL_ROWS _Pragma("omp parallel for collapse(2) schedule(static)")
+#define HMO_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define HMO_PARALLEL_ROWS
+#define HMO_PARALLEL_FOR
+#endif
 
 #define MAXF(a, b) ((a) < (b) ? (b) : (a))
 #define IMIN(a, b) ((a) < (b) ? (a) : (b))
 #define IMAX(a, b) ((a) > (b) ? (a) : (b))
+
+int hmo_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 
 /* src/cuda/cudaUtils.cuh:18-28: clamp to [0,255], then truncate (the +0.5 is
  * commented out in the reference). */
@@ -81,6 +105,7 @@ int hmo_remap_offset(const uint8_t* src, int src_w, int src_h, int src_c, void* 
     if (roi_x + roi_w > remap_w || roi_y + roi_h > remap_h) return -1;
   }
   const uint8_t zero[4] = {0, 0, 0, 0};
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b) {
     for (int y = 0; y < roi_h; ++y) {
       for (int x = 0; x < roi_w; ++x) {
@@ -116,6 +141,7 @@ int hmo_remap_offset(const uint8_t* src, int src_w, int src_h, int src_c, void* 
 int hmo_remap_dest_map(const uint8_t* src, int src_w, int src_h, int c, uint8_t* dst, int dst_w, int dst_h,
                        const uint16_t* map_x, const uint16_t* map_y, int remap_w, int remap_h, int off_x, int off_y,
                        int image_index, const uint8_t* dest_map, int batch) {
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b) {
     for (int y = 0; y < remap_h; ++y) {
       for (int x = 0; x < remap_w; ++x) {
@@ -143,6 +169,7 @@ int hmo_remap_hard_seam_n(int n, const uint8_t* const* srcs, const int* src_w, c
                           const uint16_t* const* map_x, const uint16_t* const* map_y, const int* off_x,
                           const int* off_y, const int* map_w, const int* map_h, const uint8_t* dest_map, uint8_t* dst,
                           int dst_w, int dst_h, int batch) {
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b) {
     for (int y = 0; y < dst_h; ++y) {
       for (int x = 0; x < dst_w; ++x) {
@@ -169,6 +196,7 @@ int hmo_remap_hard_seam_n(int n, const uint8_t* const* srcs, const int* src_w, c
 int hmo_copy_roi(const void* src, int src_is_float, int src_w, int src_h, int src_c, int region_w, int region_h,
                  int src_roi_x, int src_roi_y, void* dst, int dst_is_float, int dst_w, int dst_h, int dst_c,
                  int off_x, int off_y, int batch) {
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b) {
     for (int y = 0; y < region_h; ++y) {
       for (int x = 0; x < region_w; ++x) {
@@ -214,6 +242,7 @@ int hmo_copy_roi(const void* src, int src_is_float, int src_w, int src_h, int sr
  * 0 if none; alpha = max over all in-range taps. */
 void hmo_downsample(const float* in, int in_w, int in_h, float* out, int out_w, int out_h, int c, int batch) {
   const int sum_c = c < 3 ? c : 3;
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b)
     for (int y = 0; y < out_h; ++y)
       for (int x = 0; x < out_w; ++x) {
@@ -245,6 +274,7 @@ void hmo_downsample(const float* in, int in_w, int in_h, float* out, int out_w, 
  * to M interleaved channels as FusedBatchedDownsampleMaskN
  * (cudaBlendN.cu:19-48) and FusedBatchedDownsampleMask3 (cudaBlend3.cu:220-262). */
 void hmo_downsample_mask(const float* in, int in_w, int in_h, float* out, int out_w, int out_h, int m) {
+  HMO_PARALLEL_FOR
   for (int y = 0; y < out_h; ++y)
     for (int x = 0; x < out_w; ++x) {
       float sums[HMO_MAX_IMAGES];
@@ -313,6 +343,7 @@ static float up_alpha_aware(const float* low, int low_w, int low_h, int c, int b
  * (:446-449); alpha is copied from the high level. */
 void hmo_laplacian(const float* high, int hw, int hh, const float* low, int lw, int lh, float* lap, int c,
                             int batch) {
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b)
     for (int y = 0; y < hh; ++y)
       for (int x = 0; x < hw; ++x) {
@@ -346,6 +377,7 @@ void hmo_laplacian(const float* high, int hw, int hh, const float* low, int lw, 
  * alpha copied from the Laplacian. */
 void hmo_reconstruct(const float* low, int lw, int lh, const float* lap, int hw, int hh, float* out, int c,
                               int batch) {
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b)
     for (int y = 0; y < hh; ++y)
       for (int x = 0; x < hw; ++x) {
@@ -366,6 +398,7 @@ void hmo_reconstruct(const float* low, int lw, int lh, const float* lap, int hw,
 /* BatchedBlendKernel (src/cuda/cudaBlend.cu:520-576). */
 static void blend2_level(const float* l1, const float* l2, const float* mask, float* out, int w, int h, int c,
                          int batch) {
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b)
     for (int y = 0; y < h; ++y)
       for (int x = 0; x < w; ++x) {
@@ -391,6 +424,7 @@ static void blend2_level(const float* l1, const float* l2, const float* mask, fl
 /* BatchedBlendKernelN (src/cuda/cudaBlendN.cu:199-288). */
 static void blend_n_level(int n, const float* const* laps, const float* mask, float* out, int w, int h, int c,
                           int batch) {
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b)
     for (int y = 0; y < h; ++y)
       for (int x = 0; x < w; ++x) {
@@ -447,6 +481,7 @@ static void blend_n_level(int n, const float* const* laps, const float* mask, fl
 /* BatchedBlendKernel3 (src/cuda/cudaBlend3.cu:376-531). */
 static void blend3_level(const float* l1, const float* l2, const float* l3, const float* mask, float* out, int w,
                          int h, int c, int batch) {
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b)
     for (int y = 0; y < h; ++y)
       for (int x = 0; x < w; ++x) {
@@ -600,11 +635,20 @@ int hmo_blend3(const float* img1, const float* img2, const float* img3, const fl
 /* mask padding: CanvasManager::convertMaskMat (canvasManager.cpp:63-101)     */
 /* = cv::copyMakeBorder(BORDER_REPLICATE) on the right/bottom                */
 /* ------------------------------------------------------------------------- */
-static uint8_t* pad_replicate(const uint8_t* seam, int sw, int sh, int cw, int ch) {
-  if (sw > cw || sh > ch || sw < 1 || sh < 1) return NULL;
-  uint8_t* out = (uint8_t*)malloc((size_t)cw * ch);
+int hmo_pad_replicate(const uint8_t* seam, int sw, int sh, uint8_t* out, int cw, int ch) {
+  /* canvasManager.cpp:69-70 asserts the seam is no larger than the canvas. */
+  if (sw > cw || sh > ch || sw < 1 || sh < 1) return -1;
   for (int y = 0; y < ch; ++y)
     for (int x = 0; x < cw; ++x) out[(size_t)y * cw + x] = seam[(size_t)IMIN(y, sh - 1) * sw + IMIN(x, sw - 1)];
+  return 0;
+}
+
+static uint8_t* pad_replicate(const uint8_t* seam, int sw, int sh, int cw, int ch) {
+  uint8_t* out = (uint8_t*)malloc((size_t)cw * ch);
+  if (hmo_pad_replicate(seam, sw, sh, out, cw, ch) != 0) {
+    free(out);
+    return NULL;
+  }
   return out;
 }
 
@@ -929,6 +973,7 @@ int hmo_pano3_canvas_size(const hmo_pano3_cfg* cfg, int* canvas_w, int* canvas_h
 static void fused_remap_to_full3(const hmo_pano3_cfg* cfg, const uint8_t* const* in, float* const* full, int fw,
                                  int fh) {
   const int c = cfg->channels;
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < cfg->batch; ++b)
     for (int y = 0; y < fh; ++y)
       for (int x = 0; x < fw; ++x)
@@ -947,6 +992,7 @@ static void fused_remap_to_full3(const hmo_pano3_cfg* cfg, const uint8_t* const*
  * canvas width; unmapped / out-of-range pixels are left untouched. */
 static void fused_hard_seam3(const hmo_pano3_cfg* cfg, const uint8_t* const* in, uint8_t* canvas, int cw, int ch) {
   const int c = cfg->channels;
+  HMO_PARALLEL_ROWS
   for (int b = 0; b < cfg->batch; ++b)
     for (int y = 0; y < ch; ++y)
       for (int x = 0; x < cw; ++x) {

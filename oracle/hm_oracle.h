@@ -45,6 +45,11 @@ typedef struct hmo_dump {
   float* reconstruct[HMO_MAX_LEVELS];
 } hmo_dump;
 
+/* Threads of the per-pixel stages (OpenMP over rows); n <= 0 keeps the
+ * current setting.  Returns the number now in use.  Results do not depend on
+ * it: every output pixel is computed by one thread from its own inputs. */
+int hmo_set_threads(int n);
+
 /* ---- casts: src/cuda/cudaUtils.cuh:18-28 (truncating round_to_uchar) ---- */
 uint8_t hmo_round_to_uchar(float x);
 
@@ -97,6 +102,13 @@ int hmo_blend_n(int n, const float* const* imgs, const float* mask, float* out, 
 /* cudaBatchedLaplacianBlendWithContext3 (src/cuda/cudaBlend3.cu:881-1206). */
 int hmo_blend3(const float* img1, const float* img2, const float* img3, const float* mask, float* out, int w, int h,
                int c, int levels, int batch, hmo_dump* dump);
+
+/* ---- seam padding: CanvasManager::convertMaskMat (src/pano/canvasManager.cpp:63-86,
+ * src/pano/canvasManagerN.h:31-51) = cv::copyMakeBorder(mask, 0, ch-sh, 0,
+ * cw-sw, BORDER_REPLICATE), i.e. out[y][x] = seam[min(y,sh-1)][min(x,sw-1)]
+ * (right/bottom only).  out is cw x ch; -1 when the seam is larger than the
+ * canvas (the reference asserts) or empty. ---- */
+int hmo_pad_replicate(const uint8_t* seam, int sw, int sh, uint8_t* out, int cw, int ch);
 
 /* ---- 2-image pano: CudaStitchPano ctor + process_impl ----
  * (src/pano/cudaPano.inl:19-94, 167-418; canvasManager.cpp:16-101) */

@@ -80,6 +80,8 @@ def lib():
         L.hmo_round_to_uchar.restype = ctypes.c_uint8
         L.hmo_round_to_uchar.argtypes = [ctypes.c_float]
         L.hmo_pyramid_sizes.argtypes = [_I, _I, _I, _I, _P, _P]
+        L.hmo_pad_replicate.argtypes = [_P, _I, _I, _P, _I, _I]
+        L.hmo_set_threads.argtypes = [_I]
         L.hmo_remap_offset.argtypes = [_P, _I, _I, _I, _P, _I, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I,
                                        _I]
         L.hmo_remap_dest_map.argtypes = [_P, _I, _I, _I, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _P, _I]
@@ -111,6 +113,21 @@ def _ptr(a: np.ndarray) -> int:
 
 def round_to_uchar(x: float) -> int:
     return int(lib().hmo_round_to_uchar(ctypes.c_float(x)))
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's per-pixel stages (0 keeps the setting); returns the count in use."""
+    return int(lib().hmo_set_threads(n))
+
+
+def pad_replicate(seam: np.ndarray, canvas_w: int, canvas_h: int) -> Optional[np.ndarray]:
+    """The seam padded to canvas_w x canvas_h (right/bottom replicate), or
+    None when the oracle refuses it (seam larger than the canvas, or empty)."""
+    seam = np.ascontiguousarray(seam, np.uint8)
+    out = np.zeros((canvas_h, canvas_w), np.uint8)
+    sh, sw = seam.shape
+    rc = lib().hmo_pad_replicate(_ptr(seam), sw, sh, _ptr(out), canvas_w, canvas_h)
+    return out if rc == 0 else None
 
 
 def pyramid_sizes(w: int, h: int, levels: int, clamp: bool):

@@ -160,6 +160,39 @@ def test_pano2_invalid_masks_status(hm, torch_gpu):
     assert pano.status != 0
 
 
+@pytest.mark.parametrize("levels", [0, 1, 4, 6])
+@pytest.mark.parametrize("minimize", [True, False])
+def test_pano2_short_seam_bit_exact(hm, torch_gpu, oracle, levels, minimize):
+    """A seam narrower and shorter than the canvas is padded by replicating its
+    last column and row (CanvasManager::convertMaskMat, canvasManager.cpp:63-86).
+    The crop ends inside the overlap, so the replicated column decides the
+    seam over the rest of the blend region."""
+    import copy
+
+    import hm_synth
+
+    rig = hm_synth.small_rig(seed=11)
+    x2, w1 = rig.pos_x[1], rig.pos_x[0] + rig.map_x[0].shape[1]
+    short = copy.copy(rig)
+    short.seam = np.ascontiguousarray(rig.seam[: rig.canvas_h - 9, : (x2 + w1) // 2])
+    frames = frames_for(rig, 1, 12)
+    _, got = run_pano(hm, torch_gpu, short, frames, levels, minimize)
+    want = oracle.pano2(short, frames[0], frames[1], levels, minimize=minimize)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_pano2_seam_larger_than_canvas_is_refused(hm, torch_gpu):
+    """canvasManager.cpp:69-70 asserts the seam fits the canvas; here it is an error status."""
+    import copy
+
+    import hm_synth
+
+    rig = hm_synth.small_rig(seed=11)
+    big = copy.copy(rig)
+    big.seam = np.zeros((rig.canvas_h, rig.canvas_w + 3), np.uint8)
+    assert hm.StitchPano(1, 2, big).status != 0
+
+
 @pytest.mark.parametrize("levels", [0, 6])
 def test_pano2_cfg0_full_size_bit_exact(hm, torch_gpu, oracle, levels):
     """2 x 1280x720 -> 2400x900 (the reference's CPU-runnable workload size)."""

@@ -137,6 +137,34 @@ def test_panoN_invalid_masks_status(hm, torch_gpu):
     assert hm.StitchPanoN(1, 2, rig).status != 0  # ControlMasksN::is_valid fails: sticky status
 
 
+@pytest.mark.parametrize("n", [3, 5])
+@pytest.mark.parametrize("levels", [0, 3, 6])
+@pytest.mark.parametrize("minimize", [True, False])
+def test_panoN_short_seam_bit_exact(hm, torch_gpu, oracle, n, levels, minimize):
+    """A seam narrower and shorter than the canvas is padded by replication on the
+    right and bottom (CanvasManagerN::convertMaskMat, canvasManagerN.h:31-51): the
+    labels of its last column extend over the rest of the canvas."""
+    import copy
+
+    rig = small_n_rig(n, 60 + n)
+    short = copy.copy(rig)
+    h, w = rig.seam.shape
+    short.seam = np.ascontiguousarray(rig.seam[: h - 7, : w - w // 5])
+    frames = frames_for(rig, 1, 21 + n)
+    _, got = run_panoN(hm, torch_gpu, short, frames, levels, minimize)
+    want = oracle.panoN(short, frames, levels, minimize=minimize)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_panoN_seam_larger_than_canvas_is_refused(hm, torch_gpu):
+    import copy
+
+    rig = small_n_rig(3, 3)
+    big = copy.copy(rig)
+    big.seam = np.zeros((rig.seam.shape[0] + 1, rig.seam.shape[1]), np.uint8)
+    assert hm.StitchPanoN(1, 2, big).status != 0
+
+
 @pytest.mark.parametrize("levels", [0, 6])
 def test_panoN_cfg2_n3_full_size_bit_exact(hm, torch_gpu, oracle, levels):
     """3 x 3840x2160 -> 6200x2300 against the oracle."""

@@ -315,6 +315,94 @@ def test_copy_roi_casts(oracle):
 
 
 # ---------------------------------------------------------------------------
+# seam padding: CanvasManager::convertMaskMat (src/pano/canvasManager.cpp:63-86,
+# src/pano/canvasManagerN.h:31-51) = cv::copyMakeBorder(..., BORDER_REPLICATE)
+# on the right and bottom only
+# ---------------------------------------------------------------------------
+def test_pad_replicate_closed_form(oracle):
+    """A 3x2 seam with distinct values padded to 6x5, written out by hand:
+    the last column repeats to the right, the last row repeats down.  Left/top
+    padding, BORDER_REFLECT (3 2 1 ...), BORDER_WRAP (1 2 3 ...), a constant
+    border or a clamp to sw instead of sw-1 all give a different array."""
+    seam = np.uint8([[1, 2, 3],
+                     [4, 5, 6]])
+    want = np.uint8([[1, 2, 3, 3, 3, 3],
+                     [4, 5, 6, 6, 6, 6],
+                     [4, 5, 6, 6, 6, 6],
+                     [4, 5, 6, 6, 6, 6],
+                     [4, 5, 6, 6, 6, 6]])
+    np.testing.assert_array_equal(oracle.pad_replicate(seam, 6, 5), want)
+
+
+def test_pad_replicate_equals_numpy_edge_mode(oracle):
+    """numpy's 'edge' padding is the same definition, written independently."""
+    rng = np.random.default_rng(3)
+    for sw, sh, cw, ch in [(1, 1, 4, 3), (7, 5, 7, 9), (7, 5, 12, 5), (13, 11, 40, 17), (9, 9, 9, 9)]:
+        seam = rng.integers(0, 256, size=(sh, sw), dtype=np.uint8)
+        got = oracle.pad_replicate(seam, cw, ch)
+        np.testing.assert_array_equal(got, np.pad(seam, ((0, ch - sh), (0, cw - sw)), mode="edge"))
+
+
+def test_pad_replicate_refuses_larger_or_empty_seam(oracle):
+    """canvasManager.cpp:69-70 asserts the mask is no larger than the canvas."""
+    seam = np.zeros((4, 6), np.uint8)
+    assert oracle.pad_replicate(seam, 5, 4) is None
+    assert oracle.pad_replicate(seam, 6, 3) is None
+    assert oracle.pad_replicate(np.zeros((0, 0), np.uint8), 4, 4) is None
+
+
+def _cropped(rig, dw, dh):
+    import copy
+
+    r = copy.copy(rig)
+    h, w = rig.seam.shape
+    r.seam = np.ascontiguousarray(rig.seam[: h - dh, : w - dw])
+    return r
+
+
+def _padded(rig):
+    import copy
+
+    r = copy.copy(rig)
+    ch = max(py + m.shape[0] for py, m in zip(rig.pos_y, rig.map_x))
+    cw = max(px + m.shape[1] for px, m in zip(rig.pos_x, rig.map_x))
+    sh, sw = rig.seam.shape
+    r.seam = np.pad(rig.seam, ((0, ch - sh), (0, cw - sw)), mode="edge")
+    return r
+
+
+@pytest.mark.parametrize("levels,minimize", [(0, True), (3, True), (3, False)])
+def test_pano2_short_seam_is_padded_by_replication(oracle, levels, minimize):
+    """A seam narrower and shorter than the canvas stitches like the seam
+    padded beforehand with numpy's edge mode; the crop ends inside the overlap
+    so the replicated last column moves the seam."""
+    import hm_synth
+
+    rig = hm_synth.small_rig(seed=11)
+    x2, w1 = rig.pos_x[1], rig.pos_x[0] + rig.map_x[0].shape[1]
+    dw = rig.canvas_w - (x2 + w1) // 2
+    short = _cropped(rig, dw, 9)
+    f = frames_for(rig, 1, 5)
+    got = oracle.pano2(short, f[0], f[1], levels, minimize=minimize)
+    want = oracle.pano2(_padded(short), f[0], f[1], levels, minimize=minimize)
+    np.testing.assert_array_equal(got, want)
+    assert (got != oracle.pano2(rig, f[0], f[1], levels, minimize=minimize)).any()
+
+
+@pytest.mark.parametrize("levels", [0, 3])
+def test_panoN_short_seam_is_padded_by_replication(oracle, levels):
+    import hm_synth
+
+    rig = hm_synth.small_rig(n=3, seed=4, canvas=(520, 72))
+    short = _cropped(rig, rig.canvas_w // 5, 7)
+    f = frames_for(rig, 1, 8)
+    got = oracle.panoN(short, f, levels)
+    want = oracle.panoN(_padded(short), f, levels)
+    np.testing.assert_array_equal(got, want)
+    assert (got != oracle.panoN(rig, f, levels)).any()
+
+
+# ---------------------------------------------------------------------------
 # pano classes (tests/python/test_pytorch_pano.py and src/pano/*_test.cpp restated)
 # ---------------------------------------------------------------------------
 def test_pano2_hard_seam_selection(oracle):
