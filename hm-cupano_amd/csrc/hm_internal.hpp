@@ -18,10 +18,12 @@
 
 namespace hm {
 
-// Device allocation owned by a host object.
+// Device allocation owned by a host object, or a view of memory owned
+// elsewhere (view(): never freed here).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = false;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -29,12 +31,19 @@ struct DevBuf {
   hipError_t alloc(size_t n) {
     reset();
     bytes = n;
+    owned = true;
     return hipMalloc(&p, n ? n : 1);
   }
+  void view(void* q, size_t n) {
+    reset();
+    p = q;
+    bytes = n;
+  }
   void reset() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = false;
   }
   template <typename T>
   T* as() const {
@@ -147,7 +156,12 @@ class Pyramid {
  public:
   // kind: hmdev::kBlend2 (1-channel mask, level count unclamped),
   //       hmdev::kBlend3 or hmdev::kBlendN (N-channel mask, stop at a side < 2).
-  hipError_t init(int kind, int n, int channels, int width, int height, int levels, int batch, bool debug);
+  // With an arena, levels >= 1 are views of it (256-byte aligned, in order)
+  // and nothing is allocated; hipErrorOutOfMemory when they do not fit
+  // (arena_need then holds the bytes they need).  Debug mode needs no arena.
+  hipError_t init(int kind, int n, int channels, int width, int height, int levels, int batch, bool debug,
+                  void* arena = nullptr, size_t arena_bytes = 0);
+  size_t arena_need = 0;
   hipError_t build_mask_pyramid(const float* mask0, hipStream_t s);
   // g0: n level-0 images (uint8 if g0_u8 else float), tight, batch-major.
   hipError_t run(const void* const* g0, bool g0_u8, const Level0Out& out, hipStream_t s, LaunchTimer* timer);
@@ -178,7 +192,7 @@ class Pyramid {
   DevBuf lap[hmdev::kMaxImages][hmdev::kMaxLevels]; // debug
   DevBuf blend[hmdev::kMaxLevels];                  // debug
   DevBuf r0_debug;                                  // debug copy of level-0 reconstruction
-  std::vector<hmdev::LevelDesc> level_table;        // levels 1..levels-1 (host; launches copy what they need)
+  hmdev::LevelDesc level_table[hmdev::kMaxLevels] = {};  // levels 1..levels-1 (host; launches copy what they need)
   size_t allocation_size = 0;
   int mask_channels() const { return kind == hmdev::kBlend2 ? 1 : n; }
 

@@ -440,7 +440,7 @@ hipError_t Pano2::launch_remap_down(const Pano2L0& a, hipStream_t s, int* n_firs
   ra.r = a;
   const int n_out = std::min(5, pyr.levels - 1);
   for (int i = 0; i < 2; ++i) ra.dn.g0[i] = g0[i].as<uint8_t>();
-  fill_down_args(ra.dn, pyr.level_table.data(), bw, bh, n_out);
+  fill_down_args(ra.dn, pyr.level_table, bw, bh, n_out);
   const dim3 grid((unsigned)(ra.dn.tiles_x * ra.dn.tiles_y), 1u, (unsigned)batch);
   if (hm::g_recorder) return hm::g_recorder->fail("remap_down");
   timer.begin(s);

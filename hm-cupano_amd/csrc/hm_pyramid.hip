@@ -3,6 +3,8 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <sstream>
 #include <type_traits>
 #include <vector>
@@ -456,10 +458,11 @@ std::string LaunchTimer::report() const {
 // Pyramid
 // ---------------------------------------------------------------------------
 hipError_t Pyramid::init(int kind_, int n_, int channels, int width, int height, int levels_req, int batch_,
-                         bool debug_) {
+                         bool debug_, void* arena, size_t arena_bytes) {
   if ((kind_ != kBlend2 && kind_ != kBlend3 && kind_ != kBlendN) || n_ < 2 || n_ > kMaxImages ||
       (channels != 3 && channels != 4) || batch_ < 1)
     return hipErrorInvalidValue;
+  if (arena && debug_) return hipErrorInvalidValue;
   if (kind_ == kBlend2 && n_ != 2) return hipErrorInvalidValue;
   if (kind_ == kBlend3 && n_ != 3) return hipErrorInvalidValue;
   kind = kind_;
@@ -474,16 +477,34 @@ hipError_t Pyramid::init(int kind_, int n_, int channels, int width, int height,
     wn[l].reset();
     touch[l].reset();
   }
+  // Arena layout: every level's buffers in the order below, each at a
+  // 256-byte boundary.  First the size check, then the views.
+  const auto round_up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  arena_need = 0;
+  for (int l = 1; l < levels; ++l) {
+    const size_t px = (size_t)W[l] * H[l];
+    const size_t img = round_up(px * c * batch * sizeof(float));
+    arena_need += (n + 1) * img + round_up(px * mask_channels() * sizeof(float));
+    if (kind == kBlendN && n > 2) arena_need += round_up(px * n * sizeof(float));
+  }
+  if (arena && arena_need > arena_bytes) return hipErrorOutOfMemory;
+  char* cursor = static_cast<char*>(arena);
+  const auto take = [&](DevBuf& b, size_t bytes) -> hipError_t {
+    if (!arena) return b.alloc(bytes);
+    b.view(cursor, bytes);
+    cursor += round_up(bytes);
+    return hipSuccess;
+  };
   for (int l = 0; l < levels; ++l) {
     const size_t px = (size_t)W[l] * H[l];
     const size_t img = px * c * batch * sizeof(float);
     if (l > 0) {
-      for (int i = 0; i < n; ++i) HM_RETURN_IF_ERROR(g[i][l].alloc(img));
-      HM_RETURN_IF_ERROR(mask[l].alloc(px * mask_channels() * sizeof(float)));
-      HM_RETURN_IF_ERROR(r[l].alloc(img));
+      for (int i = 0; i < n; ++i) HM_RETURN_IF_ERROR(take(g[i][l], img));
+      HM_RETURN_IF_ERROR(take(mask[l], px * mask_channels() * sizeof(float)));
+      HM_RETURN_IF_ERROR(take(r[l], img));
       allocation_size += n * img + img + px * mask_channels() * sizeof(float);
       if (kind == kBlendN && n > 2 && !debug) {  // the slot path of three or more images
-        HM_RETURN_IF_ERROR(wn[l].alloc(px * n * sizeof(float)));
+        HM_RETURN_IF_ERROR(take(wn[l], px * n * sizeof(float)));
         allocation_size += px * n * sizeof(float);
       }
     }
@@ -505,7 +526,7 @@ hipError_t Pyramid::init(int kind_, int n_, int channels, int width, int height,
 }
 
 void Pyramid::build_level_table() {
-  level_table.assign(levels > 1 ? levels - 1 : 0, LevelDesc{});
+  for (int l = 0; l < kMaxLevels; ++l) level_table[l] = LevelDesc{};
   for (int l = 1; l < levels; ++l) {
     LevelDesc& d = level_table[l - 1];
     for (int i = 0; i < kMaxImages; ++i) {
@@ -577,7 +598,7 @@ hipError_t Pyramid::build_mask_pyramid(const float* mask0_, hipStream_t s) {
 
 hipError_t Pyramid::run_down(const void* const* g0, bool g0_u8, int from, hipStream_t s, LaunchTimer* timer) {
   // Up to five levels per launch from aligned tiles.
-  const LevelDesc* table = level_table.data();
+  const LevelDesc* table = level_table;
   while (from + 1 < levels) {
     const int n_out = std::min(5, levels - 1 - from);
     const void* in[kMaxImages];
@@ -824,3 +845,64 @@ extern "C" int hmBlendContextLevelPtr(const hmBlendContext* ctx, int what, int i
 }
 
 extern "C" void hmBlendContextDestroy(hmBlendContext* ctx) { delete ctx; }
+
+// ===========================================================================
+// Allocation-free blend: the pyramid levels live in `scratch`, a file-scope
+// device array (one copy per GPU, held until the process ends).
+// ===========================================================================
+static __device__ __align__(256) unsigned char scratch[HM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::atomic<void*> g_scratch[kMaxDevices];  // device address of `scratch`, per device
+std::mutex g_scratch_mu;
+
+// The device address of `scratch` on the current device, looked up once per
+// device.  The array's name is a host-side stand-in: only this address may go
+// to kernels or copies.
+hipError_t scratch_ptr(void** out) {
+  int dev = -1;
+  HM_RETURN_IF_ERROR(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  void* p = g_scratch[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      HM_RETURN_IF_ERROR(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      if (!p) return hipErrorInvalidSymbol;
+      g_scratch[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return hipSuccess;
+}
+}  // namespace
+
+extern "C" int hmBlendRunNoAlloc(int kind, int n_images, int width, int height, int num_levels, int batch_size,
+                                 int channels, const float* const* d_images, const float* d_mask, float* d_output,
+                                 hipStream_t stream) {
+  if (kind != HM_BLEND_TWO && kind != HM_BLEND_THREE && kind != HM_BLEND_N) return hipErrorInvalidValue;
+  if (width < 1 || height < 1 || num_levels < 1) return hipErrorInvalidValue;
+  if (!d_images || !d_mask || !d_output) return hipErrorInvalidValue;
+  void* arena = nullptr;
+  HM_RETURN_IF_ERROR(scratch_ptr(&arena));
+  // Host state only (fixed-size arrays); the device buffers are views of the scratch.
+  hm::Pyramid pyr;
+  const int k = kind == HM_BLEND_TWO ? hmdev::kBlend2 : (kind == HM_BLEND_THREE ? hmdev::kBlend3 : hmdev::kBlendN);
+  HM_RETURN_IF_ERROR(pyr.init(k, n_images, channels, width, height, num_levels, batch_size, false, arena,
+                              HM_NOALLOC_SCRATCH_BYTES));
+  const void* g0[hmdev::kMaxImages];
+  for (int i = 0; i < pyr.n; ++i) {
+    if (!d_images[i]) return hipErrorInvalidValue;
+    if (pyr.c == 4 && (reinterpret_cast<uintptr_t>(d_images[i]) & 15u)) return hipErrorInvalidValue;
+    g0[i] = d_images[i];
+  }
+  if (pyr.c == 4 && (reinterpret_cast<uintptr_t>(d_output) & 15u)) return hipErrorInvalidValue;
+  // Every level is written in full before it is read (mask pyramid, down pass,
+  // up pass), so nothing in the scratch needs clearing.
+  HM_RETURN_IF_ERROR(pyr.build_mask_pyramid(d_mask, stream));
+  hm::Level0Out out;
+  out.f32 = d_output;
+  return pyr.run(g0, false, out, stream, nullptr);
+}

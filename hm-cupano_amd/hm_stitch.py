@@ -170,6 +170,7 @@ def lib() -> ctypes.CDLL:
         "hmBlendRun": (i, [vp, ctypes.POINTER(vp), vp, vp, vp]),
         "hmBlendContextLevelPtr": (i, [vp, i, i, i, ctypes.POINTER(vp)]),
         "hmBlendContextDestroy": (None, [vp]),
+        "hmBlendRunNoAlloc": (i, [i, i, i, i, i, i, i, ctypes.POINTER(vp), vp, vp, vp]),
         "hmStitchPanoCreate": (i, [i, i, ctypes.POINTER(hmControlMasks2), i, i, i, i, ctypes.POINTER(vp)]),
         "hmStitchPanoProcess": (i, [vp, S, S, S, vp]),
         "hmStitchPanoNCreate": (i, [i, i, ctypes.POINTER(hmControlMasksN), i, i, i, ctypes.POINTER(vp)]),
@@ -627,6 +628,18 @@ class BlendContext:
         p = ctypes.c_void_p()
         check(lib().hmBlendContextLevelPtr(self._h, what, image, level, ctypes.byref(p)), "hmBlendContextLevelPtr")
         return p.value or 0
+
+
+NOALLOC_SCRATCH_BYTES = 128 << 20  # HM_NOALLOC_SCRATCH_BYTES
+
+
+def blend_run_no_alloc(kind: int, n_images: int, width: int, height: int, num_levels: int, batch_size: int,
+                       channels: int, images: Sequence, mask, out, stream=None) -> int:
+    """hmBlendRunNoAlloc: the blend of BlendContext(...).run(...) with its pyramid in the
+    library's static device scratch (no device allocation).  Returns the hipError_t code."""
+    arr = (ctypes.c_void_p * len(images))(*[t.data_ptr() for t in images])
+    return lib().hmBlendRunNoAlloc(kind, n_images, width, height, num_levels, batch_size, channels, arr,
+                                   mask.data_ptr(), out.data_ptr(), _stream(stream))
 
 
 def remap_ex_offset(src, src_type, dest, dest_type, map_x, map_y, batch, remap_w, remap_h, off_x, off_y,

@@ -133,6 +133,26 @@ int hmBlendRun(hmBlendContext* ctx, const float* const* d_images, const float* d
 int hmBlendContextLevelPtr(const hmBlendContext* ctx, int what, int image, int level, const float** ptr);
 void hmBlendContextDestroy(hmBlendContext* ctx);
 
+/* Bytes of the device scratch that hmBlendRunNoAlloc carves its pyramid
+ * levels from: one array per GPU, held from the first call on that GPU until
+ * the process ends. */
+#define HM_NOALLOC_SCRATCH_BYTES (128u << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Equivalent to hmBlendContextCreate(kind, n_images,
+ * width, height, num_levels, batch_size, channels, 0, &c) + hmBlendRun(c,
+ * d_images, d_mask, d_output, stream) + hmBlendContextDestroy(c), same
+ * kernels and same bits, but it calls no hipMalloc/hipFree: levels >= 1 and
+ * the mask pyramid live in a static device array of HM_NOALLOC_SCRATCH_BYTES.
+ * Returns hipErrorOutOfMemory when the pyramid does not fit it (a 2-image,
+ * 4-channel blend of 1200 x 3300 at 6 levels, batch 1, needs about 69 MB).
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (other streams or threads) race.  Nothing in it needs clearing: every
+ * level is written in full before it is read. */
+int hmBlendRunNoAlloc(int kind, int n_images, int width, int height, int num_levels, int batch_size, int channels,
+                      const float* const* d_images, const float* d_mask, float* d_output, hipStream_t stream);
+
 /* ------------------------------------------------------------------------- */
 /* Tier 2: pano classes                                                      */
 /* ------------------------------------------------------------------------- */
