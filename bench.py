@@ -118,39 +118,63 @@ def measured_traffic():
         return json.load(f)["kernels"]
 
 
-def cpu_baseline(rig, f1, f2, levels, budget_s=12.0):
-    """The CPU oracle (single thread) on the same workload, bounded in time."""
+def host_threads():
+    """Threads for the all-cores CPU baseline: the cores this process may run on, capped by
+    OMP_NUM_THREADS when the environment sets it (a shared host gives a job a share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def time_oracle(hm_oracle, rig, f1, f2, levels, threads, budget_s, min_frames=3):
+    """Median seconds per frame of the oracle's stitch on `threads` threads: frames until
+    `budget_s` has passed and at least `min_frames` ran."""
+    hm_oracle.set_threads(threads)
+    hm_oracle.pano2(rig, f1, f2, levels)  # warm-up (page-in of the buffers)
+    ts = []
+    t0 = time.perf_counter()
+    while len(ts) < min_frames or time.perf_counter() - t0 < budget_s:
+        t = time.perf_counter()
+        hm_oracle.pano2(rig, f1, f2, levels)
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), len(ts)
+
+
+def cpu_baseline(hm_synth, rig, f1, f2, levels):
+    """The CPU oracle on the benchmark workload (8500x3300) and on the 2400x900 workload,
+    single-threaded and on every host core (OpenMP over rows), in the same run."""
     import hm_oracle
 
     hm_oracle.lib()
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        hm_oracle.pano2(rig, f1, f2, levels)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    mpix = rig.canvas_w * rig.canvas_h * n / el / 1e6
-    return {"value": round(mpix, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": f"{n} frame(s) of the same 8500x3300 L={levels} stitch on the oracle in {el:.1f} s",
-            "fps": round(n / el, 4)}
+    cores = host_threads()
+    rig0 = hm_synth.config_rig("cfg0")
+    g1 = hm_synth.make_frame(rig0.src_h, rig0.src_w, 11)[None]
+    g2 = hm_synth.make_frame(rig0.src_h, rig0.src_w, 12)[None]
+    runs = {}
+    for name, r, a, b, budget in (("8500x3300", rig, f1, f2, 5.0), ("2400x900", rig0, g1, g2, 2.0)):
+        for th in (1, cores):
+            sec, n = time_oracle(hm_oracle, r, a, b, levels, th, budget)
+            runs[f"{name}_threads{th}"] = {"ms_per_frame": round(sec * 1e3, 2), "frames": n,
+                                           "Mpix_s": round(r.canvas_w * r.canvas_h / sec / 1e6, 3),
+                                           "fps": round(1 / sec, 3)}
+            log(f"cpu baseline {name} L={levels}, {th} thread(s): {sec * 1e3:.1f} ms/frame over {n} frames")
+    hm_oracle.set_threads(cores)
+    main = runs[f"8500x3300_threads{cores}"]
+    return {"value": main["Mpix_s"], "unit": "Mpix/s", "cores": cores, "kind": "port",
+            "sample": f"the oracle's 8500x3300 L={levels} stitch of the benchmark frames, median of "
+                      f"{main['frames']} frames on {cores} threads (OpenMP over rows); the same at 1 thread "
+                      f"and for 2 x 1280x720 -> 2400x900 in `runs`",
+            "fps": main["fps"], "nproc": os.cpu_count(), "runs": runs}
 
 
 def kt_of(r):
     return r["kt"]
 
 
-# The mixed served load: 32 rigs drawn from 5 seeded configs (2 to 8 images,
-# 1920x1080 to 5312x2988 inputs, canvases 2000 to 14000 wide, levels 0/4/6).
-SERVED_CONFIGS = [
-    # name, class, (n, src_w, src_h, canvas_w, canvas_h, overlap, focal, seed), levels, rigs, input sets
-    ("2img_5312x2988_8500x3300_L6", "2", None, 6, 6, 2),
-    ("2img_1920x1080_2000x700_L4", "2", (2, 1920, 1080, 2000, 700, 0.25, 1300.0, 5), 4, 8, 2),
-    ("3img_2704x1520_6000x2000_L4", "3", (3, 2704, 1520, 6000, 2000, 0.25, 1500.0, 77), 4, 6, 2),
-    ("4img_1920x1080_4000x1500_L0", "N", (4, 1920, 1080, 4000, 1500, 0.25, 1100.0, 6), 0, 8, 2),
-    ("8img_3840x2160_14000x2600_L6", "N", (8, 3840, 2160, 14000, 2600, 0.25, 3500.0, 1234), 6, 4, 1),
-]
+SERVED_CONFIGS = None  # set from hm_synth in main(): the 32-rig served load
+BATCH_MIX = None
 
 
 def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_closed, seed=7):
@@ -162,12 +186,8 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
     rigs = []  # dict(name, kind, pano, inputs=[[tensors]], canvases=[t, t], mpix)
     for name, kind, geo, levels, count, n_sets in SERVED_CONFIGS:
         log(f"served load: generating {name} ({count} rigs)")
-        rig = hm_synth.config_rig("cfg1") if geo is None else hm_synth.cylinder_rig(
-            geo[0], geo[1], geo[2], geo[3], geo[4], overlap=geo[5], focal=geo[6], seed=geo[7])
-        sets = []
-        for k in range(n_sets):
-            sets.append([torch.from_numpy(hm_synth.make_frame(rig.src_h, rig.src_w, 1000 * k + 17 * i)[None]).cuda()
-                         for i in range(rig.n)])
+        rig = hm_synth.served_rig(name)
+        sets = [[torch.from_numpy(f).cuda() for f in hm_synth.served_frames(rig, k)] for k in range(n_sets)]
         tuned = None
         for _ in range(count):
             if kind == "2":
@@ -247,8 +267,13 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
             k += 1
     arrivals.sort()
     lat = []
+    log(f"served load: open loop, {len(arrivals)} Poisson arrivals over {seconds:.0f} s")
     t_start = time.perf_counter()
+    next_log = 10.0
     for tag, (t_arr, ri, k) in enumerate(arrivals):
+        if t_arr >= next_log:
+            log(f"served load: open loop at {t_arr:.0f} s, {tag} frames submitted")
+            next_log += 10.0
         while True:  # sleep (yielding the CPU to the scheduler thread) until the arrival time
             dt = t_arr - (time.perf_counter() - t_start)
             if dt <= 0:
@@ -283,12 +308,6 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
         "graphs_captured": st["graphs_captured"], "graph_launches": st["graph_launches"],
         "batched_mixed": batched,
     }
-
-
-# The 16-frame mixed batch: 6 frames of 2 x 5312x2988 -> 8500x3300 L=6, 6 of
-# 3 x 2704x1520 -> 6000x2000 L=4 and 4 of 4 x 1920x1080 -> 4000x1500 L=0, one
-# frame per resident rig of the served load.
-BATCH_MIX = [("2img_5312x2988_8500x3300_L6", 6), ("3img_2704x1520_6000x2000_L4", 6), ("4img_1920x1080_4000x1500_L0", 4)]
 
 
 def batched_mixed_benchmark(torch, hm_stitch, rigs, stream, passes=20):
@@ -464,7 +483,7 @@ def main():
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-frames", type=int, default=200)
-    ap.add_argument("--served-seconds", type=float, default=5.0,
+    ap.add_argument("--served-seconds", type=float, default=60.0,
                     help="open-loop duration of the 32-rig served load (0: skip the served measurement)")
     ap.add_argument("--served-streams", type=int, default=4)
     ap.add_argument("--served-frames", type=int, default=640, help="frames of the closed-loop served/plain runs")
@@ -488,6 +507,8 @@ def main():
     import hm_stitch
     import hm_synth
 
+    global SERVED_CONFIGS, BATCH_MIX
+    SERVED_CONFIGS, BATCH_MIX = hm_synth.SERVED_CONFIGS, hm_synth.BATCH_MIX
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.calibrate_copy:
         # A plain elementwise copy kernel of known size: b = a + 0 over 1 GiB of int32
@@ -637,7 +658,7 @@ def main():
                                           args.served_frames)
     if not args.no_cpu:
         log("timing the CPU oracle on the same workload")
-        line["cpu_baseline"] = cpu_baseline(rig, f1, f2, main_l)
+        line["cpu_baseline"] = cpu_baseline(hm_synth, rig, f1, f2, main_l)
     print(json.dumps(line), flush=True)
 
 

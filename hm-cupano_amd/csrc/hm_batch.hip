@@ -247,6 +247,7 @@ int hmBatchCreate(const hmBatchCall* calls, int num_calls, hmBatch** out) {
   for (int i = 0; i < num_calls; ++i) {
     if (!calls[i].pano || calls[i].pano->status != hipSuccess) return hipErrorInvalidValue;
     if (calls[i].pano->channels != 4 || calls[i].pano->debug) return hipErrorNotSupported;
+    if (calls[i].pano->served.load()) return hipErrorNotSupported;  // a server's thread launches it
     for (int j = 0; j < i; ++j)  // one object's scratch serves one call of a pass
       if (calls[j].pano == calls[i].pano) return hipErrorInvalidValue;
   }

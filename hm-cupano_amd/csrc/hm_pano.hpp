@@ -1,6 +1,7 @@
 // hm_pano.hpp -- state shared by the 2-, 3- and N-image pano objects.
 #pragma once
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -29,6 +30,10 @@ struct hmStitchPano {
 
   // Launch-shape knobs (hmTuning); every setting computes the same output.
   hmTuning tuning{4, 2, 1, 0, 1, 0, 0};
+  // Set while a server holds the object (hmServerAddRig .. RemoveRig/Destroy):
+  // its scheduler thread then owns the launches and the captured graphs bake
+  // in the launch shapes, so tuning and batch recording are refused.
+  std::atomic<bool> served{false};
   // Which knobs matter for this object (the autotuner sweeps only those).
   virtual bool tunes_remap_px() const { return false; }
   bool tunes_tail() const { return !hard_seam && n_images == 2 && num_levels >= 4; }

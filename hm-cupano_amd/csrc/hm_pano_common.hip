@@ -125,6 +125,7 @@ static bool tuning_ok(const hmTuning& t) {
 
 extern "C" int hmStitchPanoSetTuning(hmStitchPano* pano, const hmTuning* tuning) {
   if (!pano || !tuning || !tuning_ok(*tuning)) return hipErrorInvalidValue;
+  if (pano->served.load()) return hipErrorNotSupported;
   pano->tuning = *tuning;
   pano->apply_tuning();
   return hipSuccess;
@@ -135,6 +136,7 @@ extern "C" int hmStitchPanoAutotune(hmStitchPano* pano, const hmSurface* inputs,
                                     float* best_ms_per_frame) {
   if (!pano || !inputs || frames < 1) return hipErrorInvalidValue;
   if (pano->status != hipSuccess) return pano->status;
+  if (pano->served.load()) return hipErrorNotSupported;
   HM_RETURN_IF_ERROR(pano->prepare());  // allocations and streams first: they decide the knobs
   const hmTuning start = pano->tuning;
   const bool remap = pano->tunes_remap_px();

@@ -58,6 +58,17 @@ struct LevelDesc {
 // differ from the general form only in the sign of a zero, which no later
 // step (comparisons, sums with non-zero terms, conversion to uint8) can see.
 // ---------------------------------------------------------------------------
+// Orders one wavefront's LDS accesses where its lanes overwrite entries that
+// other lanes of the same wavefront have just read (or read what they have
+// just written).  The hardware executes a wavefront's LDS instructions in
+// order; the barrier keeps the scheduler from moving instructions across it,
+// and the wavefront-scope fence (no instruction) keeps the compiler from
+// reordering the memory accesses themselves.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ float4 rgb_of(const float4& p) { return make_float4(p.x, p.y, p.z, 0.0f); }
 __device__ __forceinline__ float4 half_pair(const float4& p, const float4& q) {
   return make_float4(p.x * 0.5f + q.x * 0.5f, p.y * 0.5f + q.y * 0.5f, p.z * 0.5f + q.z * 0.5f, 0.0f);
@@ -204,12 +215,12 @@ __device__ __forceinline__ void down_rest(const Args& a, int tx, int ty, int b, 
         }
       }
     }
-    __builtin_amdgcn_wave_barrier();  // the level's reads stay before its writes
+    wave_lds_order();  // the level's reads stay before its writes
     if (active) {
 #pragma unroll
       for (int i = 0; i < N; ++i) s1[i][ly * oedge + lx] = r[i];
     }
-    __builtin_amdgcn_wave_barrier();  // and its writes before the next level's reads
+    wave_lds_order();  // and its writes before the next level's reads
     edge = oedge;
   }
 }
@@ -383,7 +394,7 @@ __device__ __forceinline__ void down_slots(const DownFusedArgs<T0, N>& a, const 
       }
     }
     if (last)
-      __builtin_amdgcn_wave_barrier();  // the level's reads stay before its writes
+      wave_lds_order();  // the level's reads stay before its writes
     else
       __syncthreads();
     if (active) {
@@ -391,7 +402,7 @@ __device__ __forceinline__ void down_slots(const DownFusedArgs<T0, N>& a, const 
       for (int s = 0; s < S; ++s) s1[s][ly * oedge + lx] = r[s];
     }
     if (last)
-      __builtin_amdgcn_wave_barrier();  // and its writes before the next level's reads
+      wave_lds_order();  // and its writes before the next level's reads
     else
       __syncthreads();
     edge = oedge;

@@ -248,9 +248,9 @@ extern "C" int hmServerAddRig(hmServer* sv, hmStitchPano* pano, int* rig_id) {
   if (e != hipSuccess) return e;
   const size_t bytes = pano->device_bytes();
   std::lock_guard<std::mutex> lk(sv->mu);
-  for (const auto& kv : sv->rigs)
-    if (kv.second.pano == pano) return hipErrorInvalidValue;  // already served
   if (sv->used + bytes > sv->budget) return hipErrorOutOfMemory;
+  bool idle = false;
+  if (!pano->served.compare_exchange_strong(idle, true)) return hipErrorInvalidValue;  // already served
   Rig r;
   r.pano = pano;
   r.bytes = bytes;
@@ -270,6 +270,7 @@ extern "C" int hmServerRemoveRig(hmServer* sv, int rig_id) {
   sv->idle_cv.wait(lk, [&] { return it->second.queued == 0 && !it->second.busy; });
   sv->drop_graphs(it->second);
   sv->used -= it->second.bytes;
+  it->second.pano->served.store(false);
   sv->rigs.erase(it);
   return hipSuccess;
 }
@@ -335,7 +336,10 @@ extern "C" void hmServerDestroy(hmServer* sv) {
   }
   sv->cv.notify_all();
   if (sv->th.joinable()) sv->th.join();
-  for (auto& kv : sv->rigs) sv->drop_graphs(kv.second);
+  for (auto& kv : sv->rigs) {
+    sv->drop_graphs(kv.second);
+    kv.second.pano->served.store(false);
+  }
   for (auto& sl : sv->slots) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);

@@ -239,3 +239,41 @@ def small_rig(n: int = 2, seed: int = 7, src=(160, 64), canvas=(420, 72), overla
     2-image minimized blend, which needs x2 >= 128 and w1 + 128 <= canvas)."""
     return cylinder_rig(n, src[0], src[1], canvas[0], canvas[1], overlap=overlap, focal=src[0] * 0.8,
                         pitch_deg=pitch_deg, seed=seed)
+
+
+# The mixed served load: 32 rigs drawn from 5 seeded configs (2 to 8 images,
+# 1920x1080 to 5312x2988 inputs, canvases 2000 to 14000 wide, levels 0/4/6).
+# Per config: name, pano class ("2", "3" or "N"), cylinder geometry (n, src_w,
+# src_h, canvas_w, canvas_h, overlap, focal, seed; None = the 8500x3300 rig),
+# levels, rigs in the mix, distinct input sets per rig.
+SERVED_CONFIGS = [
+    ("2img_5312x2988_8500x3300_L6", "2", None, 6, 6, 2),
+    ("2img_1920x1080_2000x700_L4", "2", (2, 1920, 1080, 2000, 700, 0.25, 1300.0, 5), 4, 8, 2),
+    ("3img_2704x1520_6000x2000_L4", "3", (3, 2704, 1520, 6000, 2000, 0.25, 1500.0, 77), 4, 6, 2),
+    ("4img_1920x1080_4000x1500_L0", "N", (4, 1920, 1080, 4000, 1500, 0.25, 1100.0, 6), 0, 8, 2),
+    ("8img_3840x2160_14000x2600_L6", "N", (8, 3840, 2160, 14000, 2600, 0.25, 3500.0, 1234), 6, 4, 1),
+]
+
+# The 16-frame mixed batch: 6 frames of 2 x 5312x2988 -> 8500x3300 L=6, 6 of
+# 3 x 2704x1520 -> 6000x2000 L=4 and 4 of 4 x 1920x1080 -> 4000x1500 L=0.
+BATCH_MIX = [("2img_5312x2988_8500x3300_L6", 6), ("3img_2704x1520_6000x2000_L4", 6),
+             ("4img_1920x1080_4000x1500_L0", 4)]
+
+
+def served_config(name: str):
+    """(kind, levels, rigs in the mix, input sets) of a served-load config."""
+    c = next(c for c in SERVED_CONFIGS if c[0] == name)
+    return c[1], c[3], c[4], c[5]
+
+
+def served_rig(name: str) -> Rig:
+    """The rig geometry of a served-load config."""
+    geo = next(c for c in SERVED_CONFIGS if c[0] == name)[2]
+    if geo is None:
+        return config_rig("cfg1")
+    return cylinder_rig(geo[0], geo[1], geo[2], geo[3], geo[4], overlap=geo[5], focal=geo[6], seed=geo[7])
+
+
+def served_frames(rig: Rig, input_set: int) -> List[np.ndarray]:
+    """Input set k of a served rig: one uint8 BGRA frame [1, H, W, 4] per camera."""
+    return [make_frame(rig.src_h, rig.src_w, 1000 * input_set + 17 * i)[None] for i in range(rig.n)]

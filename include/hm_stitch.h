@@ -223,6 +223,9 @@ typedef struct hmTuning {
   int fuse_down;   /* 1: the 2-image blend-input remap runs inside the first down-pass launch */
 } hmTuning;
 int hmStitchPanoGetTuning(const hmStitchPano* pano, hmTuning* tuning);
+/* SetTuning and Autotune return hipErrorNotSupported while a server holds the
+ * object (between hmServerAddRig and hmServerRemoveRig / hmServerDestroy):
+ * tune before adding the rig. */
 int hmStitchPanoSetTuning(hmStitchPano* pano, const hmTuning* tuning);
 /* The project's autotuner: times every setting of the knobs that matter for
  * this object on the caller's frame (inputs as for the class's process;
@@ -319,7 +322,9 @@ typedef struct hmServerStats {
 
 int hmServerCreate(int n_streams, size_t device_budget_bytes, int use_graphs, hmServer** out);
 /* Prepares the pano (hmStitchPanoPrepare) and admits it if its device bytes
- * fit the remaining budget (else hipErrorOutOfMemory, not admitted). */
+ * fit the remaining budget (else hipErrorOutOfMemory, not admitted).  A pano
+ * is held by one server at a time (else hipErrorInvalidValue); while held, its
+ * tuning is fixed and it cannot join an hmBatch (hipErrorNotSupported). */
 int hmServerAddRig(hmServer* server, hmStitchPano* pano, int* rig_id);
 /* Waits for the rig's requests, then stops serving it (the pano survives). */
 int hmServerRemoveRig(hmServer* server, int rig_id);

@@ -638,6 +638,7 @@ int hmo_blend3(const float* img1, const float* img2, const float* img3, const fl
 int hmo_pad_replicate(const uint8_t* seam, int sw, int sh, uint8_t* out, int cw, int ch) {
   /* canvasManager.cpp:69-70 asserts the seam is no larger than the canvas. */
   if (sw > cw || sh > ch || sw < 1 || sh < 1) return -1;
+  HMO_PARALLEL_FOR
   for (int y = 0; y < ch; ++y)
     for (int x = 0; x < cw; ++x) out[(size_t)y * cw + x] = seam[(size_t)IMIN(y, sh - 1) * sw + IMIN(x, sw - 1)];
   return 0;
@@ -704,6 +705,7 @@ int hmo_pano2_process(const hmo_pano2_cfg* cfg, const uint8_t* in1, const uint8_
   const int bw = minimize ? blend_width : cw;
   const int bh = ch;
   float* mask = (float*)malloc((size_t)bw * bh * sizeof(float));
+  HMO_PARALLEL_FOR
   for (int y = 0; y < bh; ++y)
     for (int x = 0; x < bw; ++x) mask[(size_t)y * bw + x] = (float)padded[(size_t)y * cw + bx0 + x];
   free(padded);

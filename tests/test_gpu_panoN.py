@@ -156,6 +156,32 @@ def test_panoN_short_seam_bit_exact(hm, torch_gpu, oracle, n, levels, minimize):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("n", [6, 8])
+@pytest.mark.parametrize("levels", [3, 6])
+@pytest.mark.parametrize("minimize", [True, False])
+def test_panoN_more_than_four_images_per_tile_bit_exact(hm, torch_gpu, oracle, n, levels, minimize):
+    """Every image covers the whole canvas (identity maps, all at (0, 0)), so every tile of the
+    down and up passes meets all n > 4 images: the down pass carries them in several slot chunks
+    per tile (the chunks before the last included) and the up pass takes its per-pixel path.
+    Label bands plus a wavy boundary in the seam, and transparent holes that differ per image."""
+    w, h = 200, 120
+    rng = np.random.default_rng(n * 10 + levels)
+    ys, xs = np.mgrid[0:h, 0:w]
+    seam = np.clip(((xs + 9 * np.sin(ys / 7.0)) * n) // w, 0, n - 1).astype(np.uint8)
+    rig = IdRig([(w, h)] * n, [(0, 0)] * n, seam)
+    imgs = []
+    for i in range(n):
+        im = rng.integers(0, 256, (1, h, w, 4)).astype(np.uint8)
+        im[..., 3] = 255
+        y0, x0 = rng.integers(0, h // 2), rng.integers(0, w // 2)
+        im[:, y0: y0 + h // 4, x0: x0 + w // 4, 3] = 0
+        imgs.append(im)
+    assert all((mx != 65535).all() for mx in rig.map_x)  # each image valid on every tile
+    _, got = run_panoN(hm, torch_gpu, rig, imgs, levels, minimize)
+    want = oracle.panoN(rig, imgs, levels, minimize=minimize)
+    np.testing.assert_array_equal(got, want)
+
+
 def test_panoN_seam_larger_than_canvas_is_refused(hm, torch_gpu):
     import copy
 

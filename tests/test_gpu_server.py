@@ -157,3 +157,33 @@ def test_server_request_errors(hm, torch_gpu, oracle):
     assert res[1]["status"] != 0 and res[2]["status"] == 0
     np.testing.assert_array_equal(good.cpu().numpy(), oracle.panoN(rig, fr, 2))
     server.close()
+
+
+def test_served_pano_refuses_tuning_and_batching(hm, torch_gpu):
+    """While a server holds a pano its scheduler thread launches it and the captured graphs
+    carry its launch shapes: tuning, autotuning, a batch over it and a second server are
+    refused until the rig is removed."""
+    import hm_synth
+
+    torch = torch_gpu
+    rig = hm_synth.small_rig(seed=44)
+    pano = hm.StitchPano(1, 4, rig)
+    dev = to_dev(torch, frames_for(rig, 1, 2))
+    canvas = pano.alloc_canvas()
+    server = hm.Server(n_streams=2)
+    rid = server.add_rig(pano)
+    other = hm.Server(n_streams=1)
+    with pytest.raises(hm.HmError):
+        other.add_rig(pano)
+    with pytest.raises(hm.HmError):
+        pano.set_tuning(4, 3, 2)
+    with pytest.raises(hm.HmError):
+        pano.autotune(dev, canvas, frames=1)
+    with pytest.raises(hm.HmError):
+        hm.Batch([(pano, dev, canvas)])
+    server.remove_rig(rid)
+    pano.set_tuning(4, 3, 2)
+    assert other.add_rig(pano) >= 0
+    other.close()
+    pano.set_tuning(4, 2, 1)
+    server.close()
