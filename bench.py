@@ -29,7 +29,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(rig, levels, pano_levels, tail_base=2):
+def algorithmic_bytes(rig, levels, pano_levels, tail_base=2, fuse_up=0):
     """Minimum HBM bytes per frame of each kernel label (what it must read + write).
 
     Maps are read as one packed 32-bit word per pixel; each source image is
@@ -96,9 +96,20 @@ def algorithmic_bytes(rig, levels, pano_levels, tail_base=2):
         quad_levels = range(1, tail_base)
     else:
         quad_levels = range(1, pano_levels)
+    fused = fuse_up and pano_levels >= 2 and 1 in quad_levels
+    if fused:  # level 1 runs inside the level-0 launch
+        quad_levels = [k for k in quad_levels if k != 1]
     quad = sum(level_bytes(k) for k in quad_levels)
     if quad:
         out["up_quad"] = quad
+    if fused:
+        # levels 1 and 0 in one launch: both level-0 images, the uint8 mask and the canvas;
+        # level 1's images and mask; level 2's images and R (taps); R_1 stays in LDS
+        b = 2 * A[0] * 4 + A[0] * 1 + A[0] * 4 + 2 * A[1] * 16 + A[1] * 4
+        if pano_levels > 2:
+            b += 2 * A[2] * 16 + A[2] * 16
+        out["up_l10"] = b
+        return out
     # level 0: both level-0 images, the uint8 mask, level-1 taps and R_1 in, canvas out
     out["up_level0"] = 2 * A[0] * 4 + A[0] * 1 + A[0] * 4
     if pano_levels > 1:
@@ -581,7 +592,8 @@ def main():
         rep = pano.timing_report(reset=True)
         pano.set_timing(False)
         kernels = {k: {"avg_ms": v[0] / v[1], "per_frame_ms": v[0] / kt, "launches": v[1]} for k, v in rep.items()}
-        alg = algorithmic_bytes(rig, levels, len(pano.levels()) if levels else 0, tuning["tail_base"])
+        alg = algorithmic_bytes(rig, levels, len(pano.levels()) if levels else 0, tuning["tail_base"],
+                                tuning.get("fuse_up", 0))
         for k, b in alg.items():
             if k in kernels:
                 per_frame = kernels[k]["launches"] / kt

@@ -219,12 +219,14 @@ hipError_t record_call(const hmBatchCall& c, LaunchRecorder& rec) {
   HM_RETURN_IF_ERROR(p->prepare());
   HM_RETURN_IF_ERROR(hipDeviceSynchronize());
   // Launch shapes the batched kernel carries: remap tiles of 4 pixels x 1 or 2
-  // rows, no fused remap+down, one workgroup per canvas tile.  The output is
+  // rows, no fused remap+down, levels 1 and 0 of the up pass in two launches,
+  // one workgroup per canvas tile.  The output is
   // the same for every tuning.
   const hmTuning saved = p->tuning;
   p->tuning.remap_px = 4;
   if (p->tuning.remap_rows != 2) p->tuning.remap_rows = 1;
   p->tuning.fuse_down = 0;
+  p->tuning.fuse_up = 0;
   p->tuning.canvas_grid = 0;
   p->apply_tuning();
   g_recorder = &rec;

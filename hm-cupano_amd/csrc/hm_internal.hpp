@@ -127,6 +127,11 @@ struct UpQuadLaunch {
   size_t wn_plane = 0;                       // floats per plane
 };
 hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch& u, int batch, hipStream_t s);
+// Two images: levels 1 and 0 in one launch (hmdev::up_l10_kernel).  u is the
+// level-0 launch; mask1 the level-1 mask; g2/r2 level 2 (nullptr when level 1
+// is the coarsest).
+hipError_t launch_up_l10(int c, bool g0_u8, const UpQuadLaunch& u, const float* mask1, const float* const* g2,
+                         const float* r2, int w2, int h2, int batch, hipStream_t s);
 
 // The tail of the up pass for two images: levels kTailBase .. levels-1 in
 // one launch (hmdev::up_tail_kernel), giving R at kTailBase.
@@ -175,6 +180,7 @@ class Pyramid {
   // debug mode, and for one or two images) the pyramid is dense.
   hipError_t set_supports(const hmdev::ImgRect* s0);
   int tail_base = kTailBase;  // two images: levels tail_base.. in one launch (0: never)
+  bool fuse_up = true;        // two images: levels 1 and 0 in one launch
   hmdev::ImgRect S[hmdev::kMaxLevels][hmdev::kMaxImages] = {};  // supports
   hmdev::ImgRect Z[hmdev::kMaxLevels][hmdev::kMaxImages] = {};  // valid regions
 

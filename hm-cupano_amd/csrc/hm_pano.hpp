@@ -29,7 +29,7 @@ struct hmStitchPano {
   hm::LaunchTimer* timer_or_null() { return timer.enabled() ? &timer : nullptr; }
 
   // Launch-shape knobs (hmTuning); every setting computes the same output.
-  hmTuning tuning{4, 2, 1, 0, 1, 0, 0};
+  hmTuning tuning{4, 2, 1, 0, 1, 0, 0, 1};
   // Set while a server holds the object (hmServerAddRig .. RemoveRig/Destroy):
   // its scheduler thread then owns the launches and the captured graphs bake
   // in the launch shapes, so tuning and batch recording are refused.
@@ -38,7 +38,11 @@ struct hmStitchPano {
   virtual bool tunes_remap_px() const { return false; }
   bool tunes_tail() const { return !hard_seam && n_images == 2 && num_levels >= 4; }
   bool tunes_overlap() const { return side != nullptr; }
-  void apply_tuning() { pyr.tail_base = tuning.tail_base; }
+  bool tunes_fuse_up() const { return !hard_seam && n_images == 2 && num_levels >= 2; }
+  void apply_tuning() {
+    pyr.tail_base = tuning.tail_base;
+    pyr.fuse_up = tuning.fuse_up != 0;
+  }
 
   // Work that does not feed the blend (the canvas outside the written-back
   // region) runs on a second stream, forked from and joined back into the

@@ -120,7 +120,7 @@ static bool tuning_ok(const hmTuning& t) {
          (t.remap_rows == 1 || t.remap_rows == 2 || t.remap_rows == 4) &&
          (t.canvas_grid == 0 || t.canvas_grid == 256 || t.canvas_grid == 512 || t.canvas_grid == 1024 ||
           t.canvas_grid == 2048) &&
-         (t.fuse_down == 0 || t.fuse_down == 1);
+         (t.fuse_down == 0 || t.fuse_down == 1) && (t.fuse_up == 0 || t.fuse_up == 1);
 }
 
 extern "C" int hmStitchPanoSetTuning(hmStitchPano* pano, const hmTuning* tuning) {
@@ -149,6 +149,7 @@ extern "C" int hmStitchPanoAutotune(hmStitchPano* pano, const hmSurface* inputs,
   std::vector<int> gds = remap && pano->tunes_overlap() ? std::vector<int>{0, 256, 512, 1024, 2048}
                                                         : std::vector<int>{start.canvas_grid};
   std::vector<int> fds = remap ? std::vector<int>{0, 1} : std::vector<int>{start.fuse_down};
+  std::vector<int> fus = pano->tunes_fuse_up() ? std::vector<int>{0, 1} : std::vector<int>{start.fuse_up};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HM_RETURN_IF_ERROR(hipEventCreate(&e0));
   hipError_t err = hipEventCreate(&e1);
@@ -182,14 +183,16 @@ extern "C" int hmStitchPanoAutotune(hmStitchPano* pano, const hmSurface* inputs,
   for (int px : pxs)
     for (int ry : rys)
       for (int xc : xcs)
-        for (int fd : fds) trial(hmTuning{px, start.tail_base, start.overlap, xc, ry, start.canvas_grid, fd});
+        for (int fd : fds)
+          trial(hmTuning{px, start.tail_base, start.overlap, xc, ry, start.canvas_grid, fd, start.fuse_up});
   const hmTuning shape = chosen;
   for (int tb : tbs)
     for (int ov : ovs)
-      for (int gd : gds) {
-        if (!ov && gd) continue;  // the grid knob applies to the second stream only
-        trial(hmTuning{shape.remap_px, tb, ov, shape.remap_xcd, shape.remap_rows, gd, shape.fuse_down});
-      }
+      for (int gd : gds)
+        for (int fu : fus) {
+          if (!ov && gd) continue;  // the grid knob applies to the second stream only
+          trial(hmTuning{shape.remap_px, tb, ov, shape.remap_xcd, shape.remap_rows, gd, shape.fuse_down, fu});
+        }
   // Confirmation: the four best settings, timed again three times each over
   // four times as many frames, interleaved; the best of the minimum wins (one
   // short trial is noisy next to differences of a few per cent).

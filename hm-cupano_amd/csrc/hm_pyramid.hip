@@ -396,6 +396,70 @@ hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch
   });
 }
 
+hipError_t launch_up_l10(int c, bool g0_u8, const UpQuadLaunch& u, const float* mask1, const float* const* g2,
+                         const float* r2, int w2, int h2, int batch, hipStream_t s) {
+  if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
+  if (u.mask0_mode != kMask0F32 && u.mask0_mode != kMask0U8) return hipErrorInvalidValue;
+  if (g_recorder) return hipErrorNotSupported;  // the batched pass does not carry this launch
+  auto go = [&](auto T0v, auto Cv, auto M) -> hipError_t {
+    using T0 = decltype(T0v);
+    constexpr int kC = decltype(Cv)::value, kM = decltype(M)::value;
+    UpL10Args<T0> A;
+    UpQuadArgs<T0, 2>& a = A.l0;
+    for (int i = 0; i < 2; ++i) {
+      a.g0[i] = static_cast<const T0*>(u.g0[i]);
+      a.g1[i] = u.g1[i];
+      A.g2[i] = g2 ? g2[i] : nullptr;
+    }
+    a.mask0 = u.mask0;
+    a.r1 = nullptr;
+    a.w0 = u.w0;
+    a.h0 = u.h0;
+    a.w1 = u.w1;
+    a.h1 = u.h1;
+    a.f32_out = u.f32_out;
+    a.u8_out = u.u8_out;
+    a.u8_pitch = u.u8_pitch;
+    a.u8_w = u.u8_w;
+    a.u8_h = u.u8_h;
+    a.ox = u.ox;
+    a.oy = u.oy;
+    a.win_x0 = u.win_x;
+    a.win_y0 = u.win_y;
+    a.win_x1 = u.win_x + u.win_w - 1;
+    a.win_y1 = u.win_y + u.win_h - 1;
+    a.qx0 = a.win_x0 >> 1;
+    a.qy0 = a.win_y0 >> 1;
+    a.nqx = (a.win_x1 >> 1) - a.qx0 + 1;
+    a.nqy = (a.win_y1 >> 1) - a.qy0 + 1;
+    a.tiles_x = (a.nqx + 63) / 64;
+    a.tiles_y = (a.nqy + 3) / 4;
+    for (int i = 0; i < 2; ++i) a.z[i] = u.z[i];
+    a.wn = nullptr;
+    a.wn_plane = 0;
+    A.mask1 = mask1;
+    A.r2 = r2;
+    A.w2 = w2;
+    A.h2 = h2;
+    const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+    if (r2)
+      hipLaunchKernelGGL((up_l10_kernel<T0, kC, kM, true>), grid, dim3(64, 4, 1), 0, s, A);
+    else
+      hipLaunchKernelGGL((up_l10_kernel<T0, kC, kM, false>), grid, dim3(64, 4, 1), 0, s, A);
+    return hipGetLastError();
+  };
+  using std::integral_constant;
+  auto by_mode = [&](auto T0v, auto Cv) -> hipError_t {
+    return u.mask0_mode == kMask0U8 ? go(T0v, Cv, integral_constant<int, kMask0U8>{})
+                                    : go(T0v, Cv, integral_constant<int, kMask0F32>{});
+  };
+  auto by_c = [&](auto T0v) -> hipError_t {
+    return c == 4 ? by_mode(T0v, integral_constant<int, 4>{}) : by_mode(T0v, integral_constant<int, 3>{});
+  };
+  if (c != 3 && c != 4) return hipErrorInvalidValue;
+  return g0_u8 ? by_c(uint8_t{}) : by_c(float{});
+}
+
 // ---------------------------------------------------------------------------
 // launch timing
 // ---------------------------------------------------------------------------
@@ -655,8 +719,11 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
     top = tb - 1;
   }
   // The other levels one launch each, coarsest first, one thread per 2x2
-  // quad; level 0 goes straight to the output.
+  // quad; level 0 goes straight to the output.  The 2-image blend: levels 1
+  // and 0 in one launch (fuse_up) when level 1 is not the tail's.
+  const bool l10 = kind == kBlend2 && fuse_up && top >= 1 && levels >= 2;
   for (int k = top; k >= 0; --k) {
+    if (l10 && k == 1) continue;  // computed inside the level-0 launch
     const bool has_next = k + 1 < levels;
     UpQuadLaunch u;
     for (int i = 0; i < n; ++i) {
@@ -694,6 +761,13 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
     u.win_h = fy1[k] - fy0[k] + 1;
     for (int i = 0; i < n; ++i) u.z[i] = Z[k][i];
     if (timer) timer->begin(s);
+    if (l10 && k == 0) {
+      const float* g2[2] = {levels > 2 ? g[0][2].as<float>() : nullptr, levels > 2 ? g[1][2].as<float>() : nullptr};
+      HM_RETURN_IF_ERROR(launch_up_l10(c, g0_u8, u, mask[1].as<float>(), g2, levels > 2 ? r[2].as<float>() : nullptr,
+                                       levels > 2 ? W[2] : 0, levels > 2 ? H[2] : 0, batch, s));
+      if (timer) timer->end(s, "up_l10");
+      continue;
+    }
     HM_RETURN_IF_ERROR(launch_up_quad(c, n, kind, k == 0 && g0_u8, u, batch, s));
     if (timer) timer->end(s, k == 0 ? "up_level0" : "up_quad");
   }

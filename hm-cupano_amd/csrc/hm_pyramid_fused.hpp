@@ -1347,4 +1347,176 @@ __global__ void __launch_bounds__(256) up_quad_kernel(UpQuadArgs<T0, N> a) {
   up_quad_body<T0, C, N, KIND, MODE, NEXT>(kernel_arg<UpQuadArgs<T0, N>>(), hw_blk(), lds);
 }
 
+// ---------------------------------------------------------------------------
+// two images: levels 1 and 0 of the up pass in one launch
+// ---------------------------------------------------------------------------
+// A level-0 workgroup (64 x 4 quads) reads R1 and G1 only on the
+// (64 + 1) x (4 + 1) level-1 footprint of its tile.  Here it computes R1 on
+// that footprint itself -- the Laplacians of level 1, their blend and the
+// reconstruction from level 2, with level 2's (32 + 2) x (2 + 2) footprint
+// staged in LDS -- and keeps it in LDS, so R1 never goes to memory and the
+// level-1 launch disappears.  Every R1 value is the level-1 launch's (the same
+// functions on the same taps: the footprint positions are clamped to the level
+// as that launch's staged taps are).  Footprint positions outside the level-1
+// window use level-2 data outside the level-2 window (not written this
+// frame); no level-0 pixel of the window reads them.
+constexpr int kL2TapW = 32 + 2, kL2TapH = 2 + 2, kL2Taps = kL2TapW * kL2TapH;
+
+template <typename T0>
+struct UpL10Args {
+  UpQuadArgs<T0, 2> l0;  // level 0 (g1: level 1's Gaussians; r1 is not read)
+  const float* mask1;    // level-1 mask
+  const float* g2[2];    // level-2 Gaussians (NEXT1)
+  const float* r2;       // level-2 reconstruction (NEXT1)
+  int w2, h2;
+};
+
+__host__ __device__ constexpr int up_l10_lds_float4() { return 3 * kQuadTaps + 3 * kL2Taps; }
+
+// NEXT1: level 1 has a coarser level (false: levels == 2, level 1 is blended only).
+// Runs on 64 x 4 threads; lds: up_l10_lds_float4() float4.
+template <typename T0, int C, int MODE, bool NEXT1>
+__device__ __forceinline__ void up_l10_body(const UpL10Args<T0>& A, Blk blk, float4* lds) {
+  const UpQuadArgs<T0, 2>& a = A.l0;
+  const int t = blk.x;
+  const int tqx = a.qx0 + (t % a.tiles_x) * 64, tqy = a.qy0 + (t / a.tiles_x) * 4;
+  const int qx = tqx + (int)threadIdx.x;
+  const int qy = tqy + (int)threadIdx.y;
+  const int b = blk.z;
+  const bool active = qx < a.qx0 + a.nqx && qy < a.qy0 + a.nqy;
+  const int tid = (int)threadIdx.y * 64 + (int)threadIdx.x;
+  constexpr int kRounds = (kQuadTaps + kWgThreads - 1) / kWgThreads;
+  float4* lr1 = lds;                  // R1 on the level-1 footprint
+  float4* lg1 = lds + kQuadTaps;      // G1 of image i at lg1 + i * kQuadTaps
+  float4* l2 = lds + 3 * kQuadTaps;   // level 2: R2, then G2 of each image
+  const int bx2 = tqx >> 1, by2 = tqy >> 1;
+  // Every load of the workgroup first: level 2's footprint, level 1's
+  // footprint (both Gaussians and the mask), the quad's level-0 pixels and mask.
+  float4 s2[3];
+  if constexpr (NEXT1) {
+    const int p = min(tid, kL2Taps - 1);
+    const int px = min(bx2 + p % kL2TapW, A.w2 - 1), py = min(by2 + p / kL2TapW, A.h2 - 1);
+    const size_t tap = ((size_t)b * A.h2 + py) * A.w2 + px;
+    s2[0] = ld_px<C>(A.r2, tap);
+    s2[1] = ld_px<C>(A.g2[0], tap);
+    s2[2] = ld_px<C>(A.g2[1], tap);
+  }
+  float4 s1[kRounds][2];
+  float m1[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int p = min(tid + r * kWgThreads, kQuadTaps - 1);
+    const int px = min(tqx + p % kQuadTapW, a.w1 - 1), py = min(tqy + p / kQuadTapW, a.h1 - 1);
+    const size_t tap = ((size_t)b * a.h1 + py) * a.w1 + px;
+    s1[r][0] = ld_px<C>(a.g1[0], tap);
+    s1[r][1] = ld_px<C>(a.g1[1], tap);
+    m1[r] = A.mask1[(size_t)py * a.w1 + px];
+  }
+  float4 g[4][2];
+  float mk[4];
+  bool in[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+    in[e] = x >= a.win_x0 && x <= a.win_x1 && y >= a.win_y0 && y <= a.win_y1;
+    const int cx = in[e] ? x : a.win_x0, cy = in[e] ? y : a.win_y0;
+    const size_t pix = ((size_t)b * a.h0 + cy) * a.w0 + cx;
+    g[e][0] = ld_px<C>(a.g0[0], pix);
+    g[e][1] = ld_px<C>(a.g0[1], pix);
+    const size_t mp = (size_t)cy * a.w0 + cx;
+    if constexpr (MODE == kMask0U8)
+      mk[e] = (float)static_cast<const uint8_t*>(a.mask0)[mp];
+    else
+      mk[e] = static_cast<const float*>(a.mask0)[mp];
+  }
+  if constexpr (NEXT1) {
+    if (tid < kL2Taps) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) l2[k * kL2Taps + tid] = s2[k];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int p = tid + r * kWgThreads;
+    if (p < kQuadTaps) {
+      lg1[p] = s1[r][0];
+      lg1[kQuadTaps + p] = s1[r][1];
+    }
+  }
+  __syncthreads();
+  // R1 on the footprint (the level-1 launch's quad arithmetic at each position)
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int p = tid + r * kWgThreads;
+    if (p >= kQuadTaps) continue;
+    const int x1 = min(tqx + p % kQuadTapW, a.w1 - 1), y1 = min(tqy + p / kQuadTapW, a.h1 - 1);
+    const int e1 = (x1 & 1) | ((y1 & 1) << 1);
+    float4 Lv[2] = {s1[r][0], s1[r][1]};
+    float4 rt[4];
+    if constexpr (NEXT1) {
+      const int t00 = ((y1 >> 1) - by2) * kL2TapW + ((x1 >> 1) - bx2);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const float4* gp = l2 + (i + 1) * kL2Taps + t00;
+        const float4 gt[4] = {gp[0], gp[1], gp[kL2TapW], gp[kL2TapW + 1]};
+        const float4 up = up_quad_lap<C>(e1, gt);
+        Lv[i] = make_float4(Lv[i].x - up.x, Lv[i].y - up.y, Lv[i].z - up.z, Lv[i].w);
+      }
+      const float4* rp = l2 + t00;
+      rt[0] = rp[0];
+      rt[1] = rp[1];
+      rt[2] = rp[kL2TapW];
+      rt[3] = rp[kL2TapW + 1];
+    }
+    const float4 B = blend2_px<C>(Lv[0], Lv[1], m1[r]);
+    float4 R = B;
+    if constexpr (NEXT1) {
+      const float4 up = up_quad_rec<C>(e1, rt);
+      R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+    }
+    lr1[p] = R;
+  }
+  __syncthreads();
+  if (!active) return;  // past the window: nothing to store
+  // Level 0 from the footprint in LDS, as the dense staged path of up_quad_body.
+  const int t0 = (int)threadIdx.y * kQuadTapW + (int)threadIdx.x;  // tap (qx, qy) in the footprint
+  float4 rt[4], gt[2][4];
+#pragma unroll
+  for (int i = 0; i <= 2; ++i) {
+    const float4* tp = lds + i * kQuadTaps + t0;
+    float4* dst = i == 0 ? rt : gt[i - 1];
+    dst[0] = tp[0];
+    dst[1] = tp[1];
+    dst[2] = tp[kQuadTapW];
+    dst[3] = tp[kQuadTapW + 1];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (!in[e]) continue;
+    const int x = 2 * qx + (e & 1), y = 2 * qy + (e >> 1);
+    float4 Lv[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const float4 up = up_quad_lap<C>(e, gt[i]);
+      Lv[i] = make_float4(g[e][i].x - up.x, g[e][i].y - up.y, g[e][i].z - up.z, g[e][i].w);
+    }
+    const float4 B = blend2_px<C>(Lv[0], Lv[1], mk[e]);
+    const float4 up = up_quad_rec<C>(e, rt);
+    const float4 R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+    if (a.f32_out) st_px<C>(a.f32_out, ((size_t)b * a.h0 + y) * a.w0 + x, R);
+    if (a.u8_out) {
+      const int X = x + a.ox, Y = y + a.oy;
+      if (X < a.u8_w && Y < a.u8_h)
+        st_px_u8<C>(a.u8_out + ((size_t)b * a.u8_h + Y) * a.u8_pitch + (size_t)X * C, R);
+    }
+  }
+}
+
+template <typename T0, int C, int MODE, bool NEXT1>
+__global__ void __launch_bounds__(256) up_l10_kernel(UpL10Args<T0> a) {
+  __shared__ float4 lds[up_l10_lds_float4()];
+  (void)a;
+  up_l10_body<T0, C, MODE, NEXT1>(kernel_arg<UpL10Args<T0>>(), hw_blk(), lds);
+}
+
 }  // namespace hmdev

@@ -86,7 +86,7 @@ class hmBatchCall(ctypes.Structure):
 class hmTuning(ctypes.Structure):
     _fields_ = [("remap_px", ctypes.c_int), ("tail_base", ctypes.c_int), ("overlap", ctypes.c_int),
                 ("remap_xcd", ctypes.c_int), ("remap_rows", ctypes.c_int), ("canvas_grid", ctypes.c_int),
-                ("fuse_down", ctypes.c_int)]
+                ("fuse_down", ctypes.c_int), ("fuse_up", ctypes.c_int)]
 
 
 class hmControlFilesInfo(ctypes.Structure):
@@ -340,8 +340,8 @@ class StitchPano:
         return {k: getattr(t, k) for k, _ in hmTuning._fields_}
 
     def set_tuning(self, remap_px: int = 4, tail_base: int = 2, overlap: int = 1, remap_xcd: int = 0,
-                   remap_rows: int = 1, canvas_grid: int = 0, fuse_down: int = 0) -> None:
-        t = hmTuning(remap_px, tail_base, overlap, remap_xcd, remap_rows, canvas_grid, fuse_down)
+                   remap_rows: int = 1, canvas_grid: int = 0, fuse_down: int = 0, fuse_up: int = 1) -> None:
+        t = hmTuning(remap_px, tail_base, overlap, remap_xcd, remap_rows, canvas_grid, fuse_down, fuse_up)
         check(lib().hmStitchPanoSetTuning(self._h, ctypes.byref(t)), "hmStitchPanoSetTuning")
 
     def autotune(self, inputs, canvas, frames: int = 10, fused: bool = True, stream=None):

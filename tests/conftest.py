@@ -83,3 +83,25 @@ def patterned_u8(w, h, idx):
     img[0, ..., 2] = base + (xs + ys) % 23
     img[0, ..., 3] = 255
     return img
+
+
+def scatter_far_entries(rig, seed, rows=0.15, frac=0.3):
+    """A copy of `rig` whose maps hold, on some rows, entries far outside the
+    source (x or y up to 65534, or the 65535 sentinel) among the regular ones:
+    out-of-range pixels of the reference's remap, and 64-entry segments whose
+    spread is too wide for the compact map layout (its 4-byte fallback)."""
+    import copy
+
+    rng = np.random.default_rng(seed)
+    r = copy.copy(rig)
+    r.map_x = [m.copy() for m in rig.map_x]
+    r.map_y = [m.copy() for m in rig.map_y]
+    for mx, my in zip(r.map_x, r.map_y):
+        h, w = mx.shape
+        for y in rng.choice(h, size=max(1, int(rows * h)), replace=False):
+            cols = rng.choice(w, size=max(1, int(frac * w)), replace=False)
+            kind = rng.integers(0, 3, size=cols.size)
+            mx[y, cols[kind == 0]] = rng.integers(1000, 65535, size=int((kind == 0).sum()))
+            my[y, cols[kind == 1]] = rng.integers(1000, 65536, size=int((kind == 1).sum()))
+            mx[y, cols[kind == 2]] = 65535
+    return r

@@ -90,13 +90,15 @@ def assert_same(got, want, what):
 # ---------------------------------------------------------------------------
 # 2 x 5312x2988 -> 8500x3300, levels 6, launch shapes
 # ---------------------------------------------------------------------------
-# (remap px, tail base, overlap, XCD bands, rows, canvas grid, fused down): what the
-# autotuner picks on this workload (first two), the fused remap+down launch, the
-# persistent canvas grids (each workgroup walks several tiles), and a shape far
-# from the default.
+# (remap px, tail base, overlap, XCD bands, rows, canvas grid, fused down[, fused
+# levels 1 and 0]): settings the autotuner picks on this workload, the fused
+# up-pass levels after each tail base, the fused remap+down launch, the persistent
+# canvas grids (each workgroup walks several tiles), and a shape far from the default.
 CFG1_TUNINGS = [
-    (4, 3, 2, 0, 2, 0, 0),
-    (4, 3, 2, 1, 2, 0, 0),
+    (4, 3, 2, 0, 2, 0, 0, 0),
+    (4, 3, 2, 1, 2, 0, 0, 1),
+    (4, 2, 2, 1, 2, 0, 0, 1),
+    (4, 0, 2, 0, 2, 0, 0, 1),
     (4, 2, 1, 1, 1, 0, 1),
     (4, 3, 2, 1, 2, 256, 0),
     (8, 0, 0, 0, 4, 2048, 0),

@@ -9,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import frames_for
+from conftest import frames_for, scatter_far_entries
 
 pytestmark = pytest.mark.gpu
 
@@ -178,6 +178,20 @@ def test_pano2_short_seam_bit_exact(hm, torch_gpu, oracle, levels, minimize):
     frames = frames_for(rig, 1, 12)
     _, got = run_pano(hm, torch_gpu, short, frames, levels, minimize)
     want = oracle.pano2(short, frames[0], frames[1], levels, minimize=minimize)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("levels", [0, 3])
+@pytest.mark.parametrize("minimize", [True, False])
+def test_pano2_far_map_entries_bit_exact(hm, torch_gpu, oracle, levels, minimize):
+    """Map entries far outside the source (and sentinels) scattered over some rows: the
+    out-of-range rules of the remap, and map segments kept in 4-byte form."""
+    import hm_synth
+
+    rig = scatter_far_entries(hm_synth.small_rig(seed=13), seed=5)
+    frames = frames_for(rig, 1, 14)
+    _, got = run_pano(hm, torch_gpu, rig, frames, levels, minimize)
+    want = oracle.pano2(rig, frames[0], frames[1], levels, minimize=minimize)
     np.testing.assert_array_equal(got, want)
 
 

@@ -5,7 +5,7 @@ pixel types, hard and soft seams; bit-exact throughout."""
 import numpy as np
 import pytest
 
-from conftest import IdRig, const_u8, frames_for
+from conftest import IdRig, const_u8, frames_for, scatter_far_entries
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +49,16 @@ def test_pano3_small_rig_bit_exact(hm, torch_gpu, oracle, levels, fused, channel
     _, got, prior = run_pano3(hm, torch_gpu, rig, frames, levels, fused, channels)
     want = oracle.pano3(rig, frames, levels, fused=fused, canvas=prior)
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("levels", [0, 3])
+@pytest.mark.parametrize("fused", [True, False])
+def test_pano3_far_map_entries_bit_exact(hm, torch_gpu, oracle, levels, fused):
+    """Map entries far outside the source (and sentinels) scattered over some rows."""
+    rig = scatter_far_entries(small3(17), seed=9)
+    frames = frames_for(rig, 1, 19)
+    _, got, prior = run_pano3(hm, torch_gpu, rig, frames, levels, fused)
+    np.testing.assert_array_equal(got, oracle.pano3(rig, frames, levels, fused=fused, canvas=prior))
 
 
 def test_pano3_batch_pitched(hm, torch_gpu, oracle):

@@ -6,7 +6,7 @@ fused multiply-add on either side."""
 import numpy as np
 import pytest
 
-from conftest import IdRig, const_u8, frames_for, patterned_u8
+from conftest import IdRig, const_u8, frames_for, patterned_u8, scatter_far_entries
 
 pytestmark = pytest.mark.gpu
 
@@ -179,6 +179,17 @@ def test_panoN_more_than_four_images_per_tile_bit_exact(hm, torch_gpu, oracle, n
     assert all((mx != 65535).all() for mx in rig.map_x)  # each image valid on every tile
     _, got = run_panoN(hm, torch_gpu, rig, imgs, levels, minimize)
     want = oracle.panoN(rig, imgs, levels, minimize=minimize)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [3, 5])
+@pytest.mark.parametrize("levels", [0, 3])
+def test_panoN_far_map_entries_bit_exact(hm, torch_gpu, oracle, n, levels):
+    """Map entries far outside the source (and sentinels) scattered over some rows."""
+    rig = scatter_far_entries(small_n_rig(n, 70 + n), seed=n)
+    frames = frames_for(rig, 1, 31 + n)
+    _, got = run_panoN(hm, torch_gpu, rig, frames, levels)
+    want = oracle.panoN(rig, frames, levels)
     np.testing.assert_array_equal(got, want)
 
 

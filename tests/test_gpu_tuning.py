@@ -25,17 +25,18 @@ def test_pano2_every_tuning_is_bit_exact(hm, torch_gpu, oracle, levels):
     pano = hm.StitchPano(1, levels, rig)
     dev = to_dev(torch, frames)
     shapes = itertools.product((2, 4, 8), (0, 1), (1, 2, 4), (0, 1))  # remap pixels, XCD order, rows, fused down
-    rest = itertools.product((0, 1, 2, 3), (0, 1, 2), (0, 256, 2048), (0, 1))  # tail, overlap, canvas grid, fused
-    settings = [(px, 2, 1, xc, ry, 0, fd) for px, xc, ry, fd in shapes] + \
-        [(4, tb, ov, 1, 2, gd, fd) for tb, ov, gd, fd in rest]
-    for px, tb, ov, xc, ry, gd, fd in settings:
-        pano.set_tuning(px, tb, ov, xc, ry, gd, fd)
+    # tail, overlap, canvas grid, fused down, levels 1 and 0 fused
+    rest = itertools.product((0, 1, 2, 3), (0, 1, 2), (0, 256, 2048), (0, 1), (0, 1))
+    settings = [(px, 2, 1, xc, ry, 0, fd, 1) for px, xc, ry, fd in shapes] + \
+        [(4, tb, ov, 1, 2, gd, fd, fu) for tb, ov, gd, fd, fu in rest]
+    for px, tb, ov, xc, ry, gd, fd, fu in settings:
+        pano.set_tuning(px, tb, ov, xc, ry, gd, fd, fu)
         canvas = torch.full(want.shape, 33, dtype=torch.uint8, device="cuda")
         assert pano.process(dev[0], dev[1], canvas) == 0
         torch.cuda.synchronize()
         np.testing.assert_array_equal(canvas.cpu().numpy(), want,
                                       err_msg=f"px={px} tail={tb} overlap={ov} xcd={xc} rows={ry} grid={gd} "
-                                              f"fuse={fd}")
+                                              f"fuse={fd} fuse_up={fu}")
 
 
 def test_multi_image_overlap_knob_is_bit_exact(hm, torch_gpu, oracle):
@@ -96,4 +97,6 @@ def test_invalid_tuning_is_refused(hm, torch_gpu):
     with pytest.raises(hm.HmError):
         pano.set_tuning(4, 7, 1)
     assert pano.get_tuning() == {"remap_px": 4, "tail_base": 2, "overlap": 1, "remap_xcd": 0, "remap_rows": 1,
-                                 "canvas_grid": 0, "fuse_down": 0}
+                                 "canvas_grid": 0, "fuse_down": 0, "fuse_up": 1}
+    with pytest.raises(hm.HmError):
+        pano.set_tuning(4, 2, 1, 0, 1, 0, 0, 2)
