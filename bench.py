@@ -613,14 +613,21 @@ def main():
     dk = cands[dom]
     mt = measured_traffic()
     traffic = None
+    traffic_low = None
     if mt and dom in mt:
         traffic = int(mt[dom]["traffic_bytes_per_frame"] / mt[dom]["launches_per_frame"])
+        if "traffic_bytes_lower_bound" in mt[dom]:
+            traffic_low = int(mt[dom]["traffic_bytes_lower_bound"] / mt[dom]["launches_per_frame"])
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(dk["achieved_GBps"], 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(dk["achieved_GBps"] / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": round(dk["avg_ms"], 5)}
     if traffic:
+        # traffic: read requests taken as 128 bytes (the calibrated x2 FETCH_SIZE scale, an upper
+        # bound for gathers); traffic_low: max(64-byte requests, algorithmic reads) + writes
         roofline["traffic_GBps"] = round(traffic / (dk["avg_ms"] * 1e-3) / 1e9, 1)
         roofline["traffic_frac"] = round(traffic / (dk["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+        if traffic_low:
+            roofline["traffic_low"] = traffic_low
     # the whole frame: every kernel's bytes over the frame time
     frame_alg = sum(v["alg_bytes"] * v["launches"] / kt_of(r) for v in cands.values())
     frame = {"alg_bytes": int(frame_alg), "alg_GBps": round(frame_alg / (r["ms"] * 1e-3) / 1e9, 1)}
@@ -629,6 +636,10 @@ def main():
         ft = sum(mt[k]["traffic_bytes_per_frame"] for k in cands)
         frame.update({"traffic_bytes": int(ft), "traffic_GBps": round(ft / (r["ms"] * 1e-3) / 1e9, 1),
                       "traffic_frac": round(ft / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)})
+        if all("traffic_bytes_lower_bound" in mt[k] for k in cands):
+            fl = sum(mt[k]["traffic_bytes_lower_bound"] for k in cands)
+            frame.update({"traffic_bytes_low": int(fl),
+                          "traffic_frac_low": round(fl / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)})
     line = {
         "metric": "pano Mpix/s + fps @8500x3300 L=6; p50/p99 frame latency, served vs plain",
         "value": round(mpix, 1),
