@@ -235,6 +235,12 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
         hm_stitch.check(plain_one(rigs[ri], k), "plain process")
     torch.cuda.synchronize()
     per_cfg = {}
+    marks = {}
+
+    def mark():  # both clocks: a kernel trace's timestamps follow one of them
+        return [time.clock_gettime_ns(time.CLOCK_MONOTONIC), time.clock_gettime_ns(time.CLOCK_BOOTTIME)]
+
+    marks["plain_start"] = mark()
     t0 = time.perf_counter()
     for ri, k in order:
         t1 = time.perf_counter()
@@ -242,6 +248,7 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
         torch.cuda.synchronize()
         per_cfg.setdefault(rigs[ri]["name"], []).append(time.perf_counter() - t1)
     t_plain = time.perf_counter() - t0
+    marks["plain_end"] = mark()
     plain_ref = [r["canvases"][0].clone() for r in rigs]  # each rig's canvas 0 after its last even frame
     torch.cuda.synchronize()
 
@@ -258,11 +265,13 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
         hm_stitch.check(submit(ri, k, i), "submit")
     hm_stitch.check(server.wait_idle(), "served warm-up")
     server.poll(1 << 16)
+    marks["served_start"] = mark()
     t0 = time.perf_counter()
     for i, (ri, k) in enumerate(order):
         hm_stitch.check(submit(ri, k, i), "submit")
     hm_stitch.check(server.wait_idle(), "served run")
     t_served = time.perf_counter() - t0
+    marks["served_end"] = mark()
     server.poll(1 << 16)
     torch.cuda.synchronize()
     equal = all(bool(torch.equal(r["canvases"][0], ref)) for r, ref in zip(rigs, plain_ref))
@@ -317,6 +326,7 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
                       "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
                       "latency_ms_max": round(float(lat.max()), 3)},
         "graphs_captured": st["graphs_captured"], "graph_launches": st["graph_launches"],
+        "phase_clock_ns": marks,
         "batched_mixed": batched,
     }
 
