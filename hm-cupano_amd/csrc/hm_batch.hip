@@ -248,7 +248,8 @@ int hmBatchCreate(const hmBatchCall* calls, int num_calls, hmBatch** out) {
   if (!calls || num_calls < 1) return hipErrorInvalidValue;
   for (int i = 0; i < num_calls; ++i) {
     if (!calls[i].pano || calls[i].pano->status != hipSuccess) return hipErrorInvalidValue;
-    if (calls[i].pano->channels != 4 || calls[i].pano->debug) return hipErrorNotSupported;
+    if (calls[i].pano->channels != 4 || calls[i].pano->debug || calls[i].pano->pipeline_f32)
+      return hipErrorNotSupported;
     if (calls[i].pano->served.load()) return hipErrorNotSupported;  // a server's thread launches it
     for (int j = 0; j < i; ++j)  // one object's scratch serves one call of a pass
       if (calls[j].pano == calls[i].pano) return hipErrorInvalidValue;

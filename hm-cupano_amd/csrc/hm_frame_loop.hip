@@ -143,6 +143,7 @@ extern "C" int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* 
     return hipErrorInvalidValue;
   if (pano->status != hipSuccess) return pano->status;
   if (n_inputs != pano->n_images) return hipErrorInvalidValue;
+  if (pano->pipeline_f32) return hipErrorNotSupported;  // BGR byte frames feed the uint8 pipeline
   const int C = pano->channels, B = pano->batch;
   const int cw = pano->canvas_w, ch = pano->canvas_h;
   for (int i = 0; i < n_inputs; ++i)

@@ -16,6 +16,10 @@ struct hmStitchPano {
   int batch = 1;
   int num_levels = 0;
   int channels = 4;
+  // Float pipeline (CudaStitchPano*<float4|float3, ...>): float inputs and
+  // canvas; g0 then holds the float full buffers of the reference.
+  bool pipeline_f32 = false;
+  hm::DevBuf fout;  // float pipeline: the blend's output (N- and 3-image classes)
   int canvas_w = 0, canvas_h = 0;
   bool hard_seam = false;
   bool debug = false;
@@ -67,7 +71,7 @@ struct hmStitchPano {
   virtual hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int fused,
                                  hipStream_t s) = 0;
   size_t base_device_bytes() const {
-    size_t b = mask0.bytes + pyr.allocation_size;
+    size_t b = mask0.bytes + pyr.allocation_size + fout.bytes;
     for (const auto& g : g0) b += g.bytes;
     return b;
   }
@@ -84,5 +88,31 @@ hipError_t upload_packed_map(const uint16_t* map_x, const uint16_t* map_y, int w
 bool pad_replicate(const uint8_t* src, int sw, int sh, int cw, int ch, std::vector<uint8_t>& out);
 
 bool surface_ok(const hmSurface& s, int channels, bool need_vec_align);
+
+// ---- float pipelines (hm_pano_f32.hip) ------------------------------------
+// channels | kPipelineF32 selects the float pipeline at construction.
+constexpr int kPipelineF32 = HM_PIPELINE_F32;
+// Splits a constructor's channels argument: false when it is not 3 or 4
+// (optionally with kPipelineF32).
+inline bool parse_channels(int arg, int* channels, bool* f32) {
+  *f32 = (arg & kPipelineF32) != 0;
+  *channels = arg & ~kPipelineF32;
+  return *channels == 3 || *channels == 4;
+}
+// A float surface is valid for c channels: pointer, pitch and 4-byte alignment.
+bool surface_ok_f32(const hmSurface& s, int channels);
+hmdev::Surf f32_tight(float* p, int w, int h, int channels);
+hipError_t f32_zero(const hmdev::Surf& s, int channels, int batch, hipStream_t st);
+// mode 0: BatchedRemapKernelExOffset(ROI); 1: ...WithDestMap (labels are the
+// canvas-wide seam, image_index the label written); 2: mapped pixels only
+// (FusedRemapToFullKernel3).  roi is in map coordinates.
+hipError_t f32_remap(int mode, int channels, const hmdev::Surf& src, const uint32_t* map, int mw, int mh, int off_x,
+                     int off_y, int roi_x, int roi_y, int roi_w, int roi_h, const hmdev::Surf& dst, int image_index,
+                     const uint8_t* labels, int batch, hipStream_t st);
+hipError_t f32_hard_n(int n, int channels, const hmSurface* inputs, const DevBuf* maps, const int* px, const int* py,
+                      const int* mw, const int* mh, const uint8_t* labels, int labels_w, const hmdev::Surf& dst,
+                      int batch, hipStream_t st);
+hipError_t f32_copy(int channels, const hmdev::Surf& src, int region_w, int region_h, int sx, int sy,
+                    const hmdev::Surf& dst, int ox, int oy, int batch, hipStream_t st);
 
 }  // namespace hm

@@ -182,12 +182,15 @@ struct Pano2 : hmStitchPano {
   hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int,
                          hipStream_t s) override {
     if (status != hipSuccess) return (hipError_t)status;
-    const hipError_t e = n_inputs == 2 && inputs ? process(inputs[0], inputs[1], canvas, s) : hipErrorInvalidValue;
+    const hipError_t e = !(n_inputs == 2 && inputs) ? hipErrorInvalidValue
+                         : pipeline_f32             ? process_f32(inputs[0], inputs[1], canvas, s)
+                                                    : process(inputs[0], inputs[1], canvas, s);
     if (e != hipSuccess) status = e;  // CudaStitchPano::process makes errors sticky (cudaPano.inl:468-470)
     return e;
   }
   size_t device_bytes() const override { return base_device_bytes() + map[0].bytes + map[1].bytes + seam.bytes + mask_u8.bytes; }
   hipError_t process(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s);
+  hipError_t process_f32(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s);
   hipError_t run_blend(const hmSurface& canvas, hipStream_t s, int down_from = 0);
   hipError_t launch_remap_down(const Pano2L0& a, hipStream_t s, int* n_first);
 };
@@ -273,11 +276,15 @@ hipError_t Pano2::setup(const hmControlMasks2& m, int minimize_blend) {
       mask_host[(size_t)y * bw + x] = (float)v;
     }
   for (int i = 0; i < 2; ++i) {
-    HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+    // uint8 level-0 inputs, or the float pipeline's full buffers
+    HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch * (pipeline_f32 ? sizeof(float) : 1)));
     // Columns the per-frame kernel never writes are the zero of the
     // reference's per-frame memset; clear them once.
     HM_RETURN_IF_ERROR(hipMemset(g0[i].p, 0, g0[i].bytes));
   }
+  // The float pipeline's blend output (the reference writes it over full
+  // buffer 1: the same values, but the level-0 inputs stay for the dumps).
+  if (pipeline_f32) HM_RETURN_IF_ERROR(fout.alloc((size_t)bw * bh * channels * batch * sizeof(float)));
   return hipSuccess;
 }
 
@@ -288,7 +295,7 @@ hipError_t Pano2::ensure_pyramid() {
   HM_RETURN_IF_ERROR(hipMemcpy(mask_u8.p, mask_u8_host.data(), mask_u8_host.size(), hipMemcpyHostToDevice));
   pyr.mask0_fast = mask_u8.p;  // float(seam byte) == the float mask, 1 byte per pixel
   pyr.mask0_mode = kMask0U8;
-  if (l0.write_canvas) HM_RETURN_IF_ERROR(ensure_side());
+  if (l0.write_canvas && !pipeline_f32) HM_RETURN_IF_ERROR(ensure_side());
   return hipSuccess;
 }
 
@@ -453,6 +460,53 @@ hipError_t Pano2::launch_remap_down(const Pano2L0& a, hipStream_t s, int* n_firs
   return hipGetLastError();
 }
 
+// The float pipeline, in the reference's order (cudaPano.inl:167-411): the
+// canvas cleared, each remap onto the canvas and its blend columns copied
+// into a float full buffer (or, without minimize_blend, each image remapped
+// straight into its buffer), the blend into full buffer 1, and its copy back
+// into the canvas.
+hipError_t Pano2::process_f32(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s) {
+  const int c = channels;
+  if (!hm::surface_ok_f32(in1, c) || !hm::surface_ok_f32(in2, c) || !hm::surface_ok_f32(canvas, c))
+    return hipErrorInvalidValue;
+  if ((int)canvas.width != canvas_w || (int)canvas.height != canvas_h) return hipErrorInvalidValue;
+  if (hm::g_recorder) return hipErrorNotSupported;  // the batched pass carries the uint8 pipeline only
+  const Surf s1 = to_surf(in1), s2 = to_surf(in2), cv = to_surf(canvas);
+  const uint32_t* m1 = map[0].as<uint32_t>();
+  const uint32_t* m2 = map[1].as<uint32_t>();
+  HM_RETURN_IF_ERROR(hm::f32_zero(cv, c, batch, s));  // cudaPano.inl:186-187
+  if (hard_seam) {
+    // cudaPano.inl:262-276, :345-359: image 1 where the seam is 1, image 2 where it is 0
+    HM_RETURN_IF_ERROR(hm::f32_remap(1, c, s1, m1, mw[0], mh[0], x1, y1, 0, 0, mw[0], mh[0], cv, 1,
+                                     seam.as<uint8_t>(), batch, s));
+    return hm::f32_remap(1, c, s2, m2, mw[1], mh[1], x2, y2, 0, 0, mw[1], mh[1], cv, 0, seam.as<uint8_t>(), batch, s);
+  }
+  HM_RETURN_IF_ERROR(ensure_pyramid());
+  const Surf f1 = hm::f32_tight(g0[0].as<float>(), bw, bh, c), f2 = hm::f32_tight(g0[1].as<float>(), bw, bh, c);
+  HM_RETURN_IF_ERROR(hm::f32_zero(f1, c, batch, s));
+  HM_RETURN_IF_ERROR(hm::f32_zero(f2, c, batch, s));
+  if (minimize) {
+    const int roi_w = bw - kOverlapPad;
+    // cudaPano.inl:210-237: remap 1 onto the canvas, its blend columns into full 1
+    HM_RETURN_IF_ERROR(hm::f32_remap(0, c, s1, m1, mw[0], mh[0], x1, y1, 0, 0, mw[0], mh[0], cv, 0, nullptr, batch, s));
+    HM_RETURN_IF_ERROR(hm::f32_copy(c, cv, roi_w, bh, x2 - kOverlapPad - x1, 0, f1, x1, 0, batch, s));
+    // :291-318: remap 2 onto the canvas, its blend columns into full 2
+    HM_RETURN_IF_ERROR(hm::f32_remap(0, c, s2, m2, mw[1], mh[1], x2, y2, 0, 0, mw[1], mh[1], cv, 0, nullptr, batch, s));
+    HM_RETURN_IF_ERROR(hm::f32_copy(c, cv, roi_w, bh, x2, 0, f2, x1 + kOverlapPad, 0, batch, s));
+  } else {
+    // :242-255, :323-336
+    HM_RETURN_IF_ERROR(hm::f32_remap(0, c, s1, m1, mw[0], mh[0], x1, y1, 0, 0, mw[0], mh[0], f1, 0, nullptr, batch, s));
+    HM_RETURN_IF_ERROR(hm::f32_remap(0, c, s2, m2, mw[1], mh[1], x2, y2, 0, 0, mw[1], mh[1], f2, 0, nullptr, batch, s));
+  }
+  // The blend (cudaPano.inl:384-392), then the copy back (:401-411).
+  const void* g[2] = {g0[0].p, g0[1].p};
+  hm::Level0Out out;
+  out.f32 = fout.as<float>();
+  HM_RETURN_IF_ERROR(pyr.run(g, false, out, s, timer_or_null()));
+  return hm::f32_copy(c, hm::f32_tight(fout.as<float>(), bw, bh, c), bw, bh, 0, 0, cv,
+                      minimize ? x2 - kOverlapPad : 0, 0, batch, s);
+}
+
 hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s, int down_from) {
   const void* g[2] = {g0[0].p, g0[1].p};
   hm::Level0Out out;
@@ -484,10 +538,10 @@ extern "C" int hmStitchPanoCreate(int batch_size, int num_levels, const hmContro
   p->n_images = 2;
   p->batch = batch_size;
   p->num_levels = num_levels;
-  p->channels = channels;
+  const bool channels_ok = hm::parse_channels(channels, &p->channels, &p->pipeline_f32);
   if (!masks) {
     p->status = hipErrorFileNotFound;
-  } else if (batch_size < 1 || num_levels < 0 || num_levels > hmdev::kMaxLevels || (channels != 3 && channels != 4)) {
+  } else if (batch_size < 1 || num_levels < 0 || num_levels > hmdev::kMaxLevels || !channels_ok) {
     p->status = hipErrorInvalidValue;
   } else {
     p->status = p->setup(*masks, minimize_blend);

@@ -40,12 +40,15 @@ struct Pano3 : hmStitchPano {
   hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int fused,
                          hipStream_t s) override {
     if (status != hipSuccess) return (hipError_t)status;
-    const hipError_t e = n_inputs == 3 && inputs ? process(inputs, canvas, fused != 0, s) : hipErrorInvalidValue;
+    const hipError_t e = !(n_inputs == 3 && inputs) ? hipErrorInvalidValue
+                         : pipeline_f32             ? process_f32(inputs, canvas, fused != 0, s)
+                                                    : process(inputs, canvas, fused != 0, s);
     if (e != hipSuccess) status = e;  // both process paths make errors sticky (cudaPano3.inl:415-421)
     return e;
   }
   size_t device_bytes() const override { return base_device_bytes() + map[0].bytes + map[1].bytes + map[2].bytes + seam.bytes + labels.bytes; }
   hipError_t process(const hmSurface* inputs, const hmSurface& canvas, bool fused, hipStream_t s);
+  hipError_t process_f32(const hmSurface* inputs, const hmSurface& canvas, bool fused, hipStream_t s);
   MultiGeom geom() const;
 };
 
@@ -93,10 +96,12 @@ hipError_t Pano3::setup(const hmControlMasks3& m) {
     }
   // Cleared once: the sparse fast path writes only each image's valid region,
   // so the rest stays zero (what the dense pyramid sees there).
+  const size_t es = pipeline_f32 ? sizeof(float) : 1;  // uint8 inputs, or the float pipeline's full buffers
   for (int i = 0; i < 3; ++i) {
-    HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+    HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch * es));
     HM_RETURN_IF_ERROR(hipMemset(g0[i].p, 0, g0[i].bytes));
   }
+  if (pipeline_f32) HM_RETURN_IF_ERROR(fout.alloc((size_t)bw * bh * channels * batch * sizeof(float)));
   return hipSuccess;
 }
 
@@ -111,8 +116,50 @@ hipError_t Pano3::ensure_pyramid() {
   ImgRect s0[3];
   for (int i = 0; i < 3; ++i) s0[i] = ImgRect{pos_x[i], pos_y[i], pos_x[i] + mw[i] - 1, pos_y[i] + mh[i] - 1};
   HM_RETURN_IF_ERROR(pyr.set_supports(s0));
-  if (bw < canvas_w) HM_RETURN_IF_ERROR(ensure_side());
+  if (bw < canvas_w && !pipeline_f32) HM_RETURN_IF_ERROR(ensure_side());
   return hipSuccess;
+}
+
+// The float pipeline, in the reference's order: process_optimized
+// (cudaPano3_optimized.inl:19-163, fused) or the remap + copy path
+// (cudaPano3.inl:186-402).
+hipError_t Pano3::process_f32(const hmSurface* inputs, const hmSurface& canvas, bool fused, hipStream_t s) {
+  const int c = channels;
+  for (int i = 0; i < 3; ++i)
+    if (!hm::surface_ok_f32(inputs[i], c)) return hipErrorInvalidValue;
+  if (!hm::surface_ok_f32(canvas, c) || (int)canvas.width != canvas_w || (int)canvas.height != canvas_h)
+    return hipErrorInvalidValue;
+  if (hm::g_recorder) return hipErrorNotSupported;  // the batched pass carries the uint8 pipeline only
+  const Surf cv = to_surf(canvas);
+  if (c == 4) HM_RETURN_IF_ERROR(hm::f32_zero(cv, c, batch, s));  // RGBA canvases are cleared first
+  if (hard_seam) {
+    if (fused)  // FusedRemapHardSeam3: unmapped pixels are left alone
+      return hm::f32_hard_n(3, c, inputs, map, pos_x, pos_y, mw, mh, seam.as<uint8_t>(), canvas_w, cv, batch, s);
+    for (int i = 0; i < 3; ++i)  // dest-map remaps in image order
+      HM_RETURN_IF_ERROR(hm::f32_remap(1, c, to_surf(inputs[i]), map[i].as<uint32_t>(), mw[i], mh[i], pos_x[i],
+                                       pos_y[i], 0, 0, mw[i], mh[i], cv, i, seam.as<uint8_t>(), batch, s));
+    return hipSuccess;
+  }
+  HM_RETURN_IF_ERROR(ensure_pyramid());
+  for (int i = 0; i < 3; ++i) {
+    const Surf full = hm::f32_tight(g0[i].as<float>(), bw, bh, c);
+    HM_RETURN_IF_ERROR(hm::f32_zero(full, c, batch, s));
+    const Surf src = to_surf(inputs[i]);
+    const uint32_t* mp = map[i].as<uint32_t>();
+    if (fused) {  // FusedRemapToFullKernel3: mapped, in-range pixels of each box
+      HM_RETURN_IF_ERROR(hm::f32_remap(2, c, src, mp, mw[i], mh[i], pos_x[i], pos_y[i], 0, 0, mw[i], mh[i], full, 0,
+                                       nullptr, batch, s));
+    } else {  // remap onto the canvas, then the image's box into its full buffer
+      HM_RETURN_IF_ERROR(hm::f32_remap(0, c, src, mp, mw[i], mh[i], pos_x[i], pos_y[i], 0, 0, mw[i], mh[i], cv, 0,
+                                       nullptr, batch, s));
+      HM_RETURN_IF_ERROR(hm::f32_copy(c, cv, mw[i], mh[i], pos_x[i], pos_y[i], full, pos_x[i], pos_y[i], batch, s));
+    }
+  }
+  const void* g[3] = {g0[0].p, g0[1].p, g0[2].p};
+  hm::Level0Out out;
+  out.f32 = fout.as<float>();
+  HM_RETURN_IF_ERROR(pyr.run(g, false, out, s, timer_or_null()));
+  return hm::f32_copy(c, hm::f32_tight(fout.as<float>(), bw, bh, c), bw, bh, 0, 0, cv, 0, 0, batch, s);
 }
 
 MultiGeom Pano3::geom() const {
@@ -213,10 +260,10 @@ extern "C" int hmStitchPano3Create(int batch_size, int num_levels, const hmContr
   *out = p;
   p->batch = batch_size;
   p->num_levels = num_levels;
-  p->channels = channels;
+  const bool channels_ok = hm::parse_channels(channels, &p->channels, &p->pipeline_f32);
   if (!masks) {
     p->status = hipErrorFileNotFound;
-  } else if (batch_size < 1 || num_levels < 0 || num_levels > kMaxLevels || (channels != 3 && channels != 4)) {
+  } else if (batch_size < 1 || num_levels < 0 || num_levels > kMaxLevels || !channels_ok) {
     p->status = hipErrorInvalidValue;
   } else {
     p->status = p->setup(*masks);

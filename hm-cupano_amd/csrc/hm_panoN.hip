@@ -92,7 +92,8 @@ struct PanoN : hmStitchPano {
   hipError_t process_any(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, int,
                          hipStream_t s) override {
     if (status != hipSuccess) return (hipError_t)status;
-    return process(inputs, n_inputs, canvas, s);  // CudaStitchPanoN::process: errors are not sticky
+    // CudaStitchPanoN::process: errors are not sticky
+    return pipeline_f32 ? process_f32(inputs, n_inputs, canvas, s) : process(inputs, n_inputs, canvas, s);
   }
   size_t device_bytes() const override {
     size_t b = base_device_bytes() + seam.bytes + labels.bytes;
@@ -100,6 +101,7 @@ struct PanoN : hmStitchPano {
     return b;
   }
   hipError_t process(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, hipStream_t s);
+  hipError_t process_f32(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, hipStream_t s);
   template <int N>
   hipError_t launch_region(const MultiRegion& r, const hmSurface* inputs, const hmSurface& canvas, hipStream_t s,
                            const char* name);
@@ -162,10 +164,12 @@ hipError_t PanoN::setup(const hmControlMasksN& m, int minimize_blend) {
     }
   // Cleared once: the sparse fast path writes only each image's valid region,
   // so the rest stays zero (what the dense pyramid sees there).
+  const size_t es = pipeline_f32 ? sizeof(float) : 1;  // uint8 inputs, or the float pipeline's full buffers
   for (int i = 0; i < n; ++i) {
-    HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch));
+    HM_RETURN_IF_ERROR(g0[i].alloc((size_t)bw * bh * channels * batch * es));
     HM_RETURN_IF_ERROR(hipMemset(g0[i].p, 0, g0[i].bytes));
   }
+  if (pipeline_f32) HM_RETURN_IF_ERROR(fout.alloc((size_t)bw * bh * channels * batch * sizeof(float)));
   return hipSuccess;
 }
 
@@ -181,8 +185,56 @@ hipError_t PanoN::ensure_pyramid() {
   for (int i = 0; i < n_images; ++i)
     s0[i] = ImgRect{pos_x[i] - bx0, pos_y[i] - by0, pos_x[i] + mw[i] - 1 - bx0, pos_y[i] + mh[i] - 1 - by0};
   HM_RETURN_IF_ERROR(pyr.set_supports(s0));
-  if (use_min) HM_RETURN_IF_ERROR(ensure_side());
+  if (use_min && !pipeline_f32) HM_RETURN_IF_ERROR(ensure_side());
   return hipSuccess;
+}
+
+// The float pipeline, in the reference's order (cudaPanoN.inl:489-685).
+hipError_t PanoN::process_f32(const hmSurface* inputs, int n_inputs, const hmSurface& canvas, hipStream_t s) {
+  const int c = channels;
+  if (!canvas.d_ptr) return hipErrorInvalidDevicePointer;
+  if (!inputs || n_inputs != n_images) return hipErrorInvalidValue;
+  for (int i = 0; i < n_images; ++i)
+    if (!hm::surface_ok_f32(inputs[i], c)) return hipErrorInvalidValue;
+  if (!hm::surface_ok_f32(canvas, c) || (int)canvas.width != canvas_w || (int)canvas.height != canvas_h)
+    return hipErrorInvalidValue;
+  if (hm::g_recorder) return hipErrorNotSupported;  // the batched pass carries the uint8 pipeline only
+  const Surf cv = to_surf(canvas);
+  if (hard_seam || use_min) {  // cleared canvas, hard seam (:508-541, :545-588)
+    HM_RETURN_IF_ERROR(hm::f32_zero(cv, c, batch, s));
+    HM_RETURN_IF_ERROR(hm::f32_hard_n(n_images, c, inputs, map, pos_x, pos_y, mw, mh, seam.as<uint8_t>(), canvas_w,
+                                      cv, batch, s));
+    if (hard_seam) return hipSuccess;
+  }
+  HM_RETURN_IF_ERROR(ensure_pyramid());
+  for (int i = 0; i < n_images; ++i) {
+    const Surf full = hm::f32_tight(g0[i].as<float>(), bw, bh, c);
+    HM_RETURN_IF_ERROR(hm::f32_zero(full, c, batch, s));
+    const Surf src = to_surf(inputs[i]);
+    const uint32_t* mp = map[i].as<uint32_t>();
+    if (use_min) {  // only the image's intersection with the blend ROI (:590-618)
+      const int ix0 = std::max(pos_x[i], blend_roi.x), iy0 = std::max(pos_y[i], blend_roi.y);
+      const int ix1 = std::min(pos_x[i] + mw[i], blend_roi.x + blend_roi.w);
+      const int iy1 = std::min(pos_y[i] + mh[i], blend_roi.y + blend_roi.h);
+      if (ix1 > ix0 && iy1 > iy0)
+        HM_RETURN_IF_ERROR(hm::f32_remap(0, c, src, mp, mw[i], mh[i], pos_x[i] - blend_roi.x, pos_y[i] - blend_roi.y,
+                                         ix0 - pos_x[i], iy0 - pos_y[i], ix1 - ix0, iy1 - iy0, full, 0, nullptr, batch,
+                                         s));
+    } else {  // remap_soft (:644-660)
+      HM_RETURN_IF_ERROR(hm::f32_remap(0, c, src, mp, mw[i], mh[i], pos_x[i], pos_y[i], 0, 0, mw[i], mh[i], full, 0,
+                                       nullptr, batch, s));
+    }
+  }
+  const void* g[kMaxImages];
+  for (int i = 0; i < n_images; ++i) g[i] = g0[i].p;
+  hm::Level0Out out;
+  out.f32 = fout.as<float>();
+  HM_RETURN_IF_ERROR(pyr.run(g, false, out, s, timer_or_null()));
+  const Surf fo = hm::f32_tight(fout.as<float>(), bw, bh, c);
+  if (use_min)  // the write ROI back into the canvas (:628-640)
+    return hm::f32_copy(c, fo, write_roi.w, write_roi.h, write_roi.x - blend_roi.x, write_roi.y - blend_roi.y, cv,
+                        write_roi.x, write_roi.y, batch, s);
+  return hm::f32_copy(c, fo, bw, bh, 0, 0, cv, 0, 0, batch, s);  // (:668-681)
 }
 
 template <int N>
@@ -308,10 +360,10 @@ extern "C" int hmStitchPanoNCreate(int batch_size, int num_levels, const hmContr
   *out = p;
   p->batch = batch_size;
   p->num_levels = num_levels;
-  p->channels = channels;
+  const bool channels_ok = hm::parse_channels(channels, &p->channels, &p->pipeline_f32);
   if (!masks) {
     p->status = hipErrorFileNotFound;
-  } else if (batch_size < 1 || num_levels < 0 || num_levels > kMaxLevels || (channels != 3 && channels != 4)) {
+  } else if (batch_size < 1 || num_levels < 0 || num_levels > kMaxLevels || !channels_ok) {
     p->status = hipErrorInvalidValue;
   } else {
     p->status = p->setup(*masks, minimize_blend && num_levels > 0);

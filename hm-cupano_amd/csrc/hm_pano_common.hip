@@ -275,7 +275,9 @@ extern "C" int hmStitchPanoReadLevel(hmStitchPano* pano, int what, int image, in
   HM_RETURN_IF_ERROR(hipStreamSynchronize(stream));
   switch (what) {
     case HM_LEVEL_GAUSS:
-      if (level == 0) {
+      if (level == 0 && pano->pipeline_f32) {
+        src = pano->g0[image].p;  // the float pipeline's full buffer
+      } else if (level == 0) {
         HM_RETURN_IF_ERROR(tmp.alloc(need));
         const size_t n = img_floats;
         hipLaunchKernelGGL(u8_to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
@@ -362,14 +364,17 @@ extern "C" int hmStitchPanoDumpPyramid(hmStitchPano* pano, const char* directory
   for (int i = 0; i < p.n; ++i) g0[i] = pano->g0[i].p;
   hm::Level0Out out;
   out.f32 = t.r0_debug.as<float>();
-  HM_RETURN_IF_ERROR(t.run(g0, true, out, stream, nullptr));
+  HM_RETURN_IF_ERROR(t.run(g0, !pano->pipeline_f32, out, stream, nullptr));
   // level-0 Gaussians as float
   hm::DevBuf g0f[hmdev::kMaxImages];
   const size_t n0 = (size_t)t.W[0] * t.H[0] * t.c * t.batch;
   for (int i = 0; i < p.n; ++i) {
     HM_RETURN_IF_ERROR(g0f[i].alloc(n0 * sizeof(float)));
-    hipLaunchKernelGGL(u8_to_f32_kernel, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, stream,
-                       pano->g0[i].as<uint8_t>(), g0f[i].as<float>(), n0);
+    if (pano->pipeline_f32)
+      HM_RETURN_IF_ERROR(hipMemcpyAsync(g0f[i].p, pano->g0[i].p, n0 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    else
+      hipLaunchKernelGGL(u8_to_f32_kernel, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, stream,
+                         pano->g0[i].as<uint8_t>(), g0f[i].as<float>(), n0);
   }
   HM_RETURN_IF_ERROR(hipGetLastError());
   HM_RETURN_IF_ERROR(hipStreamSynchronize(stream));

@@ -5,6 +5,10 @@
 // The pano classes do not use them: they run the fused kernels of
 // hm_pano2.hip / hm_panoN.hip.  Pixels travel as float4 in registers (uint8
 // to float is exact); stores convert back with the reference's perform_cast.
+#include <hip/hip_fp16.h>
+
+#include <type_traits>
+
 #include "hm_internal.hpp"
 #include "hm_stitch.h"
 
@@ -14,54 +18,67 @@ namespace {
 
 const dim3 kBlock(64, 4, 1);
 
+// Element types of pixel components.
+enum Elem { kU8 = 0, kF32 = 1, kF16 = 2 };
+
 Surf to_surf(const hmSurface& s) { return Surf{static_cast<uint8_t*>(s.d_ptr), s.width, s.height, s.pitch}; }
 
-int channels_of(int t) { return (t == HM_PIX_U8C3 || t == HM_PIX_F32C3) ? 3 : 4; }
-bool is_float(int t) { return t == HM_PIX_F32C3 || t == HM_PIX_F32C4; }
-bool valid_type(int t) { return t >= HM_PIX_U8C3 && t <= HM_PIX_F32C4; }
-size_t px_bytes(int t) { return (size_t)channels_of(t) * (is_float(t) ? 4 : 1); }
-
-template <int C, bool F>
-__device__ __forceinline__ float4 load_px(const Surf& s, int b, int x, int y) {
-  const uint8_t* row = row_ptr(s, b, y);
-  if constexpr (F) {
-    const float* p = reinterpret_cast<const float*>(row) + (size_t)x * C;
-    return make_float4(p[0], p[1], p[2], C == 4 ? p[3] : 255.0f);
-  } else {
-    const uint8_t* p = row + (size_t)x * C;
-    return make_float4((float)p[0], (float)p[1], (float)p[2], C == 4 ? (float)p[3] : 255.0f);
+bool valid_type(int t) { return t >= HM_PIX_U8C3 && t <= HM_PIX_F16C4; }
+int channels_of(int t) {
+  switch (t) {
+    case HM_PIX_U8C1: case HM_PIX_F32C1: case HM_PIX_F16C1: return 1;
+    case HM_PIX_U8C3: case HM_PIX_F32C3: case HM_PIX_F16C3: return 3;
+    default: return 4;
   }
 }
+int elem_of(int t) {
+  switch (t) {
+    case HM_PIX_F32C1: case HM_PIX_F32C3: case HM_PIX_F32C4: return kF32;
+    case HM_PIX_F16C1: case HM_PIX_F16C3: case HM_PIX_F16C4: return kF16;
+    default: return kU8;
+  }
+}
+size_t px_bytes(int t) { return (size_t)channels_of(t) * (elem_of(t) == kF32 ? 4 : (elem_of(t) == kF16 ? 2 : 1)); }
 
-// perform_cast into the destination type: float stores are exact copies,
-// uint8 stores truncate (round_to_uchar) when the source was float.
-template <int C, bool F>
-__device__ __forceinline__ void store_px(const Surf& s, int b, int x, int y, float4 v, bool src_float) {
+// One pixel as float4 (every component type widens exactly); 3 channels read
+// alpha 255 (the 3 -> 4 cast), 1 channel only x.
+template <int C, int E>
+__device__ __forceinline__ float4 load_px(const Surf& s, int b, int x, int y) {
+  const uint8_t* row = row_ptr(s, b, y);
+  float v[4] = {0.0f, 0.0f, 0.0f, 255.0f};
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    if constexpr (E == kF32)
+      v[k] = reinterpret_cast<const float*>(row)[(size_t)x * C + k];
+    else if constexpr (E == kF16)
+      v[k] = __half2float(reinterpret_cast<const __half*>(row)[(size_t)x * C + k]);
+    else
+      v[k] = (float)row[(size_t)x * C + k];
+  }
+  return make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// perform_cast into the destination type (src/cuda/cudaUtils.cuh:80-213):
+// float stores are exact, half stores round to nearest even, uint8 stores
+// truncate (round_to_uchar) when the source was float or half and copy when
+// it was uint8.
+template <int C, int E>
+__device__ __forceinline__ void store_px(const Surf& s, int b, int x, int y, float4 v, bool src_u8) {
   uint8_t* row = row_ptr(s, b, y);
-  if constexpr (F) {
-    float* p = reinterpret_cast<float*>(row) + (size_t)x * C;
-    p[0] = v.x;
-    p[1] = v.y;
-    p[2] = v.z;
-    if (C == 4) p[3] = v.w;
-  } else {
-    uint8_t* p = row + (size_t)x * C;
-    if (src_float) {
-      p[0] = to_u8(v.x);
-      p[1] = to_u8(v.y);
-      p[2] = to_u8(v.z);
-      if (C == 4) p[3] = to_u8(v.w);
-    } else {
-      p[0] = (uint8_t)v.x;
-      p[1] = (uint8_t)v.y;
-      p[2] = (uint8_t)v.z;
-      if (C == 4) p[3] = (uint8_t)v.w;
-    }
+  const float c[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    if constexpr (E == kF32)
+      reinterpret_cast<float*>(row)[(size_t)x * C + k] = c[k];
+    else if constexpr (E == kF16)
+      reinterpret_cast<__half*>(row)[(size_t)x * C + k] = __float2half(c[k]);
+    else
+      row[(size_t)x * C + k] = src_u8 ? (uint8_t)c[k] : to_u8(c[k]);
   }
 }
 
 // BatchedRemapKernelExOffset / ...ROI (src/cuda/cudaRemap.cu:61-198).
-template <int C, bool FI, bool FO>
+template <int C, int EI, int EO>
 __global__ void __launch_bounds__(256)
     remap_offset_kernel(Surf src, Surf dst, const uint16_t* mx, const uint16_t* my, int remap_w, int remap_h, int off_x,
                         int off_y, int roi_x, int roi_y, int roi_w, int roi_h, int no_unmapped_write) {
@@ -77,15 +94,15 @@ __global__ void __launch_bounds__(256)
   const int sx = mx[mi], sy = my[mi];
   if (C == 4 && no_unmapped_write && sx == kUnmapped) return;
   if (sx < (int)src.w && sy < (int)src.h) {
-    store_px<C, FO>(dst, b, dx, dy, load_px<C, FI>(src, b, sx, sy), FI);
+    store_px<C, EO>(dst, b, dx, dy, load_px<C, EI>(src, b, sx, sy), EI == kU8);
   } else if (!no_unmapped_write || sx != kUnmapped) {
     const float a = (C == 4) ? (sx == kUnmapped ? 0.0f : 255.0f) : 0.0f;
-    store_px<C, FO>(dst, b, dx, dy, make_float4(0.0f, 0.0f, 0.0f, a), false);
+    store_px<C, EO>(dst, b, dx, dy, make_float4(0.0f, 0.0f, 0.0f, a), true);
   }
 }
 
 // BatchedRemapKernelExOffsetWithDestMap (cudaRemap.cu:204-246).
-template <int C, bool F>
+template <int C, int E>
 __global__ void __launch_bounds__(256)
     remap_dest_map_kernel(Surf src, Surf dst, const uint16_t* mx, const uint16_t* my, int image_index,
                           const uint8_t* dest_map, int remap_w, int remap_h, int off_x, int off_y) {
@@ -99,8 +116,8 @@ __global__ void __launch_bounds__(256)
   const size_t mi = (size_t)y * remap_w + x;
   const int sx = mx[mi], sy = my[mi];
   float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (sx < (int)src.w && sy < (int)src.h) v = load_px<C, F>(src, b, sx, sy);
-  store_px<C, F>(dst, b, dx, dy, v, false);
+  if (sx < (int)src.w && sy < (int)src.h) v = load_px<C, E>(src, b, sx, sy);
+  store_px<C, E>(dst, b, dx, dy, v, E == kU8);
 }
 
 struct HardSeamN {
@@ -112,7 +129,7 @@ struct HardSeamN {
 };
 
 // BatchedRemapHardSeamKernelN (cudaRemap.cu:252-294).
-template <int C, bool F>
+template <int C, int E>
 __global__ void __launch_bounds__(256) hard_seam_n_kernel(HardSeamN a, const uint8_t* dest_map, Surf dst) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -125,11 +142,12 @@ __global__ void __launch_bounds__(256) hard_seam_n_kernel(HardSeamN a, const uin
   const size_t mi = (size_t)ry * a.w[i] + rx;
   const int sx = a.mx[i][mi], sy = a.my[i][mi];
   if (sx == kUnmapped) return;
-  if (sx < (int)a.src[i].w && sy < (int)a.src[i].h) store_px<C, F>(dst, b, x, y, load_px<C, F>(a.src[i], b, sx, sy), false);
+  if (sx < (int)a.src[i].w && sy < (int)a.src[i].h)
+    store_px<C, E>(dst, b, x, y, load_px<C, E>(a.src[i], b, sx, sy), E == kU8);
 }
 
 // copyRoiKernelBatched (src/cuda/cudaMakeFull.cu:40-69).
-template <int CI, bool FI, int CO, bool FO>
+template <int CI, int EI, int CO, int EO>
 __global__ void __launch_bounds__(256) copy_roi_kernel(Surf src, int region_w, int region_h, int sx0, int sy0,
                                                        Surf dst, int ox, int oy) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -140,21 +158,25 @@ __global__ void __launch_bounds__(256) copy_roi_kernel(Surf src, int region_w, i
   if (!(sx < (int)src.w && sy < (int)src.h)) return;
   const int dx = ox + x, dy = oy + y;
   if (!(dx < (int)dst.w && dy < (int)dst.h)) return;
-  store_px<CO, FO>(dst, b, dx, dy, load_px<CI, FI>(src, b, sx, sy), FI);
+  store_px<CO, EO>(dst, b, dx, dy, load_px<CI, EI>(src, b, sx, sy), EI == kU8);
 }
 
-template <int CI, bool FI>
-hipError_t copy_dispatch_out(int dt, const Surf& s, int rw, int rh, int sx, int sy, const Surf& d, int ox, int oy,
-                             int batch, hipStream_t st) {
-  const dim3 g = hm::grid_for(rw, rh, batch, kBlock);
-  switch (dt) {
-    case HM_PIX_U8C3: hipLaunchKernelGGL((copy_roi_kernel<CI, FI, 3, false>), g, kBlock, 0, st, s, rw, rh, sx, sy, d, ox, oy); break;
-    case HM_PIX_U8C4: hipLaunchKernelGGL((copy_roi_kernel<CI, FI, 4, false>), g, kBlock, 0, st, s, rw, rh, sx, sy, d, ox, oy); break;
-    case HM_PIX_F32C3: hipLaunchKernelGGL((copy_roi_kernel<CI, FI, 3, true>), g, kBlock, 0, st, s, rw, rh, sx, sy, d, ox, oy); break;
-    case HM_PIX_F32C4: hipLaunchKernelGGL((copy_roi_kernel<CI, FI, 4, true>), g, kBlock, 0, st, s, rw, rh, sx, sy, d, ox, oy); break;
+// Calls f(integral_constant C, integral_constant E) for a pixel type.
+template <typename F>
+hipError_t with_type(int t, F&& f) {
+  using std::integral_constant;
+  switch (t) {
+    case HM_PIX_U8C1: return f(integral_constant<int, 1>{}, integral_constant<int, kU8>{});
+    case HM_PIX_U8C3: return f(integral_constant<int, 3>{}, integral_constant<int, kU8>{});
+    case HM_PIX_U8C4: return f(integral_constant<int, 4>{}, integral_constant<int, kU8>{});
+    case HM_PIX_F32C1: return f(integral_constant<int, 1>{}, integral_constant<int, kF32>{});
+    case HM_PIX_F32C3: return f(integral_constant<int, 3>{}, integral_constant<int, kF32>{});
+    case HM_PIX_F32C4: return f(integral_constant<int, 4>{}, integral_constant<int, kF32>{});
+    case HM_PIX_F16C1: return f(integral_constant<int, 1>{}, integral_constant<int, kF16>{});
+    case HM_PIX_F16C3: return f(integral_constant<int, 3>{}, integral_constant<int, kF16>{});
+    case HM_PIX_F16C4: return f(integral_constant<int, 4>{}, integral_constant<int, kF16>{});
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 bool surf_ok(const hmSurface& s, int type) {
@@ -173,27 +195,23 @@ extern "C" int hmRemapExOffsetRoi(hmSurface src, hmPixelType src_type, hmSurface
   if (roi_x + roi_w > remap_w || roi_y + roi_h > remap_h) return hipErrorInvalidValue;
   if (!valid_type(src_type) || !valid_type(dest_type) || channels_of(src_type) != channels_of(dest_type))
     return hipErrorInvalidValue;
-  if (is_float(src_type) && !is_float(dest_type)) return hipErrorInvalidValue;  // not instantiated upstream
+  // dest: the source type, or float for a uint8 source (the instantiated pairs)
+  if (src_type != dest_type && !(elem_of(src_type) == kU8 && elem_of(dest_type) == kF32)) return hipErrorInvalidValue;
   if (!surf_ok(src, src_type) || !surf_ok(dest, dest_type) || !d_map_x || !d_map_y || batch_size < 1)
     return hipErrorInvalidValue;
   const Surf s = to_surf(src), d = to_surf(dest);
   const dim3 g = hm::grid_for(roi_w, roi_h, batch_size, kBlock);
-  const int c = channels_of(src_type);
-  const bool fi = is_float(src_type), fo = is_float(dest_type);
-#define HM_REMAP_LAUNCH(C, FI, FO)                                                                                  \
-  hipLaunchKernelGGL((remap_offset_kernel<C, FI, FO>), g, kBlock, 0, stream, s, d, d_map_x, d_map_y, remap_w,      \
-                     remap_h, offset_x, offset_y, roi_x, roi_y, roi_w, roi_h, no_unmapped_write)
-  if (c == 4) {
-    if (fi) HM_REMAP_LAUNCH(4, true, true);
-    else if (fo) HM_REMAP_LAUNCH(4, false, true);
-    else HM_REMAP_LAUNCH(4, false, false);
-  } else {
-    if (fi) HM_REMAP_LAUNCH(3, true, true);
-    else if (fo) HM_REMAP_LAUNCH(3, false, true);
-    else HM_REMAP_LAUNCH(3, false, false);
-  }
-#undef HM_REMAP_LAUNCH
-  return hipGetLastError();
+  const int eo = elem_of(dest_type);
+  return with_type(src_type, [&](auto Cv, auto EIv) -> hipError_t {
+    constexpr int C = decltype(Cv)::value, EI = decltype(EIv)::value;
+    if (eo == kF32 && EI != kF32)
+      hipLaunchKernelGGL((remap_offset_kernel<C, EI, kF32>), g, kBlock, 0, stream, s, d, d_map_x, d_map_y, remap_w,
+                         remap_h, offset_x, offset_y, roi_x, roi_y, roi_w, roi_h, no_unmapped_write);
+    else
+      hipLaunchKernelGGL((remap_offset_kernel<C, EI, EI>), g, kBlock, 0, stream, s, d, d_map_x, d_map_y, remap_w,
+                         remap_h, offset_x, offset_y, roi_x, roi_y, roi_w, roi_h, no_unmapped_write);
+    return hipGetLastError();
+  });
 }
 
 extern "C" int hmRemapExOffset(hmSurface src, hmPixelType src_type, hmSurface dest, hmPixelType dest_type,
@@ -214,13 +232,12 @@ extern "C" int hmRemapExOffsetWithDestMap(hmSurface src, hmSurface dest, hmPixel
   if (remap_w <= 0 || remap_h <= 0) return hipSuccess;
   const Surf s = to_surf(src), d = to_surf(dest);
   const dim3 g = hm::grid_for(remap_w, remap_h, batch_size, kBlock);
-  switch (type) {
-    case HM_PIX_U8C3: hipLaunchKernelGGL((remap_dest_map_kernel<3, false>), g, kBlock, 0, stream, s, d, d_map_x, d_map_y, this_image_index, d_dest_image_map, remap_w, remap_h, offset_x, offset_y); break;
-    case HM_PIX_U8C4: hipLaunchKernelGGL((remap_dest_map_kernel<4, false>), g, kBlock, 0, stream, s, d, d_map_x, d_map_y, this_image_index, d_dest_image_map, remap_w, remap_h, offset_x, offset_y); break;
-    case HM_PIX_F32C3: hipLaunchKernelGGL((remap_dest_map_kernel<3, true>), g, kBlock, 0, stream, s, d, d_map_x, d_map_y, this_image_index, d_dest_image_map, remap_w, remap_h, offset_x, offset_y); break;
-    default: hipLaunchKernelGGL((remap_dest_map_kernel<4, true>), g, kBlock, 0, stream, s, d, d_map_x, d_map_y, this_image_index, d_dest_image_map, remap_w, remap_h, offset_x, offset_y); break;
-  }
-  return hipGetLastError();
+  return with_type(type, [&](auto Cv, auto Ev) -> hipError_t {
+    constexpr int C = decltype(Cv)::value, E = decltype(Ev)::value;
+    hipLaunchKernelGGL((remap_dest_map_kernel<C, E>), g, kBlock, 0, stream, s, d, d_map_x, d_map_y, this_image_index,
+                       d_dest_image_map, remap_w, remap_h, offset_x, offset_y);
+    return hipGetLastError();
+  });
 }
 
 extern "C" int hmRemapHardSeamN(int n_images, const hmSurface* inputs, const uint16_t* const* d_map_x,
@@ -244,13 +261,11 @@ extern "C" int hmRemapHardSeamN(int n_images, const hmSurface* inputs, const uin
   }
   const Surf d = to_surf(dest);
   const dim3 g = hm::grid_for((int)dest.width, (int)dest.height, batch_size, kBlock);
-  switch (type) {
-    case HM_PIX_U8C3: hipLaunchKernelGGL((hard_seam_n_kernel<3, false>), g, kBlock, 0, stream, a, d_dest_image_map, d); break;
-    case HM_PIX_U8C4: hipLaunchKernelGGL((hard_seam_n_kernel<4, false>), g, kBlock, 0, stream, a, d_dest_image_map, d); break;
-    case HM_PIX_F32C3: hipLaunchKernelGGL((hard_seam_n_kernel<3, true>), g, kBlock, 0, stream, a, d_dest_image_map, d); break;
-    default: hipLaunchKernelGGL((hard_seam_n_kernel<4, true>), g, kBlock, 0, stream, a, d_dest_image_map, d); break;
-  }
-  return hipGetLastError();
+  return with_type(type, [&](auto Cv, auto Ev) -> hipError_t {
+    constexpr int C = decltype(Cv)::value, E = decltype(Ev)::value;
+    hipLaunchKernelGGL((hard_seam_n_kernel<C, E>), g, kBlock, 0, stream, a, d_dest_image_map, d);
+    return hipGetLastError();
+  });
 }
 
 extern "C" int hmCopyRoi(hmSurface src, hmPixelType src_type, int region_w, int region_h, int src_roi_x, int src_roi_y,
@@ -260,13 +275,22 @@ extern "C" int hmCopyRoi(hmSurface src, hmPixelType src_type, int region_w, int 
       batch_size < 1)
     return hipErrorInvalidValue;
   if (region_w <= 0 || region_h <= 0) return hipSuccess;
+  // 1-channel types copy to 1 channel; 3 and 4 channels convert (alpha 255 / dropped).
+  if ((channels_of(src_type) == 1) != (channels_of(dest_type) == 1)) return hipErrorInvalidValue;
   const Surf s = to_surf(src), d = to_surf(dest);
-  switch (src_type) {
-    case HM_PIX_U8C3: return copy_dispatch_out<3, false>(dest_type, s, region_w, region_h, src_roi_x, src_roi_y, d, offset_x, offset_y, batch_size, stream);
-    case HM_PIX_U8C4: return copy_dispatch_out<4, false>(dest_type, s, region_w, region_h, src_roi_x, src_roi_y, d, offset_x, offset_y, batch_size, stream);
-    case HM_PIX_F32C3: return copy_dispatch_out<3, true>(dest_type, s, region_w, region_h, src_roi_x, src_roi_y, d, offset_x, offset_y, batch_size, stream);
-    default: return copy_dispatch_out<4, true>(dest_type, s, region_w, region_h, src_roi_x, src_roi_y, d, offset_x, offset_y, batch_size, stream);
-  }
+  const dim3 g = hm::grid_for(region_w, region_h, batch_size, kBlock);
+  return with_type(src_type, [&](auto CIv, auto EIv) -> hipError_t {
+    constexpr int CI = decltype(CIv)::value, EI = decltype(EIv)::value;
+    return with_type(dest_type, [&](auto COv, auto EOv) -> hipError_t {
+      constexpr int CO = decltype(COv)::value, EO = decltype(EOv)::value;
+      if constexpr ((CI == 1) == (CO == 1)) {
+        hipLaunchKernelGGL((copy_roi_kernel<CI, EI, CO, EO>), g, kBlock, 0, stream, s, region_w, region_h, src_roi_x,
+                           src_roi_y, d, offset_x, offset_y);
+        return hipGetLastError();
+      }
+      return hipErrorInvalidValue;
+    });
+  });
 }
 
 extern "C" int hmMakeFull(hmSurface src, hmPixelType src_type, int region_w, int region_h, int src_roi_x,

@@ -20,6 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhm_stitch.so")
 
 HM_PIX_U8C3, HM_PIX_U8C4, HM_PIX_F32C3, HM_PIX_F32C4 = 0, 1, 2, 3
+HM_PIX_U8C1, HM_PIX_F32C1, HM_PIX_F16C1, HM_PIX_F16C3, HM_PIX_F16C4 = 4, 5, 6, 7, 8
 HM_BLEND_TWO, HM_BLEND_THREE, HM_BLEND_N = 2, 3, 0
 HM_LEVEL_GAUSS, HM_LEVEL_MASK, HM_LEVEL_LAPLACIAN, HM_LEVEL_BLEND, HM_LEVEL_RECONSTRUCT = range(5)
 
@@ -273,11 +274,15 @@ def _u16(a: np.ndarray) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # pano classes
 # ---------------------------------------------------------------------------
+HM_PIPELINE_F32 = 0x100
+
+
 class StitchPano:
-    """CudaStitchPano<uchar4|uchar3, float4|float3> (src/pano/cudaPano.h:68-115)."""
+    """CudaStitchPano<uchar4|uchar3, float4|float3> (src/pano/cudaPano.h:68-115), or with
+    float_pipeline=True CudaStitchPano<float4|float3, float4|float3>: float inputs and canvas."""
 
     def __init__(self, batch_size: int, num_levels: int, rig, minimize_blend: bool = True, channels: int = 4,
-                 debug: bool = False):
+                 debug: bool = False, float_pipeline: bool = False):
         L = lib()
         self._keep = [_u16(rig.map_x[0]), _u16(rig.map_y[0]), _u16(rig.map_x[1]), _u16(rig.map_y[1]),
                       np.ascontiguousarray(rig.seam, dtype=np.uint8)]
@@ -292,8 +297,9 @@ class StitchPano:
         m.ypos[0], m.ypos[1] = float(rig.pos_y[0]), float(rig.pos_y[1])
         self._masks = m
         h = ctypes.c_void_p()
-        rc = L.hmStitchPanoCreate(batch_size, num_levels, ctypes.byref(m), 1, int(minimize_blend), 0, channels,
-                                  ctypes.byref(h))
+        self.float_pipeline = float_pipeline
+        rc = L.hmStitchPanoCreate(batch_size, num_levels, ctypes.byref(m), 1, int(minimize_blend), 0,
+                                  channels | (HM_PIPELINE_F32 if float_pipeline else 0), ctypes.byref(h))
         if not h.value:
             raise HmError(rc, "hmStitchPanoCreate")
         self._h = h
@@ -321,7 +327,8 @@ class StitchPano:
     def alloc_canvas(self, device="cuda"):
         torch = _torch()
         w, h = self.canvas_size()
-        return torch.empty((self.batch_size, h, w, self.channels), dtype=torch.uint8, device=device)
+        dt = torch.float32 if getattr(self, "float_pipeline", False) else torch.uint8
+        return torch.empty((self.batch_size, h, w, self.channels), dtype=dt, device=device)
 
     def process(self, in1, in2, canvas, stream=None) -> int:
         return lib().hmStitchPanoProcess(self._h, surface(in1), surface(in2), surface(canvas), _stream(stream))
@@ -402,7 +409,7 @@ class StitchPanoN(StitchPano):
     0..n-1)."""
 
     def __init__(self, batch_size: int, num_levels: int, rig, minimize_blend: bool = True, channels: int = 4,
-                 debug: bool = False):
+                 debug: bool = False, float_pipeline: bool = False):
         L = lib()
         n = len(rig.map_x)
         m = hmControlMasksN()
@@ -421,8 +428,9 @@ class StitchPanoN(StitchPano):
         self._masks = m
         self.n_images = n
         h = ctypes.c_void_p()
-        rc = L.hmStitchPanoNCreate(batch_size, num_levels, ctypes.byref(m), int(minimize_blend), 1, channels,
-                                   ctypes.byref(h))
+        self.float_pipeline = float_pipeline
+        rc = L.hmStitchPanoNCreate(batch_size, num_levels, ctypes.byref(m), int(minimize_blend), 1,
+                                   channels | (HM_PIPELINE_F32 if float_pipeline else 0), ctypes.byref(h))
         if not h.value:
             raise HmError(rc, "hmStitchPanoNCreate")
         self._h = h
@@ -445,7 +453,8 @@ class StitchPano3(StitchPano):
 
     `rig` carries three maps, integer positions and the seam (labels 0..2)."""
 
-    def __init__(self, batch_size: int, num_levels: int, rig, channels: int = 4, debug: bool = False):
+    def __init__(self, batch_size: int, num_levels: int, rig, channels: int = 4, debug: bool = False,
+                 float_pipeline: bool = False):
         L = lib()
         m = hmControlMasks3()
         self._keep = []
@@ -461,7 +470,9 @@ class StitchPano3(StitchPano):
         m.seam_h, m.seam_w = seam.shape
         self._masks = m
         h = ctypes.c_void_p()
-        rc = L.hmStitchPano3Create(batch_size, num_levels, ctypes.byref(m), 1, channels, ctypes.byref(h))
+        self.float_pipeline = float_pipeline
+        rc = L.hmStitchPano3Create(batch_size, num_levels, ctypes.byref(m), 1,
+                                   channels | (HM_PIPELINE_F32 if float_pipeline else 0), ctypes.byref(h))
         if not h.value:
             raise HmError(rc, "hmStitchPano3Create")
         self._h = h

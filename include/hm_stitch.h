@@ -44,12 +44,17 @@ typedef struct hmSurface {
 } hmSurface;
 
 /* Pixel types of the template instantiations the reference ships
- * (src/cuda/cudaRemap.cu:507-554). */
+ * (src/cuda/cudaRemap.cu:507-554, cudaMakeFull.cu:291-313). */
 typedef enum hmPixelType {
   HM_PIX_U8C3 = 0,  /* uchar3 */
   HM_PIX_U8C4 = 1,  /* uchar4 */
   HM_PIX_F32C3 = 2, /* float3 */
-  HM_PIX_F32C4 = 3  /* float4 */
+  HM_PIX_F32C4 = 3, /* float4 */
+  HM_PIX_U8C1 = 4,  /* uchar1 */
+  HM_PIX_F32C1 = 5, /* float1 */
+  HM_PIX_F16C1 = 6, /* __half */
+  HM_PIX_F16C3 = 7, /* half3 */
+  HM_PIX_F16C4 = 8  /* half4 */
 } hmPixelType;
 
 /* ------------------------------------------------------------------------- */
@@ -57,7 +62,9 @@ typedef enum hmPixelType {
 /* ------------------------------------------------------------------------- */
 
 /* batched_remap_kernel_ex_offset (src/cuda/cudaRemap.h:41-54,
- * cudaRemap.cu:326-344).  src must be a uint8 type; the default pixel is
+ * cudaRemap.cu:326-344).  src and dest have the same channel count; dest is
+ * the source type, or float for a uint8 source (the reference instantiates
+ * uchar3/4 -> uchar or float, float1/3/4 -> float).  The default pixel is
  * all-zero (T_in{}), as every pano class passes. */
 int hmRemapExOffset(hmSurface src, hmPixelType src_type, hmSurface dest, hmPixelType dest_type,
                     const uint16_t* d_map_x, const uint16_t* d_map_y, int batch_size, int remap_w, int remap_h,
@@ -71,7 +78,8 @@ int hmRemapExOffsetRoi(hmSurface src, hmPixelType src_type, hmSurface dest, hmPi
                        int no_unmapped_write, hipStream_t stream);
 
 /* batched_remap_kernel_ex_offset_with_dest_map (src/cuda/cudaRemap.h:91-105,
- * cudaRemap.cu:396-415).  dest_image_map is unpitched, dest.width wide. */
+ * cudaRemap.cu:396-415).  dest_image_map is unpitched, dest.width wide.
+ * Any pixel type (the reference: uchar1/3/4, float1/3/4, __half). */
 int hmRemapExOffsetWithDestMap(hmSurface src, hmSurface dest, hmPixelType type, const uint16_t* d_map_x,
                                const uint16_t* d_map_y, int this_image_index, const uint8_t* d_dest_image_map,
                                int batch_size, int remap_w, int remap_h, int offset_x, int offset_y,
@@ -86,7 +94,9 @@ int hmRemapHardSeamN(int n_images, const hmSurface* inputs, const uint16_t* cons
                      hipStream_t stream);
 
 /* copy_roi_batched (src/cuda/cudaMakeFull.h:61-86, cudaMakeFull.cu:241-258):
- * typed ROI copy, float -> uint8 truncating. */
+ * typed ROI copy with perform_cast (src/cuda/cudaUtils.cuh:80-213): float or
+ * half -> uint8 clamps and truncates, -> half rounds to nearest even, 3 -> 4
+ * channels sets alpha 255, 4 -> 3 drops it; 1-channel types copy to 1 channel. */
 int hmCopyRoi(hmSurface src, hmPixelType src_type, int region_w, int region_h, int src_roi_x, int src_roi_y,
               hmSurface dest, hmPixelType dest_type, int offset_x, int offset_y, int batch_size,
               hipStream_t stream);
@@ -174,6 +184,13 @@ typedef struct hmControlMasks2 {
 } hmControlMasks2;
 
 typedef struct hmStitchPano hmStitchPano;
+
+/* The pano constructors' `channels` argument: 3 or 4 for the uint8 pipeline
+ * (CudaStitchPano*<uchar3|uchar4, float3|float4>), or-ed with
+ * HM_PIPELINE_F32 for the float pipeline (CudaStitchPano*<float3|float4,
+ * float3|float4>: float inputs and a float canvas, no truncation, as in
+ * src/pano/cudaPano_test.cpp:155, cudaPanoN_test.cpp:154, cudaPano3_test.cpp:173). */
+#define HM_PIPELINE_F32 0x100
 
 /* CudaStitchPano<uchar4|uchar3, float4|float3> ctor (src/pano/cudaPano.h:71-77,
  * cudaPano.inl:19-94).  channels = 4 (uchar4/float4) or 3 (uchar3/float3).

@@ -68,18 +68,86 @@ static size_t pix_index(int b, int x, int y, int w, int h, int c) {
   return (((size_t)b * (size_t)h + (size_t)y) * (size_t)w + (size_t)x) * (size_t)c;
 }
 
-/* Write a u8 pixel (src_c channels) into dst with perform_cast semantics
- * (cudaUtils.cuh:80-213): u8->u8 same channels copies; u8x3->x4 sets
- * alpha 255; u8x4->x3 drops alpha; u8->float is exact. */
-static void store_cast_u8(const uint8_t* sp, int src_c, void* dst, int dst_is_float, size_t di, int dst_c) {
-  if (dst_is_float) {
-    float* d = (float*)dst + di;
-    for (int c = 0; c < 3; ++c) d[c] = (float)sp[c];
-    if (dst_c == 4) d[3] = (src_c == 4) ? (float)sp[3] : 255.0f;
-  } else {
-    uint8_t* d = (uint8_t*)dst + di;
-    for (int c = 0; c < 3; ++c) d[c] = sp[c];
-    if (dst_c == 4) d[3] = (src_c == 4) ? sp[3] : 255;
+/* IEEE binary16 <-> binary32.  Half to float is exact; float to half rounds
+ * to nearest, ties to even, like static_cast<__half>(float) in the
+ * reference's perform_cast (cudaUtils.cuh:43-47). */
+float hmo_f16_to_f32(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const int exp = (h >> 10) & 0x1f;
+  uint32_t man = h & 0x3ffu, bits;
+  if (exp == 0x1f) {
+    bits = sign | 0x7f800000u | (man << 13);
+  } else if (exp != 0) {
+    bits = sign | ((uint32_t)(exp - 15 + 127) << 23) | (man << 13);
+  } else if (man == 0) {
+    bits = sign;
+  } else { /* subnormal: normalise */
+    int e = -14;
+    while (!(man & 0x400u)) {
+      man <<= 1;
+      --e;
+    }
+    bits = sign | ((uint32_t)(e + 127) << 23) | ((man & 0x3ffu) << 13);
+  }
+  float f;
+  memcpy(&f, &bits, 4);
+  return f;
+}
+
+uint16_t hmo_f32_to_f16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  const uint16_t sign = (uint16_t)((x >> 16) & 0x8000u);
+  const uint32_t ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) /* inf, nan (quiet) */
+    return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u | ((ax >> 13) & 0x3ffu) : 0u));
+  if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* rounds to >= 65520: inf */
+  if (ax < 0x38800000u) { /* below the smallest normal half: subnormal or zero */
+    const int e = (int)(ax >> 23);
+    if (e < 102) return sign; /* below half the smallest subnormal (2^-25) */
+    const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+    const int shift = 126 - e; /* value = m * 2^(e-150); in units of 2^-24: m >> (126 - e) */
+    uint32_t r = m >> shift;
+    const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (r & 1u))) ++r;
+    return (uint16_t)(sign | r);
+  }
+  /* normal: keep 10 mantissa bits, round to nearest even (a carry may bump the exponent) */
+  uint32_t r = ((ax >> 13) - ((127 - 15) << 10));
+  const uint32_t rem = ax & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) ++r;
+  return (uint16_t)(sign | r);
+}
+
+static size_t et_size(int et) { return et == HMO_F32 ? 4 : (et == HMO_F16 ? 2 : 1); }
+
+/* Pixel idx (in elements) of a c-channel image as float; 3 channels read
+ * alpha 255 (the 3 -> 4 cast), 1 channel only v[0]. */
+static void load_any(const void* base, int et, size_t idx, int c, float v[4]) {
+  v[1] = v[2] = 0.0f;
+  v[3] = 255.0f;
+  for (int k = 0; k < c; ++k) {
+    if (et == HMO_F32)
+      v[k] = ((const float*)base)[idx + k];
+    else if (et == HMO_F16)
+      v[k] = hmo_f16_to_f32(((const uint16_t*)base)[idx + k]);
+    else
+      v[k] = (float)((const uint8_t*)base)[idx + k];
+  }
+}
+
+/* perform_cast into a dst_c-channel pixel of type et (cudaUtils.cuh:80-213):
+ * to uint8 a float or half source is clamped and truncated (round_to_uchar),
+ * a uint8 source copied; to half rounds to nearest even; to float is exact;
+ * 3 -> 4 gives alpha 255 (v[3]), 4 -> 3 drops alpha. */
+static void store_any(void* base, int et, size_t idx, int dst_c, const float v[4], int src_et) {
+  for (int k = 0; k < dst_c; ++k) {
+    if (et == HMO_F32)
+      ((float*)base)[idx + k] = v[k];
+    else if (et == HMO_F16)
+      ((uint16_t*)base)[idx + k] = hmo_f32_to_f16(v[k]);
+    else
+      ((uint8_t*)base)[idx + k] = src_et == HMO_U8 ? (uint8_t)v[k] : hmo_round_to_uchar(v[k]);
   }
 }
 
@@ -90,10 +158,10 @@ static void store_cast_u8(const uint8_t* sp, int src_c, void* dst, int dst_is_fl
 /* BatchedRemapKernelExOffset (src/cuda/cudaRemap.cu:61-123) and its ROI form
  * (:129-198) with the launcher's ROI validation (:364-369). The default pixel
  * is T_in{} (all zero), as every pano class passes (cudaPano.inl:176). */
-int hmo_remap_offset(const uint8_t* src, int src_w, int src_h, int src_c, void* dst, int dst_is_float, int dst_w,
-                     int dst_h, int dst_c, const uint16_t* map_x, const uint16_t* map_y, int remap_w, int remap_h,
-                     int off_x, int off_y, int roi_x, int roi_y, int roi_w, int roi_h, int no_unmapped_write,
-                     int batch) {
+int hmo_remap_offset_t(const void* src, int src_et, int src_w, int src_h, int src_c, void* dst, int dst_et,
+                       int dst_w, int dst_h, int dst_c, const uint16_t* map_x, const uint16_t* map_y, int remap_w,
+                       int remap_h, int off_x, int off_y, int roi_x, int roi_y, int roi_w, int roi_h,
+                       int no_unmapped_write, int batch) {
   if (roi_w < 0) {
     roi_x = 0;
     roi_y = 0;
@@ -104,7 +172,6 @@ int hmo_remap_offset(const uint8_t* src, int src_w, int src_h, int src_c, void* 
     if (roi_x < 0 || roi_y < 0) return -1;
     if (roi_x + roi_w > remap_w || roi_y + roi_h > remap_h) return -1;
   }
-  const uint8_t zero[4] = {0, 0, 0, 0};
   HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b) {
     for (int y = 0; y < roi_h; ++y) {
@@ -117,17 +184,14 @@ int hmo_remap_offset(const uint8_t* src, int src_w, int src_h, int src_c, void* 
         const size_t mi = (size_t)my * (size_t)remap_w + (size_t)mx;
         const int sx = map_x[mi], sy = map_y[mi];
         if (dst_c == 4 && no_unmapped_write && sx == HMO_UNMAPPED) continue;
+        float v[4];
         if (sx < src_w && sy < src_h) {
-          store_cast_u8(src + pix_index(b, sx, sy, src_w, src_h, src_c), src_c, dst, dst_is_float, di, dst_c);
+          load_any(src, src_et, pix_index(b, sx, sy, src_w, src_h, src_c), src_c, v);
+          store_any(dst, dst_et, di, dst_c, v, src_et);
         } else if (!no_unmapped_write || sx != HMO_UNMAPPED) {
-          store_cast_u8(zero, dst_c, dst, dst_is_float, di, dst_c);
-          if (dst_c == 4) {
-            const int a = (sx == HMO_UNMAPPED) ? 0 : 255;
-            if (dst_is_float)
-              ((float*)dst)[di + 3] = (float)a;
-            else
-              ((uint8_t*)dst)[di + 3] = (uint8_t)a;
-          }
+          v[0] = v[1] = v[2] = 0.0f;
+          v[3] = (sx == HMO_UNMAPPED) ? 0.0f : 255.0f; /* alpha of a 4-channel default pixel */
+          store_any(dst, dst_et, di, dst_c, v, HMO_U8);
         }
       }
     }
@@ -135,12 +199,22 @@ int hmo_remap_offset(const uint8_t* src, int src_w, int src_h, int src_c, void* 
   return 0;
 }
 
+int hmo_remap_offset(const uint8_t* src, int src_w, int src_h, int src_c, void* dst, int dst_is_float, int dst_w,
+                     int dst_h, int dst_c, const uint16_t* map_x, const uint16_t* map_y, int remap_w, int remap_h,
+                     int off_x, int off_y, int roi_x, int roi_y, int roi_w, int roi_h, int no_unmapped_write,
+                     int batch) {
+  return hmo_remap_offset_t(src, HMO_U8, src_w, src_h, src_c, dst, dst_is_float ? HMO_F32 : HMO_U8, dst_w, dst_h,
+                            dst_c, map_x, map_y, remap_w, remap_h, off_x, off_y, roi_x, roi_y, roi_w, roi_h,
+                            no_unmapped_write, batch);
+}
+
 /* BatchedRemapKernelExOffsetWithDestMap (src/cuda/cudaRemap.cu:204-246):
  * writes only where dest_map[destY*dst_w+destX] == image_index; unmapped or
  * out-of-range sources write the default pixel (all zero, alpha 0 too). */
-int hmo_remap_dest_map(const uint8_t* src, int src_w, int src_h, int c, uint8_t* dst, int dst_w, int dst_h,
-                       const uint16_t* map_x, const uint16_t* map_y, int remap_w, int remap_h, int off_x, int off_y,
-                       int image_index, const uint8_t* dest_map, int batch) {
+int hmo_remap_dest_map_t(const void* src, int et, int src_w, int src_h, int c, void* dst, int dst_w, int dst_h,
+                         const uint16_t* map_x, const uint16_t* map_y, int remap_w, int remap_h, int off_x, int off_y,
+                         int image_index, const uint8_t* dest_map, int batch) {
+  const size_t es = et_size(et);
   HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b) {
     for (int y = 0; y < remap_h; ++y) {
@@ -150,11 +224,11 @@ int hmo_remap_dest_map(const uint8_t* src, int src_w, int src_h, int c, uint8_t*
         if (dest_map[(size_t)dy * dst_w + dx] != image_index) continue;
         const size_t mi = (size_t)y * remap_w + x;
         const int sx = map_x[mi], sy = map_y[mi];
-        uint8_t* d = dst + pix_index(b, dx, dy, dst_w, dst_h, c);
+        uint8_t* d = (uint8_t*)dst + pix_index(b, dx, dy, dst_w, dst_h, c) * es;
         if (sx < src_w && sy < src_h) {
-          memcpy(d, src + pix_index(b, sx, sy, src_w, src_h, c), (size_t)c);
+          memcpy(d, (const uint8_t*)src + pix_index(b, sx, sy, src_w, src_h, c) * es, (size_t)c * es);
         } else {
-          memset(d, 0, (size_t)c);
+          memset(d, 0, (size_t)c * es);
         }
       }
     }
@@ -162,13 +236,21 @@ int hmo_remap_dest_map(const uint8_t* src, int src_w, int src_h, int c, uint8_t*
   return 0;
 }
 
+int hmo_remap_dest_map(const uint8_t* src, int src_w, int src_h, int c, uint8_t* dst, int dst_w, int dst_h,
+                       const uint16_t* map_x, const uint16_t* map_y, int remap_w, int remap_h, int off_x, int off_y,
+                       int image_index, const uint8_t* dest_map, int batch) {
+  return hmo_remap_dest_map_t(src, HMO_U8, src_w, src_h, c, dst, dst_w, dst_h, map_x, map_y, remap_w, remap_h, off_x,
+                              off_y, image_index, dest_map, batch);
+}
+
 /* BatchedRemapHardSeamKernelN (src/cuda/cudaRemap.cu:252-294): per canvas
  * pixel pick the seam's image, gather; sentinel / out-of-range leave the
  * destination untouched. */
-int hmo_remap_hard_seam_n(int n, const uint8_t* const* srcs, const int* src_w, const int* src_h, int c,
-                          const uint16_t* const* map_x, const uint16_t* const* map_y, const int* off_x,
-                          const int* off_y, const int* map_w, const int* map_h, const uint8_t* dest_map, uint8_t* dst,
-                          int dst_w, int dst_h, int batch) {
+int hmo_remap_hard_seam_n_t(int n, const void* const* srcs, int et, const int* src_w, const int* src_h, int c,
+                            const uint16_t* const* map_x, const uint16_t* const* map_y, const int* off_x,
+                            const int* off_y, const int* map_w, const int* map_h, const uint8_t* dest_map, void* dst,
+                            int dst_w, int dst_h, int batch) {
+  const size_t es = et_size(et);
   HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b) {
     for (int y = 0; y < dst_h; ++y) {
@@ -181,8 +263,8 @@ int hmo_remap_hard_seam_n(int n, const uint8_t* const* srcs, const int* src_w, c
         const int sx = map_x[i][mi], sy = map_y[i][mi];
         if (sx == HMO_UNMAPPED) continue;
         if (sx < src_w[i] && sy < src_h[i]) {
-          memcpy(dst + pix_index(b, x, y, dst_w, dst_h, c), srcs[i] + pix_index(b, sx, sy, src_w[i], src_h[i], c),
-                 (size_t)c);
+          memcpy((uint8_t*)dst + pix_index(b, x, y, dst_w, dst_h, c) * es,
+                 (const uint8_t*)srcs[i] + pix_index(b, sx, sy, src_w[i], src_h[i], c) * es, (size_t)c * es);
         }
       }
     }
@@ -190,12 +272,21 @@ int hmo_remap_hard_seam_n(int n, const uint8_t* const* srcs, const int* src_w, c
   return 0;
 }
 
+int hmo_remap_hard_seam_n(int n, const uint8_t* const* srcs, const int* src_w, const int* src_h, int c,
+                          const uint16_t* const* map_x, const uint16_t* const* map_y, const int* off_x,
+                          const int* off_y, const int* map_w, const int* map_h, const uint8_t* dest_map, uint8_t* dst,
+                          int dst_w, int dst_h, int batch) {
+  return hmo_remap_hard_seam_n_t(n, (const void* const*)srcs, HMO_U8, src_w, src_h, c, map_x, map_y, off_x, off_y,
+                                 map_w, map_h, dest_map, dst, dst_w, dst_h, batch);
+}
+
 /* copyRoiKernelBatched (src/cuda/cudaMakeFull.cu:40-69) with perform_cast:
  * u8->float exact, float->u8 truncating (cudaUtils.cuh:118-127), float->float
- * and u8->u8 copies; channel conversion as in cudaUtils.cuh:141-183. */
-int hmo_copy_roi(const void* src, int src_is_float, int src_w, int src_h, int src_c, int region_w, int region_h,
-                 int src_roi_x, int src_roi_y, void* dst, int dst_is_float, int dst_w, int dst_h, int dst_c,
-                 int off_x, int off_y, int batch) {
+ * and u8->u8 copies, half as float; channel conversion as in
+ * cudaUtils.cuh:141-183. */
+int hmo_copy_roi_t(const void* src, int src_et, int src_w, int src_h, int src_c, int region_w, int region_h,
+                   int src_roi_x, int src_roi_y, void* dst, int dst_et, int dst_w, int dst_h, int dst_c, int off_x,
+                   int off_y, int batch) {
   HMO_PARALLEL_ROWS
   for (int b = 0; b < batch; ++b) {
     for (int y = 0; y < region_h; ++y) {
@@ -204,32 +295,20 @@ int hmo_copy_roi(const void* src, int src_is_float, int src_w, int src_h, int sr
         if (!(sx < src_w && sy < src_h)) continue;
         const int dx = off_x + x, dy = off_y + y;
         if (!(dx < dst_w && dy < dst_h)) continue;
-        const size_t si = pix_index(b, sx, sy, src_w, src_h, src_c);
-        const size_t di = pix_index(b, dx, dy, dst_w, dst_h, dst_c);
         float v[4];
-        if (src_is_float) {
-          const float* s = (const float*)src + si;
-          for (int c = 0; c < src_c; ++c) v[c] = s[c];
-        } else {
-          const uint8_t* s = (const uint8_t*)src + si;
-          for (int c = 0; c < src_c; ++c) v[c] = (float)s[c];
-        }
-        if (src_c == 3) v[3] = 255.0f;
-        if (dst_is_float) {
-          float* d = (float*)dst + di;
-          for (int c = 0; c < dst_c; ++c) d[c] = v[c];
-        } else {
-          uint8_t* d = (uint8_t*)dst + di;
-          if (src_is_float) {
-            for (int c = 0; c < dst_c; ++c) d[c] = hmo_round_to_uchar(v[c]);
-          } else {
-            for (int c = 0; c < dst_c; ++c) d[c] = (uint8_t)v[c];
-          }
-        }
+        load_any(src, src_et, pix_index(b, sx, sy, src_w, src_h, src_c), src_c, v);
+        store_any(dst, dst_et, pix_index(b, dx, dy, dst_w, dst_h, dst_c), dst_c, v, src_et);
       }
     }
   }
   return 0;
+}
+
+int hmo_copy_roi(const void* src, int src_is_float, int src_w, int src_h, int src_c, int region_w, int region_h,
+                 int src_roi_x, int src_roi_y, void* dst, int dst_is_float, int dst_w, int dst_h, int dst_c,
+                 int off_x, int off_y, int batch) {
+  return hmo_copy_roi_t(src, src_is_float ? HMO_F32 : HMO_U8, src_w, src_h, src_c, region_w, region_h, src_roi_x,
+                        src_roi_y, dst, dst_is_float ? HMO_F32 : HMO_U8, dst_w, dst_h, dst_c, off_x, off_y, batch);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -665,9 +744,9 @@ int hmo_pano2_canvas_size(const hmo_pano2_cfg* cfg, int* canvas_w, int* canvas_h
   return 0;
 }
 
-int hmo_pano2_process(const hmo_pano2_cfg* cfg, const uint8_t* in1, const uint8_t* in2, uint8_t* canvas,
-                      hmo_dump* dump) {
+int hmo_pano2_process(const hmo_pano2_cfg* cfg, const void* in1, const void* in2, void* canvas, hmo_dump* dump) {
   const int c = cfg->channels, batch = cfg->batch;
+  const int pe = cfg->pipeline_f32 ? HMO_F32 : HMO_U8; /* element type of inputs and canvas */
   if ((c != 3 && c != 4) || batch < 1 || cfg->levels < 0) return -1;
   int cw, ch;
   hmo_pano2_canvas_size(cfg, &cw, &ch);
@@ -689,14 +768,14 @@ int hmo_pano2_process(const hmo_pano2_cfg* cfg, const uint8_t* in1, const uint8_
   uint8_t* padded = pad_replicate(cfg->seam, cfg->seam_w, cfg->seam_h, cw, ch);
   if (!padded) return -1;
   const size_t cpx = (size_t)cw * ch;
-  memset(canvas, 0, cpx * c * batch); /* cudaPano.inl:186-187 */
+  memset(canvas, 0, cpx * c * batch * et_size(pe)); /* cudaPano.inl:186-187 */
 
   if (hard) {
     /* cudaPano.inl:262-276 and :345-359 (indices inverted at load time). */
-    hmo_remap_dest_map(in1, cfg->src_w[0], cfg->src_h[0], c, canvas, cw, ch, cfg->map_x[0], cfg->map_y[0],
-                       cfg->map_w[0], cfg->map_h[0], x1, y1, 1, padded, batch);
-    hmo_remap_dest_map(in2, cfg->src_w[1], cfg->src_h[1], c, canvas, cw, ch, cfg->map_x[1], cfg->map_y[1],
-                       cfg->map_w[1], cfg->map_h[1], x2, y2, 0, padded, batch);
+    hmo_remap_dest_map_t(in1, pe, cfg->src_w[0], cfg->src_h[0], c, canvas, cw, ch, cfg->map_x[0], cfg->map_y[0],
+                         cfg->map_w[0], cfg->map_h[0], x1, y1, 1, padded, batch);
+    hmo_remap_dest_map_t(in2, pe, cfg->src_w[1], cfg->src_h[1], c, canvas, cw, ch, cfg->map_x[1], cfg->map_y[1],
+                         cfg->map_w[1], cfg->map_h[1], x2, y2, 0, padded, batch);
     free(padded);
     return 0;
   }
@@ -715,28 +794,28 @@ int hmo_pano2_process(const hmo_pano2_cfg* cfg, const uint8_t* in1, const uint8_
 
   if (minimize) {
     /* cudaPano.inl:210-237 */
-    hmo_remap_offset(in1, cfg->src_w[0], cfg->src_h[0], c, canvas, 0, cw, ch, c, cfg->map_x[0], cfg->map_y[0],
-                     cfg->map_w[0], cfg->map_h[0], x1, y1, 0, 0, -1, -1, 0, batch);
+    hmo_remap_offset_t(in1, pe, cfg->src_w[0], cfg->src_h[0], c, canvas, pe, cw, ch, c, cfg->map_x[0],
+                       cfg->map_y[0], cfg->map_w[0], cfg->map_h[0], x1, y1, 0, 0, -1, -1, 0, batch);
     memset(full1, 0, fpx * sizeof(float));
-    hmo_copy_roi(canvas, 0, cw, ch, c, roi_w, bh, roi1_x, 0, full1, 1, bw, bh, c, x1, 0, batch);
+    hmo_copy_roi_t(canvas, pe, cw, ch, c, roi_w, bh, roi1_x, 0, full1, HMO_F32, bw, bh, c, x1, 0, batch);
     /* cudaPano.inl:291-318 */
-    hmo_remap_offset(in2, cfg->src_w[1], cfg->src_h[1], c, canvas, 0, cw, ch, c, cfg->map_x[1], cfg->map_y[1],
-                     cfg->map_w[1], cfg->map_h[1], x2, y2, 0, 0, -1, -1, 0, batch);
+    hmo_remap_offset_t(in2, pe, cfg->src_w[1], cfg->src_h[1], c, canvas, pe, cw, ch, c, cfg->map_x[1],
+                       cfg->map_y[1], cfg->map_w[1], cfg->map_h[1], x2, y2, 0, 0, -1, -1, 0, batch);
     memset(full2, 0, fpx * sizeof(float));
-    hmo_copy_roi(canvas, 0, cw, ch, c, roi_w, bh, x2, 0, full2, 1, bw, bh, c, rm2_xpos, 0, batch);
+    hmo_copy_roi_t(canvas, pe, cw, ch, c, roi_w, bh, x2, 0, full2, HMO_F32, bw, bh, c, rm2_xpos, 0, batch);
   } else {
     /* cudaPano.inl:242-255 and :323-336 */
-    hmo_remap_offset(in1, cfg->src_w[0], cfg->src_h[0], c, full1, 1, bw, bh, c, cfg->map_x[0], cfg->map_y[0],
-                     cfg->map_w[0], cfg->map_h[0], x1, y1, 0, 0, -1, -1, 0, batch);
-    hmo_remap_offset(in2, cfg->src_w[1], cfg->src_h[1], c, full2, 1, bw, bh, c, cfg->map_x[1], cfg->map_y[1],
-                     cfg->map_w[1], cfg->map_h[1], x2, y2, 0, 0, -1, -1, 0, batch);
+    hmo_remap_offset_t(in1, pe, cfg->src_w[0], cfg->src_h[0], c, full1, HMO_F32, bw, bh, c, cfg->map_x[0],
+                       cfg->map_y[0], cfg->map_w[0], cfg->map_h[0], x1, y1, 0, 0, -1, -1, 0, batch);
+    hmo_remap_offset_t(in2, pe, cfg->src_w[1], cfg->src_h[1], c, full2, HMO_F32, bw, bh, c, cfg->map_x[1],
+                       cfg->map_y[1], cfg->map_w[1], cfg->map_h[1], x2, y2, 0, 0, -1, -1, 0, batch);
   }
   /* cudaPano.inl:384-392 (output aliases full1) and :401-411 (write-back). */
   float* blended = (float*)malloc(fpx * sizeof(float));
   int rc = hmo_blend2(full1, full2, mask, blended, bw, bh, c, cfg->levels, batch, dump);
   if (rc == 0)
-    hmo_copy_roi(blended, 1, bw, bh, c, bw, bh, 0, 0, canvas, 0, cw, ch, c, minimize ? x2 - OVERLAP_PAD : 0, 0,
-                 batch);
+    hmo_copy_roi_t(blended, HMO_F32, bw, bh, c, bw, bh, 0, 0, canvas, pe, cw, ch, c, minimize ? x2 - OVERLAP_PAD : 0, 0,
+                   batch);
   free(blended);
   free(full1);
   free(full2);
@@ -873,8 +952,10 @@ int hmo_panoN_rois(const hmo_panoN_cfg* cfg, int* write_roi, int* blend_roi) {
   return 0;
 }
 
-int hmo_panoN_process(const hmo_panoN_cfg* cfg, const uint8_t* const* inputs, uint8_t* canvas, hmo_dump* dump) {
+int hmo_panoN_process(const hmo_panoN_cfg* cfg, const void* const* inputs, void* canvas, hmo_dump* dump) {
   const int n = cfg->n, c = cfg->channels, batch = cfg->batch;
+  const int pe = cfg->pipeline_f32 ? HMO_F32 : HMO_U8; /* element type of inputs and canvas */
+  const size_t ces = et_size(pe);
   if (n < 2 || n > HMO_MAX_IMAGES || (c != 3 && c != 4) || batch < 1 || cfg->levels < 0) return -1;
   int cw, ch;
   hmo_panoN_canvas_size(cfg, &cw, &ch);
@@ -885,9 +966,9 @@ int hmo_panoN_process(const hmo_panoN_cfg* cfg, const uint8_t* const* inputs, ui
 
   if (cfg->levels == 0) {
     /* cudaPanoN.inl:508-541 */
-    memset(canvas, 0, cpx * c * batch);
-    hmo_remap_hard_seam_n(n, inputs, cfg->src_w, cfg->src_h, c, cfg->map_x, cfg->map_y, cfg->pos_x, cfg->pos_y,
-                          cfg->map_w, cfg->map_h, padded, canvas, cw, ch, batch);
+    memset(canvas, 0, cpx * c * batch * ces);
+    hmo_remap_hard_seam_n_t(n, inputs, pe, cfg->src_w, cfg->src_h, c, cfg->map_x, cfg->map_y, cfg->pos_x, cfg->pos_y,
+                            cfg->map_w, cfg->map_h, padded, canvas, cw, ch, batch);
     free(padded);
     return 0;
   }
@@ -916,9 +997,9 @@ int hmo_panoN_process(const hmo_panoN_cfg* cfg, const uint8_t* const* inputs, ui
 
   if (use_min) {
     /* cudaPanoN.inl:545-641 */
-    memset(canvas, 0, cpx * c * batch);
-    hmo_remap_hard_seam_n(n, inputs, cfg->src_w, cfg->src_h, c, cfg->map_x, cfg->map_y, cfg->pos_x, cfg->pos_y,
-                          cfg->map_w, cfg->map_h, padded, canvas, cw, ch, batch);
+    memset(canvas, 0, cpx * c * batch * ces);
+    hmo_remap_hard_seam_n_t(n, inputs, pe, cfg->src_w, cfg->src_h, c, cfg->map_x, cfg->map_y, cfg->pos_x, cfg->pos_y,
+                            cfg->map_w, cfg->map_h, padded, canvas, cw, ch, batch);
     for (int i = 0; i < n; ++i) {
       rect_t img = {cfg->pos_x[i], cfg->pos_y[i], cfg->map_w[i], cfg->map_h[i]};
       rect_t inter = rect_and(img, blend_roi);
@@ -929,23 +1010,23 @@ int hmo_panoN_process(const hmo_panoN_cfg* cfg, const uint8_t* const* inputs, ui
         roi.w = inter.w;
         roi.h = inter.h;
       }
-      rc = hmo_remap_offset(inputs[i], cfg->src_w[i], cfg->src_h[i], c, full[i], 1, bw, bh, c, cfg->map_x[i],
-                            cfg->map_y[i], cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i] - blend_roi.x,
-                            cfg->pos_y[i] - blend_roi.y, roi.x, roi.y, roi.w, roi.h, 0, batch);
+      rc = hmo_remap_offset_t(inputs[i], pe, cfg->src_w[i], cfg->src_h[i], c, full[i], HMO_F32, bw, bh, c,
+                              cfg->map_x[i], cfg->map_y[i], cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i] - blend_roi.x,
+                              cfg->pos_y[i] - blend_roi.y, roi.x, roi.y, roi.w, roi.h, 0, batch);
       if (rc) break;
     }
     if (!rc) rc = hmo_blend_n(n, (const float* const*)full, mask, out, bw, bh, c, cfg->levels, batch, dump);
     if (!rc)
-      hmo_copy_roi(out, 1, bw, bh, c, write_roi.w, write_roi.h, write_roi.x - blend_roi.x, write_roi.y - blend_roi.y,
-                   canvas, 0, cw, ch, c, write_roi.x, write_roi.y, batch);
+      hmo_copy_roi_t(out, HMO_F32, bw, bh, c, write_roi.w, write_roi.h, write_roi.x - blend_roi.x,
+                     write_roi.y - blend_roi.y, canvas, pe, cw, ch, c, write_roi.x, write_roi.y, batch);
   } else {
     /* cudaPanoN.inl:644-684 (the canvas is not cleared on this path). */
     for (int i = 0; i < n; ++i)
-      hmo_remap_offset(inputs[i], cfg->src_w[i], cfg->src_h[i], c, full[i], 1, bw, bh, c, cfg->map_x[i],
-                       cfg->map_y[i], cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], 0, 0, -1, -1, 0,
-                       batch);
+      hmo_remap_offset_t(inputs[i], pe, cfg->src_w[i], cfg->src_h[i], c, full[i], HMO_F32, bw, bh, c, cfg->map_x[i],
+                         cfg->map_y[i], cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], 0, 0, -1, -1, 0,
+                         batch);
     rc = hmo_blend_n(n, (const float* const*)full, mask, out, bw, bh, c, cfg->levels, batch, dump);
-    if (!rc) hmo_copy_roi(out, 1, bw, bh, c, bw, bh, 0, 0, canvas, 0, cw, ch, c, 0, 0, batch);
+    if (!rc) hmo_copy_roi_t(out, HMO_F32, bw, bh, c, bw, bh, 0, 0, canvas, pe, cw, ch, c, 0, 0, batch);
   }
   for (int i = 0; i < n; ++i) free(full[i]);
   free(out);
@@ -972,7 +1053,7 @@ int hmo_pano3_canvas_size(const hmo_pano3_cfg* cfg, int* canvas_w, int* canvas_h
 
 /* FusedRemapToFullKernel3 (cudaFusedKernels3.cu:25-119): over the full
  * buffers' extent, each image's box writes only mapped, in-range pixels. */
-static void fused_remap_to_full3(const hmo_pano3_cfg* cfg, const uint8_t* const* in, float* const* full, int fw,
+static void fused_remap_to_full3(const hmo_pano3_cfg* cfg, const void* const* in, int pe, float* const* full, int fw,
                                  int fh) {
   const int c = cfg->channels;
   HMO_PARALLEL_ROWS
@@ -985,15 +1066,17 @@ static void fused_remap_to_full3(const hmo_pano3_cfg* cfg, const uint8_t* const*
           const size_t mi = (size_t)ry * cfg->map_w[i] + rx;
           const int sx = cfg->map_x[i][mi], sy = cfg->map_y[i][mi];
           if (sx == HMO_UNMAPPED || !(sx < cfg->src_w[i] && sy < cfg->src_h[i])) continue;
-          store_cast_u8(in[i] + pix_index(b, sx, sy, cfg->src_w[i], cfg->src_h[i], c), c, full[i], 1,
-                        pix_index(b, x, y, fw, fh, c), c);
+          float v[4];
+          load_any(in[i], pe, pix_index(b, sx, sy, cfg->src_w[i], cfg->src_h[i], c), c, v);
+          store_any(full[i], HMO_F32, pix_index(b, x, y, fw, fh, c), c, v, pe);
         }
 }
 
 /* FusedRemapHardSeam3 (cudaFusedKernels3.cu:124-217): seam indexed with the
  * canvas width; unmapped / out-of-range pixels are left untouched. */
-static void fused_hard_seam3(const hmo_pano3_cfg* cfg, const uint8_t* const* in, uint8_t* canvas, int cw, int ch) {
+static void fused_hard_seam3(const hmo_pano3_cfg* cfg, const void* const* in, int pe, void* canvas, int cw, int ch) {
   const int c = cfg->channels;
+  const size_t es = et_size(pe);
   HMO_PARALLEL_ROWS
   for (int b = 0; b < cfg->batch; ++b)
     for (int y = 0; y < ch; ++y)
@@ -1005,13 +1088,14 @@ static void fused_hard_seam3(const hmo_pano3_cfg* cfg, const uint8_t* const* in,
         const size_t mi = (size_t)ry * cfg->map_w[i] + rx;
         const int sx = cfg->map_x[i][mi], sy = cfg->map_y[i][mi];
         if (sx == HMO_UNMAPPED || !(sx < cfg->src_w[i] && sy < cfg->src_h[i])) continue;
-        memcpy(canvas + pix_index(b, x, y, cw, ch, c), in[i] + pix_index(b, sx, sy, cfg->src_w[i], cfg->src_h[i], c),
-               (size_t)c);
+        memcpy((uint8_t*)canvas + pix_index(b, x, y, cw, ch, c) * es,
+               (const uint8_t*)in[i] + pix_index(b, sx, sy, cfg->src_w[i], cfg->src_h[i], c) * es, (size_t)c * es);
       }
 }
 
-int hmo_pano3_process(const hmo_pano3_cfg* cfg, const uint8_t* const* inputs, uint8_t* canvas, hmo_dump* dump) {
+int hmo_pano3_process(const hmo_pano3_cfg* cfg, const void* const* inputs, void* canvas, hmo_dump* dump) {
   const int c = cfg->channels, batch = cfg->batch;
+  const int pe = cfg->pipeline_f32 ? HMO_F32 : HMO_U8; /* element type of inputs and canvas */
   if ((c != 3 && c != 4) || batch < 1 || cfg->levels < 0) return -1;
   int cw, ch;
   hmo_pano3_canvas_size(cfg, &cw, &ch);
@@ -1021,14 +1105,15 @@ int hmo_pano3_process(const hmo_pano3_cfg* cfg, const uint8_t* const* inputs, ui
    * (CanvasManager3::convertMaskMat, canvasManager3.cpp:107-128). */
   if (hard && (cfg->seam_w != cw || cfg->seam_h < ch)) return -1;
   if (!hard && cfg->seam_h < ch) return -1;
-  if (c == 4) memset(canvas, 0, (size_t)cw * ch * c * batch); /* cudaPano3_optimized.inl:36-39 */
+  if (c == 4) memset(canvas, 0, (size_t)cw * ch * c * batch * et_size(pe)); /* cudaPano3_optimized.inl:36-39 */
   if (hard) {
     if (cfg->fused) {
-      fused_hard_seam3(cfg, inputs, canvas, cw, ch);
+      fused_hard_seam3(cfg, inputs, pe, canvas, cw, ch);
     } else {
       for (int i = 0; i < 3; ++i) /* cudaPano3.inl: dest-map remaps in image order */
-        hmo_remap_dest_map(inputs[i], cfg->src_w[i], cfg->src_h[i], c, canvas, cw, ch, cfg->map_x[i], cfg->map_y[i],
-                           cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], i, cfg->seam, batch);
+        hmo_remap_dest_map_t(inputs[i], pe, cfg->src_w[i], cfg->src_h[i], c, canvas, cw, ch, cfg->map_x[i],
+                             cfg->map_y[i], cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], i, cfg->seam,
+                             batch);
     }
     return 0;
   }
@@ -1043,21 +1128,21 @@ int hmo_pano3_process(const hmo_pano3_cfg* cfg, const uint8_t* const* inputs, ui
   float* full[3];
   for (int i = 0; i < 3; ++i) full[i] = (float*)calloc(fpx, sizeof(float));
   if (cfg->fused) {
-    fused_remap_to_full3(cfg, inputs, full, bw, bh);
+    fused_remap_to_full3(cfg, inputs, pe, full, bw, bh);
   } else {
     /* cudaPano3.inl:210-305: remap i onto the canvas, then copy its box into full i. */
     for (int i = 0; i < 3; ++i) {
-      hmo_remap_offset(inputs[i], cfg->src_w[i], cfg->src_h[i], c, canvas, 0, cw, ch, c, cfg->map_x[i],
-                       cfg->map_y[i], cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], 0, 0, -1, -1, 0,
-                       batch);
+      hmo_remap_offset_t(inputs[i], pe, cfg->src_w[i], cfg->src_h[i], c, canvas, pe, cw, ch, c, cfg->map_x[i],
+                         cfg->map_y[i], cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], 0, 0, -1, -1, 0,
+                         batch);
       memset(full[i], 0, fpx * sizeof(float));
-      hmo_copy_roi(canvas, 0, cw, ch, c, cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], full[i], 1, bw,
-                   bh, c, cfg->pos_x[i], cfg->pos_y[i], batch);
+      hmo_copy_roi_t(canvas, pe, cw, ch, c, cfg->map_w[i], cfg->map_h[i], cfg->pos_x[i], cfg->pos_y[i], full[i],
+                     HMO_F32, bw, bh, c, cfg->pos_x[i], cfg->pos_y[i], batch);
     }
   }
   float* out = (float*)calloc(fpx, sizeof(float));
   int rc = hmo_blend3(full[0], full[1], full[2], mask, out, bw, bh, c, cfg->levels, batch, dump);
-  if (!rc) hmo_copy_roi(out, 1, bw, bh, c, bw, bh, 0, 0, canvas, 0, cw, ch, c, 0, 0, batch);
+  if (!rc) hmo_copy_roi_t(out, HMO_F32, bw, bh, c, bw, bh, 0, 0, canvas, pe, cw, ch, c, 0, 0, batch);
   for (int i = 0; i < 3; ++i) free(full[i]);
   free(out);
   free(mask);

@@ -28,6 +28,12 @@ extern "C" {
 
 #define HMO_UNMAPPED 65535
 
+/* Element types of pipeline images (the reference's uchar, float and __half
+ * vectors). */
+#define HMO_U8 0
+#define HMO_F32 1
+#define HMO_F16 2
+
 /* Per-level pyramid dump (gauss/mask/lap/blend/reconstruct) -- mirrors the
  * layout written by CudaStitchPano::dump_soft_blend_pyramid
  * (src/pano/cudaPano.inl:420-458).  Each pointer array has `levels` entries;
@@ -59,7 +65,27 @@ uint8_t hmo_round_to_uchar(float x);
  *                        (src/cuda/cudaBlendN.cu:675-691). Returns levels. */
 int hmo_pyramid_sizes(int w, int h, int levels, int clamp_small_sides, int* widths, int* heights);
 
-/* ---- remap family: src/cuda/cudaRemap.cu ---- */
+/* IEEE binary16 conversions: exact widening, round-to-nearest-even narrowing. */
+float hmo_f16_to_f32(uint16_t h);
+uint16_t hmo_f32_to_f16(float f);
+
+/* ---- remap family: src/cuda/cudaRemap.cu ----
+ * The _t forms take any element type (HMO_U8 / HMO_F32 / HMO_F16) and 1, 3
+ * or 4 channels; the others are their uint8-source shorthands. */
+int hmo_remap_offset_t(const void* src, int src_et, int src_w, int src_h, int src_c, void* dst, int dst_et,
+                       int dst_w, int dst_h, int dst_c, const uint16_t* map_x, const uint16_t* map_y, int remap_w,
+                       int remap_h, int off_x, int off_y, int roi_x, int roi_y, int roi_w, int roi_h,
+                       int no_unmapped_write, int batch);
+int hmo_remap_dest_map_t(const void* src, int et, int src_w, int src_h, int c, void* dst, int dst_w, int dst_h,
+                         const uint16_t* map_x, const uint16_t* map_y, int remap_w, int remap_h, int off_x, int off_y,
+                         int image_index, const uint8_t* dest_map, int batch);
+int hmo_remap_hard_seam_n_t(int n, const void* const* srcs, int et, const int* src_w, const int* src_h, int c,
+                            const uint16_t* const* map_x, const uint16_t* const* map_y, const int* off_x,
+                            const int* off_y, const int* map_w, const int* map_h, const uint8_t* dest_map, void* dst,
+                            int dst_w, int dst_h, int batch);
+int hmo_copy_roi_t(const void* src, int src_et, int src_w, int src_h, int src_c, int region_w, int region_h,
+                   int src_roi_x, int src_roi_y, void* dst, int dst_et, int dst_w, int dst_h, int dst_c, int off_x,
+                   int off_y, int batch);
 /* BatchedRemapKernelExOffset / ...ROI (cudaRemap.cu:61-198, host :326-390).
  * roi_w < 0 selects the whole map (non-ROI launcher). dst_is_float selects a
  * float destination (perform_cast uchar->float), else uint8. */
@@ -121,10 +147,11 @@ typedef struct hmo_pano2_cfg {
   int pos_x[2], pos_y[2];    /* normalized integer positions */
   const uint8_t* seam;       /* post-load seam (1 = image 1), seam_w x seam_h */
   int seam_w, seam_h;
+  int pipeline_f32;           /* 1: float pipeline images and canvas (CudaStitchPano*<float4|float3, ...>) */
 } hmo_pano2_cfg;
 int hmo_pano2_canvas_size(const hmo_pano2_cfg* cfg, int* canvas_w, int* canvas_h);
-int hmo_pano2_process(const hmo_pano2_cfg* cfg, const uint8_t* in1, const uint8_t* in2, uint8_t* canvas,
-                      hmo_dump* dump);
+/* in1/in2/canvas: uint8, or float when cfg->pipeline_f32 */
+int hmo_pano2_process(const hmo_pano2_cfg* cfg, const void* in1, const void* in2, void* canvas, hmo_dump* dump);
 
 /* ---- N-image pano: CudaStitchPanoN ctor + process ----
  * (src/pano/cudaPanoN.inl:12-370, 489-685; canvasManagerN.h:13-79) */
@@ -137,11 +164,12 @@ typedef struct hmo_panoN_cfg {
   int pos_x[HMO_MAX_IMAGES], pos_y[HMO_MAX_IMAGES];
   const uint8_t* seam;       /* labels 0..n-1, seam_w x seam_h */
   int seam_w, seam_h;
+  int pipeline_f32;           /* 1: float pipeline images and canvas (CudaStitchPano*<float4|float3, ...>) */
 } hmo_panoN_cfg;
 int hmo_panoN_canvas_size(const hmo_panoN_cfg* cfg, int* canvas_w, int* canvas_h);
 /* ROIs chosen by the ctor (for tests): write/blend ROI as x,y,w,h. */
 int hmo_panoN_rois(const hmo_panoN_cfg* cfg, int* write_roi, int* blend_roi);
-int hmo_panoN_process(const hmo_panoN_cfg* cfg, const uint8_t* const* inputs, uint8_t* canvas, hmo_dump* dump);
+int hmo_panoN_process(const hmo_panoN_cfg* cfg, const void* const* inputs, void* canvas, hmo_dump* dump);
 
 /* ---- 3-image pano: CudaStitchPano3 ctor + process ----
  * (src/pano/cudaPano3.inl:19-110, 183-422; cudaPano3_optimized.inl:19-163;
@@ -156,9 +184,10 @@ typedef struct hmo_pano3_cfg {
   int pos_x[3], pos_y[3];
   const uint8_t* seam;       /* labels 0..2, seam_w x seam_h */
   int seam_w, seam_h;
+  int pipeline_f32;           /* 1: float pipeline images and canvas (CudaStitchPano*<float4|float3, ...>) */
 } hmo_pano3_cfg;
 int hmo_pano3_canvas_size(const hmo_pano3_cfg* cfg, int* canvas_w, int* canvas_h);
-int hmo_pano3_process(const hmo_pano3_cfg* cfg, const uint8_t* const* inputs, uint8_t* canvas, hmo_dump* dump);
+int hmo_pano3_process(const hmo_pano3_cfg* cfg, const void* const* inputs, void* canvas, hmo_dump* dump);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,7 @@ class hmo_pano2_cfg(ctypes.Structure):
         ("src_w", _I * 2), ("src_h", _I * 2), ("map_w", _I * 2), ("map_h", _I * 2),
         ("map_x", _P * 2), ("map_y", _P * 2), ("pos_x", _I * 2), ("pos_y", _I * 2),
         ("seam", _P), ("seam_w", _I), ("seam_h", _I),
+        ("pipeline_f32", _I),
     ]
 
 
@@ -52,6 +53,7 @@ class hmo_panoN_cfg(ctypes.Structure):
         ("map_x", _P * MAX_IMAGES), ("map_y", _P * MAX_IMAGES),
         ("pos_x", _I * MAX_IMAGES), ("pos_y", _I * MAX_IMAGES),
         ("seam", _P), ("seam_w", _I), ("seam_h", _I),
+        ("pipeline_f32", _I),
     ]
 
 
@@ -61,6 +63,7 @@ class hmo_pano3_cfg(ctypes.Structure):
         ("src_w", _I * 3), ("src_h", _I * 3), ("map_w", _I * 3), ("map_h", _I * 3),
         ("map_x", _P * 3), ("map_y", _P * 3), ("pos_x", _I * 3), ("pos_y", _I * 3),
         ("seam", _P), ("seam_w", _I), ("seam_h", _I),
+        ("pipeline_f32", _I),
     ]
 
 
@@ -82,6 +85,15 @@ def lib():
         L.hmo_pyramid_sizes.argtypes = [_I, _I, _I, _I, _P, _P]
         L.hmo_pad_replicate.argtypes = [_P, _I, _I, _P, _I, _I]
         L.hmo_set_threads.argtypes = [_I]
+        L.hmo_f16_to_f32.restype = ctypes.c_float
+        L.hmo_f16_to_f32.argtypes = [ctypes.c_uint16]
+        L.hmo_f32_to_f16.restype = ctypes.c_uint16
+        L.hmo_f32_to_f16.argtypes = [ctypes.c_float]
+        L.hmo_remap_offset_t.argtypes = [_P, _I, _I, _I, _I, _P, _I, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _I,
+                                         _I, _I, _I]
+        L.hmo_remap_dest_map_t.argtypes = [_P, _I, _I, _I, _I, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _P, _I]
+        L.hmo_remap_hard_seam_n_t.argtypes = [_I, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I]
+        L.hmo_copy_roi_t.argtypes = [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _I, _I, _I, _I, _I, _I, _I]
         L.hmo_remap_offset.argtypes = [_P, _I, _I, _I, _P, _I, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I,
                                        _I]
         L.hmo_remap_dest_map.argtypes = [_P, _I, _I, _I, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _P, _I]
@@ -109,6 +121,22 @@ def lib():
 
 def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
+
+
+U8, F32, F16 = 0, 1, 2
+
+
+def _et(a: np.ndarray) -> int:
+    """Element type code of an image array (uint8, float32 or float16)."""
+    return {np.dtype(np.uint8): U8, np.dtype(np.float32): F32, np.dtype(np.float16): F16}[a.dtype]
+
+
+def f16_to_f32(h: int) -> float:
+    return float(lib().hmo_f16_to_f32(h))
+
+
+def f32_to_f16(f: float) -> int:
+    return int(lib().hmo_f32_to_f16(ctypes.c_float(f)))
 
 
 def round_to_uchar(x: float) -> int:
@@ -197,9 +225,9 @@ def remap_offset(src: np.ndarray, dst: np.ndarray, map_x, map_y, off_x, off_y, n
     mx = np.ascontiguousarray(map_x, np.uint16)
     my = np.ascontiguousarray(map_y, np.uint16)
     rx, ry, rw, rh = roi if roi is not None else (0, 0, -1, -1)
-    return lib().hmo_remap_offset(_ptr(src), sw, sh, sc, _ptr(dst), int(dst.dtype == np.float32), dw, dh, dc,
-                                  _ptr(mx), _ptr(my), mx.shape[1], mx.shape[0], off_x, off_y, rx, ry, rw, rh,
-                                  int(no_unmapped_write), b)
+    return lib().hmo_remap_offset_t(_ptr(src), _et(src), sw, sh, sc, _ptr(dst), _et(dst), dw, dh, dc,
+                                    _ptr(mx), _ptr(my), mx.shape[1], mx.shape[0], off_x, off_y, rx, ry, rw, rh,
+                                    int(no_unmapped_write), b)
 
 
 def remap_dest_map(src, dst, map_x, map_y, off_x, off_y, image_index, dest_map):
@@ -208,8 +236,8 @@ def remap_dest_map(src, dst, map_x, map_y, off_x, off_y, image_index, dest_map):
     mx = np.ascontiguousarray(map_x, np.uint16)
     my = np.ascontiguousarray(map_y, np.uint16)
     dm = np.ascontiguousarray(dest_map, np.uint8)
-    return lib().hmo_remap_dest_map(_ptr(src), sw, sh, c, _ptr(dst), dw, dh, _ptr(mx), _ptr(my), mx.shape[1],
-                                    mx.shape[0], off_x, off_y, image_index, _ptr(dm), b)
+    return lib().hmo_remap_dest_map_t(_ptr(src), _et(src), sw, sh, c, _ptr(dst), dw, dh, _ptr(mx), _ptr(my),
+                                      mx.shape[1], mx.shape[0], off_x, off_y, image_index, _ptr(dm), b)
 
 
 def remap_hard_seam_n(srcs, maps_x, maps_y, offsets, dest_map, dst):
@@ -228,14 +256,15 @@ def remap_hard_seam_n(srcs, maps_x, maps_y, offsets, dest_map, dst):
     mw = (_I * n)(*[m.shape[1] for m in mxs])
     mh = (_I * n)(*[m.shape[0] for m in mxs])
     dm = np.ascontiguousarray(dest_map, np.uint8)
-    return lib().hmo_remap_hard_seam_n(n, sp, sw, sh, c, mxp, myp, ox, oy, mw, mh, _ptr(dm), _ptr(dst), dw, dh, b)
+    return lib().hmo_remap_hard_seam_n_t(n, sp, _et(dst), sw, sh, c, mxp, myp, ox, oy, mw, mh, _ptr(dm), _ptr(dst),
+                                         dw, dh, b)
 
 
 def copy_roi(src, region_w, region_h, sx, sy, dst, ox, oy):
     b, sh, sw, sc = src.shape
     _, dh, dw, dc = dst.shape
-    return lib().hmo_copy_roi(_ptr(src), int(src.dtype == np.float32), sw, sh, sc, region_w, region_h, sx, sy,
-                              _ptr(dst), int(dst.dtype == np.float32), dw, dh, dc, ox, oy, b)
+    return lib().hmo_copy_roi_t(_ptr(src), _et(src), sw, sh, sc, region_w, region_h, sx, sy,
+                                _ptr(dst), _et(dst), dw, dh, dc, ox, oy, b)
 
 
 # ---------------------------------------------------------------------------
@@ -261,13 +290,15 @@ def _pano2_cfg(rig, levels, minimize, channels, batch, src_w, src_h):
 
 
 def pano2(rig, in1: np.ndarray, in2: np.ndarray, levels: int, minimize: bool = True, dump: bool = False):
-    """CudaStitchPano::process on the CPU.  in1/in2: uint8 [B,H,W,C]."""
+    """CudaStitchPano::process on the CPU.  in1/in2: uint8 [B,H,W,C], or float32 for the
+    float pipeline (CudaStitchPano<float4|float3, ...>; the canvas is then float32)."""
     b, h1, w1, c = in1.shape
     _, h2, w2, _ = in2.shape
     cfg, keep = _pano2_cfg(rig, levels, minimize, c, b, (w1, w2), (h1, h2))
+    cfg.pipeline_f32 = int(in1.dtype == np.float32)
     cw, ch = _I(), _I()
     lib().hmo_pano2_canvas_size(ctypes.byref(cfg), ctypes.byref(cw), ctypes.byref(ch))
-    canvas = np.zeros((b, ch.value, cw.value, c), np.uint8)
+    canvas = np.zeros((b, ch.value, cw.value, c), in1.dtype)
     d = None
     if dump and levels > 0:
         bw = (rig.map_x[0].shape[1] - rig.pos_x[1] + 256) if minimize else cw.value
@@ -311,9 +342,10 @@ def panoN(rig, inputs: List[np.ndarray], levels: int, minimize: bool = True, dum
     b, _, _, c = inputs[0].shape
     srcs = [np.ascontiguousarray(x) for x in inputs]
     cfg, keep = _panoN_cfg(rig, levels, minimize, c, b, srcs)
+    cfg.pipeline_f32 = int(srcs[0].dtype == np.float32)
     cw, ch = _I(), _I()
     lib().hmo_panoN_canvas_size(ctypes.byref(cfg), ctypes.byref(cw), ctypes.byref(ch))
-    canvas = np.zeros((b, ch.value, cw.value, c), np.uint8)
+    canvas = np.zeros((b, ch.value, cw.value, c), srcs[0].dtype)
     d = None
     if dump and levels > 0:
         wr, br = panoN_rois(rig, levels)
@@ -335,6 +367,7 @@ def pano3(rig, inputs: List[np.ndarray], levels: int, fused: bool = True, canvas
     cfg = hmo_pano3_cfg()
     keep = []
     cfg.batch, cfg.levels, cfg.channels, cfg.fused = b, levels, c, int(fused)
+    cfg.pipeline_f32 = int(srcs[0].dtype == np.float32)
     for i in range(3):
         mx = np.ascontiguousarray(rig.map_x[i], np.uint16)
         my = np.ascontiguousarray(rig.map_y[i], np.uint16)
@@ -348,7 +381,8 @@ def pano3(rig, inputs: List[np.ndarray], levels: int, fused: bool = True, canvas
     cfg.seam_h, cfg.seam_w = seam.shape
     cw, ch = _I(), _I()
     lib().hmo_pano3_canvas_size(ctypes.byref(cfg), ctypes.byref(cw), ctypes.byref(ch))
-    out = np.zeros((b, ch.value, cw.value, c), np.uint8) if canvas is None else np.ascontiguousarray(canvas).copy()
+    out = (np.zeros((b, ch.value, cw.value, c), srcs[0].dtype) if canvas is None
+           else np.ascontiguousarray(canvas, srcs[0].dtype).copy())
     d = Dump(3, seam.shape[1], ch.value, c, levels, b, True, 3) if (dump and levels > 0) else None
     arr = (_P * 3)(*[_ptr(x) for x in srcs])
     rc = lib().hmo_pano3_process(ctypes.byref(cfg), arr, _ptr(out), ctypes.byref(d.st) if d else None)

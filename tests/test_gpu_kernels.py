@@ -191,3 +191,109 @@ def test_copy_roi_and_make_full(hm, torch_gpu, oracle, sc, sf, dc, df):
     assert hm.make_full(dev(torch, src), code(sc, sf), 9, 8, 4, 2, 5, 3, 2, d, code(dc, df)) == 0
     torch.cuda.synchronize()
     np.testing.assert_array_equal(d.cpu().numpy(), want)
+
+
+# ---------------------------------------------------------------------------
+# float, half and 1-channel pixel types (src/cuda/cudaRemap.cu:507-554,
+# cudaMakeFull.cu:291-313)
+# ---------------------------------------------------------------------------
+def _ptype(hm, c, kind):
+    return {("u8", 1): hm.HM_PIX_U8C1, ("u8", 3): hm.HM_PIX_U8C3, ("u8", 4): hm.HM_PIX_U8C4,
+            ("f32", 1): hm.HM_PIX_F32C1, ("f32", 3): hm.HM_PIX_F32C3, ("f32", 4): hm.HM_PIX_F32C4,
+            ("f16", 1): hm.HM_PIX_F16C1, ("f16", 3): hm.HM_PIX_F16C3, ("f16", 4): hm.HM_PIX_F16C4}[(kind, c)]
+
+
+def _rand_px(rng, shape, kind):
+    if kind == "u8":
+        return rng.integers(0, 256, shape, dtype=np.uint8)
+    v = (rng.uniform(-20, 290, shape)).astype(np.float32)
+    return v.astype(np.float16) if kind == "f16" else v
+
+
+@pytest.mark.parametrize("c,kind", [(1, "f32"), (3, "f32"), (4, "f32"), (1, "u8"), (4, "f16")])
+def test_remap_ex_offset_float_and_one_channel_sources(hm, torch_gpu, oracle, c, kind):
+    """float1/3/4 -> same (and uchar1, half4): exact copies; defaults with alpha 0 / 255."""
+    torch = torch_gpu
+    rng = np.random.default_rng(c * 7 + len(kind))
+    src = _rand_px(rng, (2, 20, 30, c), kind)
+    mx, my = rand_maps(rng, 25, 17, 30, 20)
+    dst0 = np.full((2, 22, 40, c), 9, src.dtype)
+    want = dst0.copy()
+    oracle.remap_offset(src, want, mx, my, 7, -3)
+    d = dev(torch, dst0)
+    t = _ptype(hm, c, kind)
+    assert hm.remap_ex_offset(dev(torch, src), t, d, t, dev(torch, mx), dev(torch, my), 2, 25, 17, 7, -3, False) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d.cpu().numpy(), want)
+
+
+def test_remap_ex_offset_refuses_pairs_not_instantiated(hm, torch_gpu):
+    torch = torch_gpu
+    src = torch.zeros((1, 4, 4, 4), dtype=torch.float32, device="cuda")
+    dst = torch.zeros((1, 4, 4, 4), dtype=torch.uint8, device="cuda")
+    m = torch.zeros((4, 4), dtype=torch.int16, device="cuda")
+    assert hm.remap_ex_offset(src, hm.HM_PIX_F32C4, dst, hm.HM_PIX_U8C4, m, m, 1, 4, 4, 0, 0, False) != 0
+    assert hm.remap_ex_offset(src, hm.HM_PIX_F32C4, src, hm.HM_PIX_F32C3, m, m, 1, 4, 4, 0, 0, False) != 0
+
+
+@pytest.mark.parametrize("c,kind", [(1, "u8"), (1, "f32"), (1, "f16"), (3, "f32"), (4, "f32")])
+def test_remap_dest_map_other_types(hm, torch_gpu, oracle, c, kind):
+    """uchar1, float1, __half, float3, float4 dest-map remaps."""
+    torch = torch_gpu
+    rng = np.random.default_rng(11 * c + len(kind))
+    src = _rand_px(rng, (2, 20, 30, c), kind)
+    mx, my = rand_maps(rng, 25, 17, 30, 20)
+    dm = rng.integers(0, 3, (22, 40)).astype(np.uint8)
+    dst0 = np.full((2, 22, 40, c), 5, src.dtype)
+    want = dst0.copy()
+    oracle.remap_dest_map(src, want, mx, my, 4, 2, 1, dm)
+    d = dev(torch, dst0)
+    assert hm.remap_dest_map(dev(torch, src), d, _ptype(hm, c, kind), dev(torch, mx), dev(torch, my), 1,
+                             dev(torch, dm), 2, 25, 17, 4, 2) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("c", [3, 4])
+def test_remap_hard_seam_n_float(hm, torch_gpu, oracle, c):
+    torch = torch_gpu
+    rng = np.random.default_rng(40 + c)
+    dm = rng.integers(0, 4, (22, 40)).astype(np.uint8)  # label 3: no image
+    srcs = [_rand_px(rng, (2, 20, 30, c), "f32") for _ in range(3)]
+    maps = [rand_maps(rng, 25, 17, 30, 20) for _ in range(3)]
+    offs = [(0, 0), (8, 3), (15, 5)]
+    dst0 = np.full((2, 22, 40, c), 5, np.float32)
+    want = dst0.copy()
+    oracle.remap_hard_seam_n(srcs, [m[0] for m in maps], [m[1] for m in maps], offs, dm, want)
+    d = dev(torch, dst0)
+    assert hm.remap_hard_seam_n([dev(torch, s) for s in srcs], [dev(torch, m[0]) for m in maps],
+                                [dev(torch, m[1]) for m in maps], offs, [(25, 17)] * 3, dev(torch, dm), d,
+                                _ptype(hm, c, "f32"), 2) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("s,d", [((3, "f16"), (3, "u8")), ((3, "f16"), (4, "u8")), ((4, "f16"), (4, "u8")),
+                                 ((4, "f16"), (3, "u8")), ((3, "f32"), (4, "f16")), ((4, "f16"), (3, "f32")),
+                                 ((3, "u8"), (4, "f16")), ((4, "u8"), (3, "f16")), ((3, "f32"), (3, "f16"))])
+def test_copy_roi_and_make_full_half_types(hm, torch_gpu, oracle, s, d):
+    """half3/half4 copies and make_full variants: to uint8 clamps and truncates, to half rounds
+    to nearest even, 3 -> 4 sets alpha 255, 4 -> 3 drops it."""
+    torch = torch_gpu
+    rng = np.random.default_rng(hash((s, d)) & 0xFFFF)
+    src = _rand_px(rng, (2, 15, 21, s[0]), s[1])
+    ddt = {"u8": np.uint8, "f32": np.float32, "f16": np.float16}[d[1]]
+    dst0 = np.full((2, 18, 26, d[0]), 3, ddt)
+    want = dst0.copy()
+    oracle.copy_roi(src, 13, 9, 4, 3, want, 6, 5)
+    dd = dev(torch, dst0)
+    assert hm.copy_roi(dev(torch, src), _ptype(hm, *s), 13, 9, 4, 3, dd, _ptype(hm, *d), 6, 5, 2) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint8), want.view(np.uint8))
+    # simple_make_full_batch: the destination cleared, then the copy
+    want = np.zeros_like(dst0)
+    oracle.copy_roi(src, 13, 9, 4, 3, want, 6, 5)
+    dd = dev(torch, dst0)
+    assert hm.make_full(dev(torch, src), _ptype(hm, *s), 13, 9, 4, 3, 6, 5, 2, dd, _ptype(hm, *d)) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint8), want.view(np.uint8))

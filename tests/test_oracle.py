@@ -565,3 +565,153 @@ def test_pano3_soft_constant_images_blend_to_selection(oracle):
     np.testing.assert_array_equal(out[0, :, 4:20], np.broadcast_to(np.uint8([200, 0, 0, 255]), (h, 16, 4)))
     np.testing.assert_array_equal(out[0, :, 76:92], np.broadcast_to(np.uint8([0, 0, 100, 255]), (h, 16, 4)))
     assert (out[..., 3] == 255).all()
+
+
+# ---------------------------------------------------------------------------
+# half conversions and float pipelines
+# ---------------------------------------------------------------------------
+def test_f16_widening_is_exact_for_every_half(oracle):
+    """All 65536 binary16 patterns against numpy's float16 -> float32 (NaN payloads aside)."""
+    bits = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    want = bits.view(np.float16).astype(np.float32)
+    got = np.array([oracle.f16_to_f32(int(b)) for b in bits], np.float32)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    np.testing.assert_array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
+
+
+def test_f32_to_f16_rounds_to_nearest_even_like_numpy(oracle):
+    rng = np.random.default_rng(5)
+    vals = np.concatenate([
+        rng.uniform(-70000, 70000, 4000), rng.uniform(-1, 1, 2000) * 1e-4, rng.uniform(-1, 1, 2000) * 1e-7,
+        np.float32([0.0, -0.0, 65504.0, 65519.99, 65520.0, 1e9, -1e9, np.inf, -np.inf, 5.960464e-08, 2.980232e-08,
+                    2.98023259e-08, 1.0 + 2 ** -11, 1.0 + 3 * 2 ** -11, 2048.5, 2049.5, 0.1, 255.0, 254.99])]).astype(np.float32)
+    halves = (np.arange(65536, dtype=np.uint32).astype(np.uint16).view(np.float16))
+    finite = halves[np.isfinite(halves)].astype(np.float32)
+    mids = ((finite[:-1].astype(np.float64) + finite[1:].astype(np.float64)) / 2).astype(np.float32)
+    vals = np.concatenate([vals, finite[::7], mids[::5]])
+    with np.errstate(over="ignore"):
+        want = vals.astype(np.float16).view(np.uint16)
+    got = np.array([oracle.f32_to_f16(float(v)) for v in vals], np.uint16)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_remap_float_and_half_sources(oracle):
+    """Float sources copy exactly; half sources widen exactly; uint8 destinations from float
+    truncate; sentinel / out-of-range give the default pixel (alpha 0 / 255)."""
+    src = np.float32([[[[1.5, 2.25, 300.0, 0.75], [-3.0, 7.0, 8.0, 9.0]]]])  # 1 x 1 x 2 x 4
+    mx = np.uint16([[1, 0, 65535, 5]])
+    my = np.uint16([[0, 0, 0, 0]])
+    d = np.full((1, 1, 4, 4), 7.0, np.float32)
+    oracle.remap_offset(src, d, mx, my, 0, 0)
+    np.testing.assert_array_equal(d[0, 0], [[-3, 7, 8, 9], [1.5, 2.25, 300, 0.75], [0, 0, 0, 0], [0, 0, 0, 255]])
+    du = np.zeros((1, 1, 4, 4), np.uint8)
+    oracle.copy_roi(d, 4, 1, 0, 0, du, 0, 0)
+    np.testing.assert_array_equal(du[0, 0], [[0, 7, 8, 9], [1, 2, 255, 0], [0, 0, 0, 0], [0, 0, 0, 255]])
+    h = src.astype(np.float16)
+    dh = np.zeros((1, 1, 4, 3), np.float32)
+    oracle.copy_roi(h, 2, 1, 0, 0, dh, 0, 0)  # half4 -> float3 drops alpha
+    np.testing.assert_array_equal(dh[0, 0, :2], h[0, 0, :, :3].astype(np.float32))
+    f1 = np.float32([[[[1.25], [2.5]]]])
+    d1 = np.zeros((1, 1, 3, 1), np.float32)
+    oracle.remap_dest_map(f1, d1, np.uint16([[1, 0, 9]]), np.uint16([[0, 0, 0]]), 0, 0, 2, np.uint8([[2, 1, 2]]))
+    np.testing.assert_array_equal(d1[0, 0, :, 0], [2.5, 0.0, 0.0])
+
+
+def _float_frames(frames):
+    return [f.astype(np.float32) for f in frames]
+
+
+@pytest.mark.parametrize("levels,minimize", [(0, True), (3, True), (3, False), (1, True)])
+def test_pano2_float_pipeline_truncates_to_uint8_pipeline(oracle, levels, minimize):
+    """Integer-valued float inputs: the float pipeline computes the same floats as the uint8
+    one, which only adds the final truncation (round_to_uchar)."""
+    import hm_synth
+
+    rig = hm_synth.small_rig(seed=11)
+    f = frames_for(rig, 1, 5)
+    u8 = oracle.pano2(rig, f[0], f[1], levels, minimize=minimize)
+    fl = oracle.pano2(rig, *_float_frames(f), levels, minimize=minimize)
+    assert fl.dtype == np.float32
+    np.testing.assert_array_equal(np.clip(fl, 0, 255).astype(np.uint8), u8)
+    if levels == 0:
+        np.testing.assert_array_equal(fl, u8.astype(np.float32))
+
+
+@pytest.mark.parametrize("levels,minimize", [(0, True), (3, True), (3, False)])
+def test_panoN_float_pipeline_truncates_to_uint8_pipeline(oracle, levels, minimize):
+    import hm_synth
+
+    rig = hm_synth.small_rig(n=3, seed=4, canvas=(520, 72))
+    f = frames_for(rig, 1, 8)
+    u8 = oracle.panoN(rig, f, levels, minimize=minimize)
+    fl = oracle.panoN(rig, _float_frames(f), levels, minimize=minimize)
+    np.testing.assert_array_equal(np.clip(fl, 0, 255).astype(np.uint8), u8)
+
+
+@pytest.mark.parametrize("levels,fused", [(0, True), (0, False), (3, True), (3, False)])
+def test_pano3_float_pipeline_truncates_to_uint8_pipeline(oracle, levels, fused):
+    import hm_synth
+
+    rig = hm_synth.small_rig(n=3, seed=15, src=(120, 56), canvas=(330, 64))
+    f = frames_for(rig, 1, 9)
+    u8 = oracle.pano3(rig, f, levels, fused=fused)
+    fl = oracle.pano3(rig, _float_frames(f), levels, fused=fused)
+    np.testing.assert_array_equal(np.clip(fl, 0, 255).astype(np.uint8), u8)
+
+
+def pattern_f4(w, h, idx):
+    """make_pattern_image_f4 of src/pano/cudaPano_test.cpp:45-58 (also cudaPanoN_test.cpp)."""
+    return patterned_u8(w, h, idx).astype(np.float32)
+
+
+def test_pano2_float_minimize_matches_full(oracle):
+    """src/pano/cudaPano_test.cpp:141-204: CudaStitchPano<float4, float4>, minimized = full within 1e-4."""
+    w, h, x2 = 384, 64, 192
+    cw = w + x2
+    seam = np.zeros((h, cw), np.uint8)
+    seam[:, : cw // 2] = 1
+    rig = IdRig([(w, h), (w, h)], [(0, 0), (x2, 0)], seam)
+    i1, i2 = pattern_f4(w, h, 0), pattern_f4(w, h, 1)
+    full, dfull = oracle.pano2(rig, i1, i2, 4, minimize=False, dump=True)
+    mini, dmini = oracle.pano2(rig, i1, i2, 4, minimize=True, dump=True)
+    assert np.abs(full - mini).max() <= 1e-4
+    assert dfull.sizes[0] == (cw, h) and dmini.sizes[0][0] < cw
+
+
+PANO_N_FLOAT_CASES = {
+    # src/pano/cudaPanoN_test.cpp:192-299: (image w, h, positions, canvas w, seam columns of label 1, images)
+    "two_way_overlap": (384, 384, [(0, 0), (0, 0)], 384, (192, 384), 2),
+    "no_overlap": (192, 384, [(0, 0), (192, 0)], 384, (192, 384), 2),
+    "gap": (128, 384, [(0, 0), (256, 0)], 384, (192, 384), 2),
+    "multi_seam_intersection": (384, 384, [(0, 0)] * 4, 384, None, 4),
+}
+
+
+@pytest.mark.parametrize("case", sorted(PANO_N_FLOAT_CASES))
+def test_panoN_float_reference_cases(oracle, case):
+    w, h, pos, cw, band, n = PANO_N_FLOAT_CASES[case]
+    seam = np.zeros((h, cw), np.uint8)
+    if band:
+        seam[:, band[0]: band[1]] = 1
+    else:  # a 64 x 64 square of four quadrant labels 0..3 at the centre
+        cx, cy = w // 2, h // 2
+        for y in range(cy - 32, cy + 32):
+            for x in range(cx - 32, cx + 32):
+                seam[y, x] = (1 if y >= cy else 0) * 2 + (1 if x >= cx else 0)
+    rig = IdRig([(w, h)] * n, pos, seam)
+    imgs = [pattern_f4(w, h, i) for i in range(n)]
+    full = oracle.panoN(rig, imgs, 4, minimize=False)
+    mini = oracle.panoN(rig, imgs, 4, minimize=True)
+    assert np.abs(full - mini).max() <= 1e-4
+    if case == "gap":  # the uncovered columns [128, 256) stay transparent
+        assert (mini[0, :, 128:256, 3] == 0).all()
+
+
+def test_pano3_float_one_hot_selects_middle_image(oracle):
+    """src/pano/cudaPano3_test.cpp:133-196: 1x1 images 30 / 60 / 90, label 1, levels 1."""
+    seam = np.uint8([[1]])
+    rig = IdRig([(1, 1)] * 3, [(0, 0)] * 3, seam)
+    imgs = [np.float32([[[[v, v, v, 255.0]]]]) for v in (30.0, 60.0, 90.0)]
+    out = oracle.pano3(rig, imgs, 1)
+    np.testing.assert_allclose(out[0, 0, 0], [60.0, 60.0, 60.0, 255.0], atol=1e-3)
