@@ -33,6 +33,23 @@ using BaseScalar_t = typename BaseScalar<T>::type;
 template <typename T>
 constexpr int num_channels_v = (int)(sizeof(T) / sizeof(BaseScalar_t<T>));
 
+// The pano constructors' channels argument for a pipeline pixel type: the
+// channel count, with HM_PIPELINE_F32 for a float pipeline.
+template <typename T>
+constexpr int pano_channels_arg() {
+  return num_channels_v<T> | (std::is_same_v<BaseScalar_t<T>, float> ? HM_PIPELINE_F32 : 0);
+}
+
+// The pipeline / compute pairs the pano classes instantiate: uchar with the
+// float of the same channel count, or float throughout.
+template <typename T_pipeline, typename T_compute>
+constexpr bool pano_types_ok() {
+  return (std::is_same_v<T_pipeline, uchar4> && std::is_same_v<T_compute, float4>) ||
+         (std::is_same_v<T_pipeline, uchar3> && std::is_same_v<T_compute, float3>) ||
+         (std::is_same_v<T_pipeline, float4> && std::is_same_v<T_compute, float4>) ||
+         (std::is_same_v<T_pipeline, float3> && std::is_same_v<T_compute, float3>);
+}
+
 // The C-ABI pixel code of a pixel type.
 template <typename T>
 constexpr hmPixelType hm_pixel_type() {

@@ -2,8 +2,8 @@
 //
 // The constructor, process(), status(), canvas_width/height(), batch_size()
 // and dump_soft_blend_pyramid() of hm-cupano's src/pano/cudaPano.h:68-115, over the gfx950
-// library's C ABI (include/hm_stitch.h).  T_pipeline is uchar4 or uchar3
-// with T_compute float4 or float3.  Like the reference, constructor errors
+// library's C ABI (include/hm_stitch.h).  T_pipeline is uchar4 / uchar3 with
+// T_compute float4 / float3, or float4 / float3 for both (the float pipeline).  Like the reference, constructor errors
 // are sticky in status() and process() moves the canvas in and back out.
 #pragma once
 
@@ -23,9 +23,8 @@ namespace cuda {
 
 template <typename T_pipeline, typename T_compute>
 class CudaStitchPano {
-  static_assert((std::is_same_v<T_pipeline, uchar4> && std::is_same_v<T_compute, float4>) ||
-                    (std::is_same_v<T_pipeline, uchar3> && std::is_same_v<T_compute, float3>),
-                "CudaStitchPano supports <uchar4, float4> and <uchar3, float3>");
+  static_assert(pano_types_ok<T_pipeline, T_compute>(),
+                "CudaStitchPano supports <uchar4, float4>, <uchar3, float3>, <float4, float4> and <float3, float3>");
 
  public:
   CudaStitchPano(int batch_size, int num_levels, const ControlMasks& control_masks, bool quiet = false,
@@ -52,7 +51,7 @@ class CudaStitchPano {
     }
     hmStitchPano* h = nullptr;
     const int rc = hmStitchPanoCreate(batch_size, num_levels, &m, quiet, minimize_blend, max_output_width,
-                                      num_channels_v<T_pipeline>, &h);
+                                      pano_channels_arg<T_pipeline>(), &h);
     handle_.reset(h);
     if (rc != hipSuccess) status_ = CudaStatus(static_cast<hipError_t>(rc), "CudaStitchPano construction failed");
   }

@@ -22,9 +22,8 @@ namespace cuda {
 
 template <typename T_pipeline, typename T_compute>
 class CudaStitchPano3 {
-  static_assert((std::is_same_v<T_pipeline, uchar4> && std::is_same_v<T_compute, float4>) ||
-                    (std::is_same_v<T_pipeline, uchar3> && std::is_same_v<T_compute, float3>),
-                "CudaStitchPano3 supports <uchar4, float4> and <uchar3, float3>");
+  static_assert(pano_types_ok<T_pipeline, T_compute>(),
+                "CudaStitchPano3 supports <uchar4, float4>, <uchar3, float3>, <float4, float4> and <float3, float3>");
 
  public:
   using pipeline_type = T_pipeline;
@@ -50,7 +49,7 @@ class CudaStitchPano3 {
     m.seam_w = control_masks.whole_seam_mask_image.cols;
     m.seam_h = control_masks.whole_seam_mask_image.rows;
     hmStitchPano* h = nullptr;
-    const int rc = hmStitchPano3Create(batch_size, num_levels, &m, quiet, num_channels_v<T_pipeline>, &h);
+    const int rc = hmStitchPano3Create(batch_size, num_levels, &m, quiet, pano_channels_arg<T_pipeline>(), &h);
     handle_.reset(h);
     if (rc != hipSuccess) status_ = CudaStatus(static_cast<hipError_t>(rc), "CudaStitchPano3 construction failed");
   }

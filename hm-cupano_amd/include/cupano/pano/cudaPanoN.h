@@ -24,9 +24,8 @@ namespace cuda {
 
 template <typename T_pipeline, typename T_compute>
 class CudaStitchPanoN {
-  static_assert((std::is_same_v<T_pipeline, uchar4> && std::is_same_v<T_compute, float4>) ||
-                    (std::is_same_v<T_pipeline, uchar3> && std::is_same_v<T_compute, float3>),
-                "CudaStitchPanoN supports <uchar4, float4> and <uchar3, float3>");
+  static_assert(pano_types_ok<T_pipeline, T_compute>(),
+                "CudaStitchPanoN supports <uchar4, float4>, <uchar3, float3>, <float4, float4> and <float3, float3>");
 
  public:
   CudaStitchPanoN(int batch_size, int num_levels, const ControlMasksN& control_masks, bool minimize_blend,
@@ -55,7 +54,7 @@ class CudaStitchPanoN {
     m.seam_h = control_masks.whole_seam_mask_indexed.rows;
     hmStitchPano* h = nullptr;
     const int rc = hmStitchPanoNCreate(batch_size, num_levels, &m, minimize_blend, quiet,
-                                       num_channels_v<T_pipeline>, &h);
+                                       pano_channels_arg<T_pipeline>(), &h);
     handle_.reset(h);
     n_images_ = n;
     if (rc != hipSuccess) status_ = CudaStatus(static_cast<hipError_t>(rc), "CudaStitchPanoN construction failed");

@@ -2,8 +2,14 @@
 // CudaStitchPano3 through the reference's class interfaces (ControlMasks* ->
 // ctor -> process(..., stream, canvas)), restating cases of
 // src/pano/cudaPano_test.cpp, src/pano/cudaPano3_test.cpp and
-// tests/python/test_pytorch_pano.py with a uint8 pipeline.  Prints one line
-// per case; exit code 0 = pass.
+// tests/python/test_pytorch_pano.py with uint8 pipelines, and the float tests
+// of src/pano/cudaPano_test.cpp:141-204, cudaPanoN_test.cpp:192-299 and
+// cudaPano3_test.cpp:133-196 with float pipelines.  Prints one line per case;
+// exit code 0 = pass.  With a directory argument, the float canvases are also
+// written there as raw float32 (<case>.f32, B x H x W x 4) for comparison with
+// the oracle.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
@@ -130,9 +136,83 @@ std::vector<uchar4> download(CudaMat<uchar4>& m, int w, int h) {
   return host;
 }
 
+// ---- float pipelines ----
+const char* g_dump_dir = nullptr;
+
+void dump(const char* name, const std::vector<float4>& img) {
+  if (!g_dump_dir) return;
+  const std::string path = std::string(g_dump_dir) + "/" + name + ".f32";
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) return;
+  std::fwrite(img.data(), sizeof(float4), img.size(), f);
+  std::fclose(f);
+}
+
+// The reference tests' pattern as float, plus `frac` on the colour channels.
+std::vector<float4> patterned_f(int w, int h, int idx, float frac) {
+  const auto u = patterned(w, h, idx);
+  std::vector<float4> img(u.size());
+  for (size_t i = 0; i < u.size(); ++i) img[i] = make_float4(u[i].x + frac, u[i].y + frac, u[i].z + frac, u[i].w);
+  return img;
+}
+
+float max_abs_diff(const std::vector<float4>& a, const std::vector<float4>& b) {
+  if (a.empty() || a.size() != b.size()) return 1e30f;
+  float d = 0.0f;
+  for (size_t i = 0; i < a.size(); ++i) {
+    d = std::max(d, std::fabs(a[i].x - b[i].x));
+    d = std::max(d, std::fabs(a[i].y - b[i].y));
+    d = std::max(d, std::fabs(a[i].z - b[i].z));
+    d = std::max(d, std::fabs(a[i].w - b[i].w));
+  }
+  return d;
+}
+
+template <typename P>
+std::vector<P> download_t(CudaMat<P>& m) {
+  std::vector<P> host((size_t)m.width() * m.height());
+  if (m.download(host.data()) != hipSuccess) return {};
+  return host;
+}
+
+template <typename P>
+std::vector<P> stitch2_f(const ControlMasks& masks, bool minimize, const std::vector<P>& a, const std::vector<P>& b,
+                         int w, int h, int* level0_w) {
+  CudaStitchPano<P, P> pano(1, 4, masks, /*quiet=*/true, minimize);
+  if (!pano.status().ok()) return {};
+  CudaMat<P> in1(a.data(), w, h), in2(b.data(), w, h);
+  auto r = pano.process(in1, in2, nullptr,
+                        std::make_unique<CudaMat<P>>(1, pano.canvas_width(), pano.canvas_height()));
+  if (!r.ok()) return {};
+  auto out = r.ConsumeValueOrDie();
+  if (level0_w) {
+    int levels = 0, lw[16] = {0}, lh[16] = {0};  // at most 16 levels
+    *level0_w = hmStitchPanoLevels(pano.native_handle(), &levels, lw, lh) == 0 ? lw[0] : -1;
+  }
+  return download_t(*out);
+}
+
+std::vector<float4> stitchN_f(const ControlMasksN& masks, bool minimize, const std::vector<std::vector<float4>>& imgs,
+                              int w, int h) {
+  CudaStitchPanoN<float4, float4> pano(1, 4, masks, minimize, /*quiet=*/true);
+  if (!pano.status().ok()) return {};
+  std::vector<std::unique_ptr<CudaMat<float4>>> ins;
+  std::vector<const CudaMat<float4>*> ptrs;
+  for (const auto& im : imgs) {
+    ins.push_back(std::make_unique<CudaMat<float4>>(im.data(), w, h));
+    ptrs.push_back(ins.back().get());
+  }
+  auto r = pano.process(ptrs, nullptr,
+                        std::make_unique<CudaMat<float4>>(1, pano.canvas_width(), pano.canvas_height()));
+  if (!r.ok()) return {};
+  auto out = r.ConsumeValueOrDie();
+  return download_t(*out);
+}
+
 }  // namespace
 
-int main() {
+int main(int argc, char** argv) {
+  if (argc > 1) g_dump_dir = argv[1];
   // tests/python/test_pytorch_pano.py:109-152: hard seam and single-level soft seam select exactly.
   {
     const int w = 256, h = 32, x2 = 128, cwid = w + x2;
@@ -270,6 +350,112 @@ int main() {
       ok = ok && host.size() == 1 && same(host[0], v[1]);
     }
     check(ok, "CudaStitchPano3 one-hot label 1 selects image 1 (fused and unfused)");
+  }
+  // src/pano/cudaPano_test.cpp:141-204 with a float pipeline: the minimized and
+  // full-canvas blends agree within 1e-4 and the minimized level 0 is narrower.
+  {
+    const int w = 384, h = 64, x2 = 192, cwid = w + x2;
+    HostMat<uint8_t> seam(h, cwid, 0);
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < cwid / 2; ++x) seam.at(y, x) = 1;
+    const ControlMasks masks = make_masks(w, h, x2, seam);
+    for (float frac : {0.0f, 0.375f}) {
+      const auto i1 = patterned_f(w, h, 0, frac), i2 = patterned_f(w, h, 1, frac);
+      int lw_full = 0, lw_mini = 0;
+      const auto full = stitch2_f(masks, false, i1, i2, w, h, &lw_full);
+      const auto mini = stitch2_f(masks, true, i1, i2, w, h, &lw_mini);
+      dump(frac == 0.0f ? "pano2_float_mini" : "pano2_float_mini_frac", mini);
+      check(full.size() == (size_t)cwid * h && max_abs_diff(full, mini) <= 1e-4f && lw_full == cwid &&
+                lw_mini > 0 && lw_mini < cwid,
+            frac == 0.0f ? "CudaStitchPano<float4, float4>: minimized equals full within 1e-4, narrower level 0"
+                         : "CudaStitchPano<float4, float4> with fractional inputs: minimized equals full within 1e-4");
+    }
+    // float3: the same blend on three channels equals the float4 one's colour channels.
+    std::vector<float3> a3, b3;
+    const auto i1 = patterned_f(w, h, 0, 0.375f), i2 = patterned_f(w, h, 1, 0.375f);
+    for (size_t i = 0; i < i1.size(); ++i) {
+      a3.push_back(make_float3(i1[i].x, i1[i].y, i1[i].z));
+      b3.push_back(make_float3(i2[i].x, i2[i].y, i2[i].z));
+    }
+    const auto full3 = stitch2_f(masks, false, a3, b3, w, h, nullptr);
+    const auto mini3 = stitch2_f(masks, true, a3, b3, w, h, nullptr);
+    std::vector<float4> full4(full3.size()), mini4(mini3.size());
+    for (size_t i = 0; i < full3.size(); ++i) full4[i] = make_float4(full3[i].x, full3[i].y, full3[i].z, 0.0f);
+    for (size_t i = 0; i < mini3.size(); ++i) mini4[i] = make_float4(mini3[i].x, mini3[i].y, mini3[i].z, 0.0f);
+    dump("pano2_float3_mini_frac", mini4);
+    check(full3.size() == (size_t)cwid * h && max_abs_diff(full4, mini4) <= 1e-4f,
+          "CudaStitchPano<float3, float3>: minimized equals full within 1e-4");
+  }
+  // src/pano/cudaPanoN_test.cpp:192-299 with a float pipeline: two-way overlap,
+  // no overlap, a gap (transparent uncovered columns) and a four-way seam
+  // intersection; minimized and full blends agree within 1e-4.
+  {
+    struct NCase {
+      const char* name;
+      int w, h, n, cw;
+      int xs[4];
+      int band0, band1;  // columns of label 1; band0 < 0: four quadrant labels
+    };
+    const NCase cases[] = {
+        {"panoN_float_two_way_overlap", 384, 384, 2, 384, {0, 0, 0, 0}, 192, 384},
+        {"panoN_float_no_overlap", 192, 384, 2, 384, {0, 192, 0, 0}, 192, 384},
+        {"panoN_float_gap", 128, 384, 2, 384, {0, 256, 0, 0}, 192, 384},
+        {"panoN_float_multi_seam_intersection", 384, 384, 4, 384, {0, 0, 0, 0}, -1, -1},
+    };
+    for (const NCase& c : cases) {
+      ControlMasksN masks;
+      masks.img_col.resize(c.n);
+      masks.img_row.resize(c.n);
+      std::vector<std::vector<float4>> imgs;
+      for (int i = 0; i < c.n; ++i) {
+        identity(c.w, c.h, &masks.img_col[i], &masks.img_row[i]);
+        masks.positions.push_back(SpatialTiff{(float)c.xs[i], 0.0f});
+        imgs.push_back(patterned_f(c.w, c.h, i, 0.0f));
+      }
+      HostMat<uint8_t> seam(c.h, c.cw, 0);
+      if (c.band0 >= 0) {
+        for (int y = 0; y < c.h; ++y)
+          for (int x = c.band0; x < c.band1; ++x) seam.at(y, x) = 1;
+      } else {
+        const int cx = c.w / 2, cy = c.h / 2;
+        for (int y = cy - 32; y < cy + 32; ++y)
+          for (int x = cx - 32; x < cx + 32; ++x) seam.at(y, x) = (uint8_t)((y >= cy ? 2 : 0) + (x >= cx ? 1 : 0));
+      }
+      masks.whole_seam_mask_indexed = seam;
+      const auto full = stitchN_f(masks, false, imgs, c.w, c.h);
+      const auto mini = stitchN_f(masks, true, imgs, c.w, c.h);
+      dump(c.name, mini);
+      bool ok = full.size() == (size_t)c.cw * c.h && max_abs_diff(full, mini) <= 1e-4f;
+      if (ok && c.xs[1] == 256)
+        for (int y = 0; y < c.h; ++y)
+          for (int x = 128; x < 256; ++x) ok = ok && mini[(size_t)y * c.cw + x].w == 0.0f;
+      check(ok, c.name);
+    }
+  }
+  // src/pano/cudaPano3_test.cpp:133-196 with a float pipeline: a 1x1 one-hot
+  // label 1 selects image 1's value (within 1e-3), fused and unfused.
+  {
+    ControlMasks3 masks;
+    for (auto* m : {&masks.img0_col, &masks.img0_row, &masks.img1_col, &masks.img1_row, &masks.img2_col,
+                    &masks.img2_row})
+      *m = HostMat<uint16_t>(1, 1, 0);
+    masks.whole_seam_mask_image = HostMat<uint8_t>(1, 1, 1);
+    masks.positions = {SpatialTiff{0, 0}, SpatialTiff{0, 0}, SpatialTiff{0, 0}};
+    const float4 v[3] = {make_float4(30, 30, 30, 255), make_float4(60, 60, 60, 255), make_float4(90, 90, 90, 255)};
+    CudaMat<float4> i0(&v[0], 1, 1), i1(&v[1], 1, 1), i2(&v[2], 1, 1);
+    bool ok = true;
+    for (bool fused : {true, false}) {
+      CudaStitchPano3<float4, float4> pano(1, 1, masks, true);
+      auto r = pano.process(i0, i1, i2, nullptr, std::make_unique<CudaMat<float4>>(1, 1, 1), fused);
+      if (!r.ok()) {
+        ok = false;
+        continue;
+      }
+      auto out = r.ConsumeValueOrDie();
+      const auto host = download_t(*out);
+      ok = ok && host.size() == 1 && max_abs_diff(host, {v[1]}) <= 1e-3f;
+    }
+    check(ok, "CudaStitchPano3<float4, float4> one-hot label 1 selects image 1 (fused and unfused)");
   }
   std::printf("%d failure(s)\n", g_failures);
   return g_failures ? 1 : 0;
