@@ -190,6 +190,7 @@ extern "C" int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* 
   int cb_error = 0;
   hipError_t err = hipSuccess;
   const auto t0 = std::chrono::steady_clock::now();
+  auto t_first = t0;  // the first frame's delivery
 
   // Drains slot `s`: waits for its download, times it, hands it to the sink.
   auto drain = [&](int s) -> hipError_t {
@@ -207,7 +208,10 @@ extern "C" int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* 
     }
     const int rc = sink(user, sl.frame, sl.host_out.p, (uint32_t)cw, (uint32_t)(ch * B), (uint32_t)out_pitch);
     sl.frame = -1;
-    if (stats) stats->frames += 1;
+    if (stats) {
+      if (stats->frames == 0) t_first = std::chrono::steady_clock::now();
+      stats->frames += 1;
+    }
     if (rc != 0) {
       cb_error = rc;
       return hipErrorUnknown;
@@ -281,7 +285,13 @@ extern "C" int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* 
   (void)hipStreamSynchronize(up.s);
   (void)hipStreamSynchronize(down.s);
   if (stats) {
-    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const auto t_end = std::chrono::steady_clock::now();
+    stats->seconds = std::chrono::duration<double>(t_end - t0).count();
+    if (stats->frames > 0) {
+      stats->first_frame_seconds = std::chrono::duration<double>(t_first - t0).count();
+      stats->steady_seconds = std::chrono::duration<double>(t_end - t_first).count();
+      stats->steady_frames = stats->frames - 1;
+    }
     if (timed) {
       stats->upload_ms = up_ms / timed;
       stats->stitch_ms = comp_ms / timed;

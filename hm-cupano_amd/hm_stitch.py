@@ -130,7 +130,8 @@ class hmFrameLoopStats(ctypes.Structure):
     _fields_ = [("frames", ctypes.c_int64), ("seconds", ctypes.c_double), ("upload_ms", ctypes.c_double),
                 ("stitch_ms", ctypes.c_double), ("download_ms", ctypes.c_double),
                 ("bytes_up_per_frame", ctypes.c_uint64), ("bytes_down_per_frame", ctypes.c_uint64),
-                ("callback_error", ctypes.c_int)]
+                ("callback_error", ctypes.c_int), ("first_frame_seconds", ctypes.c_double),
+                ("steady_seconds", ctypes.c_double), ("steady_frames", ctypes.c_int64)]
 
 
 FRAME_SOURCE = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p),

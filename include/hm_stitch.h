@@ -440,6 +440,12 @@ typedef struct hmFrameLoopStats {
   double upload_ms, stitch_ms, download_ms; /* mean per-frame stream times */
   uint64_t bytes_up_per_frame, bytes_down_per_frame;
   int callback_error;
+  /* The pipeline once filled: the first frame reaches the sink after every slot
+   * has been filled once by the source (first_frame_seconds from the start);
+   * steady_frames = frames - 1 delivered after it, in steady_seconds. */
+  double first_frame_seconds;
+  double steady_seconds;
+  int64_t steady_frames;
 } hmFrameLoopStats;
 /* options may be NULL (3 slots, until the source ends, default transfers). */
 int hmFrameLoopRun(hmStitchPano* pano, int n_inputs, const uint32_t* in_w, const uint32_t* in_h,

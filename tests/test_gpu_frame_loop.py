@@ -49,6 +49,10 @@ def test_two_image_loop_matches_oracle(hm, torch_gpu, oracle, slots, levels):
     w, h = pano.canvas_size()
     assert st["bytes_down_per_frame"] == w * h * 3
     assert st["bytes_up_per_frame"] == 2 * rig.src_w * rig.src_h * 3
+    # the steady part: every frame after the first, within the call's wall time
+    assert st["steady_frames"] == 6
+    assert 0 < st["first_frame_seconds"] and 0 < st["steady_seconds"]
+    assert st["first_frame_seconds"] + st["steady_seconds"] <= st["seconds"] + 1e-6
 
 
 @pytest.mark.parametrize("upload,download", [(1, 0), (0, 1), (1, 1)])
