@@ -35,11 +35,8 @@
 #include <string>
 #include <vector>
 
-#include "hm_batch.hpp"
+#include "hm_batch_ops.hpp"
 #include "hm_pano.hpp"
-#include "hm_pano2_device.hpp"
-#include "hm_pano_device.hpp"
-#include "hm_pyramid_fused.hpp"
 
 namespace hm {
 thread_local LaunchRecorder* g_recorder = nullptr;
@@ -50,124 +47,6 @@ using namespace hm;
 
 namespace {
 
-constexpr int kStageItems = 16;
-constexpr int kStageBlob = 3584;
-
-struct StageItem {
-  int op;
-  uint32_t off;    // argument block in the blob
-  uint32_t first;  // first work unit
-  uint32_t wgs;    // workgroups along x of the recorded launch
-};
-
-// Kernel arguments of one stage launch (under the 4 KiB argument limit).
-struct StageArgs {
-  int n_items;
-  uint32_t total;  // work units: sum of wgs * batch
-  int pad[2];
-  StageItem items[kStageItems];
-  alignas(16) uint8_t blob[kStageBlob];
-};
-static_assert(sizeof(StageArgs) <= 4000, "stage arguments must fit the kernel-argument segment");
-
-// The stage's arguments are read through the kernel-argument segment pointer
-// (constant memory: scalar loads).  Taking typed views of different bodies'
-// argument blocks in the by-value parameter instead makes the compiler copy
-// the whole parameter into scratch memory.
-template <typename T>
-using kptr = const __attribute__((address_space(4))) T*;
-
-template <typename A>
-__device__ __forceinline__ const A& arg_at(kptr<StageArgs> s, uint32_t off) {
-  return *(const A*)(s->blob + off);  // constant memory seen through a generic pointer
-}
-
-// The op table: (op, body call).  Each op has a persistent kernel of its
-// own, so that a body runs with its own register allocation (a kernel that
-// carried several bodies would take the largest one's, and with it the
-// lowest occupancy, for all of them).
-#define HM_BATCH_OPS(X)                                                                                       \
-  X(kOpP2L0_4_1, 0, (pano2_level0_body<4, 4, 1>(arg_at<Pano2L0>(s, off), blk, (int)wgs)))                    \
-  X(kOpP2L0_4_2, 0, (pano2_level0_body<4, 4, 2>(arg_at<Pano2L0>(s, off), blk, (int)wgs)))                    \
-  X(kOpP2Hard_4_1, 0, (pano2_hard_body<4, 4, 1>(arg_at<Pano2Hard>(s, off), blk)))                            \
-  X(kOpP2Hard_4_2, 0, (pano2_hard_body<4, 4, 2>(arg_at<Pano2Hard>(s, off), blk)))                            \
-  X(kOpMi3, 0, (multi_inputs_body<4, 3, false>(arg_at<MultiArgs<3>>(s, off), blk)))                          \
-  X(kOpMi3DM, 0, (multi_inputs_body<4, 3, true>(arg_at<MultiArgs<3>>(s, off), blk)))                         \
-  X(kOpMi4, 0, (multi_inputs_body<4, 4, false>(arg_at<MultiArgs<4>>(s, off), blk)))                          \
-  X(kOpMc3, 0, (multi_canvas_body<4, 3>(arg_at<MultiArgs<3>>(s, off), blk, lds)))                            \
-  X(kOpMc4, 0, (multi_canvas_body<4, 4>(arg_at<MultiArgs<4>>(s, off), blk, lds)))                            \
-  X(kOpDown2U8, 1, (down_fused_body<uint8_t, 4, 2>(arg_at<DownFusedArgs<uint8_t, 2>>(s, off), blk, lds)))    \
-  X(kOpDown2F, 1, (down_fused_body<float, 4, 2>(arg_at<DownFusedArgs<float, 2>>(s, off), blk, lds)))         \
-  X(kOpDown3U8, 1, (down_fused_slots_body<uint8_t, 4, 3>(arg_at<DownFusedArgs<uint8_t, 3>>(s, off), blk, lds))) \
-  X(kOpDown3F, 1, (down_fused_slots_body<float, 4, 3>(arg_at<DownFusedArgs<float, 3>>(s, off), blk, lds)))   \
-  X(kOpDown4U8, 1, (down_fused_slots_body<uint8_t, 4, 4>(arg_at<DownFusedArgs<uint8_t, 4>>(s, off), blk, lds))) \
-  X(kOpDown4F, 1, (down_fused_slots_body<float, 4, 4>(arg_at<DownFusedArgs<float, 4>>(s, off), blk, lds)))   \
-  X(kOpTail2, 2, (up_tail_body<4, 2, kBlend2>(arg_at<UpTailArgs<2>>(s, off), blk, lds)))                     \
-  X(kOpUq2F, 2, (up_quad_body<float, 4, 2, kBlend2, kMask0F32, true>(arg_at<UpQuadArgs<float, 2>>(s, off), blk, lds))) \
-  X(kOpUq2FLast, 2,                                                                                           \
-    (up_quad_body<float, 4, 2, kBlend2, kMask0F32, false>(arg_at<UpQuadArgs<float, 2>>(s, off), blk, lds)))       \
-  X(kOpUq2U8, 2,                                                                                              \
-    (up_quad_body<uint8_t, 4, 2, kBlend2, kMask0U8, true>(arg_at<UpQuadArgs<uint8_t, 2>>(s, off), blk, lds)))     \
-  X(kOpUq2U8Last, 2,                                                                                          \
-    (up_quad_body<uint8_t, 4, 2, kBlend2, kMask0U8, false>(arg_at<UpQuadArgs<uint8_t, 2>>(s, off), blk, lds)))    \
-  X(kOpUq3F, 3, (up_quad_body<float, 4, 3, kBlend3, kMask0F32, true>(arg_at<UpQuadArgs<float, 3>>(s, off), blk, lds))) \
-  X(kOpUq3FLast, 3,                                                                                           \
-    (up_quad_body<float, 4, 3, kBlend3, kMask0F32, false>(arg_at<UpQuadArgs<float, 3>>(s, off), blk, lds)))       \
-  X(kOpUq3L, 3,                                                                                               \
-    (up_quad_body<uint8_t, 4, 3, kBlend3, kMask0Label, true>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk, lds)))  \
-  X(kOpUq3LLast, 3,                                                                                           \
-    (up_quad_body<uint8_t, 4, 3, kBlend3, kMask0Label, false>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk, lds))) \
-  X(kOpUqN3F, 3, (up_quad_body<float, 4, 3, kBlendN, kMask0F32, true>(arg_at<UpQuadArgs<float, 3>>(s, off), blk, lds))) \
-  X(kOpUqN3FLast, 3,                                                                                          \
-    (up_quad_body<float, 4, 3, kBlendN, kMask0F32, false>(arg_at<UpQuadArgs<float, 3>>(s, off), blk, lds)))       \
-  X(kOpUqN3L, 3,                                                                                              \
-    (up_quad_body<uint8_t, 4, 3, kBlendN, kMask0Label, true>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk, lds)))  \
-  X(kOpUqN3LLast, 3,                                                                                          \
-    (up_quad_body<uint8_t, 4, 3, kBlendN, kMask0Label, false>(arg_at<UpQuadArgs<uint8_t, 3>>(s, off), blk, lds))) \
-  X(kOpUqN4F, 3, (up_quad_body<float, 4, 4, kBlendN, kMask0F32, true>(arg_at<UpQuadArgs<float, 4>>(s, off), blk, lds))) \
-  X(kOpUqN4FLast, 3,                                                                                          \
-    (up_quad_body<float, 4, 4, kBlendN, kMask0F32, false>(arg_at<UpQuadArgs<float, 4>>(s, off), blk, lds)))       \
-  X(kOpUqN4L, 3,                                                                                              \
-    (up_quad_body<uint8_t, 4, 4, kBlendN, kMask0Label, true>(arg_at<UpQuadArgs<uint8_t, 4>>(s, off), blk, lds)))  \
-  X(kOpUqN4LLast, 3,                                                                                          \
-    (up_quad_body<uint8_t, 4, 4, kBlendN, kMask0Label, false>(arg_at<UpQuadArgs<uint8_t, 4>>(s, off), blk, lds)))
-
-// Runs on 64 x 4 threads; every body uses all 256 (see Blk).  The
-// workgroups walk the launch's work-list with a stride of the grid size.
-template <int OP>
-__global__ void __launch_bounds__(256) batch_op_kernel(StageArgs stage) {
-  (void)stage;  // read through the segment pointer below
-  const kptr<StageArgs> s = (kptr<StageArgs>)__builtin_amdgcn_kernarg_segment_ptr();
-  constexpr int kLds =
-      (OP >= kOpDown2U8 && OP <= kOpDown4F)
-          ? down_lds_float4<4>()
-          : (OP == kOpTail2 ? kTailLdsFloat4
-                            : ((OP == kOpMc3 || OP == kOpMc4)
-                                   ? kCanvasGeoFloat4
-                                   : ((OP >= kOpUq2F && OP <= kOpUqN4LLast) ? quad_lds_float4<4>() : 1)));
-  __shared__ float4 lds[kLds];
-  (void)lds;
-  for (uint32_t w = blockIdx.x; w < s->total; w += gridDim.x) {
-    int j = 0;
-    for (int q = 1; q < s->n_items; ++q)
-      if (w >= s->items[q].first) j = q;
-    const uint32_t off = s->items[j].off, wgs = s->items[j].wgs;
-    const uint32_t local = w - s->items[j].first;
-    const Blk blk{(int)(local % wgs), (int)(local / wgs)};
-#define HM_OP_CALL(OPID, CLS, CALL) \
-  if constexpr (OP == OPID) CALL;
-    HM_BATCH_OPS(HM_OP_CALL)
-#undef HM_OP_CALL
-    __syncthreads();  // the next item may reuse the LDS
-  }
-}
-
-typedef void (*StageKernel)(StageArgs);
-template <int... OPS>
-constexpr std::array<StageKernel, sizeof...(OPS)> op_kernels(std::integer_sequence<int, OPS...>) {
-  return {{batch_op_kernel<OPS>...}};
-}
-const std::array<StageKernel, kNumBatchOps> kOpKernels = op_kernels(std::make_integer_sequence<int, kNumBatchOps>{});
 constexpr int kLaneStreams = 3;  // concurrent launches of one stage beside the caller's stream
 
 }  // namespace
@@ -248,8 +127,7 @@ int hmBatchCreate(const hmBatchCall* calls, int num_calls, hmBatch** out) {
   if (!calls || num_calls < 1) return hipErrorInvalidValue;
   for (int i = 0; i < num_calls; ++i) {
     if (!calls[i].pano || calls[i].pano->status != hipSuccess) return hipErrorInvalidValue;
-    if (calls[i].pano->channels != 4 || calls[i].pano->debug || calls[i].pano->pipeline_f32)
-      return hipErrorNotSupported;
+    if (calls[i].pano->debug || calls[i].pano->pipeline_f32) return hipErrorNotSupported;
     if (calls[i].pano->served.load()) return hipErrorNotSupported;  // a server's thread launches it
     for (int j = 0; j < i; ++j)  // one object's scratch serves one call of a pass
       if (calls[j].pano == calls[i].pano) return hipErrorInvalidValue;
@@ -295,7 +173,15 @@ int hmBatchCreate(const hmBatchCall* calls, int num_calls, hmBatch** out) {
       for (const auto& r : free_rec)
         if ((size_t)r.main_before == k) group.push_back(&r);
     }
-    for (int op = 0; op < kNumBatchOps; ++op) {
+    std::vector<int> ops;  // the bodies of the group, in the carried order
+    for_each_batch_op([&](int op) {
+      for (const RecordedLaunch* rp : group)
+        if (rp->op == op) {
+          ops.push_back(op);
+          break;
+        }
+    });
+    for (const int op : ops) {
       StageArgs cur{};
       uint32_t bump = 0;  // next free byte of cur.blob
       auto flush = [&]() {
@@ -367,7 +253,7 @@ static hipError_t launch_lane(hmBatch* b, hipStream_t stream, int k, bool* forke
   HM_RETURN_IF_ERROR(hipStreamWaitEvent(b->lane, b->ev_lane_fork, 0));
   for (const auto& l : b->free_launches) {
     if (l.stage != k) continue;
-    hipLaunchKernelGGL(kOpKernels[l.op], dim3((unsigned)b->grid_for(l)), dim3(64, 4, 1), 0, b->lane, l.args);
+    hipLaunchKernelGGL(batch_kernel(l.op), dim3((unsigned)b->grid_for(l)), dim3(64, 4, 1), 0, b->lane, l.args);
     HM_RETURN_IF_ERROR(hipGetLastError());
   }
   *forked = true;
@@ -392,7 +278,7 @@ static hipError_t launch_stages(hmBatch* b, hipStream_t stream) {
       const int q = (int)(j - i);  // 0: the caller's stream
       hipStream_t st = q == 0 ? stream : b->side[(q - 1) % kLaneStreams];
       const hmBatch::Launch& l = b->launches[j];
-      hipLaunchKernelGGL(kOpKernels[l.op], dim3((unsigned)b->grid_for(l)), dim3(64, 4, 1), 0, st, l.args);
+      hipLaunchKernelGGL(batch_kernel(l.op), dim3((unsigned)b->grid_for(l)), dim3(64, 4, 1), 0, st, l.args);
       HM_RETURN_IF_ERROR(hipGetLastError());
     }
     for (int q = 0; q < n_side; ++q) {

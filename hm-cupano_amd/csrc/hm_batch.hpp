@@ -20,100 +20,103 @@
 
 namespace hm {
 
-// The kernel bodies of the batched pass: 4-channel pipelines of the 2-image
-// class (any levels, remap tuning 4 pixels x 1 or 2 rows), the 3-image class
-// and the N-image class with 3 or 4 images.
-enum BatchOp : int {
-  kOpNone = -1,
-  kOpP2L0_4_1 = 0,  // pano2_level0_body<4, 4, 1>
-  kOpP2L0_4_2,      // pano2_level0_body<4, 4, 2>
-  kOpP2Hard_4_1,    // pano2_hard_body<4, 4, 1>
-  kOpP2Hard_4_2,    // pano2_hard_body<4, 4, 2>
-  kOpDown2U8,       // down_fused_body<uint8_t, 4, 2>
-  kOpDown2F,        // down_fused_body<float, 4, 2>
-  kOpDown3U8,       // down_fused_slots_body<uint8_t, 4, 3>
-  kOpDown3F,
-  kOpDown4U8,       // down_fused_slots_body<uint8_t, 4, 4>
-  kOpDown4F,
-  kOpTail2,         // up_tail_body<4, 2, kBlend2>
-  kOpUq2F,          // up_quad_body<float, 4, 2, kBlend2, kMask0F32, NEXT>
-  kOpUq2FLast,
-  kOpUq2U8,         // up_quad_body<uint8_t, 4, 2, kBlend2, kMask0U8, NEXT>
-  kOpUq2U8Last,
-  kOpUq3F,          // up_quad_body<float, 4, 3, kBlend3, kMask0F32, NEXT>
-  kOpUq3FLast,
-  kOpUq3L,          // up_quad_body<uint8_t, 4, 3, kBlend3, kMask0Label, NEXT>
-  kOpUq3LLast,
-  kOpUqN3F,         // up_quad_body<float, 4, 3, kBlendN, kMask0F32, NEXT>
-  kOpUqN3FLast,
-  kOpUqN3L,         // up_quad_body<uint8_t, 4, 3, kBlendN, kMask0Label, NEXT>
-  kOpUqN3LLast,
-  kOpUqN4F,         // the same with 4 images
-  kOpUqN4FLast,
-  kOpUqN4L,
-  kOpUqN4LLast,
-  kOpMi3,           // multi_inputs_body<4, 3, false>
-  kOpMi3DM,         // multi_inputs_body<4, 3, true>
-  kOpMi4,           // multi_inputs_body<4, 4, false>
-  kOpMc3,           // multi_canvas_body<4, 3>
-  kOpMc4,           // multi_canvas_body<4, 4>
-  kNumBatchOps
+// The kernel bodies of the batched pass: the uint8 pipelines of the 2-image
+// class (remap tuning 4 pixels x 1 or 2 rows), the 3-image class and the
+// N-image class with 2 to 8 images, with 3 or 4 channels.  An op names one
+// body with its template parameters: family (bits 8+), NEXT (bit 6), a
+// family-specific flag A (bit 5), 4 channels (bit 4) and an image or row
+// count n (bits 0-3).
+enum BatchFamily : int {
+  kFamP2L0 = 1,  // pano2_level0_body<C, 4, RY>                 n = RY
+  kFamP2Hard,    // pano2_hard_body<C, 4, RY>                   n = RY
+  kFamDown,      // down_fused_body (N = 2) or down_fused_slots_body<T0, C, N>; A: T0 = uint8_t, else float
+  kFamTail2,     // up_tail_body<C, 2, kBlend2>
+  kFamUq2,       // up_quad_body<T0, C, 2, kBlend2, M, NEXT>; A: uint8 level and mask (M = kMask0U8), else float (kMask0F32)
+  kFamUq3,       // up_quad_body<T0, C, 3, kBlend3, M, NEXT>; A: uint8 level and labels (kMask0Label), else float
+  kFamUqN,       // up_quad_body<T0, C, N, kBlendN, M, NEXT>; A as for kFamUq3
+  kFamMi,        // multi_inputs_body<C, N, DM>; A: DM
+  kFamMc,        // multi_canvas_body<C, N>
 };
+constexpr int kOpNone = -1;
+constexpr int batch_op(int fam, int c, int a, int next, int n) {
+  return fam << 8 | next << 6 | a << 5 | (c == 4 ? 16 : 0) | n;
+}
+constexpr int op_fam(int op) { return op >> 8; }
+constexpr int op_c(int op) { return (op & 16) ? 4 : 3; }
+constexpr int op_a(int op) { return (op >> 5) & 1; }
+constexpr int op_next(int op) { return (op >> 6) & 1; }
+constexpr int op_n(int op) { return op & 15; }
+
+// Every op the batched pass carries, in a fixed order (the order of a
+// stage's launches).
+template <typename F>
+constexpr void for_each_batch_op(F&& f) {
+  for (int c = 3; c <= 4; ++c) {
+    for (int ry = 1; ry <= 2; ++ry) {
+      f(batch_op(kFamP2L0, c, 0, 0, ry));
+      f(batch_op(kFamP2Hard, c, 0, 0, ry));
+    }
+    for (int n = 2; n <= 8; ++n) {
+      f(batch_op(kFamMi, c, 0, 0, n));
+      f(batch_op(kFamMc, c, 0, 0, n));
+    }
+    f(batch_op(kFamMi, c, 1, 0, 3));
+    for (int a = 0; a <= 1; ++a)
+      for (int n = 2; n <= 8; ++n) f(batch_op(kFamDown, c, a, 0, n));
+    f(batch_op(kFamTail2, c, 0, 0, 2));
+    for (int a = 0; a <= 1; ++a)
+      for (int next = 0; next <= 1; ++next) {
+        f(batch_op(kFamUq2, c, a, next, 2));
+        f(batch_op(kFamUq3, c, a, next, 3));
+        for (int n = 2; n <= 8; ++n) f(batch_op(kFamUqN, c, a, next, n));
+      }
+  }
+}
+constexpr int count_batch_ops() {
+  int k = 0;
+  for_each_batch_op([&](int) { ++k; });
+  return k;
+}
+constexpr int kNumBatchOps = count_batch_ops();
 
 // ---- template parameters -> op (kOpNone: not carried) ----------------------
+constexpr bool batch_channels(int c) { return c == 3 || c == 4; }
 template <int C, int PX, int RY>
 constexpr int op_pano2_level0() {
-  return (C == 4 && PX == 4) ? (RY == 1 ? kOpP2L0_4_1 : (RY == 2 ? kOpP2L0_4_2 : kOpNone)) : kOpNone;
+  return (batch_channels(C) && PX == 4 && (RY == 1 || RY == 2)) ? batch_op(kFamP2L0, C, 0, 0, RY) : kOpNone;
 }
 template <int C, int PX, int RY>
 constexpr int op_pano2_hard() {
-  return (C == 4 && PX == 4) ? (RY == 1 ? kOpP2Hard_4_1 : (RY == 2 ? kOpP2Hard_4_2 : kOpNone)) : kOpNone;
+  return (batch_channels(C) && PX == 4 && (RY == 1 || RY == 2)) ? batch_op(kFamP2Hard, C, 0, 0, RY) : kOpNone;
 }
 template <typename T0, int C, int N>
 constexpr int op_down() {
   constexpr bool u8 = std::is_same<T0, uint8_t>::value;
-  if (C != 4) return kOpNone;
-  if (N == 2) return u8 ? kOpDown2U8 : kOpDown2F;
-  if (N == 3) return u8 ? kOpDown3U8 : kOpDown3F;
-  if (N == 4) return u8 ? kOpDown4U8 : kOpDown4F;
-  return kOpNone;
+  return (batch_channels(C) && N >= 2 && N <= 8) ? batch_op(kFamDown, C, u8, 0, N) : kOpNone;
 }
 template <int C, int N, int KIND>
 constexpr int op_up_tail() {
-  return (C == 4 && N == 2 && KIND == hmdev::kBlend2) ? kOpTail2 : kOpNone;
+  return (batch_channels(C) && N == 2 && KIND == hmdev::kBlend2) ? batch_op(kFamTail2, C, 0, 0, 2) : kOpNone;
 }
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT>
 constexpr int op_up_quad() {
   constexpr bool u8 = std::is_same<T0, uint8_t>::value;
-  if (C != 4) return kOpNone;
+  if (!batch_channels(C)) return kOpNone;
   // MODE values: 0 float mask, 1 uint8 mask, 2 labels (hmdev::Mask0Mode)
-  if (N == 2 && KIND == hmdev::kBlend2) {
-    if (!u8 && MODE == 0) return NEXT ? kOpUq2F : kOpUq2FLast;
-    if (u8 && MODE == 1) return NEXT ? kOpUq2U8 : kOpUq2U8Last;
-    return kOpNone;
-  }
-  if (N == 3 && KIND == hmdev::kBlend3) {
-    if (!u8 && MODE == 0) return NEXT ? kOpUq3F : kOpUq3FLast;
-    if (u8 && MODE == 2) return NEXT ? kOpUq3L : kOpUq3LLast;
-    return kOpNone;
-  }
-  if ((N == 3 || N == 4) && KIND == hmdev::kBlendN) {
-    if (!u8 && MODE == 0) return N == 3 ? (NEXT ? kOpUqN3F : kOpUqN3FLast) : (NEXT ? kOpUqN4F : kOpUqN4FLast);
-    if (u8 && MODE == 2) return N == 3 ? (NEXT ? kOpUqN3L : kOpUqN3LLast) : (NEXT ? kOpUqN4L : kOpUqN4LLast);
-  }
+  if (KIND == hmdev::kBlend2)
+    return (N == 2 && (u8 ? MODE == 1 : MODE == 0)) ? batch_op(kFamUq2, C, u8, NEXT, 2) : kOpNone;
+  const bool mode_ok = u8 ? MODE == 2 : MODE == 0;
+  if (KIND == hmdev::kBlend3) return (N == 3 && mode_ok) ? batch_op(kFamUq3, C, u8, NEXT, 3) : kOpNone;
+  if (KIND == hmdev::kBlendN && N >= 2 && N <= 8 && mode_ok) return batch_op(kFamUqN, C, u8, NEXT, N);
   return kOpNone;
 }
 template <int C, int N, bool DM>
 constexpr int op_multi_inputs() {
-  if (C != 4) return kOpNone;
-  if (N == 3) return DM ? kOpMi3DM : kOpMi3;
-  if (N == 4 && !DM) return kOpMi4;
-  return kOpNone;
+  return (batch_channels(C) && N >= 2 && N <= 8 && (!DM || N == 3)) ? batch_op(kFamMi, C, DM, 0, N) : kOpNone;
 }
 template <int C, int N>
 constexpr int op_multi_canvas() {
-  if (C != 4) return kOpNone;
-  return N == 3 ? kOpMc3 : (N == 4 ? kOpMc4 : kOpNone);
+  return (batch_channels(C) && N >= 2 && N <= 8) ? batch_op(kFamMc, C, 0, 0, N) : kOpNone;
 }
 
 // ---- the recorder -----------------------------------------------------------

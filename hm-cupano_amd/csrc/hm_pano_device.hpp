@@ -428,7 +428,7 @@ hipError_t launch_multi(const MultiGeom& g, const MultiRegion& r, int channels, 
     case kMultiInputs:
       if (hm::g_recorder)
         return c4 ? hm::g_recorder->add(hm::op_multi_inputs<4, N, false>(), grid, kBlock, s, a, "multi_inputs")
-                  : hm::g_recorder->fail("multi_inputs");
+                  : hm::g_recorder->add(hm::op_multi_inputs<3, N, false>(), grid, kBlock, s, a, "multi_inputs");
       if (c4)
         hipLaunchKernelGGL((multi_inputs_kernel<4, N, false>), grid, kBlock, 0, s, a);
       else
@@ -437,7 +437,7 @@ hipError_t launch_multi(const MultiGeom& g, const MultiRegion& r, int channels, 
     case kMultiInputsDM:
       if (hm::g_recorder)
         return c4 ? hm::g_recorder->add(hm::op_multi_inputs<4, N, true>(), grid, kBlock, s, a, "multi_inputs")
-                  : hm::g_recorder->fail("multi_inputs");
+                  : hm::g_recorder->add(hm::op_multi_inputs<3, N, true>(), grid, kBlock, s, a, "multi_inputs");
       if (c4)
         hipLaunchKernelGGL((multi_inputs_kernel<4, N, true>), grid, kBlock, 0, s, a);
       else
@@ -446,7 +446,7 @@ hipError_t launch_multi(const MultiGeom& g, const MultiRegion& r, int channels, 
     default:
       if (hm::g_recorder)
         return c4 ? hm::g_recorder->add(hm::op_multi_canvas<4, N>(), grid, kBlock, s, a, "multi_canvas")
-                  : hm::g_recorder->fail("multi_canvas");
+                  : hm::g_recorder->add(hm::op_multi_canvas<3, N>(), grid, kBlock, s, a, "multi_canvas");
       if (c4)
         hipLaunchKernelGGL((multi_canvas_kernel<4, N>), grid, kBlock, 0, s, a);
       else

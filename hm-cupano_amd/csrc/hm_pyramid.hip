@@ -310,7 +310,7 @@ hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int bat
   a.tiles_y = (u.wy1 - u.wy0 + kTailTile) / kTailTile;
   const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
   if (g_recorder) {
-    const int op = c == 4 && kind == kBlend2 ? kOpTail2 : kOpNone;
+    const int op = kind != kBlend2 ? kOpNone : (c == 4 ? op_up_tail<4, 2, kBlend2>() : op_up_tail<3, 2, kBlend2>());
     return g_recorder->add(op, grid, dim3(256), s, a, "up_tail");
   }
   if (c == 4) {

@@ -371,10 +371,10 @@ void hmServerDestroy(hmServer* server);
  * written where they are at create time, like a captured graph); each frame's
  * output is bit-identical to its own process().
  *
- * Carried: 4-channel pipelines of the 2-image class, the 3-image class and the
- * N-image class with 3 or 4 images (any levels).  Anything else, a pano in
- * debug mode, or one pano in two calls: hipErrorNotSupported /
- * hipErrorInvalidValue at create time.
+ * Carried: the uint8 pipelines (3 or 4 channels) of the 2-image class, the
+ * 3-image class and the N-image class with 2 to 8 images (any levels).  A
+ * float pipeline, a pano in debug mode or being served, or one pano in two
+ * calls: hipErrorNotSupported / hipErrorInvalidValue at create time.
  * ------------------------------------------------------------------------- */
 typedef struct hmBatch hmBatch;
 typedef struct hmBatchCall {

@@ -38,6 +38,42 @@ def make_calls(hm):
     return out
 
 
+def make_calls_wide(hm):
+    """A mix of 3-channel pipelines of every class and of N-image rigs with 2 and
+    5 to 8 images: (name, pano, rig, levels, kind, fused, batch, channels)."""
+    import hm_synth
+
+    def n_rig(n, seed):
+        return hm_synth.small_rig(n=n, seed=seed, src=(120, 56), canvas=(90 * n + 60, 64))
+
+    out = []
+    r = hm_synth.small_rig(seed=61)
+    out.append(("pano2_c3_L5", hm.StitchPano(1, 5, r, channels=3), r, 5, "2", True, 1, 3))
+    r = hm_synth.small_rig(seed=62)
+    out.append(("pano2_c3_L0_b2", hm.StitchPano(2, 0, r, channels=3), r, 0, "2", True, 2, 3))
+    r = n_rig(3, 63)
+    out.append(("panoN3_c3_L3", hm.StitchPanoN(1, 3, r, channels=3), r, 3, "N", True, 1, 3))
+    r = n_rig(5, 64)
+    out.append(("panoN5_c3_L4", hm.StitchPanoN(1, 4, r, channels=3), r, 4, "N", True, 1, 3))
+    r = hm_synth.small_rig(n=3, seed=65, src=(120, 56), canvas=(330, 64))
+    out.append(("pano3_c3_L2_fused", hm.StitchPano3(1, 2, r, channels=3), r, 2, "3", True, 1, 3))
+    r = hm_synth.small_rig(n=3, seed=66, src=(120, 56), canvas=(330, 64))
+    out.append(("pano3_c3_L3_unfused", hm.StitchPano3(1, 3, r, channels=3), r, 3, "3", False, 1, 3))
+    r = n_rig(2, 67)
+    out.append(("panoN2_L3", hm.StitchPanoN(1, 3, r), r, 3, "N", True, 1, 4))
+    r = n_rig(5, 68)
+    out.append(("panoN5_L0", hm.StitchPanoN(1, 0, r), r, 0, "N", True, 1, 4))
+    r = n_rig(6, 69)
+    out.append(("panoN6_L4_b2", hm.StitchPanoN(2, 4, r), r, 4, "N", True, 2, 4))
+    r = n_rig(7, 70)
+    out.append(("panoN7_L2", hm.StitchPanoN(1, 2, r), r, 2, "N", True, 1, 4))
+    r = n_rig(8, 71)
+    out.append(("panoN8_L6", hm.StitchPanoN(1, 6, r), r, 6, "N", True, 1, 4))
+    r = n_rig(8, 72)
+    out.append(("panoN8_c3_L5", hm.StitchPanoN(1, 5, r, channels=3), r, 5, "N", True, 1, 3))
+    return out
+
+
 def oracle_out(oracle, kind, rig, frames, levels, fused, prior):
     if kind == "2":
         return oracle.pano2(rig, frames[0], frames[1], levels)
@@ -48,10 +84,11 @@ def oracle_out(oracle, kind, rig, frames, levels, fused, prior):
 
 def build(hm, torch, calls, seed):
     bufs = []
-    for k, (name, pano, rig, levels, kind, fused, b) in enumerate(calls):
-        fr = frames_for(rig, b, seed + 10 * k)
+    for k, (name, pano, rig, levels, kind, fused, b, *ch) in enumerate(calls):
+        c = ch[0] if ch else 4
+        fr = frames_for(rig, b, seed + 10 * k, c)
         w, h = pano.canvas_size()
-        prior = np.full((b, h, w, 4), 7 + k, np.uint8)
+        prior = np.full((b, h, w, c), 7 + k, np.uint8)
         bufs.append((fr, to_dev(torch, fr), torch.from_numpy(prior).cuda(), prior))
     torch.cuda.synchronize()
     batch = hm.Batch([(c[1], bufs[k][1], bufs[k][2], c[5]) for k, c in enumerate(calls)])
@@ -59,7 +96,7 @@ def build(hm, torch, calls, seed):
 
 
 def check(oracle, calls, bufs, msg):
-    for k, (name, pano, rig, levels, kind, fused, b) in enumerate(calls):
+    for k, (name, pano, rig, levels, kind, fused, b, *_) in enumerate(calls):
         fr, _, canvas, prior = bufs[k]
         want = oracle_out(oracle, kind, rig, fr, levels, fused, prior)
         np.testing.assert_array_equal(canvas.cpu().numpy(), want, err_msg=f"{name} {msg}")
@@ -76,6 +113,55 @@ def test_batch_mixed_matches_alone(hm, torch_gpu, oracle, use_graph):
     assert batch.run(use_graph=use_graph) == 0
     torch.cuda.synchronize()
     check(oracle, calls, bufs, f"graph={use_graph}")
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_three_channels_and_up_to_eight_images(hm, torch_gpu, oracle, use_graph):
+    """3-channel pipelines of every class and N-image rigs of 2 and 5 to 8 images in one pass."""
+    torch = torch_gpu
+    calls = make_calls_wide(hm)
+    batch, bufs = build(hm, torch, calls, 500)
+    info = batch.info()
+    assert info["launches"] < info["recorded_launches"]
+    assert batch.run(use_graph=use_graph) == 0
+    torch.cuda.synchronize()
+    check(oracle, calls, bufs, f"graph={use_graph}")
+
+
+def test_batch_more_than_four_images_per_tile(hm, torch_gpu, oracle):
+    """Six images over the whole canvas (identity maps, all at (0, 0)): every tile
+    meets all six, so the batched down pass runs several slot chunks per tile and
+    the up pass its per-pixel path; beside it an 8-image rig with a 2-image one."""
+    import hm_synth
+    from conftest import IdRig
+
+    torch = torch_gpu
+    w, h, n = 160, 72, 6
+    ys, xs = np.mgrid[0:h, 0:w]
+    seam = np.clip(((xs + 7 * np.sin(ys / 5.0)) * n) // w, 0, n - 1).astype(np.uint8)
+    rig = IdRig([(w, h)] * n, [(0, 0)] * n, seam)
+    rng = np.random.default_rng(81)
+    imgs = []
+    for i in range(n):
+        im = rng.integers(0, 256, (1, h, w, 4), dtype=np.uint8)
+        im[..., 3] = 255
+        im[0, 10 * i: 10 * i + 6, 20:40, 3] = 0  # a transparent hole per image
+        imgs.append(im)
+    pano6 = hm.StitchPanoN(1, 4, rig)
+    r8 = hm_synth.small_rig(n=8, seed=82, src=(120, 56), canvas=(780, 64))
+    pano8 = hm.StitchPanoN(1, 3, r8)
+    fr8 = frames_for(r8, 1, 83)
+    r2 = hm_synth.small_rig(seed=84)
+    pano2 = hm.StitchPano(1, 4, r2)
+    fr2 = frames_for(r2, 1, 85)
+    cv = [p.alloc_canvas() for p in (pano6, pano8, pano2)]
+    batch = hm.Batch([(pano6, to_dev(torch, imgs), cv[0]), (pano8, to_dev(torch, fr8), cv[1]),
+                      (pano2, to_dev(torch, fr2), cv[2])])
+    assert batch.run(use_graph=True) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(cv[0].cpu().numpy(), oracle.panoN(rig, imgs, 4))
+    np.testing.assert_array_equal(cv[1].cpu().numpy(), oracle.panoN(r8, fr8, 3))
+    np.testing.assert_array_equal(cv[2].cpu().numpy(), oracle.pano2(r2, fr2[0], fr2[1], 4))
 
 
 @pytest.mark.parametrize("grid", [1, 7, 64])
@@ -144,13 +230,9 @@ def test_batch_errors(hm, torch_gpu):
     canvas = pano.alloc_canvas()
     with pytest.raises(hm.HmError):  # one object twice in a pass
         hm.Batch([(pano, dev, canvas), (pano, dev, canvas)])
-    r3c = hm_synth.small_rig(seed=52)
-    p3c = hm.StitchPano(1, 3, r3c, channels=3)
-    with pytest.raises(hm.HmError):  # 3-channel pipelines are not carried
-        hm.Batch([(p3c, to_dev(torch, frames_for(r3c, 1, 2, channels=3)), p3c.alloc_canvas())])
     r5 = hm_synth.small_rig(n=5, seed=53, src=(120, 56), canvas=(510, 64))
-    p5 = hm.StitchPanoN(1, 3, r5)
-    with pytest.raises(hm.HmError):  # five images: not carried
+    p5 = hm.StitchPanoN(1, 3, r5, debug=True)
+    with pytest.raises(hm.HmError):  # debug mode (every level kept) is not carried
         hm.Batch([(p5, to_dev(torch, frames_for(r5, 1, 3)), p5.alloc_canvas())])
     # a failed create leaves the panos usable on their own
     assert pano.process(dev[0], dev[1], canvas) == 0
