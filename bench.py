@@ -260,21 +260,39 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
         r = rigs[ri]
         return server.submit(ids[ri], r["inputs"][k % len(r["inputs"])], r["canvases"][k % 2], tag=tag)
 
-    # served: warm-up (captures the graphs), then everything at once
-    for i, (ri, k) in enumerate(order[: len(rigs) * 4]):
-        hm_stitch.check(submit(ri, k, i), "submit")
-    hm_stitch.check(server.wait_idle(), "served warm-up")
-    server.poll(1 << 16)
-    marks["served_start"] = mark()
-    t0 = time.perf_counter()
-    for i, (ri, k) in enumerate(order):
-        hm_stitch.check(submit(ri, k, i), "submit")
-    hm_stitch.check(server.wait_idle(), "served run")
-    t_served = time.perf_counter() - t0
-    marks["served_end"] = mark()
-    server.poll(1 << 16)
-    torch.cuda.synchronize()
-    equal = all(bool(torch.equal(r["canvases"][0], ref)) for r, ref in zip(rigs, plain_ref))
+    # served, for each scheduling mode: warm-up (captures the graphs), then everything at once.
+    # "graphs": each request alone on a free stream, its rig's graph replayed; "batchedK":
+    # continuous batching, the ready requests of up to K rigs as one batched pass.
+    modes = {}
+    equal = True
+    for mode, max_batch in (("graphs", 0), ("batched8", 8), ("batched32", 32)):
+        server.set_batching(max_batch)
+        for i, (ri, k) in enumerate(order[: len(rigs) * 4]):
+            hm_stitch.check(submit(ri, k, i), "submit")
+        hm_stitch.check(server.wait_idle(), "served warm-up")
+        server.poll(1 << 16)
+        st0 = server.stats()
+        marks[f"served_{mode}_start"] = mark()
+        t0 = time.perf_counter()
+        for i, (ri, k) in enumerate(order):
+            hm_stitch.check(submit(ri, k, i), "submit")
+        hm_stitch.check(server.wait_idle(), "served run")
+        dt = time.perf_counter() - t0
+        marks[f"served_{mode}_end"] = mark()
+        server.poll(1 << 16)
+        st1 = server.stats()
+        torch.cuda.synchronize()
+        equal = equal and all(bool(torch.equal(r["canvases"][0], ref)) for r, ref in zip(rigs, plain_ref))
+        passes = st1["batched_passes"] - st0["batched_passes"]
+        modes[mode] = {"fps": round(frames_closed / dt, 1), "Mpix_s": round(mpix_closed / dt, 1),
+                       "vs_plain": round(t_plain / dt, 3), "batched_passes": passes,
+                       "frames_per_pass": round((st1["batched_frames"] - st0["batched_frames"]) / passes, 2)
+                       if passes else None,
+                       "pass_graph_launches": st1["pass_graph_launches"] - st0["pass_graph_launches"]}
+        log(f"served load: closed loop {mode}: {modes[mode]}")
+    best = max(modes, key=lambda m: modes[m]["fps"])
+    t_served = frames_closed / modes[best]["fps"]
+    server.set_batching({"graphs": 0, "batched8": 8, "batched32": 32}[best])
 
     # open loop: Poisson arrivals, 30 fps per rig
     arrivals = []
@@ -287,7 +305,7 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
             k += 1
     arrivals.sort()
     lat = []
-    log(f"served load: open loop, {len(arrivals)} Poisson arrivals over {seconds:.0f} s")
+    log(f"served load: open loop ({best}), {len(arrivals)} Poisson arrivals over {seconds:.0f} s")
     t_start = time.perf_counter()
     next_log = 10.0
     for tag, (t_arr, ri, k) in enumerate(arrivals):
@@ -318,9 +336,11 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
         "plain_fps": round(plain_fps, 1), "plain_Mpix_s": round(mpix_closed / t_plain, 1),
         "served_fps": round(served_fps, 1), "served_Mpix_s": round(mpix_closed / t_served, 1),
         "served_vs_plain": round(served_fps / plain_fps, 3),
+        "served_mode": best,
+        "served_modes": modes,
         "served_equals_plain_output": equal,
         "plain_ms_per_frame": {k: round(1e3 * float(np.median(v)), 3) for k, v in per_cfg.items()},
-        "open_loop": {"seconds": seconds, "offered_fps": round(len(arrivals) / seconds, 1),
+        "open_loop": {"mode": best, "seconds": seconds, "offered_fps": round(len(arrivals) / seconds, 1),
                       "achieved_fps": round(len(res) / t_open, 1), "Mpix_s": round(mpix_open / t_open, 1),
                       "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
                       "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "hm_batch.hpp"
 #include "hm_files.h"
 #include "hm_pano.hpp"
 
@@ -82,12 +83,16 @@ hipError_t hmStitchPano::ensure_side() {
   return hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
 }
 
+// While a call is recorded for a batched pass nothing is enqueued: the pass
+// orders the side-stream launches itself.
 hipError_t hmStitchPano::fork(hipStream_t s) {
+  if (hm::g_recorder) return hipSuccess;
   HM_RETURN_IF_ERROR(hipEventRecord(ev_fork, s));
   return hipStreamWaitEvent(side, ev_fork, 0);
 }
 
 hipError_t hmStitchPano::join(hipStream_t s) {
+  if (hm::g_recorder) return hipSuccess;
   HM_RETURN_IF_ERROR(hipEventRecord(ev_join, side));
   return hipStreamWaitEvent(s, ev_join, 0);
 }

@@ -12,6 +12,14 @@
 //     batching);
 //   - a rig's pipeline is captured once per (inputs, canvas) buffer set as a
 //     hipGraph and replayed, so a frame costs one graph launch on the host;
+//   - with batching on (hmServerSetBatching), the requests that are ready
+//     when a stream frees up -- frames of different rigs, of any class,
+//     canvas size, image and level count -- run as one batched pass: each
+//     kernel body of a pipeline stage is one persistent launch whose
+//     workgroups walk the tile work-list of every frame in the pass (the
+//     pass planner of hm_pass.hpp over each request's recorded launches).
+//     A pass whose composition comes back is captured as a hipGraph and
+//     replayed.  A request that is ready alone takes its rig's graph;
 //   - completion is observed by polling one event per stream; each result
 //     carries its submit, start and completion times.
 // Per-pixel work is the pano classes' own, so every frame is bit-identical
@@ -27,6 +35,7 @@
 #include <vector>
 
 #include "hm_pano.hpp"
+#include "hm_pass.hpp"
 
 namespace {
 
@@ -65,6 +74,14 @@ GraphKey key_of(const Request& r) {
 }
 
 constexpr size_t kMaxGraphsPerRig = 16;
+constexpr size_t kMaxPassGraphs = 64;    // captured batched passes, least recently used evicted
+constexpr size_t kMaxPassSightings = 4096;
+
+// One rig's recorded launches for one buffer set (a batched pass's input).
+struct Rec {
+  uint64_t id = 0;  // unique over the server's life: names the recording in pass keys
+  hm::Recording launches;
+};
 
 struct Rig {
   hmStitchPano* pano = nullptr;
@@ -72,17 +89,24 @@ struct Rig {
   bool busy = false;   // a request of this rig is on the device
   int queued = 0;      // requests waiting
   bool removing = false;
+  bool batchable = true;  // false once a recording failed (a launch the batched kernels do not carry)
   std::map<GraphKey, hipGraphExec_t> graphs;
   std::deque<GraphKey> lru;
+  std::map<GraphKey, std::unique_ptr<Rec>> recs;
+  std::deque<GraphKey> rec_lru;
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
-  Request req;
+  std::vector<Request> reqs;  // one request, or the frames of a batched pass
   int64_t t_start = 0;
+  std::unique_ptr<hm::PassStreams> ps;  // side streams of the batched passes run here
 };
+
+// A captured batched pass, keyed by the recordings it runs (in pass order).
+using PassKey = std::vector<uint64_t>;
 
 }  // namespace
 
@@ -97,25 +121,43 @@ struct hmServer {
   std::vector<hmServerResult> done;
   size_t budget = 0, used = 0;
   bool use_graphs = true;
+  int max_batch = 0;  // frames per batched pass (below 2: no batching)
   bool stop = false;
   int in_flight = 0;
   long graphs_captured = 0, graph_launches = 0, eager_launches = 0;
+  long batched_passes = 0, batched_frames = 0, pass_graphs_captured = 0, pass_graph_launches = 0;
+  uint64_t next_rec = 1;
+  hm::PassPlan plan;  // planning scratch (scheduler thread)
+  std::map<PassKey, hipGraphExec_t> pass_graphs;
+  std::deque<PassKey> pass_lru;
+  std::map<PassKey, int> pass_seen;  // compositions seen once (captured on the second sighting)
   int sticky = hipSuccess;  // first device-side error seen by the scheduler
   std::thread th;
 
   void run();
   hipError_t launch(Slot& sl, Rig& rig);
+  hipError_t launch_pass(Slot& sl, const std::vector<Rec*>& recs);
+  Rec* recording_of(Rig& rig, const Request& r);
+  void finish(Slot& sl, hipError_t status, int64_t t_done);
   void drop_graphs(Rig& rig) {
     for (auto& kv : rig.graphs) (void)hipGraphExecDestroy(kv.second);
     rig.graphs.clear();
     rig.lru.clear();
+    rig.recs.clear();
+    rig.rec_lru.clear();
+  }
+  void drop_pass_graphs() {
+    for (auto& kv : pass_graphs) (void)hipGraphExecDestroy(kv.second);
+    pass_graphs.clear();
+    pass_lru.clear();
+    pass_seen.clear();
   }
 };
 
 // Starts one request on a free stream: replays the rig's graph for this
 // buffer set (capturing it the first time), or launches eagerly.
 hipError_t hmServer::launch(Slot& sl, Rig& rig) {
-  Request& r = sl.req;
+  Request& r = sl.reqs[0];
   if (use_graphs) {
     const GraphKey k = key_of(r);
     auto it = rig.graphs.find(k);
@@ -149,6 +191,99 @@ hipError_t hmServer::launch(Slot& sl, Rig& rig) {
   return rig.pano->process_any(r.inputs.data(), (int)r.inputs.size(), r.canvas, r.fused, sl.stream);
 }
 
+// The rig's recording for the request's buffer set (recorded the first time:
+// host work only), or nullptr when the rig's pipeline cannot be batched.
+Rec* hmServer::recording_of(Rig& rig, const Request& r) {
+  if (!rig.batchable) return nullptr;
+  const GraphKey k = key_of(r);
+  auto it = rig.recs.find(k);
+  if (it != rig.recs.end()) return it->second.get();
+  std::unique_ptr<Rec> rec(new Rec);
+  if (hm::record_process(rig.pano, r.inputs.data(), (int)r.inputs.size(), r.canvas, r.fused, rec->launches) !=
+      hipSuccess) {
+    rig.batchable = false;
+    return nullptr;
+  }
+  rec->id = next_rec++;
+  if (rig.recs.size() >= kMaxGraphsPerRig) {
+    rig.recs.erase(rig.rec_lru.front());
+    rig.rec_lru.pop_front();
+  }
+  rig.rec_lru.push_back(k);
+  return rig.recs.emplace(k, std::move(rec)).first->second.get();
+}
+
+// Starts a batched pass over the recordings on the slot's stream: replays
+// the pass's graph when this composition was captured, captures it on its
+// second sighting, and launches the pass eagerly otherwise.
+hipError_t hmServer::launch_pass(Slot& sl, const std::vector<Rec*>& recs) {
+  if (!sl.ps) sl.ps.reset(new hm::PassStreams);
+  HM_RETURN_IF_ERROR(sl.ps->create());
+  PassKey key;
+  std::vector<const hm::Recording*> calls;
+  for (const Rec* r : recs) {
+    key.push_back(r->id);
+    calls.push_back(&r->launches);
+  }
+  ++batched_passes;
+  batched_frames += (long)recs.size();
+  auto it = pass_graphs.find(key);
+  if (it != pass_graphs.end()) {
+    ++pass_graph_launches;
+    return hipGraphLaunch(it->second, sl.stream);
+  }
+  HM_RETURN_IF_ERROR(hm::plan_pass(calls, plan));
+  bool capture = false;
+  if (use_graphs) {
+    if (pass_seen.size() >= kMaxPassSightings) pass_seen.clear();
+    capture = ++pass_seen[key] >= 2;
+  }
+  if (!capture) return hm::run_pass(plan, *sl.ps, 0, sl.stream);
+  hipGraph_t g = nullptr;
+  HM_RETURN_IF_ERROR(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
+  const hipError_t e = hm::run_pass(plan, *sl.ps, 0, sl.stream);
+  const hipError_t ec = hipStreamEndCapture(sl.stream, &g);
+  if (e != hipSuccess || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return e != hipSuccess ? e : ec;
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HM_RETURN_IF_ERROR(ei);
+  if (pass_graphs.size() >= kMaxPassGraphs) {
+    (void)hipGraphExecDestroy(pass_graphs[pass_lru.front()]);
+    pass_graphs.erase(pass_lru.front());
+    pass_lru.pop_front();
+  }
+  pass_graphs.emplace(key, ex);
+  pass_lru.push_back(key);
+  pass_seen.erase(key);
+  ++pass_graphs_captured;
+  ++pass_graph_launches;
+  return hipGraphLaunch(ex, sl.stream);
+}
+
+// Completes every request of the slot with `status`.
+void hmServer::finish(Slot& sl, hipError_t status, int64_t t_done) {
+  for (const Request& r : sl.reqs) {
+    hmServerResult res{};
+    res.tag = r.tag;
+    res.rig_id = r.rig;
+    res.status = status;
+    res.t_submit_ns = r.t_submit;
+    res.t_start_ns = sl.t_start;
+    res.t_done_ns = t_done;
+    done.push_back(res);
+    auto rit = rigs.find(r.rig);
+    if (rit != rigs.end()) rit->second.busy = false;
+    --in_flight;
+  }
+  if (status != hipSuccess && sticky == hipSuccess) sticky = status;
+  sl.reqs.clear();
+  sl.busy = false;
+}
+
 void hmServer::run() {
   std::unique_lock<std::mutex> lk(mu);
   while (true) {
@@ -158,51 +293,58 @@ void hmServer::run() {
       if (!sl.busy) continue;
       const hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
-      hmServerResult res{};
-      res.tag = sl.req.tag;
-      res.rig_id = sl.req.rig;
-      res.status = q;
-      res.t_submit_ns = sl.req.t_submit;
-      res.t_start_ns = sl.t_start;
-      res.t_done_ns = now_ns();
-      if (q != hipSuccess && sticky == hipSuccess) sticky = q;
-      done.push_back(res);
-      sl.busy = false;
-      auto rit = rigs.find(sl.req.rig);
-      if (rit != rigs.end()) rit->second.busy = false;
-      --in_flight;
+      finish(sl, q, now_ns());
       progressed = true;
     }
-    // 2. admissions: the oldest request whose rig is idle, on each free stream
+    // 2. admissions, on each free stream: the oldest request whose rig is
+    // idle; with batching on, together with the other ready requests of
+    // distinct idle rigs (oldest first) that the batched kernels carry.
     for (auto& sl : slots) {
       if (sl.busy) continue;
-      auto qi = std::find_if(queue.begin(), queue.end(), [&](const Request& r) {
+      auto ready = [&](const Request& r) {
         auto it = rigs.find(r.rig);
         return it != rigs.end() && !it->second.busy;
-      });
+      };
+      auto qi = std::find_if(queue.begin(), queue.end(), ready);
       if (qi == queue.end()) break;
-      Rig& rig = rigs[qi->rig];
-      sl.req = std::move(*qi);
-      queue.erase(qi);
-      --rig.queued;
-      rig.busy = true;
+      std::vector<std::deque<Request>::iterator> pick{qi};
+      std::vector<Rec*> recs;
+      if (max_batch >= 2) {
+        Rec* r0 = recording_of(rigs[qi->rig], *qi);
+        if (r0) {
+          recs.push_back(r0);
+          for (auto qj = std::next(qi); qj != queue.end() && (int)pick.size() < max_batch; ++qj) {
+            if (!ready(*qj)) continue;
+            bool dup = false;
+            for (const auto& p : pick) dup = dup || p->rig == qj->rig;
+            if (dup) continue;
+            Rec* rj = recording_of(rigs[qj->rig], *qj);
+            if (!rj) continue;
+            pick.push_back(qj);
+            recs.push_back(rj);
+          }
+        }
+      }
+      sl.reqs.clear();
+      std::vector<size_t> pos;  // queue positions of the picked requests, ascending
+      for (auto& p : pick) {
+        Rig& rig = rigs[p->rig];
+        pos.push_back((size_t)(p - queue.begin()));
+        sl.reqs.push_back(std::move(*p));
+        --rig.queued;
+        rig.busy = true;
+        ++in_flight;
+      }
+      // An erase in the middle of a deque invalidates every iterator: erase by
+      // position, the last first.
+      for (auto p = pos.rbegin(); p != pos.rend(); ++p) queue.erase(queue.begin() + (std::ptrdiff_t)*p);
       sl.busy = true;
       sl.t_start = now_ns();
-      ++in_flight;
-      hipError_t e = launch(sl, rig);
+      hipError_t e = sl.reqs.size() >= 2 ? launch_pass(sl, recs) : launch(sl, rigs[sl.reqs[0].rig]);
       if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
-      if (e != hipSuccess) {  // nothing (more) runs for it: complete it with the error now
-        hmServerResult res{};
-        res.tag = sl.req.tag;
-        res.rig_id = sl.req.rig;
-        res.status = e;
-        res.t_submit_ns = sl.req.t_submit;
-        res.t_start_ns = res.t_done_ns = sl.t_start;
-        done.push_back(res);
+      if (e != hipSuccess) {  // nothing (more) runs for them: complete them with the error now
         (void)hipStreamSynchronize(sl.stream);
-        sl.busy = false;
-        rig.busy = false;
-        --in_flight;
+        finish(sl, e, sl.t_start);
       }
       progressed = true;
     }
@@ -269,32 +411,48 @@ extern "C" int hmServerRemoveRig(hmServer* sv, int rig_id) {
   it->second.removing = true;
   sv->idle_cv.wait(lk, [&] { return it->second.queued == 0 && !it->second.busy; });
   sv->drop_graphs(it->second);
+  sv->drop_pass_graphs();  // passes that ran the rig's recordings
   sv->used -= it->second.bytes;
   it->second.pano->served.store(false);
   sv->rigs.erase(it);
   return hipSuccess;
 }
 
-extern "C" int hmServerSubmit(hmServer* sv, int rig_id, const hmSurface* inputs, int n_inputs, hmSurface canvas,
-                              int fused, uint64_t tag) {
-  if (!sv || !inputs || n_inputs < 1 || n_inputs > HM_MAX_IMAGES) return hipErrorInvalidValue;
-  Request r;
-  r.rig = rig_id;
-  r.inputs.assign(inputs, inputs + n_inputs);
-  r.canvas = canvas;
-  r.fused = fused;
-  r.tag = tag;
+extern "C" int hmServerSubmitMany(hmServer* sv, const hmServerRequest* reqs, int n) {
+  if (!sv || n < 0 || (n > 0 && !reqs)) return hipErrorInvalidValue;
+  std::vector<Request> rs((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const hmServerRequest& q = reqs[i];
+    if (!q.inputs || q.num_inputs < 1 || q.num_inputs > HM_MAX_IMAGES) return hipErrorInvalidValue;
+    Request& r = rs[(size_t)i];
+    r.rig = q.rig_id;
+    r.inputs.assign(q.inputs, q.inputs + q.num_inputs);
+    r.canvas = q.canvas;
+    r.fused = q.fused;
+    r.tag = q.tag;
+  }
   {
     std::lock_guard<std::mutex> lk(sv->mu);
-    auto it = sv->rigs.find(rig_id);
-    if (it == sv->rigs.end() || it->second.removing) return hipErrorInvalidValue;
-    if (it->second.pano->n_images != n_inputs) return hipErrorInvalidValue;
-    r.t_submit = now_ns();
-    ++it->second.queued;
-    sv->queue.push_back(std::move(r));
+    for (const Request& r : rs) {  // all or nothing
+      auto it = sv->rigs.find(r.rig);
+      if (it == sv->rigs.end() || it->second.removing) return hipErrorInvalidValue;
+      if (it->second.pano->n_images != (int)r.inputs.size()) return hipErrorInvalidValue;
+    }
+    const int64_t t = now_ns();
+    for (Request& r : rs) {
+      r.t_submit = t;
+      ++sv->rigs[r.rig].queued;
+      sv->queue.push_back(std::move(r));
+    }
   }
   sv->cv.notify_one();
   return hipSuccess;
+}
+
+extern "C" int hmServerSubmit(hmServer* sv, int rig_id, const hmSurface* inputs, int n_inputs, hmSurface canvas,
+                              int fused, uint64_t tag) {
+  const hmServerRequest r{rig_id, inputs, n_inputs, canvas, fused, tag};
+  return hmServerSubmitMany(sv, &r, 1);
 }
 
 extern "C" int hmServerPoll(hmServer* sv, hmServerResult* results, int max_results, int* n_results) {
@@ -325,6 +483,17 @@ extern "C" int hmServerGetStats(hmServer* sv, hmServerStats* st) {
   st->graphs_captured = sv->graphs_captured;
   st->graph_launches = sv->graph_launches;
   st->eager_launches = sv->eager_launches;
+  st->batched_passes = sv->batched_passes;
+  st->batched_frames = sv->batched_frames;
+  st->pass_graphs_captured = sv->pass_graphs_captured;
+  st->pass_graph_launches = sv->pass_graph_launches;
+  return hipSuccess;
+}
+
+extern "C" int hmServerSetBatching(hmServer* sv, int max_frames) {
+  if (!sv || max_frames < 0) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->max_batch = max_frames;
   return hipSuccess;
 }
 
@@ -340,6 +509,7 @@ extern "C" void hmServerDestroy(hmServer* sv) {
     sv->drop_graphs(kv.second);
     kv.second.pano->served.store(false);
   }
+  sv->drop_pass_graphs();
   for (auto& sl : sv->slots) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);

@@ -76,7 +76,14 @@ class hmServerStats(ctypes.Structure):
     _fields_ = [("n_rigs", ctypes.c_int), ("device_budget_bytes", ctypes.c_size_t),
                 ("device_used_bytes", ctypes.c_size_t), ("queued", ctypes.c_int), ("in_flight", ctypes.c_int),
                 ("graphs_captured", ctypes.c_long), ("graph_launches", ctypes.c_long),
-                ("eager_launches", ctypes.c_long)]
+                ("eager_launches", ctypes.c_long), ("batched_passes", ctypes.c_long),
+                ("batched_frames", ctypes.c_long), ("pass_graphs_captured", ctypes.c_long),
+                ("pass_graph_launches", ctypes.c_long)]
+
+
+class hmServerRequest(ctypes.Structure):
+    _fields_ = [("rig_id", ctypes.c_int), ("inputs", ctypes.POINTER(hmSurface)), ("num_inputs", ctypes.c_int),
+                ("canvas", hmSurface), ("fused", ctypes.c_int), ("tag", ctypes.c_uint64)]
 
 
 class hmBatchCall(ctypes.Structure):
@@ -189,6 +196,8 @@ def lib() -> ctypes.CDLL:
         "hmServerWaitIdle": (i, [vp]),
         "hmServerGetStats": (i, [vp, ctypes.POINTER(hmServerStats)]),
         "hmServerDestroy": (None, [vp]),
+        "hmServerSetBatching": (i, [vp, i]),
+        "hmServerSubmitMany": (i, [vp, ctypes.POINTER(hmServerRequest), i]),
         "hmBatchCreate": (i, [ctypes.POINTER(hmBatchCall), i, ctypes.POINTER(vp)]),
         "hmBatchSetGrid": (i, [vp, i]),
         "hmBatchGetInfo": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
@@ -496,13 +505,21 @@ class StitchPano3(StitchPano):
 # ---------------------------------------------------------------------------
 class Server:
     """Many rigs on one GPU (hmServer*): admission by device bytes, a scheduler
-    thread over `n_streams` streams, hipGraph replay per buffer set."""
+    thread over `n_streams` streams, hipGraph replay per buffer set, and with
+    `batch` >= 2 continuous batching of the ready requests into batched passes."""
 
-    def __init__(self, n_streams: int = 4, device_budget_bytes: int = 64 << 30, use_graphs: bool = True):
+    def __init__(self, n_streams: int = 4, device_budget_bytes: int = 64 << 30, use_graphs: bool = True,
+                 batch: int = 0):
         h = ctypes.c_void_p()
         check(lib().hmServerCreate(n_streams, device_budget_bytes, int(use_graphs), ctypes.byref(h)), "hmServerCreate")
         self._h = h
         self._keep = {}  # rig id -> pano (the caller's objects must outlive their requests)
+        if batch:
+            self.set_batching(batch)
+
+    def set_batching(self, max_frames: int) -> None:
+        """Continuous batching: ready requests of up to max_frames rigs run as one batched pass."""
+        check(lib().hmServerSetBatching(self._h, max_frames), "hmServerSetBatching")
 
     def close(self) -> None:
         h = getattr(self, "_h", None)
@@ -528,6 +545,22 @@ class Server:
     def submit(self, rig_id: int, inputs, canvas, tag: int = 0, fused: bool = True) -> int:
         arr = (hmSurface * len(inputs))(*[surface(t) for t in inputs])
         return lib().hmServerSubmit(self._h, rig_id, arr, len(inputs), surface(canvas), int(fused), tag)
+
+    def submit_many(self, requests) -> int:
+        """Queues (rig_id, inputs, canvas, tag[, fused]) requests at once, all or nothing."""
+        arr = (hmServerRequest * len(requests))()
+        keep = []
+        for q, req in zip(arr, requests):
+            rig_id, inputs, canvas, tag = req[:4]
+            ins = (hmSurface * len(inputs))(*[surface(t) for t in inputs])
+            keep.append(ins)
+            q.rig_id = rig_id
+            q.inputs = ctypes.cast(ins, ctypes.POINTER(hmSurface))
+            q.num_inputs = len(inputs)
+            q.canvas = surface(canvas)
+            q.fused = int(req[4]) if len(req) > 4 else 1
+            q.tag = tag
+        return lib().hmServerSubmitMany(self._h, arr, len(requests))
 
     def poll(self, max_results: int = 4096):
         buf = (hmServerResult * max_results)()

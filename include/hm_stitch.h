@@ -337,6 +337,10 @@ typedef struct hmServerStats {
   long graphs_captured;
   long graph_launches;
   long eager_launches;
+  long batched_passes;       /* batched passes started (hmServerSetBatching) */
+  long batched_frames;       /* requests those passes carried */
+  long pass_graphs_captured; /* passes captured as hipGraphs (a composition seen twice) */
+  long pass_graph_launches;  /* passes replayed from a captured graph */
 } hmServerStats;
 
 int hmServerCreate(int n_streams, size_t device_budget_bytes, int use_graphs, hmServer** out);
@@ -350,11 +354,35 @@ int hmServerRemoveRig(hmServer* server, int rig_id);
 /* Queues one frame; returns at once.  fused: the 3-image class's path. */
 int hmServerSubmit(hmServer* server, int rig_id, const hmSurface* inputs, int num_inputs, hmSurface canvas, int fused,
                    uint64_t tag);
+/* Queues several frames at once, in order (e.g. the frames of many rigs
+ * captured at one tick): the scheduler sees all of them together, so with
+ * batching on they can share a pass.  All or nothing: one invalid request
+ * (unknown rig, wrong input count) queues none. */
+typedef struct hmServerRequest {
+  int rig_id;
+  const hmSurface* inputs;
+  int num_inputs;
+  hmSurface canvas;
+  int fused;
+  uint64_t tag;
+} hmServerRequest;
+int hmServerSubmitMany(hmServer* server, const hmServerRequest* requests, int num_requests);
 /* Moves up to max_results completed requests out (oldest first). */
 int hmServerPoll(hmServer* server, hmServerResult* results, int max_results, int* num_results);
 /* Blocks until nothing is queued or running; returns the first error seen. */
 int hmServerWaitIdle(hmServer* server);
 int hmServerGetStats(hmServer* server, hmServerStats* stats);
+/* Continuous batching (no counterpart in the reference).  max_frames >= 2:
+ * when a stream frees up, the requests that are ready -- the oldest whose
+ * rig is idle, then up to max_frames - 1 more of other idle rigs, oldest
+ * first -- run as one batched pass (the persistent work-list kernels of
+ * hmBatch, over each request's recorded launches); a request ready alone
+ * takes its rig's graph.  A pass whose composition of (rig, buffer set)
+ * comes back is captured as a hipGraph on its second sighting and replayed.
+ * Rigs whose pipeline the batched kernels do not carry (float pipelines,
+ * debug mode) always run alone.  0 or 1: off (the default).  Output is
+ * bit-identical either way. */
+int hmServerSetBatching(hmServer* server, int max_frames);
 /* Drains the queue, then releases streams and graphs. */
 void hmServerDestroy(hmServer* server);
 

@@ -194,7 +194,8 @@ def test_cfg3_mixed_batch_full_size_every_frame_vs_oracle(hm, torch_gpu, oracle)
 # ---------------------------------------------------------------------------
 # the 32-rig served stream through hmServer
 # ---------------------------------------------------------------------------
-def test_cfg4_served_32_rigs_each_frame_equals_alone(hm, torch_gpu, oracle):
+@pytest.mark.parametrize("batch", [0, 8])
+def test_cfg4_served_32_rigs_each_frame_equals_alone(hm, torch_gpu, oracle, batch):
     import hm_synth
 
     torch = torch_gpu
@@ -237,7 +238,7 @@ def test_cfg4_served_32_rigs_each_frame_equals_alone(hm, torch_gpu, oracle):
         for c in canvases:
             c.fill_(3)
     torch.cuda.synchronize()
-    server = hm.Server(n_streams=4, device_budget_bytes=200 << 30, use_graphs=True)
+    server = hm.Server(n_streams=4, device_budget_bytes=200 << 30, use_graphs=True, batch=batch)
     ids = [server.add_rig(r[2]) for r in rigs]
     for tag, (ri, r) in enumerate(jobs):
         name, kind, pano, sets, canvases = rigs[ri]
@@ -249,6 +250,10 @@ def test_cfg4_served_32_rigs_each_frame_equals_alone(hm, torch_gpu, oracle):
     for ri, (name, kind, pano, sets, canvases) in enumerate(rigs):
         for r in range(2):
             assert_same(canvases[r].cpu().numpy(), alone[name, r % len(sets)], f"rig {ri} ({name}) frame {r}")
+    st = server.stats()
+    print(f"served stats: {st}", flush=True)
+    if batch:
+        assert st["batched_passes"] > 0
     server.close()
     del rigs, server
     release(torch)

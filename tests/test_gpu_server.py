@@ -83,6 +83,80 @@ def test_server_mixed_rigs_match_alone(hm, torch_gpu, oracle, use_graphs):
     server.close()
 
 
+@pytest.mark.parametrize("use_graphs", [True, False])
+@pytest.mark.parametrize("max_batch", [2, 8])
+def test_server_batched_passes_match_alone(hm, torch_gpu, oracle, use_graphs, max_batch):
+    """Continuous batching: requests of every class queued together run as batched passes
+    of up to max_batch rigs (replayed as graphs when a composition comes back); a float
+    pipeline rig, which the batched kernels do not carry, runs alone in the same stream of
+    requests.  Every frame equals the oracle's, rounds after rounds."""
+    import hm_synth
+
+    torch = torch_gpu
+    rigs = make_rigs(hm)
+    rf = hm_synth.small_rig(seed=17)
+    fpano = hm.StitchPano(1, 3, rf, float_pipeline=True)
+    server = hm.Server(n_streams=2, use_graphs=use_graphs, batch=max_batch)
+    ids = [server.add_rig(p) for (_, p, *_rest) in rigs]
+    fid = server.add_rig(fpano)
+    inputs = {}
+    for ri, (_, _, rig, *_r) in enumerate(rigs):
+        for s in range(2):
+            fr = frames_for(rig, 1, 300 + 100 * ri + s)
+            inputs[ri, s] = (fr, to_dev(torch, fr))
+    ffr = [f.astype(np.float32) + 0.25 for f in frames_for(rf, 1, 77)]
+    fdev = to_dev(torch, ffr)
+    fcanvas = torch.zeros((1, fpano.canvas_size()[1], fpano.canvas_size()[0], 4), dtype=torch.float32,
+                          device="cuda")
+    canvases = {}
+    for ri, (_, pano, *_r) in enumerate(rigs):
+        w, h = pano.canvas_size()
+        for s in range(2):
+            canvases[ri, s] = torch.zeros((1, h, w, 4), dtype=torch.uint8, device="cuda")
+    rounds = 6
+    tag = 0
+    for rnd in range(rounds):
+        s = rnd % 2
+        priors = {}
+        for ri in range(len(rigs)):
+            priors[ri] = np.full(canvases[ri, s].shape, 9 + rnd, np.uint8)
+            canvases[ri, s].copy_(torch.from_numpy(priors[ri]).cuda())
+        torch.cuda.synchronize()
+        jobs = {}
+        reqs = []  # one tick of every rig, queued at once so that passes form
+        for ri, (name, pano, rig, levels, kind, fused) in enumerate(rigs):
+            reqs.append((ids[ri], inputs[ri, s][1], canvases[ri, s], tag, fused))
+            jobs[tag] = ri
+            tag += 1
+        reqs.insert(3, (fid, fdev, fcanvas, tag))
+        jobs[tag] = -1
+        tag += 1
+        assert server.submit_many(reqs) == 0
+        assert server.wait_idle() == 0
+        res = server.poll()
+        assert sorted(r["tag"] for r in res) == sorted(jobs)
+        assert all(r["status"] == 0 for r in res)
+        for ri, (name, pano, rig, levels, kind, fused) in enumerate(rigs):
+            want = oracle_out(oracle, kind, rig, inputs[ri, s][0], levels, fused, priors[ri])
+            np.testing.assert_array_equal(canvases[ri, s].cpu().numpy(), want, err_msg=f"{name} round {rnd}")
+        np.testing.assert_array_equal(fcanvas.cpu().numpy(), oracle.pano2(rf, ffr[0], ffr[1], 3),
+                                      err_msg=f"float rig round {rnd}")
+    st = server.stats()
+    print(st)
+    # every tick: the six carried rigs in passes of up to max_batch, the float rig alone; the
+    # compositions of rounds 0/2/4 and 1/3/5 repeat, so with graphs rounds 4 and 5 replay them
+    per_tick = -(-len(rigs) // max_batch)
+    assert st["batched_passes"] == rounds * per_tick and st["batched_frames"] == rounds * len(rigs)
+    if use_graphs:
+        assert st["pass_graphs_captured"] == 2 * per_tick and st["pass_graph_launches"] == 4 * per_tick
+    else:
+        assert st["pass_graph_launches"] == 0
+    bad = (ids[0], inputs[0, 0][1], canvases[0, 0], 999)
+    assert server.submit_many([bad, (ids[0] + 100, inputs[0, 0][1], canvases[0, 0], 1000)]) != 0
+    assert server.stats()["queued"] == 0  # all or nothing
+    server.close()
+
+
 def test_server_graph_replays_with_eager_work_between(hm, torch_gpu, oracle):
     """One buffer set, replayed several times with new input content written in between."""
     import hm_synth
