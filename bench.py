@@ -265,8 +265,10 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
     # continuous batching, the ready requests of up to K rigs as one batched pass.
     modes = {}
     equal = True
-    for mode, max_batch in (("graphs", 0), ("batched8", 8), ("batched32", 32)):
-        server.set_batching(max_batch)
+    mode_set = {"graphs": (0, False), "batched8": (8, False), "batched32": (32, False),
+                "batched8_same": (8, True), "batched32_same": (32, True)}
+    for mode, (max_batch, same) in mode_set.items():
+        server.set_batching(max_batch, same)
         for i, (ri, k) in enumerate(order[: len(rigs) * 4]):
             hm_stitch.check(submit(ri, k, i), "submit")
         hm_stitch.check(server.wait_idle(), "served warm-up")
@@ -292,7 +294,7 @@ def served_benchmark(torch, hm_stitch, hm_synth, seconds, n_streams, frames_clos
         log(f"served load: closed loop {mode}: {modes[mode]}")
     best = max(modes, key=lambda m: modes[m]["fps"])
     t_served = frames_closed / modes[best]["fps"]
-    server.set_batching({"graphs": 0, "batched8": 8, "batched32": 32}[best])
+    server.set_batching(*mode_set[best])
 
     # open loop: Poisson arrivals, 30 fps per rig
     arrivals = []

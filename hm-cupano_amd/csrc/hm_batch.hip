@@ -42,8 +42,8 @@ thread_local LaunchRecorder* g_recorder = nullptr;
 hipError_t record_process(hmStitchPano* p, const hmSurface* inputs, int num_inputs, const hmSurface& canvas,
                           int fused, Recording& out) {
   // Launch shapes the batched kernels carry: remap tiles of 4 pixels x 1 or 2
-  // rows, no fused remap+down, levels 1 and 0 of the up pass in two launches,
-  // one workgroup per canvas tile.  The output is the same for every tuning.
+  // rows, no fused remap+down, one workgroup per canvas tile.  The output is
+  // the same for every tuning.
   if (p->debug || p->pipeline_f32) return hipErrorNotSupported;  // launches the batched kernels do not carry
   const hmTuning saved = p->tuning;
   const int status = p->status;  // a refused recording must not become the object's sticky error
@@ -51,7 +51,6 @@ hipError_t record_process(hmStitchPano* p, const hmSurface* inputs, int num_inpu
   p->tuning.remap_px = 4;
   if (p->tuning.remap_rows != 2) p->tuning.remap_rows = 1;
   p->tuning.fuse_down = 0;
-  p->tuning.fuse_up = 0;
   p->tuning.canvas_grid = 0;
   p->apply_tuning();
   p->timer.enable(false);

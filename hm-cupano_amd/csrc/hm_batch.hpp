@@ -36,6 +36,8 @@ enum BatchFamily : int {
   kFamUqN,       // up_quad_body<T0, C, N, kBlendN, M, NEXT>; A as for kFamUq3
   kFamMi,        // multi_inputs_body<C, N, DM>; A: DM
   kFamMc,        // multi_canvas_body<C, N>
+  kFamTail2L,    // up_tail_lds_body<C, 2, kBlend2, kTailLdsTile> (the tail in one load burst)
+  kFamL10,       // up_l10_body<T0, C, M, NEXT>: levels 1 and 0 of the 2-image up pass; A as for kFamUq2
 };
 constexpr int kOpNone = -1;
 constexpr int batch_op(int fam, int c, int a, int next, int n) {
@@ -64,8 +66,10 @@ constexpr void for_each_batch_op(F&& f) {
     for (int a = 0; a <= 1; ++a)
       for (int n = 2; n <= 8; ++n) f(batch_op(kFamDown, c, a, 0, n));
     f(batch_op(kFamTail2, c, 0, 0, 2));
+    f(batch_op(kFamTail2L, c, 0, 0, 2));
     for (int a = 0; a <= 1; ++a)
       for (int next = 0; next <= 1; ++next) {
+        f(batch_op(kFamL10, c, a, next, 2));
         f(batch_op(kFamUq2, c, a, next, 2));
         f(batch_op(kFamUq3, c, a, next, 3));
         for (int n = 2; n <= 8; ++n) f(batch_op(kFamUqN, c, a, next, n));
@@ -97,6 +101,16 @@ constexpr int op_down() {
 template <int C, int N, int KIND>
 constexpr int op_up_tail() {
   return (batch_channels(C) && N == 2 && KIND == hmdev::kBlend2) ? batch_op(kFamTail2, C, 0, 0, 2) : kOpNone;
+}
+template <int C, int N, int KIND>
+constexpr int op_up_tail_lds() {
+  return (batch_channels(C) && N == 2 && KIND == hmdev::kBlend2) ? batch_op(kFamTail2L, C, 0, 0, 2) : kOpNone;
+}
+template <typename T0, int C, int MODE, bool NEXT>
+constexpr int op_up_l10() {
+  constexpr bool u8 = std::is_same<T0, uint8_t>::value;
+  // MODE values: 0 float mask, 1 uint8 mask (hmdev::Mask0Mode)
+  return (batch_channels(C) && (u8 ? MODE == 1 : MODE == 0)) ? batch_op(kFamL10, C, u8, NEXT, 2) : kOpNone;
 }
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT>
 constexpr int op_up_quad() {

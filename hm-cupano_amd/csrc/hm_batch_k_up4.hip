@@ -1,12 +1,12 @@
 // hm_batch_k_up4.hip -- batched-pass kernels of the families
-// kFamUq2, kFamUq3 and kFamUqN with 4 channels: the up-pass levels.
+// kFamUq2, kFamUq3, kFamUqN and kFamL10 with 4 channels: the up-pass levels.
 #include "hm_batch_ops.hpp"
 
 namespace {
 
 constexpr bool in_part(int op) {
   const int f = hm::op_fam(op);
-  return (f == hm::kFamUq2 || f == hm::kFamUq3 || f == hm::kFamUqN) && hm::op_c(op) == 4;
+  return (f == hm::kFamUq2 || f == hm::kFamUq3 || f == hm::kFamUqN || f == hm::kFamL10) && hm::op_c(op) == 4;
 }
 constexpr int kCount = hmdev::count_ops_where(in_part);
 constexpr std::array<int, kCount> kOps = hmdev::ops_where<kCount>(in_part);

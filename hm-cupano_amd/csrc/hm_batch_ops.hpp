@@ -44,8 +44,8 @@ typedef void (*StageKernel)(StageArgs);
 // The kernel of `op` from the translation unit that compiles its family, or
 // nullptr (not carried).
 StageKernel batch_kernel_remap(int op);  // kFamP2L0, kFamP2Hard, kFamMi, kFamMc
-StageKernel batch_kernel_down(int op);   // kFamDown, kFamTail2
-StageKernel batch_kernel_up3(int op);    // kFamUq2, kFamUq3, kFamUqN with 3 channels
+StageKernel batch_kernel_down(int op);   // kFamDown, kFamTail2, kFamTail2L
+StageKernel batch_kernel_up3(int op);    // kFamUq2, kFamUq3, kFamUqN, kFamL10 with 3 channels
 StageKernel batch_kernel_up4(int op);    // the same with 4 channels
 inline StageKernel batch_kernel(int op) {
   switch (op_fam(op)) {
@@ -56,10 +56,12 @@ inline StageKernel batch_kernel(int op) {
       return batch_kernel_remap(op);
     case kFamDown:
     case kFamTail2:
+    case kFamTail2L:
       return batch_kernel_down(op);
     case kFamUq2:
     case kFamUq3:
     case kFamUqN:
+    case kFamL10:
       return op_c(op) == 4 ? batch_kernel_up4(op) : batch_kernel_up3(op);
     default:
       return nullptr;
@@ -94,6 +96,10 @@ struct OpArgsOf<OP, hm::kFamDown> {
   using type = DownFusedArgs<std::conditional_t<hm::op_a(OP) == 1, uint8_t, float>, hm::op_n(OP)>;
 };
 template <int OP> struct OpArgsOf<OP, hm::kFamTail2> { using type = UpTailArgs<2>; };
+template <int OP> struct OpArgsOf<OP, hm::kFamTail2L> { using type = UpTailArgs<2>; };
+template <int OP> struct OpArgsOf<OP, hm::kFamL10> {
+  using type = UpL10Args<std::conditional_t<hm::op_a(OP) == 1, uint8_t, float>>;
+};
 template <int OP> struct OpArgsOf<OP, hm::kFamUq2> {
   using type = UpQuadArgs<std::conditional_t<hm::op_a(OP) == 1, uint8_t, float>, 2>;
 };
@@ -110,6 +116,8 @@ constexpr int op_lds_float4() {
   constexpr int fam = hm::op_fam(OP), n = hm::op_n(OP);
   if constexpr (fam == hm::kFamDown) return down_lds_float4<down_slots_count<n>()>();
   if constexpr (fam == hm::kFamTail2) return kTailLdsFloat4;
+  if constexpr (fam == hm::kFamTail2L) return up_tail_lds_float4<2, kBlend2, kTailLdsTile>();
+  if constexpr (fam == hm::kFamL10) return up_l10_lds_float4();
   if constexpr (fam == hm::kFamMc) return kCanvasGeoFloat4;
   if constexpr (fam == hm::kFamUq2 || fam == hm::kFamUq3 || fam == hm::kFamUqN) return quad_lds_float4<n>();
   return 1;
@@ -138,6 +146,11 @@ __device__ __forceinline__ void op_body(kptr<hm::StageArgs> s, uint32_t off, Blk
       down_fused_slots_body<T0, C, N>(a, blk, lds);
   } else if constexpr (fam == hm::kFamTail2) {
     up_tail_body<C, 2, kBlend2>(a, blk, lds);
+  } else if constexpr (fam == hm::kFamTail2L) {
+    up_tail_lds_body<C, 2, kBlend2, kTailLdsTile>(a, blk, lds);
+  } else if constexpr (fam == hm::kFamL10) {
+    using T0 = std::conditional_t<A, uint8_t, float>;
+    up_l10_body<T0, C, A ? kMask0U8 : kMask0F32, NEXT>(a, blk, lds);
   } else if constexpr (fam == hm::kFamUq2) {
     using T0 = std::conditional_t<A, uint8_t, float>;
     up_quad_body<T0, C, 2, kBlend2, A ? kMask0U8 : kMask0F32, NEXT>(a, blk, lds);

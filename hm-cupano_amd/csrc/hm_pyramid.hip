@@ -285,12 +285,17 @@ hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int bat
   a.wy0 = u.wy0;
   a.wx1 = u.wx1;
   a.wy1 = u.wy1;
-  if (u.n - 1 <= kTailLds && !g_recorder) {  // (a batched pass carries the other form)
+  if (u.n - 1 <= kTailLds) {
     // every level's footprint in LDS, loaded in one burst
     constexpr int T = kTailLdsTile;
     a.tiles_x = (u.wx1 - u.wx0 + T) / T;
     a.tiles_y = (u.wy1 - u.wy0 + T) / T;
     const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+    if (g_recorder) {
+      const int op =
+          kind != kBlend2 ? kOpNone : (c == 4 ? op_up_tail_lds<4, 2, kBlend2>() : op_up_tail_lds<3, 2, kBlend2>());
+      return g_recorder->add(op, grid, dim3(256), s, a, "up_tail_lds");
+    }
     if (c == 4) {
       if (kind == kBlend2)
         hipLaunchKernelGGL((up_tail_lds_kernel<4, 2, kBlend2, T>), grid, dim3(256), 0, s, a);
@@ -400,7 +405,6 @@ hipError_t launch_up_l10(int c, bool g0_u8, const UpQuadLaunch& u, const float* 
                          const float* r2, int w2, int h2, int batch, hipStream_t s) {
   if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
   if (u.mask0_mode != kMask0F32 && u.mask0_mode != kMask0U8) return hipErrorInvalidValue;
-  if (g_recorder) return hipErrorNotSupported;  // the batched pass does not carry this launch
   auto go = [&](auto T0v, auto Cv, auto M) -> hipError_t {
     using T0 = decltype(T0v);
     constexpr int kC = decltype(Cv)::value, kM = decltype(M)::value;
@@ -442,6 +446,10 @@ hipError_t launch_up_l10(int c, bool g0_u8, const UpQuadLaunch& u, const float* 
     A.w2 = w2;
     A.h2 = h2;
     const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+    if (g_recorder) {
+      const int op = r2 ? op_up_l10<T0, kC, kM, true>() : op_up_l10<T0, kC, kM, false>();
+      return g_recorder->add(op, grid, dim3(64, 4, 1), s, A, "up_l10");
+    }
     if (r2)
       hipLaunchKernelGGL((up_l10_kernel<T0, kC, kM, true>), grid, dim3(64, 4, 1), 0, s, A);
     else

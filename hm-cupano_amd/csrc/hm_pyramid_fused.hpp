@@ -625,22 +625,32 @@ __global__ void __launch_bounds__(256) up_tail_kernel(UpTailArgs<N> a) {
 constexpr int kTailLds = 4;
 constexpr int kTailLdsTile = 16;
 
+// LDS of the burst tail, in float4: the footprints of every level (Gaussians,
+// then the mask), R of two levels, and the footprint bounds.
+template <int N, int KIND, int T>
+__host__ __device__ constexpr int up_tail_lds_float4() {
+  constexpr int M = KIND == kBlend2 ? 1 : N;
+  constexpr int RU = T / 2 + 2, CAP = T * T + kTailLds * RU * RU;
+  return N * CAP + 2 * RU * RU + (M * CAP + 3) / 4 + ((kTailLds + 1) * 4 + kTailLds + 2 + 3) / 4;
+}
+
+// Runs on 256 threads; lds: up_tail_lds_float4<N, KIND, T>() float4.
 template <int C, int N, int KIND, int T>
-__global__ void __launch_bounds__(256) up_tail_lds_kernel(UpTailArgs<N> a) {
+__device__ __forceinline__ void up_tail_lds_body(const UpTailArgs<N>& a, Blk blk, float4* lds) {
   constexpr int M = KIND == kBlend2 ? 1 : N;
   constexpr int RU = T / 2 + 2;                      // footprint edge above the base
   constexpr int CAP = T * T + kTailLds * RU * RU;    // pixels of all footprints
   constexpr int SB = (T * T + 255) / 256;            // base-level pixels per thread
   constexpr int SU = (RU * RU + 255) / 256;          // pixels per thread at each level above
-  __shared__ float4 gl[N * CAP];                     // [level][image][footprint pixel]
-  __shared__ float ml[M * CAP];                      // [level][footprint pixel][M]
-  __shared__ float4 rb[2][RU * RU];
-  __shared__ int reg[kTailLds + 1][4];               // footprint [x0, x1] x [y0, y1]
-  __shared__ int cum[kTailLds + 2];                  // pixels of the footprints before level j
-  const int t = (int)blockIdx.x;
+  float4* gl = lds;                                  // [level][image][footprint pixel]
+  float4(*rb)[RU * RU] = (float4(*)[RU * RU])(lds + N * CAP);
+  float* ml = (float*)(lds + N * CAP + 2 * RU * RU);  // [level][footprint pixel][M]
+  int(*reg)[4] = (int(*)[4])(ml + ((M * CAP + 3) & ~3));  // footprint [x0, x1] x [y0, y1]
+  int* cum = &reg[kTailLds + 1][0];                  // pixels of the footprints before level j
+  const int t = blk.x;
   const int tx = t % a.tiles_x, ty = t / a.tiles_x;
-  const int b = blockIdx.z;
-  const int tid = threadIdx.x;
+  const int b = blk.z;
+  const int tid = flat_tid();
   const int n = a.n;
   if (tid == 0) {
     int x0 = a.wx0 + tx * T, y0 = a.wy0 + ty * T;
@@ -708,7 +718,7 @@ __global__ void __launch_bounds__(256) up_tail_lds_kernel(UpTailArgs<N> a) {
     const int j = n - 1;
     const int rw = reg[j][1] - reg[j][0] + 1, area = cum[j + 1] - cum[j];
     (void)rw;
-    for (int idx = tid; idx < area; idx += blockDim.x) {
+    for (int idx = tid; idx < area; idx += kWgThreads) {
       float4 Lv[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) Lv[i] = gpx(j, i, idx);
@@ -728,7 +738,7 @@ __global__ void __launch_bounds__(256) up_tail_lds_kernel(UpTailArgs<N> a) {
     const int nx0 = reg[j + 1][0], ny0 = reg[j + 1][2], nrw = reg[j + 1][1] - nx0 + 1;
     const float4* prev = rb[cur ^ 1];
     const float* mk = ml + cum[j] * M;
-    for (int q = tid; q < nqx * nqy; q += blockDim.x) {
+    for (int q = tid; q < nqx * nqy; q += kWgThreads) {
       const int qx = qx0 + q % nqx, qy = qy0 + q / nqx;
       const int qx1 = min(qx + 1, dn.w - 1), qy1 = min(qy + 1, dn.h - 1);
       const int lx = qx - nx0, lx1 = qx1 - nx0, ly = qy - ny0, ly1 = qy1 - ny0;
@@ -767,6 +777,13 @@ __global__ void __launch_bounds__(256) up_tail_lds_kernel(UpTailArgs<N> a) {
       }
     }
   }
+}
+
+template <int C, int N, int KIND, int T>
+__global__ void __launch_bounds__(256) up_tail_lds_kernel(UpTailArgs<N> a) {
+  __shared__ float4 lds[up_tail_lds_float4<N, KIND, T>()];
+  (void)a;
+  up_tail_lds_body<C, N, KIND, T>(kernel_arg<UpTailArgs<N>>(), hw_blk(), lds);
 }
 
 // ---------------------------------------------------------------------------

@@ -79,9 +79,25 @@ constexpr size_t kMaxPassSightings = 4096;
 
 // One rig's recorded launches for one buffer set (a batched pass's input).
 struct Rec {
-  uint64_t id = 0;  // unique over the server's life: names the recording in pass keys
+  uint64_t id = 0;   // unique over the server's life: names the recording in pass keys
+  uint64_t sig = 0;  // the pipeline's structure: its bodies in launch order and their streams
   hm::Recording launches;
 };
+
+uint64_t signature_of(const hm::Recording& launches) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over (op, side stream, fork point)
+  auto mix = [&](uint64_t v) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= (v >> (8 * i)) & 0xff;
+      h *= 1099511628211ull;
+    }
+  };
+  for (const auto& l : launches) {
+    mix((uint64_t)(uint32_t)l.op);
+    mix(l.stream == nullptr ? 0u : 1u + (uint64_t)l.main_before);
+  }
+  return h;
+}
 
 struct Rig {
   hmStitchPano* pano = nullptr;
@@ -121,7 +137,8 @@ struct hmServer {
   std::vector<hmServerResult> done;
   size_t budget = 0, used = 0;
   bool use_graphs = true;
-  int max_batch = 0;  // frames per batched pass (below 2: no batching)
+  int max_batch = 0;     // frames per batched pass (below 2: no batching)
+  int batch_policy = 0;  // 0: any ready requests; 1: only requests whose pipelines have one structure
   bool stop = false;
   int in_flight = 0;
   long graphs_captured = 0, graph_launches = 0, eager_launches = 0;
@@ -205,6 +222,7 @@ Rec* hmServer::recording_of(Rig& rig, const Request& r) {
     return nullptr;
   }
   rec->id = next_rec++;
+  rec->sig = signature_of(rec->launches);
   if (rig.recs.size() >= kMaxGraphsPerRig) {
     rig.recs.erase(rig.rec_lru.front());
     rig.rec_lru.pop_front();
@@ -319,7 +337,7 @@ void hmServer::run() {
             for (const auto& p : pick) dup = dup || p->rig == qj->rig;
             if (dup) continue;
             Rec* rj = recording_of(rigs[qj->rig], *qj);
-            if (!rj) continue;
+            if (!rj || (batch_policy == 1 && rj->sig != r0->sig)) continue;
             pick.push_back(qj);
             recs.push_back(rj);
           }
@@ -490,10 +508,11 @@ extern "C" int hmServerGetStats(hmServer* sv, hmServerStats* st) {
   return hipSuccess;
 }
 
-extern "C" int hmServerSetBatching(hmServer* sv, int max_frames) {
-  if (!sv || max_frames < 0) return hipErrorInvalidValue;
+extern "C" int hmServerSetBatching(hmServer* sv, int max_frames, int policy) {
+  if (!sv || max_frames < 0 || policy < 0 || policy > 1) return hipErrorInvalidValue;
   std::lock_guard<std::mutex> lk(sv->mu);
   sv->max_batch = max_frames;
+  sv->batch_policy = policy;
   return hipSuccess;
 }
 

@@ -196,7 +196,7 @@ def lib() -> ctypes.CDLL:
         "hmServerWaitIdle": (i, [vp]),
         "hmServerGetStats": (i, [vp, ctypes.POINTER(hmServerStats)]),
         "hmServerDestroy": (None, [vp]),
-        "hmServerSetBatching": (i, [vp, i]),
+        "hmServerSetBatching": (i, [vp, i, i]),
         "hmServerSubmitMany": (i, [vp, ctypes.POINTER(hmServerRequest), i]),
         "hmBatchCreate": (i, [ctypes.POINTER(hmBatchCall), i, ctypes.POINTER(vp)]),
         "hmBatchSetGrid": (i, [vp, i]),
@@ -509,17 +509,18 @@ class Server:
     `batch` >= 2 continuous batching of the ready requests into batched passes."""
 
     def __init__(self, n_streams: int = 4, device_budget_bytes: int = 64 << 30, use_graphs: bool = True,
-                 batch: int = 0):
+                 batch: int = 0, same_structure: bool = False):
         h = ctypes.c_void_p()
         check(lib().hmServerCreate(n_streams, device_budget_bytes, int(use_graphs), ctypes.byref(h)), "hmServerCreate")
         self._h = h
         self._keep = {}  # rig id -> pano (the caller's objects must outlive their requests)
         if batch:
-            self.set_batching(batch)
+            self.set_batching(batch, same_structure)
 
-    def set_batching(self, max_frames: int) -> None:
-        """Continuous batching: ready requests of up to max_frames rigs run as one batched pass."""
-        check(lib().hmServerSetBatching(self._h, max_frames), "hmServerSetBatching")
+    def set_batching(self, max_frames: int, same_structure: bool = False) -> None:
+        """Continuous batching: ready requests of up to max_frames rigs run as one batched pass
+        (same_structure: only rigs whose pipelines have the same kernel sequence share one)."""
+        check(lib().hmServerSetBatching(self._h, max_frames, int(same_structure)), "hmServerSetBatching")
 
     def close(self) -> None:
         h = getattr(self, "_h", None)

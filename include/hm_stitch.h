@@ -380,9 +380,13 @@ int hmServerGetStats(hmServer* server, hmServerStats* stats);
  * takes its rig's graph.  A pass whose composition of (rig, buffer set)
  * comes back is captured as a hipGraph on its second sighting and replayed.
  * Rigs whose pipeline the batched kernels do not carry (float pipelines,
- * debug mode) always run alone.  0 or 1: off (the default).  Output is
- * bit-identical either way. */
-int hmServerSetBatching(hmServer* server, int max_frames);
+ * debug mode) always run alone.  0 or 1: off (the default).  policy 0: any
+ * ready requests share a pass (every class, canvas size, image and level
+ * count); policy 1: only requests whose pipelines have the same structure
+ * (class, image and level count, channels: the same kernel bodies in the
+ * same order, canvas sizes may differ), so no frame's stage waits for a
+ * longer pipeline's.  Output is bit-identical either way. */
+int hmServerSetBatching(hmServer* server, int max_frames, int policy);
 /* Drains the queue, then releases streams and graphs. */
 void hmServerDestroy(hmServer* server);
 

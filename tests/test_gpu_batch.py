@@ -128,6 +128,33 @@ def test_batch_three_channels_and_up_to_eight_images(hm, torch_gpu, oracle, use_
     check(oracle, calls, bufs, f"graph={use_graph}")
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_two_image_up_pass_shapes(hm, torch_gpu, oracle, use_graph):
+    """2-image rigs under every up-pass shape the pass carries: the burst tail (base 1..3)
+    and the per-level tail form (more levels than the burst holds), with levels 1 and 0 in
+    one launch or two, 3 and 4 channels, levels 2..8."""
+    import hm_synth
+
+    torch = torch_gpu
+    cases = [  # (levels, channels, tail_base, fuse_up)
+        (6, 4, 3, 1), (6, 4, 2, 0), (6, 3, 1, 1), (4, 4, 2, 1), (4, 3, 0, 0), (2, 4, 2, 1), (8, 4, 1, 1),
+        (8, 3, 2, 0),
+    ]
+    calls = []
+    for k, (levels, ch, tail, fu) in enumerate(cases):
+        r = hm_synth.small_rig(seed=80 + k, src=(200, 96), canvas=(520, 112))
+        p = hm.StitchPano(1, levels, r, channels=ch)
+        p.set_tuning(tail_base=tail, fuse_up=fu)
+        calls.append((f"pano2_L{levels}_c{ch}_tail{tail}_fu{fu}", p, r, levels, "2", True, 1, ch))
+    batch, bufs = build(hm, torch, calls, 900)
+    assert batch.run(use_graph=use_graph) == 0
+    torch.cuda.synchronize()
+    check(oracle, calls, bufs, f"graph={use_graph}")
+    for (levels, ch, tail, fu), (name, p, *_r) in zip(cases, calls):  # the recording leaves the tuning alone
+        t = p.get_tuning()
+        assert (t["tail_base"], t["fuse_up"]) == (tail, fu), name
+
+
 def test_batch_more_than_four_images_per_tile(hm, torch_gpu, oracle):
     """Six images over the whole canvas (identity maps, all at (0, 0)): every tile
     meets all six, so the batched down pass runs several slot chunks per tile and

@@ -157,6 +157,45 @@ def test_server_batched_passes_match_alone(hm, torch_gpu, oracle, use_graphs, ma
     server.close()
 
 
+def test_server_batching_same_structure(hm, torch_gpu, oracle):
+    """Policy 'same structure': only rigs whose pipelines launch the same kernel sequence share
+    a pass (here two 2-image L4 rigs of different canvas sizes, and two 3-image N-class L3
+    rigs); a rig of another structure (2-image L0) runs alone."""
+    import hm_synth
+
+    torch = torch_gpu
+    specs = [
+        ("2", 4, hm_synth.small_rig(seed=51)),
+        ("N", 3, hm_synth.small_rig(n=3, seed=52, src=(120, 56), canvas=(330, 64))),
+        ("2", 0, hm_synth.small_rig(seed=53)),
+        ("2", 4, hm_synth.small_rig(seed=54, src=(170, 72), canvas=(440, 80))),
+        ("N", 3, hm_synth.small_rig(n=3, seed=55, src=(100, 48), canvas=(280, 56))),
+    ]
+    rigs = []
+    for kind, levels, rig in specs:
+        pano = hm.StitchPano(1, levels, rig) if kind == "2" else hm.StitchPanoN(1, levels, rig)
+        assert pano.status == 0
+        rigs.append((kind, levels, rig, pano))
+    assert rigs[0][3].canvas_size() != rigs[3][3].canvas_size()
+    server = hm.Server(n_streams=2, use_graphs=True, batch=8, same_structure=True)
+    ids = [server.add_rig(p) for (*_r, p) in rigs]
+    for rnd in range(3):
+        frames = [frames_for(rig, 1, 600 + 10 * ri + rnd) for ri, (_, _, rig, _) in enumerate(rigs)]
+        devs = [to_dev(torch, f) for f in frames]
+        canvases = [p.alloc_canvas() for (*_r, p) in rigs]
+        torch.cuda.synchronize()
+        assert server.submit_many([(ids[i], devs[i], canvases[i], i) for i in range(len(rigs))]) == 0
+        assert server.wait_idle() == 0
+        assert all(r["status"] == 0 for r in server.poll())
+        for i, (kind, levels, rig, _) in enumerate(rigs):
+            want = oracle.pano2(rig, frames[i][0], frames[i][1], levels) if kind == "2" else \
+                oracle.panoN(rig, frames[i], levels)
+            np.testing.assert_array_equal(canvases[i].cpu().numpy(), want, err_msg=f"rig {i} round {rnd}")
+    st = server.stats()
+    assert st["batched_passes"] == 2 * 3 and st["batched_frames"] == 4 * 3
+    server.close()
+
+
 def test_server_graph_replays_with_eager_work_between(hm, torch_gpu, oracle):
     """One buffer set, replayed several times with new input content written in between."""
     import hm_synth
