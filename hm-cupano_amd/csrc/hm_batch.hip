@@ -170,8 +170,7 @@ static hipError_t launch_lane(const PassPlan& plan, PassStreams& ps, int grid, h
   HM_RETURN_IF_ERROR(hipStreamWaitEvent(ps.lane, ps.ev_lane_fork, 0));
   for (const auto& l : plan.free_launches) {
     if (l.stage != k) continue;
-    hipLaunchKernelGGL(batch_kernel(l.op), dim3((unsigned)pass_grid_for(l, grid)), dim3(64, 4, 1), 0, ps.lane,
-                       l.args);
+    hipLaunchKernelGGL(pass_kernel(l, grid), dim3((unsigned)pass_grid_for(l, grid)), dim3(64, 4, 1), 0, ps.lane, l.args);
     HM_RETURN_IF_ERROR(hipGetLastError());
   }
   *forked = true;
@@ -201,7 +200,7 @@ hipError_t run_pass(const PassPlan& plan, PassStreams& ps, int grid, hipStream_t
       const int q = (int)(j - i);  // 0: the caller's stream
       hipStream_t st = q == 0 ? stream : ps.side[(q - 1) % kLaneStreams];
       const PassLaunch& l = plan.launches[j];
-      hipLaunchKernelGGL(batch_kernel(l.op), dim3((unsigned)pass_grid_for(l, grid)), dim3(64, 4, 1), 0, st, l.args);
+      hipLaunchKernelGGL(pass_kernel(l, grid), dim3((unsigned)pass_grid_for(l, grid)), dim3(64, 4, 1), 0, st, l.args);
       HM_RETURN_IF_ERROR(hipGetLastError());
     }
     for (int q = 0; q < n_side; ++q) {

@@ -13,4 +13,4 @@ constexpr std::array<int, kCount> kOps = hmdev::ops_where<kCount>(in_part);
 
 }  // namespace
 
-hm::StageKernel hm::batch_kernel_remap(int op) { return hmdev::find_op_kernel<kOps>(op); }
+hm::StageKernel hm::batch_kernel_remap(int op, bool persist) { return hmdev::find_op_kernel<kOps>(op, persist); }

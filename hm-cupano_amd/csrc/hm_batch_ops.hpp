@@ -1,7 +1,7 @@
 // hm_batch_ops.hpp -- the persistent work-list kernel of the batched pass.
 //
-// batch_op_kernel<OP> runs one kernel body (the op, see hm_batch.hpp) over a
-// stage's work-list: every tile (x) of every batch entry (z) of every
+// batch_op_kernel<OP, PERSIST> runs one kernel body (the op, see hm_batch.hpp)
+// over a stage's work-list: every tile (x) of every batch entry (z) of every
 // recorded launch of that body, walked by the workgroups with a stride of the
 // grid size.  The items and their argument blocks travel in the kernel
 // arguments.  The kernels are compiled in several translation units, one per
@@ -42,27 +42,28 @@ static_assert(sizeof(StageArgs) <= 4000, "stage arguments must fit the kernel-ar
 typedef void (*StageKernel)(StageArgs);
 
 // The kernel of `op` from the translation unit that compiles its family, or
-// nullptr (not carried).
-StageKernel batch_kernel_remap(int op);  // kFamP2L0, kFamP2Hard, kFamMi, kFamMc
-StageKernel batch_kernel_down(int op);   // kFamDown, kFamTail2, kFamTail2L
-StageKernel batch_kernel_up3(int op);    // kFamUq2, kFamUq3, kFamUqN, kFamL10 with 3 channels
-StageKernel batch_kernel_up4(int op);    // the same with 4 channels
-inline StageKernel batch_kernel(int op) {
+// nullptr (not carried); persist: the form whose workgroups walk the list
+// (a grid smaller than the work units), else one workgroup per unit.
+StageKernel batch_kernel_remap(int op, bool persist);  // kFamP2L0, kFamP2Hard, kFamMi, kFamMc
+StageKernel batch_kernel_down(int op, bool persist);   // kFamDown, kFamTail2, kFamTail2L
+StageKernel batch_kernel_up3(int op, bool persist);    // kFamUq2, kFamUq3, kFamUqN, kFamL10 with 3 channels
+StageKernel batch_kernel_up4(int op, bool persist);    // the same with 4 channels
+inline StageKernel batch_kernel(int op, bool persist) {
   switch (op_fam(op)) {
     case kFamP2L0:
     case kFamP2Hard:
     case kFamMi:
     case kFamMc:
-      return batch_kernel_remap(op);
+      return batch_kernel_remap(op, persist);
     case kFamDown:
     case kFamTail2:
     case kFamTail2L:
-      return batch_kernel_down(op);
+      return batch_kernel_down(op, persist);
     case kFamUq2:
     case kFamUq3:
     case kFamUqN:
     case kFamL10:
-      return op_c(op) == 4 ? batch_kernel_up4(op) : batch_kernel_up3(op);
+      return op_c(op) == 4 ? batch_kernel_up4(op, persist) : batch_kernel_up3(op, persist);
     default:
       return nullptr;
   }
@@ -163,22 +164,35 @@ __device__ __forceinline__ void op_body(kptr<hm::StageArgs> s, uint32_t off, Blk
   }
 }
 
-// Runs on 64 x 4 threads; every body uses all 256 (see Blk).  The
-// workgroups walk the launch's work-list with a stride of the grid size.
+// Work unit w of the stage: its item, and the body on the item's tile.
 template <int OP>
+__device__ __forceinline__ void run_unit(kptr<hm::StageArgs> s, uint32_t w, float4* lds) {
+  int j = 0;
+  for (int q = 1; q < s->n_items; ++q)
+    if (w >= s->items[q].first) j = q;
+  const uint32_t off = s->items[j].off, wgs = s->items[j].wgs;
+  const uint32_t local = w - s->items[j].first;
+  op_body<OP>(s, off, Blk{(int)(local % wgs), (int)(local / wgs)}, wgs, lds);
+}
+
+// Runs on 64 x 4 threads; every body uses all 256 (see Blk).  PERSIST: the
+// workgroups walk the launch's work-list with a stride of the grid size;
+// otherwise the grid has one workgroup per work unit.  (The loop costs the
+// bodies registers -- the 8-image canvas writer 98 VGPRs against 34 without
+// it -- so the one-per-unit grid, the default, runs the loop-free form.)
+template <int OP, bool PERSIST>
 __global__ void __launch_bounds__(256) batch_op_kernel(hm::StageArgs stage) {
   (void)stage;  // read through the segment pointer below
   const kptr<hm::StageArgs> s = (kptr<hm::StageArgs>)__builtin_amdgcn_kernarg_segment_ptr();
   __shared__ float4 lds[op_lds_float4<OP>()];
   (void)lds;
-  for (uint32_t w = blockIdx.x; w < s->total; w += gridDim.x) {
-    int j = 0;
-    for (int q = 1; q < s->n_items; ++q)
-      if (w >= s->items[q].first) j = q;
-    const uint32_t off = s->items[j].off, wgs = s->items[j].wgs;
-    const uint32_t local = w - s->items[j].first;
-    op_body<OP>(s, off, Blk{(int)(local % wgs), (int)(local / wgs)}, wgs, lds);
-    __syncthreads();  // the next item may reuse the LDS
+  if constexpr (PERSIST) {
+    for (uint32_t w = blockIdx.x; w < s->total; w += gridDim.x) {
+      run_unit<OP>(s, w, lds);
+      __syncthreads();  // the next item may reuse the LDS
+    }
+  } else {
+    run_unit<OP>(s, blockIdx.x, lds);
   }
 }
 
@@ -199,17 +213,18 @@ constexpr std::array<int, K> ops_where(Pred pred) {
   });
   return a;
 }
-template <const auto& OPS, size_t... I>
+template <const auto& OPS, bool PERSIST, size_t... I>
 std::array<hm::StageKernel, sizeof...(I)> op_kernels(std::index_sequence<I...>) {
   static_assert(((sizeof(typename OpArgsOf<OPS[I]>::type) <= hm::kStageBlob) && ...),
                 "every body's arguments fit a stage's blob");
-  return {{batch_op_kernel<OPS[I]>...}};
+  return {{batch_op_kernel<OPS[I], PERSIST>...}};
 }
 template <const auto& OPS>
-hm::StageKernel find_op_kernel(int op) {
-  static const auto kernels = op_kernels<OPS>(std::make_index_sequence<OPS.size()>{});
+hm::StageKernel find_op_kernel(int op, bool persist) {
+  static const auto once = op_kernels<OPS, false>(std::make_index_sequence<OPS.size()>{});
+  static const auto walk = op_kernels<OPS, true>(std::make_index_sequence<OPS.size()>{});
   for (size_t i = 0; i < OPS.size(); ++i)
-    if (OPS[i] == op) return kernels[i];
+    if (OPS[i] == op) return persist ? walk[i] : once[i];
   return nullptr;
 }
 

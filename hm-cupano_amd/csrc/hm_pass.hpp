@@ -71,6 +71,12 @@ inline int pass_grid_for(const PassLaunch& l, int grid) {
   return grid > 0 ? (int)(l.args.total < (uint32_t)grid ? l.args.total : (uint32_t)grid) : (int)l.args.total;
 }
 
+// The kernel of a launch: the loop-free form when the grid has a workgroup
+// per work unit, the list-walking form for a smaller (persistent) grid.
+inline StageKernel pass_kernel(const PassLaunch& l, int grid) {
+  return batch_kernel(l.op, grid > 0 && (uint32_t)grid < l.args.total);
+}
+
 // Enqueues the pass on `stream` (stage by stage, the bodies of a stage on
 // forked streams, lane work beside them, everything joined back into
 // `stream`).  Capturable.
