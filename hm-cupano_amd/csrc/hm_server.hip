@@ -118,6 +118,7 @@ struct Slot {
   bool busy = false;
   std::vector<Request> reqs;  // one request, or the frames of a batched pass
   int64_t t_start = 0;
+  hipGraphExec_t pass_exec = nullptr;  // the captured pass running here, if any (never destroyed meanwhile)
   std::unique_ptr<hm::PassStreams> ps;  // side streams of the batched passes run here
 };
 
@@ -163,11 +164,32 @@ struct hmServer {
     rig.recs.clear();
     rig.rec_lru.clear();
   }
+  // Every captured pass (the server is idle).
   void drop_pass_graphs() {
     for (auto& kv : pass_graphs) (void)hipGraphExecDestroy(kv.second);
     pass_graphs.clear();
     pass_lru.clear();
     pass_seen.clear();
+  }
+  // The captured passes that ran one of `ids` (recordings of an idle rig: none
+  // of them is in flight).
+  void drop_pass_graphs_of(const std::vector<uint64_t>& ids) {
+    for (auto it = pass_graphs.begin(); it != pass_graphs.end();) {
+      bool hit = false;
+      for (uint64_t id : it->first) hit = hit || std::find(ids.begin(), ids.end(), id) != ids.end();
+      if (!hit) {
+        ++it;
+        continue;
+      }
+      (void)hipGraphExecDestroy(it->second);
+      pass_lru.erase(std::find(pass_lru.begin(), pass_lru.end(), it->first));
+      it = pass_graphs.erase(it);
+    }
+  }
+  bool pass_in_flight(hipGraphExec_t ex) const {
+    for (const auto& sl : slots)
+      if (sl.busy && sl.pass_exec == ex) return true;
+    return false;
   }
 };
 
@@ -248,6 +270,9 @@ hipError_t hmServer::launch_pass(Slot& sl, const std::vector<Rec*>& recs) {
   auto it = pass_graphs.find(key);
   if (it != pass_graphs.end()) {
     ++pass_graph_launches;
+    pass_lru.erase(std::find(pass_lru.begin(), pass_lru.end(), key));  // most recently used last
+    pass_lru.push_back(key);
+    sl.pass_exec = it->second;
     return hipGraphLaunch(it->second, sl.stream);
   }
   HM_RETURN_IF_ERROR(hm::plan_pass(calls, plan));
@@ -270,15 +295,22 @@ hipError_t hmServer::launch_pass(Slot& sl, const std::vector<Rec*>& recs) {
   (void)hipGraphDestroy(g);
   HM_RETURN_IF_ERROR(ei);
   if (pass_graphs.size() >= kMaxPassGraphs) {
-    (void)hipGraphExecDestroy(pass_graphs[pass_lru.front()]);
-    pass_graphs.erase(pass_lru.front());
-    pass_lru.pop_front();
+    // the least recently used pass that no stream is running
+    for (auto k = pass_lru.begin(); k != pass_lru.end(); ++k) {
+      auto victim = pass_graphs.find(*k);
+      if (pass_in_flight(victim->second)) continue;
+      (void)hipGraphExecDestroy(victim->second);
+      pass_graphs.erase(victim);
+      pass_lru.erase(k);
+      break;
+    }
   }
   pass_graphs.emplace(key, ex);
   pass_lru.push_back(key);
   pass_seen.erase(key);
   ++pass_graphs_captured;
   ++pass_graph_launches;
+  sl.pass_exec = ex;
   return hipGraphLaunch(ex, sl.stream);
 }
 
@@ -299,6 +331,7 @@ void hmServer::finish(Slot& sl, hipError_t status, int64_t t_done) {
   }
   if (status != hipSuccess && sticky == hipSuccess) sticky = status;
   sl.reqs.clear();
+  sl.pass_exec = nullptr;
   sl.busy = false;
 }
 
@@ -428,8 +461,10 @@ extern "C" int hmServerRemoveRig(hmServer* sv, int rig_id) {
   if (it == sv->rigs.end()) return hipErrorInvalidValue;
   it->second.removing = true;
   sv->idle_cv.wait(lk, [&] { return it->second.queued == 0 && !it->second.busy; });
+  std::vector<uint64_t> ids;  // the rig's recordings: the passes that ran them go too
+  for (const auto& kv : it->second.recs) ids.push_back(kv.second->id);
   sv->drop_graphs(it->second);
-  sv->drop_pass_graphs();  // passes that ran the rig's recordings
+  sv->drop_pass_graphs_of(ids);
   sv->used -= it->second.bytes;
   it->second.pano->served.store(false);
   sv->rigs.erase(it);

@@ -196,6 +196,43 @@ def test_server_batching_same_structure(hm, torch_gpu, oracle):
     server.close()
 
 
+def test_server_remove_rig_while_batched_passes_run(hm, torch_gpu, oracle):
+    """A rig leaves while passes of the other rigs (captured as graphs) are queued and running:
+    only the passes that ran the leaving rig's recordings are dropped, the others keep replaying,
+    and every frame of the remaining rigs matches the oracle."""
+    import hm_synth
+
+    torch = torch_gpu
+    rigs = [hm_synth.small_rig(seed=70 + k) for k in range(4)]
+    panos = [hm.StitchPano(1, 3, r) for r in rigs]
+    server = hm.Server(n_streams=2, use_graphs=True, batch=4, same_structure=True)
+    ids = [server.add_rig(p) for p in panos]
+    frames = [frames_for(r, 1, 800 + k) for k, r in enumerate(rigs)]
+    devs = [to_dev(torch, f) for f in frames]
+    canvases = [p.alloc_canvas() for p in panos]
+    torch.cuda.synchronize()
+    tag = 0
+    for rnd in range(3):  # compositions of all four rigs, seen often enough to be captured
+        assert server.submit_many([(ids[k], devs[k], canvases[k], tag + k) for k in range(4)]) == 0
+        tag += 4
+        assert server.wait_idle() == 0
+    # rigs 1..3 keep submitting while rig 0 leaves
+    for rnd in range(4):
+        assert server.submit_many([(ids[k], devs[k], canvases[k], tag + k) for k in range(1, 4)]) == 0
+        tag += 3
+        if rnd == 1:
+            server.remove_rig(ids[0])
+    assert server.wait_idle() == 0
+    res = server.poll(4096)
+    assert all(r["status"] == 0 for r in res)
+    for k in range(1, 4):
+        np.testing.assert_array_equal(canvases[k].cpu().numpy(), oracle.pano2(rigs[k], frames[k][0], frames[k][1], 3),
+                                      err_msg=f"rig {k}")
+    st = server.stats()
+    assert st["n_rigs"] == 3 and st["pass_graph_launches"] > 0
+    server.close()
+
+
 def test_server_graph_replays_with_eager_work_between(hm, torch_gpu, oracle):
     """One buffer set, replayed several times with new input content written in between."""
     import hm_synth
