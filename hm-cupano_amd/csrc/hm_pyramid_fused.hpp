@@ -15,6 +15,8 @@
 // (hm_pyramid_kernels.hpp), in the same order.
 #pragma once
 
+#include <type_traits>
+
 #include "hm_pyramid_kernels.hpp"
 
 namespace hmdev {
@@ -925,6 +927,143 @@ __host__ __device__ constexpr int quad_lds_float4() {
   return ((N < kQuadSlots ? N : kQuadSlots) + 1) * kQuadTaps;
 }
 
+// A uint8 pixel as loaded (4 bytes; 3 channels: packed), converted where it
+// is used: the same values ld_px gives.
+template <int C>
+__device__ __forceinline__ uint32_t ld_raw_u8(const uint8_t* base, size_t pix) {
+  if constexpr (C == 4) {
+    return *reinterpret_cast<const uint32_t*>(base + pix * 4);
+  } else {
+    const uint8_t* p = base + pix * 3;
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16);
+  }
+}
+template <int C>
+__device__ __forceinline__ float4 cvt_raw_u8(uint32_t r) {
+  return make_float4((float)(r & 0xffu), (float)((r >> 8) & 0xffu), (float)((r >> 16) & 0xffu),
+                     C == 4 ? (float)(r >> 24) : 0.0f);
+}
+
+// Level 0 of the N-image blend from the seam labels, through the tile's S
+// slot images (uint8 level 0).  The blend of a pixel is the labelled image's
+// Laplacian where that image is opaque (3 channels: always), else the first
+// slot of largest alpha (label_blend), so each pixel computes the labelled
+// slot's Laplacian only, and all slots' only where that one is transparent.
+// The slots' pixels are held as loaded (4 bytes each) until then.  Every
+// value is computed by the functions, on the inputs, of the general slot path.
+template <int C, int N, bool NEXT, int S>
+__device__ __forceinline__ void up_quad_label_slots(const UpQuadArgs<uint8_t, N>& a, const int* idx_rt, int tqx,
+                                                    int tqy, int b, float4* lds) {
+  constexpr int SA = S > 0 ? S : 1;  // array extent
+  int idx[SA];
+#pragma unroll
+  for (int s = 0; s < S; ++s) idx[s] = idx_rt[s];
+  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
+  const int qx = tqx + lx, qy = tqy + ly;
+  const QuadPix q = quad_pixels(a, qx, qy, b);
+  // the tile's taps of R and of each slot image, staged in LDS first
+  constexpr int kRounds = (kQuadTaps + kWgThreads - 1) / kWgThreads;
+  const int tid = ly * 64 + lx;
+  if (NEXT) {
+    size_t tap[kRounds];
+    float4 st[S + 1][kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      const int p = min(tid + r * kWgThreads, kQuadTaps - 1);
+      const int px = min(tqx + p % kQuadTapW, a.w1 - 1), py = min(tqy + p / kQuadTapW, a.h1 - 1);
+      tap[r] = ((size_t)b * a.h1 + py) * a.w1 + px;
+      st[0][r] = ld_px<C>(a.r1, tap[r]);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const float* g1 = a.g1[idx[s]];
+#pragma unroll
+      for (int r = 0; r < kRounds; ++r) st[s + 1][r] = ld_px<C>(g1, tap[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      const int p = tid + r * kWgThreads;
+      if (p < kQuadTaps) {
+#pragma unroll
+        for (int s = 0; s <= S; ++s) lds[s * kQuadTaps + p] = st[s][r];
+      }
+    }
+  }
+  uint32_t raw[SA][4];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const uint8_t* g0 = a.g0[idx[s]];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) raw[s][e] = ld_raw_u8<C>(g0, q.pix[e]);
+  }
+  int lab[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) lab[e] = static_cast<const uint8_t*>(a.mask0)[q.mpx[e]];
+  const int t0 = ly * kQuadTapW + lx;  // tap (qx, qy) in the staged footprint
+  if (NEXT) __syncthreads();
+  float4 rt[4];
+  if (NEXT) {
+    rt[0] = lds[t0];
+    rt[1] = lds[t0 + 1];
+    rt[2] = lds[t0 + kQuadTapW];
+    rt[3] = lds[t0 + kQuadTapW + 1];
+  }
+  // Laplacian of slot s at pixel e of the quad (zero outside its region)
+  auto lap_of = [&](int s, uint32_t rv, const ImgRect& zs, int e) -> float4 {
+    if (!in_rect(zs, q.x[e], q.y[e])) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 g = cvt_raw_u8<C>(rv);
+    if (!NEXT) return g;
+    const float4* tp = lds + (s + 1) * kQuadTaps + t0;
+    const float4 gt[4] = {tp[0], tp[1], tp[kQuadTapW], tp[kQuadTapW + 1]};
+    const float4 up = up_quad_lap<C>(e, gt);  // e is compile-time after unrolling
+    return make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+  };
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (!q.in[e]) continue;
+    // the labelled image's slot, by selects over compile-time slots
+    int sl = -1;
+    uint32_t rv = 0u;
+    ImgRect zs = {0, 0, -1, -1};
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (idx[s] == lab[e]) {
+        sl = s;
+        rv = raw[s][e];
+        zs = a.z[idx[s]];
+      }
+    }
+    float4 B = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool done = false;
+    if (sl >= 0) {
+      const float4 L = lap_of(sl, rv, zs, e);
+      if (C == 3 || L.w != 0.0f) {
+        B = C == 4 ? L : make_float4(L.x, L.y, L.z, 0.0f);
+        done = true;
+      }
+    }
+    if constexpr (C == 4) {
+      if (!done) {  // alpha fallback: the first slot of largest alpha
+        float amax = 0.0f;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const float4 L = lap_of(s, raw[s][e], a.z[idx[s]], e);
+          if (L.w > amax) {
+            amax = L.w;
+            B = L;
+          }
+        }
+      }
+    }
+    float4 R = B;
+    if (NEXT) {
+      const float4 up = up_quad_rec<C>(e, rt);
+      R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+    }
+    quad_store<C>(a, q.x[e], q.y[e], b, R);
+  }
+}
+
 // One quad of a level of the N- or 3-image blend through the tile's S slot
 // images: idx[0..S-1], or, when MASK >= 0, the images of the set MASK (a
 // compile-time set: the 3-image blend places each image's Laplacian at its
@@ -936,6 +1075,10 @@ __host__ __device__ constexpr int quad_lds_float4() {
 template <typename T0, int C, int N, int KIND, int MODE, bool NEXT, int S, int MASK = -1>
 __device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const int* idx_rt, int tqx, int tqy,
                                               int b, float4* lds) {
+  if constexpr (MODE == kMask0Label && MASK < 0 && std::is_same<T0, uint8_t>::value) {
+    up_quad_label_slots<C, N, NEXT, S>(a, idx_rt, tqx, tqy, b, lds);
+    return;
+  }
   constexpr int SA = S > 0 ? S : 1;  // array extent
   int idx[SA];
 #pragma unroll
