@@ -1218,9 +1218,7 @@ __device__ __forceinline__ void up_quad_slots(const UpQuadArgs<T0, N>& a, const 
 // dense path (one or two images) runs with a coarser level, else none.
 template <int N, int KIND, int MODE, bool NEXT>
 __host__ __device__ constexpr int up_quad_lds_float4() {
-  return ((N > 2 && NEXT && !(KIND == kBlend3 && MODE == kMask0Label)) || (N <= 2 && NEXT && MODE != kMask0Label))
-             ? quad_lds_float4<N>()
-             : 1;
+  return ((N > 2 && NEXT) || (N <= 2 && NEXT && MODE != kMask0Label)) ? quad_lds_float4<N>() : 1;
 }
 
 // NEXT: the level has a coarser level (false at the coarsest: blend only).
@@ -1237,16 +1235,17 @@ __device__ __forceinline__ void up_quad_body(const UpQuadArgs<T0, N>& a, Blk blk
   const int qy = tqy + (int)threadIdx.y;
   const int b = blk.z;
   // Three or more images: the tile's images in slots when they fit and the
-  // weights allow.  Not at level 0 of the 3-image class: there the images
-  // cover the whole canvas and the labels are coherent, so loading only the
-  // labelled image per pixel (below) moves less (measured 71 against 91 us at
-  // 6000x2000).
-  if constexpr (N > 2 && !(KIND == kBlend3 && MODE == kMask0Label)) {
+  // weights allow.  At level 0 (labels) the slot path computes only the
+  // labelled image's Laplacian per pixel (up_quad_label_slots); for the
+  // 3-image class, whose images cover the whole canvas, it beats the
+  // per-pixel path below, which loads only the labelled image but only after
+  // its label (59 against 71 us at 6000x2000).
+  if constexpr (N > 2) {
     if (MODE == kMask0Label || KIND == kBlend3 || a.wn != nullptr) {
       int idx[kQuadSlots];
       unsigned set = 0;
       const int n = tile_images<N>(a.z, 2 * tqx, 2 * tqy, 2 * tqx + 127, 2 * tqy + 7, idx, &set);
-      if constexpr (KIND == kBlend3) {
+      if constexpr (KIND == kBlend3 && MODE != kMask0Label) {
         switch (set) {  // uniform
           case 0: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 0, 0>(a, idx, tqx, tqy, b, lds); return;
           case 1: up_quad_slots<T0, C, N, KIND, MODE, NEXT, 1, 1>(a, idx, tqx, tqy, b, lds); return;
