@@ -192,6 +192,7 @@ struct Pano2 : hmStitchPano {
   hipError_t process(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s);
   hipError_t process_f32(const hmSurface& in1, const hmSurface& in2, const hmSurface& canvas, hipStream_t s);
   hipError_t run_blend(const hmSurface& canvas, hipStream_t s, int down_from = 0);
+  hm::Level0Out level0_out(const hmSurface& canvas) const;  // the level-0 reconstruction into the canvas
   hipError_t launch_remap_down(const Pano2L0& a, hipStream_t s, int* n_first);
 };
 
@@ -401,7 +402,8 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
   };
   // Blend-input columns first, on the caller's stream: the pyramid waits on
   // them.  The canvas remap forks after them (overlap 1), so it does not
-  // delay them, or at the start of the frame (overlap 2): it reads nothing the
+  // delay them, after the down pass (overlap 3), or at the start of the
+  // frame (overlap 2): it reads nothing the
   // blend columns write, and the fork's event then sits next to the previous
   // frame's join instead of between two launches of the chain.
   // With fuse_down the columns are remapped inside the first down launch.
@@ -424,6 +426,18 @@ hipError_t Pano2::process(const hmSurface& in1, const hmSurface& in2, const hmSu
     return e_roi;
   }
   if (!side) return run_blend(canvas, s, down_from);
+  if (tuning.overlap == 3 && !pyr.debug) {
+    // After the down pass: the canvas remap then shares the GPU with the up
+    // pass's latency-bound small levels rather than with the bandwidth-bound
+    // blend columns and down pass.
+    const void* g[2] = {g0[0].p, g0[1].p};
+    HM_RETURN_IF_ERROR(pyr.run_down(g, true, down_from, s, timer_or_null()));
+    HM_RETURN_IF_ERROR(fork(s));
+    const hipError_t e1 = launch_spans(0, roi_x0, roi_x1, canvas_w, side, "remap_canvas");
+    const hipError_t e2 = e1 == hipSuccess ? pyr.run_up(g, true, level0_out(canvas), s, timer_or_null()) : e1;
+    const hipError_t j = join(s);
+    return e2 != hipSuccess ? e2 : j;
+  }
   if (fork_first) {
     const hipError_t e2 = run_blend(canvas, s, down_from);
     const hipError_t j = join(s);
@@ -507,8 +521,7 @@ hipError_t Pano2::process_f32(const hmSurface& in1, const hmSurface& in2, const 
                       minimize ? x2 - kOverlapPad : 0, 0, batch, s);
 }
 
-hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s, int down_from) {
-  const void* g[2] = {g0[0].p, g0[1].p};
+hm::Level0Out Pano2::level0_out(const hmSurface& canvas) const {
   hm::Level0Out out;
   out.u8 = static_cast<uint8_t*>(canvas.d_ptr);
   out.u8_pitch = canvas.pitch;
@@ -516,6 +529,12 @@ hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s, int down_fro
   out.u8_h = canvas_h;
   out.ox = minimize ? x2 - kOverlapPad : 0;  // copy-back offset (cudaPano.inl:408)
   out.oy = 0;
+  return out;
+}
+
+hipError_t Pano2::run_blend(const hmSurface& canvas, hipStream_t s, int down_from) {
+  const void* g[2] = {g0[0].p, g0[1].p};
+  const hm::Level0Out out = level0_out(canvas);
   // Debug: the per-level path keeping every level; else the fused passes
   // (the first down_from levels already built by the fused remap).
   if (pyr.debug) return pyr.run(g, true, out, s, timer_or_null());

@@ -121,7 +121,7 @@ extern "C" int hmStitchPanoGetTuning(const hmStitchPano* pano, hmTuning* tuning)
 
 static bool tuning_ok(const hmTuning& t) {
   return (t.remap_px == 2 || t.remap_px == 4 || t.remap_px == 8) && t.tail_base >= 0 && t.tail_base <= 3 &&
-         (t.overlap >= 0 && t.overlap <= 2) && (t.remap_xcd == 0 || t.remap_xcd == 1) &&
+         (t.overlap >= 0 && t.overlap <= 3) && (t.remap_xcd == 0 || t.remap_xcd == 1) &&
          (t.remap_rows == 1 || t.remap_rows == 2 || t.remap_rows == 4) &&
          (t.canvas_grid == 0 || t.canvas_grid == 256 || t.canvas_grid == 512 || t.canvas_grid == 1024 ||
           t.canvas_grid == 2048) &&
@@ -147,7 +147,7 @@ extern "C" int hmStitchPanoAutotune(hmStitchPano* pano, const hmSurface* inputs,
   const bool remap = pano->tunes_remap_px();
   std::vector<int> pxs = remap ? std::vector<int>{2, 4, 8} : std::vector<int>{start.remap_px};
   std::vector<int> tbs = pano->tunes_tail() ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{start.tail_base};
-  std::vector<int> ovs = pano->tunes_overlap() ? (remap ? std::vector<int>{0, 1, 2} : std::vector<int>{0, 1})
+  std::vector<int> ovs = pano->tunes_overlap() ? (remap ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0, 1})
                                                 : std::vector<int>{start.overlap};
   std::vector<int> xcs = remap ? std::vector<int>{0, 1} : std::vector<int>{start.remap_xcd};
   std::vector<int> rys = remap ? std::vector<int>{1, 2, 4} : std::vector<int>{start.remap_rows};

@@ -228,6 +228,8 @@ int hmStitchPanoDeviceBytes(const hmStitchPano* pano, size_t* bytes);
  *   overlap    1: the canvas outside the blend region is remapped on a second
  *              stream overlapping the pyramid, forked after the blend inputs;
  *              2 (2-image class): the same, forked at the start of the frame;
+ *              3 (2-image class): forked after the down pass, beside the up
+ *              pass's small levels;
  *              0: everything on one stream. */
 typedef struct hmTuning {
   int remap_px;

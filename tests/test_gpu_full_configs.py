@@ -97,6 +97,8 @@ def assert_same(got, want, what):
 CFG1_TUNINGS = [
     (4, 3, 2, 0, 2, 0, 0, 0),
     (4, 3, 2, 1, 2, 0, 0, 1),
+    (4, 3, 3, 0, 1, 0, 0, 1),
+    (4, 2, 3, 1, 2, 0, 1, 1),
     (4, 2, 2, 1, 2, 0, 0, 1),
     (4, 0, 2, 0, 2, 0, 0, 1),
     (4, 2, 1, 1, 1, 0, 1),
