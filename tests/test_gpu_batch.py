@@ -155,6 +155,32 @@ def test_batch_two_image_up_pass_shapes(hm, torch_gpu, oracle, use_graph):
         assert (t["tail_base"], t["fuse_up"]) == (tail, fu), name
 
 
+def test_batch_stage_split_over_several_launches(hm, torch_gpu, oracle):
+    """More calls of one body than a launch carries: 18 2-image rigs (more than the 16 items of
+    a launch) and 6 8-image rigs (their argument blocks overflow a launch's 3.5 KB), so each
+    stage runs several launches of the same body; every frame equals the oracle's."""
+    import hm_synth
+
+    torch = torch_gpu
+    calls = []
+    for k in range(18):
+        r = hm_synth.small_rig(seed=200 + k)
+        calls.append((f"pano2_{k}", hm.StitchPano(1, 2 + k % 3, r), r, 2 + k % 3, "2", True, 1))
+    for k in range(6):
+        r = hm_synth.small_rig(n=8, seed=300 + k, src=(120, 56), canvas=(780, 64))
+        calls.append((f"panoN8_{k}", hm.StitchPanoN(1, 3, r), r, 3, "N", True, 1))
+    batch, bufs = build(hm, torch, calls, 1200)
+    info = batch.info()
+    assert info["launches"] < info["recorded_launches"]
+    for use_graph in (False, True):
+        for _, _, c, prior in bufs:
+            c.copy_(torch.from_numpy(prior).cuda())
+        torch.cuda.synchronize()
+        assert batch.run(use_graph=use_graph) == 0
+        torch.cuda.synchronize()
+        check(oracle, calls, bufs, f"graph={use_graph}")
+
+
 def test_batch_more_than_four_images_per_tile(hm, torch_gpu, oracle):
     """Six images over the whole canvas (identity maps, all at (0, 0)): every tile
     meets all six, so the batched down pass runs several slot chunks per tile and
