@@ -7,11 +7,12 @@ batch_size(), process() taking [B, H, W, C] or [H, W, C] CUDA tensors and an
 optional canvas and returning the canvas, and dump_soft_blend_pyramid().
 
 What differs, by design:
-- The library stitches the uint8 pipeline (uchar4 / uchar3).  uint8 tensors
-  go straight to it.  Tensors of another dtype are accepted when every value
-  is an integer in 0..255 (as the reference's tests feed float images); they
-  are converted to uint8 on the device and the canvas is returned in the
-  input's dtype.  Other values raise ValueError.
+- uint8 inputs run the library's uint8 pipeline (CudaStitchPano<uchar4 |
+  uchar3, float4 | float3>).  Inputs of any other real dtype run its float
+  pipeline (<float4 | float3, float4 | float3>) on float32 copies, so
+  fractional values stay fractional, and the canvas comes back in the first
+  input's dtype as cupano/ops.py:52-57 casts it: `.to(dtype)` for a floating
+  dtype, clamped to 0..255 first for an integer one.
 - `backend` and `enable_cuda_graphs` are accepted and not used: there is one
   backend, and graph replay of many rigs is hm_stitch.Server's job.
 - Work is issued on the current torch stream; as in the reference, nothing
@@ -43,20 +44,37 @@ def ensure_batched(image):
     return image
 
 
-def _as_u8(t):
-    """The uint8 view the library stitches (a packed copy when needed)."""
+def cast_like(value, dtype):
+    """cupano/ops.py:52-57: the canvas in the input's dtype."""
+    if value.dtype == dtype:
+        return value
+    if dtype.is_floating_point:
+        return value.to(dtype)
+    return value.clamp(0, 255).to(dtype)
+
+
+def _packed(t) -> bool:
+    """Rows of whole pixels, one image after another (the surface layout the library reads)."""
+    return t.stride(3) == 1 and t.stride(2) == t.shape[3] and (t.shape[0] == 1 or t.stride(0) == t.stride(1) * t.shape[1])
+
+
+def _float_pipeline(tensors) -> bool:
+    """uint8 inputs stitch in the uint8 pipeline, anything else in the float one."""
     torch = _torch()
-    if not t.is_cuda:
-        raise ValueError("inputs must be CUDA tensors")
-    if t.dtype != torch.uint8:
-        if t.is_floating_point() or t.is_complex():
-            bad = (t != torch.round(t)) | (t < 0) | (t > 255)
-        else:
-            bad = (t < 0) | (t > 255)
-        if bool(bad.any()):
-            raise ValueError("the HIP pipeline is uint8: non-uint8 inputs must hold integers in 0..255")
-        t = t.to(torch.uint8)
-    if t.stride(3) != 1 or t.stride(2) != t.shape[3] or (t.shape[0] > 1 and t.stride(0) != t.stride(1) * t.shape[1]):
+    for t in tensors:
+        if not t.is_cuda:
+            raise ValueError("inputs must be CUDA tensors")
+        if t.is_complex() or t.dtype == torch.bool:
+            raise ValueError(f"unsupported input dtype {t.dtype}")
+    return any(t.dtype != torch.uint8 for t in tensors)
+
+
+def _pipeline_input(t, f32: bool):
+    """The tensor the library reads: uint8 (uint8 pipeline) or float32, packed."""
+    torch = _torch()
+    if f32 and t.dtype != torch.float32:
+        t = t.to(torch.float32)
+    if not _packed(t):
         t = t.contiguous()
     return t
 
@@ -79,7 +97,8 @@ class _PanoBase:
         self._status = CudaStatus()
         self._batch = int(batch_size)
         self._num_levels = int(num_levels)
-        self._objs: dict = {}  # channels -> hm_stitch object
+        self._objs: dict = {}  # (channels, float pipeline) -> hm_stitch object
+        self._last: dict = {}  # channels -> the object that processed last
         self._canvas_wh = (0, 0)
 
     @property
@@ -95,21 +114,21 @@ class _PanoBase:
     def batch_size(self) -> int:
         return self._batch
 
-    def _make(self, channels: int):  # pragma: no cover - per class
+    def _make(self, channels: int, f32: bool):  # pragma: no cover - per class
         raise NotImplementedError
 
-    def _obj(self, channels: int):
-        obj = self._objs.get(channels)
+    def _obj(self, channels: int, f32: bool = False):
+        obj = self._objs.get((channels, f32))
         if obj is None:
             try:
-                obj = self._make(channels)
+                obj = self._make(channels, f32)
             except hm_stitch.HmError as e:
                 self._status = CudaStatus(e.code, f"{self._kind} construction failed (hipError {e.code})")
                 raise_if_error(self._status)
             if obj.status != 0:
                 self._status = CudaStatus(obj.status, f"{self._kind} construction failed (hipError {obj.status})")
                 raise_if_error(self._status)
-            self._objs[channels] = obj
+            self._objs[(channels, f32)] = obj
             self._canvas_wh = obj.canvas_size()
         return obj
 
@@ -132,18 +151,26 @@ class _PanoBase:
             raise ValueError(f"canvas shape {tuple(canvas.shape)} != {shape}")
         return canvas
 
-    def _run(self, obj, call, canvas, keep_prior: bool):
-        """Runs `call(u8_canvas)` into `canvas`, through a uint8 stand-in when needed."""
+    def _run(self, obj, f32: bool, call, canvas, keep_prior: bool):
+        """Runs `call(pipeline_canvas)` into `canvas`, through a stand-in of the
+        pipeline's dtype when the canvas has another dtype or layout."""
         torch = _torch()
-        packed = canvas.dtype == torch.uint8 and canvas.stride(3) == 1 and canvas.stride(2) == canvas.shape[3] and (
-            canvas.shape[0] == 1 or canvas.stride(0) == canvas.stride(1) * canvas.shape[1])
-        target = canvas if packed else (_as_u8(canvas) if keep_prior else torch.empty(
-            canvas.shape, dtype=torch.uint8, device=canvas.device))
+        if not canvas.is_cuda:
+            raise ValueError("canvas must be a CUDA tensor")
+        dt = torch.float32 if f32 else torch.uint8
+        if canvas.dtype == dt and _packed(canvas):
+            target = canvas
+        elif keep_prior:
+            target = cast_like(canvas, dt).contiguous()
+        else:
+            target = torch.empty(canvas.shape, dtype=dt, device=canvas.device)
         rc = call(target)
         if rc != 0:
             self._on_error(obj, rc)
         if target is not canvas:
-            canvas.copy_(target)
+            canvas.copy_(cast_like(target, canvas.dtype))
+        obj._processed = True
+        self._last[canvas.shape[-1]] = obj
         return canvas
 
     def _on_error(self, obj, rc: int) -> None:
@@ -159,13 +186,13 @@ class _PanoBase:
         if self._num_levels <= 0:
             raise ValueError("Soft-blend pyramid is only available when num_levels > 0")
         if channels is None:
-            used = [c for c, o in self._objs.items() if getattr(o, "_processed", False)]
+            used = list(self._last)
             if len(used) != 1:
                 raise ValueError("device and channels must be provided when multiple scratch buffers exist"
                                  if used else "No Laplacian workspace is populated yet")
             channels = used[0]
-        obj = self._objs.get(channels)
-        if obj is None or not getattr(obj, "_processed", False):
+        obj = self._last.get(channels)
+        if obj is None:
             raise ValueError(f"No soft scratch buffers available for channels={channels}")
         obj.dump_soft_blend_pyramid(str(directory))
 
@@ -188,8 +215,9 @@ class CudaStitchPano(_PanoBase):
         self._rig = _Rig([m.img1_col, m.img2_col], [m.img1_row, m.img2_row], m.positions, m.whole_seam_mask_image)
         self._first_object(quiet, "Stitched")
 
-    def _make(self, channels: int):
-        return hm_stitch.StitchPano(self._batch, self._num_levels, self._rig, self._minimize_blend, channels)
+    def _make(self, channels: int, f32: bool):
+        return hm_stitch.StitchPano(self._batch, self._num_levels, self._rig, self._minimize_blend, channels,
+                                    float_pipeline=f32)
 
     def process(self, input_image_1, input_image_2, canvas=None):
         raise_if_error(self._status)
@@ -201,12 +229,11 @@ class CudaStitchPano(_PanoBase):
             raise ValueError("Only 3- and 4-channel inputs are supported")
         if b.shape[-1] != c:
             raise ValueError("Mismatched channel counts")
-        obj = self._obj(c)
+        f32 = _float_pipeline((a, b))
+        obj = self._obj(c, f32)
         out = self._canvas_for(canvas, a, c, clear=False)  # every canvas pixel is written
-        ua, ub = _as_u8(a), _as_u8(b)
-        res = self._run(obj, lambda t: obj.process(ua, ub, t), out, keep_prior=False)
-        obj._processed = True
-        return res
+        ua, ub = _pipeline_input(a, f32), _pipeline_input(b, f32)
+        return self._run(obj, f32, lambda t: obj.process(ua, ub, t), out, keep_prior=False)
 
 
 class CudaStitchPanoN(_PanoBase):
@@ -231,8 +258,9 @@ class CudaStitchPanoN(_PanoBase):
         self._rig = _Rig(m.img_col, m.img_row, m.positions, m.whole_seam_mask_indexed)
         self._first_object(quiet, "Stitched (N-image)")
 
-    def _make(self, channels: int):
-        return hm_stitch.StitchPanoN(self._batch, self._num_levels, self._rig, self._minimize_blend, channels)
+    def _make(self, channels: int, f32: bool):
+        return hm_stitch.StitchPanoN(self._batch, self._num_levels, self._rig, self._minimize_blend, channels,
+                                     float_pipeline=f32)
 
     def process(self, inputs: Iterable, canvas=None):
         raise_if_error(self._status)
@@ -248,12 +276,11 @@ class CudaStitchPanoN(_PanoBase):
             raise ValueError("Mismatched channel counts")
         if c not in (3, 4):
             raise ValueError("Only 3- and 4-channel inputs are supported")
-        obj = self._obj(c)
+        f32 = _float_pipeline(ins)
+        obj = self._obj(c, f32)
         out = self._canvas_for(canvas, ins[0], c, clear=False)
-        us = [_as_u8(t) for t in ins]
-        res = self._run(obj, lambda t: obj.process(us, t), out, keep_prior=False)
-        obj._processed = True
-        return res
+        us = [_pipeline_input(t, f32) for t in ins]
+        return self._run(obj, f32, lambda t: obj.process(us, t), out, keep_prior=False)
 
 
 class CudaStitchPano3(_PanoBase):
@@ -275,8 +302,8 @@ class CudaStitchPano3(_PanoBase):
                          m.whole_seam_mask_image)
         self._first_object(quiet, "Stitched (3-image)")
 
-    def _make(self, channels: int):
-        return hm_stitch.StitchPano3(self._batch, self._num_levels, self._rig, channels)
+    def _make(self, channels: int, f32: bool):
+        return hm_stitch.StitchPano3(self._batch, self._num_levels, self._rig, channels, float_pipeline=f32)
 
     def process(self, input_image_0, input_image_1, input_image_2, canvas=None, fused: bool = True):
         raise_if_error(self._status)
@@ -288,9 +315,8 @@ class CudaStitchPano3(_PanoBase):
             raise ValueError("Only 3- and 4-channel inputs are supported")
         if any(t.shape[-1] != c for t in ins):
             raise ValueError("Mismatched channel counts")
-        obj = self._obj(c)
+        f32 = _float_pipeline(ins)
+        obj = self._obj(c, f32)
         out = self._canvas_for(canvas, ins[0], c, clear=True)
-        us = [_as_u8(t) for t in ins]
-        res = self._run(obj, lambda t: obj.process(us[0], us[1], us[2], t, fused=fused), out, keep_prior=True)
-        obj._processed = True
-        return res
+        us = [_pipeline_input(t, f32) for t in ins]
+        return self._run(obj, f32, lambda t: obj.process(us[0], us[1], us[2], t, fused=fused), out, keep_prior=True)

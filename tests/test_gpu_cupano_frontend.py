@@ -1,6 +1,7 @@
 """The cupano Python frontend on the GPU: the reference's own Python test cases
-(tests/python/test_pytorch_pano.py, float images holding integer values),
-bit-exact parity with the oracle for uint8 rigs, rigs loaded from control
+(tests/python/test_pytorch_pano.py, float images), bit-exact parity with the
+oracle for uint8 rigs and for float inputs with fractional values (the float
+pipeline, the canvas cast back to the input's dtype), rigs loaded from control
 directories, canvases passed in, and input errors."""
 import os
 import sys
@@ -96,8 +97,8 @@ def test_minimize_blend_changes_workspace_size(cupano, torch_gpu, tmp_path):
     full = cupano.CudaStitchPano(1, 4, masks, quiet=True, minimize_blend=False, enable_cuda_graphs=False)
     mini = cupano.CudaStitchPano(1, 4, masks, quiet=True, minimize_blend=True, enable_cuda_graphs=False)
     out_full, out_mini = full.process(i1, i2), mini.process(i1, i2)
-    # the uint8 pipeline truncates at the end: the two blends agree within 1
-    assert float((out_full - out_mini).abs().max()) <= 1.0
+    assert out_full.dtype == torch.float32
+    assert float((out_full - out_mini).abs().max()) <= 1e-5
     full.dump_soft_blend_pyramid(tmp_path / "full")
     mini.dump_soft_blend_pyramid(tmp_path / "mini")
 
@@ -240,10 +241,78 @@ def test_input_errors(cupano, torch_gpu):
         pano.process(torch.cat([f[0], f[0]]), torch.cat([f[1], f[1]]))
     with pytest.raises(ValueError, match="3- and 4-channel"):
         pano.process(f[0][..., :2], f[1][..., :2])
-    with pytest.raises(ValueError, match="uint8"):
-        pano.process(f[0].float() + 0.5, f[1].float())
+    with pytest.raises(ValueError, match="CUDA"):
+        pano.process(f[0].cpu(), f[1].cpu())
+    with pytest.raises(ValueError, match="dtype"):
+        pano.process(f[0] > 0, f[1] > 0)
     with pytest.raises(ValueError, match="HWC or BHWC"):
         pano.process(f[0][0, 0], f[1][0, 0])
     assert pano.status.ok()  # input errors are not sticky
     out = pano.process(*f)
     assert out.shape[0] == 1
+
+
+# ---- float inputs: the float pipeline, the canvas in the input's dtype (cupano/ops.py:52-57) ----
+def fractional(frames, seed):
+    """The uint8 frames plus a seeded fraction in every colour channel (alpha stays 255 or 0)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for f in frames:
+        v = f.astype(np.float32)
+        v[..., :3] += rng.uniform(0, 1, f[..., :3].shape).astype(np.float32)
+        out.append(v)
+    return out
+
+
+def cast_like_np(v, dtype):
+    return v.astype(dtype) if np.issubdtype(dtype, np.floating) else np.clip(v, 0, 255).astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float16", "int16"])
+def test_two_image_float_inputs_match_oracle(cupano, torch_gpu, oracle, dtype):
+    import hm_synth
+
+    torch = torch_gpu
+    rig = hm_synth.small_rig(seed=151)
+    f = fractional(frames_for(rig, 1, 9), 9)
+    tdt, ndt = getattr(torch, dtype), np.dtype(dtype)
+    f = [x.astype(ndt) for x in f]  # the inputs as the caller holds them
+    pano = cupano.CudaStitchPano(1, 4, masks_of(cupano, rig), quiet=True)
+    out = pano.process(*[torch.from_numpy(x).cuda() for x in f])
+    assert out.dtype == tdt
+    want = oracle.pano2(rig, f[0].astype(np.float32), f[1].astype(np.float32), 4)
+    np.testing.assert_array_equal(out.cpu().numpy(), cast_like_np(want, ndt))
+    # the uint8 object of the same pano still serves uint8 inputs
+    u = frames_for(rig, 1, 9)
+    out8 = pano.process(*[torch.from_numpy(x).cuda() for x in u])
+    np.testing.assert_array_equal(out8.cpu().numpy(), oracle.pano2(rig, u[0], u[1], 4))
+
+
+def test_n_and_three_image_float_inputs_match_oracle(cupano, torch_gpu, oracle):
+    import hm_synth
+
+    torch = torch_gpu
+    rig = hm_synth.small_rig(n=4, seed=161, src=(120, 56), canvas=(440, 64))
+    f = fractional(frames_for(rig, 1, 10), 10)
+    m = cupano.ControlMasksN()
+    m.img_col, m.img_row = list(rig.map_x), list(rig.map_y)
+    m.whole_seam_mask_indexed = rig.seam
+    m.positions = [cupano.SpatialTiff(float(x), float(y)) for x, y in zip(rig.pos_x, rig.pos_y)]
+    out = cupano.CudaStitchPanoN(1, 3, m, quiet=True).process([torch.from_numpy(x).cuda() for x in f])
+    assert out.dtype == torch.float32
+    np.testing.assert_array_equal(out.cpu().numpy(), oracle.panoN(rig, f, 3))
+
+    rig3 = hm_synth.small_rig(n=3, seed=171, src=(120, 56), canvas=(330, 64))
+    f3 = fractional(frames_for(rig3, 1, 11), 11)
+    m3 = cupano.ControlMasks3()
+    m3.img0_col, m3.img1_col, m3.img2_col = rig3.map_x
+    m3.img0_row, m3.img1_row, m3.img2_row = rig3.map_y
+    m3.whole_seam_mask_image = rig3.seam
+    m3.positions = [cupano.SpatialTiff(float(x), float(y)) for x, y in zip(rig3.pos_x, rig3.pos_y)]
+    pano3 = cupano.CudaStitchPano3(1, 3, m3, quiet=True)
+    for fused in (True, False):
+        prior = np.full((1, pano3.canvas_height(), pano3.canvas_width(), 4), 7.25, np.float32)
+        canvas = torch.from_numpy(prior.copy()).cuda()
+        out3 = pano3.process(*[torch.from_numpy(x).cuda() for x in f3], canvas=canvas, fused=fused)
+        assert out3.data_ptr() == canvas.data_ptr()
+        np.testing.assert_array_equal(out3.cpu().numpy(), oracle.pano3(rig3, f3, 3, fused=fused, canvas=prior))
