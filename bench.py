@@ -526,6 +526,11 @@ def frame_loop_benchmark(hm_stitch, rig, f1, f2, levels, frames):
             "bytes_up_per_frame": up, "bytes_down_per_frame": down, "transfers": runs}
 
 
+def pct(v, q):
+    """The q-th percentile of v in ms (4 decimals), or None with no samples."""
+    return round(float(np.percentile(v, q)), 4) if len(v) else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -696,8 +701,8 @@ def main():
         "config": {"workload": WORKLOAD, "levels": main_l, "canvas": f"{cw}x{ch}", "batch": 1,
                    "blend_roi": "minimized (overlap + 2x128)"},
         "fps": round(1e3 / r["ms"], 1),
-        "latency_ms_p50": round(float(np.percentile(r["lat"], 50)), 4),
-        "latency_ms_p99": round(float(np.percentile(r["lat"], 99)), 4),
+        "latency_ms_p50": pct(r["lat"], 50),
+        "latency_ms_p99": pct(r["lat"], 99),
         "roofline": roofline,
         "hbm_frame": frame,
         "tuning": r["tuning"],
@@ -710,8 +715,8 @@ def main():
         r0 = results[0]
         line["levels0"] = {"ms_per_step": round(r0["ms"], 5), "fps": round(1e3 / r0["ms"], 1), "tuning": r0["tuning"],
                            "value_Mpix_s": round(cw * ch / (r0["ms"] * 1e-3) / 1e6, 1),
-                           "latency_ms_p50": round(float(np.percentile(r0["lat"], 50)), 4),
-                           "latency_ms_p99": round(float(np.percentile(r0["lat"], 99)), 4),
+                           "latency_ms_p50": pct(r0["lat"], 50),
+                           "latency_ms_p99": pct(r0["lat"], 99),
                            "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                                        for k, v in r0["kernels"].items()}}
     if args.loop_frames > 0:

@@ -141,10 +141,7 @@ __device__ __forceinline__ void op_body(kptr<hm::StageArgs> s, uint32_t off, Blk
     multi_canvas_body<C, N>(a, blk, lds);
   } else if constexpr (fam == hm::kFamDown) {
     using T0 = std::conditional_t<A, uint8_t, float>;
-    if constexpr (N <= 2)
-      down_fused_body<T0, C, N>(a, blk, lds);
-    else
-      down_fused_slots_body<T0, C, N>(a, blk, lds);
+    down_any_body<T0, C, N>(a, blk, lds);
   } else if constexpr (fam == hm::kFamTail2) {
     up_tail_body<C, 2, kBlend2>(a, blk, lds);
   } else if constexpr (fam == hm::kFamTail2L) {

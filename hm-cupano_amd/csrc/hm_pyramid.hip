@@ -207,10 +207,7 @@ static hipError_t down_fused_t(const void* const* g0, int w0, int h0, int n_out,
   a.touch = touch;
   const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
   if (g_recorder) return g_recorder->add(op_down<T0, C, N>(), grid, dim3(256), s, a, "down_fused");
-  if constexpr (N <= 2)
-    hipLaunchKernelGGL((down_fused_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL((down_fused_slots_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((down_fused_kernel<T0, C, N>), grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

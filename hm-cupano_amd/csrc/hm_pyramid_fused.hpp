@@ -299,20 +299,16 @@ __device__ __forceinline__ void down_fused_body(const DownFusedArgs<T0, N>& a, B
   down_rest<C, N>(a, tx, ty, b, tid, s1);
 }
 
-template <typename T0, int C, int N>
-__global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a) {
-  __shared__ float4 lds[down_lds_float4<N>()];
-  (void)a;
-  down_fused_body<T0, C, N>(kernel_arg<DownFusedArgs<T0, N>>(), hw_blk(), lds);
-}
-
 // Three or more images: the images whose valid region meets the tile at some
 // produced level run the level chain together, up to kDownSlots at a time in
 // one set of LDS tiles; a workgroup skips the other images (their buffers
 // there are never read).  The slot count S is a template parameter (the
 // kernel switches on it, a uniform value), so all loads of the first level
-// are issued without a branch between them.
-constexpr int kDownSlots = 4;
+// are issued without a branch between them.  Two slots: on the 8-image
+// workload (14000x2600 canvas) the pass takes 107 us, against 121 us with
+// one slot, 116 us with three and 130 us with four; three images on a
+// 6200x2300 canvas: 28.6 us against 30.8 us with three slots.
+constexpr int kDownSlots = 2;
 
 // Whether the e x e tile (tx, ty) of a level meets the rectangle z.
 __host__ __device__ __forceinline__ bool tile_meets(const ImgRect& z, int tx, int ty, int e) {
@@ -448,8 +444,12 @@ __device__ __forceinline__ void down_fused_slots_body(const DownFusedArgs<T0, N>
     const bool last = act == 0;  // uniform
     switch (n) {  // uniform
       case 1: down_slots<T0, C, N, 1>(a, idx, tx, ty, b, tid, s1, last); break;
-      case 2: down_slots<T0, C, N, 2>(a, idx, tx, ty, b, tid, s1, last); break;
-      case 3: down_slots<T0, C, N, 3>(a, idx, tx, ty, b, tid, s1, last); break;
+      case 2:
+        if constexpr (K >= 2) down_slots<T0, C, N, 2>(a, idx, tx, ty, b, tid, s1, last);
+        break;
+      case 3:
+        if constexpr (K >= 3) down_slots<T0, C, N, 3>(a, idx, tx, ty, b, tid, s1, last);
+        break;
       default:
         if constexpr (K >= 4) down_slots<T0, C, N, 4>(a, idx, tx, ty, b, tid, s1, last);
         break;
@@ -457,11 +457,21 @@ __device__ __forceinline__ void down_fused_slots_body(const DownFusedArgs<T0, N>
   }
 }
 
+// Every down launch's body: all images (one or two), or the images of the
+// tile in slots.
 template <typename T0, int C, int N>
-__global__ void __launch_bounds__(256) down_fused_slots_kernel(DownFusedArgs<T0, N> a) {
+__device__ __forceinline__ void down_any_body(const DownFusedArgs<T0, N>& a, Blk blk, float4* lds) {
+  if constexpr (N <= 2)
+    down_fused_body<T0, C, N>(a, blk, lds);
+  else
+    down_fused_slots_body<T0, C, N>(a, blk, lds);
+}
+
+template <typename T0, int C, int N>
+__global__ void __launch_bounds__(256) down_fused_kernel(DownFusedArgs<T0, N> a) {
   __shared__ float4 lds[down_lds_float4<down_slots_count<N>()>()];
   (void)a;
-  down_fused_slots_body<T0, C, N>(kernel_arg<DownFusedArgs<T0, N>>(), hw_blk(), lds);
+  down_any_body<T0, C, N>(kernel_arg<DownFusedArgs<T0, N>>(), hw_blk(), lds);
 }
 
 // ---------------------------------------------------------------------------

@@ -33,33 +33,48 @@ struct DownArgs {
   int tiles_x, tiles_y;
 };
 
+// x / d for a tap count d in {0, 1, 2, 3, 4}, y = RN(1 / d) (0 for d = 0,
+// where x = +0): q0 = x * y, the residual r = x - d * q0 (exact, one fused
+// multiply-add) and q0 + r * y rounded once.  For d = 3 this is Markstein's
+// correction step, exact because y is the correctly rounded reciprocal; for
+// d = 1, 2, 4 q0 is the quotient itself (r = 0) except where x / d is
+// subnormal.  Checked on every finite float x but -0 (which a sum that
+// starts at +0 never is) against IEEE division, on the CPU: the same bits as
+// x / d, for about a third of the division sequence's instructions.
+__device__ __forceinline__ float div_count(float x, float d, float y) {
+  const float q0 = x * y;
+  const float r = __builtin_fmaf(-d, q0, x);
+  return __builtin_fmaf(r, y, q0);
+}
+
 // One coarse pixel from its 2x2 block, taps in the reference order (0,0),
 // (1,0), (0,1), (1,1): RGB mean of the in-range taps whose alpha != 0 (0 if
-// none), alpha = max over the in-range taps (cudaBlend.cu:85-134).
+// none), alpha = max over the in-range taps (cudaBlend.cu:85-134).  Written
+// without branches: a tap that does not count adds +0, which leaves a sum
+// that starts at +0 unchanged (under round to nearest it never becomes -0),
+// so the sums are the conditional sums' bits.  Taps out of range are never
+// read (p[k] need not hold a value there).
 template <int C>
 __device__ __forceinline__ float4 down_px(const float4* p, const bool* in_range) {
   float sx = 0.0f, sy = 0.0f, sz = 0.0f, alpha = 0.0f;
   int count = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    if (!in_range[k]) continue;
-    bool keep = true;
+    const bool in = in_range[k];
+    const float4 q = in ? p[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool keep = in;
     if constexpr (C == 4) {
-      alpha = alpha < p[k].w ? p[k].w : alpha;
-      keep = p[k].w != 0.0f;
+      alpha = (in && alpha < q.w) ? q.w : alpha;
+      keep = in && q.w != 0.0f;
     }
-    if (keep) {
-      sx += p[k].x;
-      sy += p[k].y;
-      sz += p[k].z;
-      ++count;
-    }
+    sx += keep ? q.x : 0.0f;
+    sy += keep ? q.y : 0.0f;
+    sz += keep ? q.z : 0.0f;
+    count += keep ? 1 : 0;
   }
-  if (count > 0) {
-    const float fc = (float)count;
-    return make_float4(div_exact(sx, fc), div_exact(sy, fc), div_exact(sz, fc), alpha);
-  }
-  return make_float4(0.0f, 0.0f, 0.0f, alpha);
+  const float y = count == 4 ? 0.25f : count == 3 ? 0x1.555556p-2f : count == 2 ? 0.5f : count == 1 ? 1.0f : 0.0f;
+  const float d = (float)count;
+  return make_float4(div_count(sx, d, y), div_count(sy, d, y), div_count(sz, d, y), alpha);
 }
 
 template <typename TIn, int C, int N>
