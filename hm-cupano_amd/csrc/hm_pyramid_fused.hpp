@@ -957,10 +957,10 @@ __device__ __forceinline__ float4 cvt_raw_u8(uint32_t r) {
 // Level 0 of the N-image blend from the seam labels, through the tile's S
 // slot images (uint8 level 0).  The blend of a pixel is the labelled image's
 // Laplacian where that image is opaque (3 channels: always), else the first
-// slot of largest alpha (label_blend), so each pixel computes the labelled
-// slot's Laplacian only, and all slots' only where that one is transparent.
-// The slots' pixels are held as loaded (4 bytes each) until then.  Every
-// value is computed by the functions, on the inputs, of the general slot path.
+// slot of largest alpha (label_blend), so each pixel loads and computes the
+// labelled slot's pixel only (the labels are read first), and all slots'
+// only where that one is transparent or not a slot.  Every value is computed
+// by the functions, on the inputs, of the general slot path.
 template <int C, int N, bool NEXT, int S>
 __device__ __forceinline__ void up_quad_label_slots(const UpQuadArgs<uint8_t, N>& a, const int* idx_rt, int tqx,
                                                     int tqy, int b, float4* lds) {
@@ -971,6 +971,9 @@ __device__ __forceinline__ void up_quad_label_slots(const UpQuadArgs<uint8_t, N>
   const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
   const int qx = tqx + lx, qy = tqy + ly;
   const QuadPix q = quad_pixels(a, qx, qy, b);
+  int lab[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) lab[e] = static_cast<const uint8_t*>(a.mask0)[q.mpx[e]];
   // the tile's taps of R and of each slot image, staged in LDS first
   constexpr int kRounds = (kQuadTaps + kWgThreads - 1) / kWgThreads;
   const int tid = ly * 64 + lx;
@@ -999,16 +1002,22 @@ __device__ __forceinline__ void up_quad_label_slots(const UpQuadArgs<uint8_t, N>
       }
     }
   }
-  uint32_t raw[SA][4];
+  // each pixel's labelled slot (-1: none) and its pixel as loaded
+  int sl[4];
+  uint32_t rv[4];
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const uint8_t* g0 = a.g0[idx[s]];
+  for (int e = 0; e < 4; ++e) {
+    sl[e] = -1;
+    const uint8_t* g = a.g0[idx[0]];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) raw[s][e] = ld_raw_u8<C>(g0, q.pix[e]);
+    for (int s = 0; s < S; ++s) {
+      if (idx[s] == lab[e]) {
+        sl[e] = s;
+        g = a.g0[idx[s]];
+      }
+    }
+    rv[e] = ld_raw_u8<C>(g, q.pix[e]);
   }
-  int lab[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) lab[e] = static_cast<const uint8_t*>(a.mask0)[q.mpx[e]];
   const int t0 = ly * kQuadTapW + lx;  // tap (qx, qy) in the staged footprint
   if (NEXT) __syncthreads();
   float4 rt[4];
@@ -1019,9 +1028,9 @@ __device__ __forceinline__ void up_quad_label_slots(const UpQuadArgs<uint8_t, N>
     rt[3] = lds[t0 + kQuadTapW + 1];
   }
   // Laplacian of slot s at pixel e of the quad (zero outside its region)
-  auto lap_of = [&](int s, uint32_t rv, const ImgRect& zs, int e) -> float4 {
+  auto lap_of = [&](int s, uint32_t v, const ImgRect& zs, int e) -> float4 {
     if (!in_rect(zs, q.x[e], q.y[e])) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 g = cvt_raw_u8<C>(rv);
+    const float4 g = cvt_raw_u8<C>(v);
     if (!NEXT) return g;
     const float4* tp = lds + (s + 1) * kQuadTaps + t0;
     const float4 gt[4] = {tp[0], tp[1], tp[kQuadTapW], tp[kQuadTapW + 1]};
@@ -1031,22 +1040,14 @@ __device__ __forceinline__ void up_quad_label_slots(const UpQuadArgs<uint8_t, N>
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     if (!q.in[e]) continue;
-    // the labelled image's slot, by selects over compile-time slots
-    int sl = -1;
-    uint32_t rv = 0u;
     ImgRect zs = {0, 0, -1, -1};
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (idx[s] == lab[e]) {
-        sl = s;
-        rv = raw[s][e];
-        zs = a.z[idx[s]];
-      }
-    }
+    for (int s = 0; s < S; ++s)
+      if (s == sl[e]) zs = a.z[idx[s]];
     float4 B = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     bool done = false;
-    if (sl >= 0) {
-      const float4 L = lap_of(sl, rv, zs, e);
+    if (sl[e] >= 0) {
+      const float4 L = lap_of(sl[e], rv[e], zs, e);
       if (C == 3 || L.w != 0.0f) {
         B = C == 4 ? L : make_float4(L.x, L.y, L.z, 0.0f);
         done = true;
@@ -1057,7 +1058,7 @@ __device__ __forceinline__ void up_quad_label_slots(const UpQuadArgs<uint8_t, N>
         float amax = 0.0f;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-          const float4 L = lap_of(s, raw[s][e], a.z[idx[s]], e);
+          const float4 L = lap_of(s, ld_raw_u8<C>(a.g0[idx[s]], q.pix[e]), a.z[idx[s]], e);
           if (L.w > amax) {
             amax = L.w;
             B = L;
