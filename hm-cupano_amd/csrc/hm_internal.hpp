@@ -133,6 +133,18 @@ hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch
 hipError_t launch_up_l10(int c, bool g0_u8, const UpQuadLaunch& u, const float* mask1, const float* const* g2,
                          const float* r2, int w2, int h2, int batch, hipStream_t s);
 
+// Three or more images, uint8 level 0 with seam labels: levels 1 and 0 in one
+// launch (hmdev::up_l10n_kernel).  u is the level-0 launch; wn1/z1 level 1's
+// weights and valid regions; g2/r2 level 2.  Valid only when
+// l10n_window_fits holds for u's window.
+hipError_t launch_up_l10n(int c, int n, const UpQuadLaunch& u, const float* wn1, size_t wn1_plane,
+                          const hmdev::ImgRect* z1, const float* const* g2, const float* r2, int w2, int h2,
+                          int batch, hipStream_t s);
+// Whether no level-0 tile of the window meets more images (hmdev::l10n_tile_set)
+// than the fused launch carries.
+bool l10n_window_fits(int n, const hmdev::ImgRect* z0, const hmdev::ImgRect* z1, int win_x, int win_y, int win_w,
+                      int win_h);
+
 // The tail of the up pass for two images: levels kTailBase .. levels-1 in
 // one launch (hmdev::up_tail_kernel), giving R at kTailBase.
 constexpr int kTailBase = 2;
@@ -180,7 +192,9 @@ class Pyramid {
   // debug mode, and for one or two images) the pyramid is dense.
   hipError_t set_supports(const hmdev::ImgRect* s0);
   int tail_base = kTailBase;  // two images: levels tail_base.. in one launch (0: never)
-  bool fuse_up = true;        // two images: levels 1 and 0 in one launch
+  bool fuse_up = true;        // levels 1 and 0 in one launch (two images; three or more with labels)
+  int l10n_key[4] = {-1, -1, -1, -1};  // the level-0 window l10n_fits was found for
+  bool l10n_fits = false;
   hmdev::ImgRect S[hmdev::kMaxLevels][hmdev::kMaxImages] = {};  // supports
   hmdev::ImgRect Z[hmdev::kMaxLevels][hmdev::kMaxImages] = {};  // valid regions
 

@@ -42,7 +42,10 @@ struct hmStitchPano {
   virtual bool tunes_remap_px() const { return false; }
   bool tunes_tail() const { return !hard_seam && n_images == 2 && num_levels >= 4; }
   bool tunes_overlap() const { return side != nullptr; }
-  bool tunes_fuse_up() const { return !hard_seam && n_images == 2 && num_levels >= 2; }
+  bool tunes_fuse_up() const {
+    return !hard_seam && ((n_images == 2 && num_levels >= 2) ||
+                          (pyr.kind == hmdev::kBlendN && n_images > 2 && num_levels >= 3 && !pipeline_f32));
+  }
   void apply_tuning() {
     pyr.tail_base = tuning.tail_base;
     pyr.fuse_up = tuning.fuse_up != 0;

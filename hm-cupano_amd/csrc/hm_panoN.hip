@@ -367,6 +367,13 @@ extern "C" int hmStitchPanoNCreate(int batch_size, int num_levels, const hmContr
     p->status = hipErrorInvalidValue;
   } else {
     p->status = p->setup(*masks, minimize_blend && num_levels > 0);
+    // Levels 1 and 0 in one launch by default for three images only: it
+    // holds more LDS per workgroup (R1 and every slot's G1 and G2 taps), which
+    // pays where tiles meet few images (3 x 3840x2160 -> 6200x2300: 51 us
+    // against 57 us for the two launches) and not with eight (14000x2600:
+    // 252 us against 213 us).  The autotuner tries both.
+    p->tuning.fuse_up = p->n_images <= 3 ? 1 : 0;
+    p->apply_tuning();
   }
   return p->status;
 }

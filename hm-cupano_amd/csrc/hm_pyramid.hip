@@ -331,6 +331,41 @@ hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int bat
   return hipGetLastError();
 }
 
+// Fills the level-0 arguments of a quad launch (window, tiles, regions).
+template <typename T0, int N>
+static void fill_quad_args(UpQuadArgs<T0, N>& a, const UpQuadLaunch& u) {
+  for (int i = 0; i < N; ++i) {
+    a.g0[i] = static_cast<const T0*>(u.g0[i]);
+    a.g1[i] = u.g1[i];
+  }
+  a.mask0 = u.mask0;
+  a.r1 = u.r1;
+  a.w0 = u.w0;
+  a.h0 = u.h0;
+  a.w1 = u.w1;
+  a.h1 = u.h1;
+  a.f32_out = u.f32_out;
+  a.u8_out = u.u8_out;
+  a.u8_pitch = u.u8_pitch;
+  a.u8_w = u.u8_w;
+  a.u8_h = u.u8_h;
+  a.ox = u.ox;
+  a.oy = u.oy;
+  a.win_x0 = u.win_x;
+  a.win_y0 = u.win_y;
+  a.win_x1 = u.win_x + u.win_w - 1;
+  a.win_y1 = u.win_y + u.win_h - 1;
+  a.qx0 = a.win_x0 >> 1;
+  a.qy0 = a.win_y0 >> 1;
+  a.nqx = (a.win_x1 >> 1) - a.qx0 + 1;
+  a.nqy = (a.win_y1 >> 1) - a.qy0 + 1;
+  a.tiles_x = (a.nqx + 63) / 64;
+  a.tiles_y = (a.nqy + 3) / 4;
+  for (int i = 0; i < N; ++i) a.z[i] = u.z[i];
+  a.wn = u.wn;
+  a.wn_plane = u.wn_plane;
+}
+
 hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch& u, int batch,
                             hipStream_t s) {
   if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
@@ -342,36 +377,7 @@ hipError_t launch_up_quad(int c, int n, int kind, bool g0_u8, const UpQuadLaunch
     auto go = [&](auto T0v, auto M) -> hipError_t {
       using T0 = decltype(T0v);
       UpQuadArgs<T0, kN> a;
-      for (int i = 0; i < kN; ++i) {
-        a.g0[i] = static_cast<const T0*>(u.g0[i]);
-        a.g1[i] = u.g1[i];
-      }
-      a.mask0 = u.mask0;
-      a.r1 = u.r1;
-      a.w0 = u.w0;
-      a.h0 = u.h0;
-      a.w1 = u.w1;
-      a.h1 = u.h1;
-      a.f32_out = u.f32_out;
-      a.u8_out = u.u8_out;
-      a.u8_pitch = u.u8_pitch;
-      a.u8_w = u.u8_w;
-      a.u8_h = u.u8_h;
-      a.ox = u.ox;
-      a.oy = u.oy;
-      a.win_x0 = u.win_x;
-      a.win_y0 = u.win_y;
-      a.win_x1 = u.win_x + u.win_w - 1;
-      a.win_y1 = u.win_y + u.win_h - 1;
-      a.qx0 = a.win_x0 >> 1;
-      a.qy0 = a.win_y0 >> 1;
-      a.nqx = (a.win_x1 >> 1) - a.qx0 + 1;
-      a.nqy = (a.win_y1 >> 1) - a.qy0 + 1;
-      a.tiles_x = (a.nqx + 63) / 64;
-      a.tiles_y = (a.nqy + 3) / 4;
-      for (int i = 0; i < kN; ++i) a.z[i] = u.z[i];
-      a.wn = u.wn;
-      a.wn_plane = u.wn_plane;
+      fill_quad_args(a, u);
       const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
       if (g_recorder) {
         const int op = a.r1 ? op_up_quad<T0, kC, kN, kK, decltype(M)::value, true>()
@@ -463,6 +469,56 @@ hipError_t launch_up_l10(int c, bool g0_u8, const UpQuadLaunch& u, const float* 
   };
   if (c != 3 && c != 4) return hipErrorInvalidValue;
   return g0_u8 ? by_c(uint8_t{}) : by_c(float{});
+}
+
+bool l10n_window_fits(int n, const ImgRect* z0, const ImgRect* z1, int win_x, int win_y, int win_w, int win_h) {
+  if (win_w <= 0 || win_h <= 0) return true;
+  const int qx0 = win_x >> 1, qy0 = win_y >> 1;
+  const int nqx = ((win_x + win_w - 1) >> 1) - qx0 + 1, nqy = ((win_y + win_h - 1) >> 1) - qy0 + 1;
+  for (int ty = 0; ty < (nqy + 3) / 4; ++ty)
+    for (int tx = 0; tx < (nqx + 63) / 64; ++tx)
+      if (__builtin_popcount(l10n_tile_set(z0, z1, n, qx0 + tx * 64, qy0 + ty * 4)) > kQuadSlots) return false;
+  return true;
+}
+
+hipError_t launch_up_l10n(int c, int n, const UpQuadLaunch& u, const float* wn1, size_t wn1_plane,
+                          const ImgRect* z1, const float* const* g2, const float* r2, int w2, int h2, int batch,
+                          hipStream_t s) {
+  if (u.win_w <= 0 || u.win_h <= 0) return hipSuccess;
+  if (u.mask0_mode != kMask0Label || n < 3 || n > kMaxImages || !wn1 || !r2 || (c != 3 && c != 4))
+    return hipErrorInvalidValue;
+  auto go = [&](auto Cv, auto Nv) -> hipError_t {
+    constexpr int kC = decltype(Cv)::value, kN = decltype(Nv)::value;
+    UpL10NArgs<kN> A;
+    fill_quad_args(A.l0, u);
+    A.l0.r1 = nullptr;
+    A.l0.wn = nullptr;
+    A.l0.wn_plane = 0;
+    A.wn1 = wn1;
+    A.wn1_plane = wn1_plane;
+    for (int i = 0; i < kN; ++i) {
+      A.z1[i] = z1[i];
+      A.g2[i] = g2[i];
+    }
+    A.r2 = r2;
+    A.w2 = w2;
+    A.h2 = h2;
+    const dim3 grid((unsigned)(A.l0.tiles_x * A.l0.tiles_y), 1u, (unsigned)batch);
+    hipLaunchKernelGGL((up_l10n_kernel<kC, kN>), grid, dim3(64, 4, 1), 0, s, A);
+    return hipGetLastError();
+  };
+  using std::integral_constant;
+  auto by_n = [&](auto Cv) -> hipError_t {
+    switch (n) {
+      case 3: return go(Cv, integral_constant<int, 3>{});
+      case 4: return go(Cv, integral_constant<int, 4>{});
+      case 5: return go(Cv, integral_constant<int, 5>{});
+      case 6: return go(Cv, integral_constant<int, 6>{});
+      case 7: return go(Cv, integral_constant<int, 7>{});
+      default: return go(Cv, integral_constant<int, 8>{});
+    }
+  };
+  return c == 4 ? by_n(integral_constant<int, 4>{}) : by_n(integral_constant<int, 3>{});
 }
 
 // ---------------------------------------------------------------------------
@@ -611,6 +667,7 @@ void Pyramid::build_level_table() {
 }
 
 hipError_t Pyramid::set_supports(const ImgRect* s0) {
+  l10n_key[0] = -1;  // the regions change: find the fused up launch's fit again
   // The debug path keeps every level dense, and so do one or two images (the
   // 2-image quad path reads every image everywhere).
   if (debug || n <= 2) return hipSuccess;
@@ -727,8 +784,21 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
   // quad; level 0 goes straight to the output.  The 2-image blend: levels 1
   // and 0 in one launch (fuse_up) when level 1 is not the tail's.
   const bool l10 = kind == kBlend2 && fuse_up && top >= 1 && levels >= 2;
+  // Three or more images with uint8 labels at level 0: levels 1 and 0 in one
+  // launch too (fuse_up), when no tile has more images than slots (checked
+  // once per window) and no batched pass is being recorded.
+  bool l10n = false;
+  if (kind == kBlendN && n > 2 && fuse_up && g0_u8 && top >= 1 && levels >= 3 && mask0_fast &&
+      mask0_mode == kMask0Label && wn[1].p && !g_recorder) {
+    const int key[4] = {fx0[0], fy0[0], fx1[0], fy1[0]};
+    if (!std::equal(key, key + 4, l10n_key)) {
+      l10n_fits = l10n_window_fits(n, Z[0], Z[1], fx0[0], fy0[0], fx1[0] - fx0[0] + 1, fy1[0] - fy0[0] + 1);
+      std::copy(key, key + 4, l10n_key);
+    }
+    l10n = l10n_fits;
+  }
   for (int k = top; k >= 0; --k) {
-    if (l10 && k == 1) continue;  // computed inside the level-0 launch
+    if ((l10 || l10n) && k == 1) continue;  // computed inside the level-0 launch
     const bool has_next = k + 1 < levels;
     UpQuadLaunch u;
     for (int i = 0; i < n; ++i) {
@@ -766,6 +836,14 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
     u.win_h = fy1[k] - fy0[k] + 1;
     for (int i = 0; i < n; ++i) u.z[i] = Z[k][i];
     if (timer) timer->begin(s);
+    if (l10n && k == 0) {
+      const float* g2[kMaxImages];
+      for (int i = 0; i < n; ++i) g2[i] = g[i][2].as<float>();
+      HM_RETURN_IF_ERROR(launch_up_l10n(c, n, u, wn[1].as<float>(), (size_t)W[1] * H[1], Z[1], g2,
+                                        r[2].as<float>(), W[2], H[2], batch, s));
+      if (timer) timer->end(s, "up_l10");
+      continue;
+    }
     if (l10 && k == 0) {
       const float* g2[2] = {levels > 2 ? g[0][2].as<float>() : nullptr, levels > 2 ? g[1][2].as<float>() : nullptr};
       HM_RETURN_IF_ERROR(launch_up_l10(c, g0_u8, u, mask[1].as<float>(), g2, levels > 2 ? r[2].as<float>() : nullptr,

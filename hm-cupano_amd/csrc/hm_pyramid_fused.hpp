@@ -954,6 +954,95 @@ __device__ __forceinline__ float4 cvt_raw_u8(uint32_t r) {
                      C == 4 ? (float)(r >> 24) : 0.0f);
 }
 
+// The labelled slot of each pixel of a quad (-1: the label is not a slot)
+// and that slot's level-0 pixel as loaded (slot 0's where there is none).
+template <int C, int N, int S>
+__device__ __forceinline__ void label_slot_pixels(const UpQuadArgs<uint8_t, N>& a, const int* idx, const QuadPix& q,
+                                                  const int* lab, int* sl, uint32_t* rv) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    sl[e] = -1;
+    const uint8_t* g = a.g0[idx[0]];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (idx[s] == lab[e]) {
+        sl[e] = s;
+        g = a.g0[idx[s]];
+      }
+    }
+    rv[e] = ld_raw_u8<C>(g, q.pix[e]);
+  }
+}
+
+// Level 0 of the label path from the tile's level-1 footprint in LDS (R at
+// lds, slot s's Gaussian at lds + (s + 1) * kQuadTaps; NEXT).  The blend of
+// a pixel is the labelled image's Laplacian where that image is opaque (3
+// channels: always), else the first slot of largest alpha (label_blend):
+// the labelled slot's pixel was loaded (sl, rv), the others are loaded only
+// where that one is transparent or not a slot.  Every value is computed by
+// the functions, on the inputs, of the general slot path.  The same steps as
+// the end of up_quad_label_slots, which keeps its own inline copy: called
+// from there, this form costs that kernel 4 more VGPRs and a wavefront per
+// SIMD (84 against 80 with eight images).
+template <int C, int N, bool NEXT, int S>
+__device__ __forceinline__ void label_level0(const UpQuadArgs<uint8_t, N>& a, const int* idx, const QuadPix& q,
+                                             const int* sl, const uint32_t* rv, const float4* lds, int b) {
+  const int t0 = (int)threadIdx.y * kQuadTapW + (int)threadIdx.x;  // tap (qx, qy) in the footprint
+  float4 rt[4];
+  if (NEXT) {
+    rt[0] = lds[t0];
+    rt[1] = lds[t0 + 1];
+    rt[2] = lds[t0 + kQuadTapW];
+    rt[3] = lds[t0 + kQuadTapW + 1];
+  }
+  // Laplacian of slot s at pixel e of the quad (zero outside its region)
+  auto lap_of = [&](int s, uint32_t v, const ImgRect& zs, int e) -> float4 {
+    if (!in_rect(zs, q.x[e], q.y[e])) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 g = cvt_raw_u8<C>(v);
+    if (!NEXT) return g;
+    const float4* tp = lds + (s + 1) * kQuadTaps + t0;
+    const float4 gt[4] = {tp[0], tp[1], tp[kQuadTapW], tp[kQuadTapW + 1]};
+    const float4 up = up_quad_lap<C>(e, gt);  // e is compile-time after unrolling
+    return make_float4(g.x - up.x, g.y - up.y, g.z - up.z, g.w);
+  };
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (!q.in[e]) continue;
+    ImgRect zs = {0, 0, -1, -1};
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      if (s == sl[e]) zs = a.z[idx[s]];
+    float4 B = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool done = false;
+    if (sl[e] >= 0) {
+      const float4 L = lap_of(sl[e], rv[e], zs, e);
+      if (C == 3 || L.w != 0.0f) {
+        B = C == 4 ? L : make_float4(L.x, L.y, L.z, 0.0f);
+        done = true;
+      }
+    }
+    if constexpr (C == 4) {
+      if (!done) {  // alpha fallback: the first slot of largest alpha
+        float amax = 0.0f;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const float4 L = lap_of(s, ld_raw_u8<C>(a.g0[idx[s]], q.pix[e]), a.z[idx[s]], e);
+          if (L.w > amax) {
+            amax = L.w;
+            B = L;
+          }
+        }
+      }
+    }
+    float4 R = B;
+    if (NEXT) {
+      const float4 up = up_quad_rec<C>(e, rt);
+      R = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+    }
+    quad_store<C>(a, q.x[e], q.y[e], b, R);
+  }
+}
+
 // Level 0 of the N-image blend from the seam labels, through the tile's S
 // slot images (uint8 level 0).  The blend of a pixel is the labelled image's
 // Laplacian where that image is opaque (3 channels: always), else the first
@@ -1687,6 +1776,183 @@ __global__ void __launch_bounds__(256) up_l10_kernel(UpL10Args<T0> a) {
   __shared__ float4 lds[up_l10_lds_float4()];
   (void)a;
   up_l10_body<T0, C, MODE, NEXT1>(kernel_arg<UpL10Args<T0>>(), hw_blk(), lds);
+}
+
+// ---------------------------------------------------------------------------
+// three or more images: levels 1 and 0 of the up pass in one launch
+// ---------------------------------------------------------------------------
+// The N-image blend with uint8 level 0 and seam labels, as up_l10_body for
+// two images: a level-0 workgroup computes R1 on its tile's (64 + 1) x
+// (4 + 1) level-1 footprint itself, from level 2's (32 + 2) x (2 + 2)
+// footprint staged in LDS, and keeps it in LDS, so R1 never goes to memory
+// and the level-1 launch disappears.  The G1 footprint staged for R1 is the
+// one level 0's Laplacians read.  Images go through the tile's slots: those
+// whose level-0 region meets the tile or whose level-1 region meets the
+// footprint, in index order -- a superset of the images that can be non-zero
+// at any pixel either level computes, so the blends are the blends over all
+// images (blendN_weighted, label_blend).  The host launches it only when no
+// tile of the window has more than kQuadSlots of them.  Every R1 value is the
+// level-1 launch's (the same functions on the same taps, footprint positions
+// clamped to the level as that launch's taps are); footprint positions
+// outside the level-1 window use level-2 data outside the level-2 window (not
+// written this frame), which no level-0 pixel of the window reads.
+template <int N>
+struct UpL10NArgs {
+  UpQuadArgs<uint8_t, N> l0;  // level 0: labels, g1 = level 1's Gaussians, z = level-0 regions (r1 not read)
+  const float* wn1;           // level 1's planar normalised weights, wn1_plane floats per image
+  size_t wn1_plane;
+  ImgRect z1[N];              // level 1's valid regions
+  const float* g2[N];         // level 2's Gaussians
+  const float* r2;            // level 2's reconstruction
+  int w2, h2;
+};
+
+// LDS of up_l10n_kernel, in float4: R1 and each slot's G1 on the level-1
+// footprint, then R2 and each slot's G2 on the level-2 footprint.
+template <int N>
+__host__ __device__ constexpr int up_l10n_lds_float4() {
+  return ((N < kQuadSlots ? N : kQuadSlots) + 1) * (kQuadTaps + kL2Taps);
+}
+
+// The image set (bit i: image i) of the level-0 tile whose first quad is
+// (tqx, tqy): images whose level-0 region z0 meets the tile's 128 x 8 pixels
+// or whose level-1 region z1 meets its level-1 footprint.
+__host__ __device__ inline unsigned l10n_tile_set(const ImgRect* z0, const ImgRect* z1, int n, int tqx, int tqy) {
+  unsigned set = 0;
+  for (int i = 0; i < n; ++i) {
+    const ImgRect& a = z0[i];
+    const ImgRect& c = z1[i];
+    const bool m0 = a.x0 <= 2 * tqx + 127 && a.x1 >= 2 * tqx && a.y0 <= 2 * tqy + 7 && a.y1 >= 2 * tqy;
+    const bool m1 = c.x0 <= tqx + kQuadTapW - 1 && c.x1 >= tqx && c.y0 <= tqy + kQuadTapH - 1 && c.y1 >= tqy;
+    if (m0 || m1) set |= 1u << i;
+  }
+  return set;
+}
+
+template <int C, int N, int S>
+__device__ __forceinline__ void up_l10n_slots(const UpL10NArgs<N>& A, const int* idx_rt, int tqx, int tqy, int b,
+                                              float4* lds) {
+  const UpQuadArgs<uint8_t, N>& a = A.l0;
+  constexpr int SA = S > 0 ? S : 1;  // array extent
+  int idx[SA];
+#pragma unroll
+  for (int s = 0; s < S; ++s) idx[s] = idx_rt[s];
+  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
+  const QuadPix q = quad_pixels(a, tqx + lx, tqy + ly, b);
+  const int tid = ly * 64 + lx;
+  constexpr int kRounds = (kQuadTaps + kWgThreads - 1) / kWgThreads;
+  float4* lr1 = lds;                         // R1 on the level-1 footprint
+  float4* lg1 = lds + kQuadTaps;             // G1 of slot s at lg1 + s * kQuadTaps
+  float4* l2 = lds + (S + 1) * kQuadTaps;    // level 2: R2, then G2 of each slot
+  const int bx2 = tqx >> 1, by2 = tqy >> 1;
+  // Every load of the workgroup first: the labels (the labelled slot's
+  // level-0 pixel depends on them), level 2's footprint, level 1's footprint
+  // (each slot's Gaussian and weight), the labelled level-0 pixels.
+  int lab[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) lab[e] = static_cast<const uint8_t*>(a.mask0)[q.mpx[e]];
+  float4 s2[S + 1];
+  {
+    const int p = min(tid, kL2Taps - 1);
+    const int px = min(bx2 + p % kL2TapW, A.w2 - 1), py = min(by2 + p / kL2TapW, A.h2 - 1);
+    const size_t tap = ((size_t)b * A.h2 + py) * A.w2 + px;
+    s2[0] = ld_px<C>(A.r2, tap);
+#pragma unroll
+    for (int s = 0; s < S; ++s) s2[s + 1] = ld_px<C>(A.g2[idx[s]], tap);
+  }
+  float4 s1[kRounds][SA];
+  float w1[kRounds][SA];
+  int x1r[kRounds], y1r[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int p = min(tid + r * kWgThreads, kQuadTaps - 1);
+    x1r[r] = min(tqx + p % kQuadTapW, a.w1 - 1);
+    y1r[r] = min(tqy + p / kQuadTapW, a.h1 - 1);
+    const size_t mp = (size_t)y1r[r] * a.w1 + x1r[r];
+    const size_t tap = (size_t)b * a.h1 * a.w1 + mp;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      s1[r][s] = ld_px<C>(a.g1[idx[s]], tap);
+      w1[r][s] = A.wn1[(size_t)idx[s] * A.wn1_plane + mp];
+    }
+  }
+  int sl[4];
+  uint32_t rv[4];
+  label_slot_pixels<C, N, S>(a, idx, q, lab, sl, rv);
+  if (tid < kL2Taps) {
+#pragma unroll
+    for (int k = 0; k <= S; ++k) l2[k * kL2Taps + tid] = s2[k];
+  }
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int p = tid + r * kWgThreads;
+    if (p < kQuadTaps) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) lg1[s * kQuadTaps + p] = s1[r][s];
+    }
+  }
+  __syncthreads();
+  // R1 on the footprint (the level-1 launch's quad arithmetic at each position)
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int p = tid + r * kWgThreads;
+    if (p >= kQuadTaps) continue;
+    const int x1 = x1r[r], y1 = y1r[r];
+    const int e1 = (x1 & 1) | ((y1 & 1) << 1);
+    const int t00 = ((y1 >> 1) - by2) * kL2TapW + ((x1 >> 1) - bx2);
+    float4 Lv[SA];
+    float w[SA];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      Lv[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (in_rect(A.z1[idx[s]], x1, y1)) {
+        const float4* gp = l2 + (s + 1) * kL2Taps + t00;
+        const float4 gt[4] = {gp[0], gp[1], gp[kL2TapW], gp[kL2TapW + 1]};
+        const float4 up = up_quad_lap<C>(e1, gt);
+        Lv[s] = make_float4(s1[r][s].x - up.x, s1[r][s].y - up.y, s1[r][s].z - up.z, s1[r][s].w);
+      }
+      w[s] = w1[r][s];
+    }
+    const float4 B = blendN_weighted<C, S>(Lv, w);
+    const float4* rp = l2 + t00;
+    const float4 rt[4] = {rp[0], rp[1], rp[kL2TapW], rp[kL2TapW + 1]};
+    const float4 up = up_quad_rec<C>(e1, rt);
+    lr1[p] = make_float4(up.x + B.x, up.y + B.y, up.z + B.z, B.w);
+  }
+  __syncthreads();
+  label_level0<C, N, true, S>(a, idx, q, sl, rv, lds, b);
+}
+
+// Runs on 64 x 4 threads; lds: up_l10n_lds_float4<N>() float4.
+template <int C, int N>
+__device__ __forceinline__ void up_l10n_body(const UpL10NArgs<N>& A, Blk blk, float4* lds) {
+  const UpQuadArgs<uint8_t, N>& a = A.l0;
+  const int t = blk.x;
+  const int tqx = a.qx0 + (t % a.tiles_x) * 64, tqy = a.qy0 + (t / a.tiles_x) * 4;
+  const unsigned set0 = l10n_tile_set(a.z, A.z1, N, tqx, tqy);  // uniform
+  unsigned set = set0;
+  int idx[kQuadSlots];
+#pragma unroll
+  for (int s = 0; s < kQuadSlots; ++s) {
+    idx[s] = set ? __builtin_ctz(set) : 0;
+    set &= set - 1u;
+  }
+  switch (__builtin_popcount(set0)) {  // uniform; at most kQuadSlots (the host checks every tile)
+    case 0: up_l10n_slots<C, N, 0>(A, idx, tqx, tqy, blk.z, lds); return;
+    case 1: up_l10n_slots<C, N, 1>(A, idx, tqx, tqy, blk.z, lds); return;
+    case 2: up_l10n_slots<C, N, 2>(A, idx, tqx, tqy, blk.z, lds); return;
+    case 3: up_l10n_slots<C, N, 3>(A, idx, tqx, tqy, blk.z, lds); return;
+    default:
+      if constexpr (N >= 4) up_l10n_slots<C, N, 4>(A, idx, tqx, tqy, blk.z, lds);
+      return;
+  }
+}
+
+template <int C, int N>
+__global__ void __launch_bounds__(256) up_l10n_kernel(UpL10NArgs<N> a) {
+  __shared__ float4 lds[up_l10n_lds_float4<N>()];
+  (void)a;
+  up_l10n_body<C, N>(kernel_arg<UpL10NArgs<N>>(), hw_blk(), lds);
 }
 
 }  // namespace hmdev

@@ -240,8 +240,9 @@ typedef struct hmTuning {
   int canvas_grid; /* workgroups of the 2-image canvas remap on the second stream: 0 = one per tile,
                       else 256, 512, 1024 or 2048 walking the tiles (a smaller share of the GPU) */
   int fuse_down;   /* 1: the 2-image blend-input remap runs inside the first down-pass launch */
-  int fuse_up;     /* 1: 2-image soft seam: levels 1 and 0 of the up pass run as one launch (level 1's
-                      reconstruction stays in LDS) */
+  int fuse_up;     /* 1: levels 1 and 0 of the up pass run as one launch (level 1's reconstruction
+                      stays in LDS): the 2-image soft seam, and the N-image class (3..8 images, uint8,
+                      levels >= 3) where no level-0 tile meets more than 4 images */
 } hmTuning;
 int hmStitchPanoGetTuning(const hmStitchPano* pano, hmTuning* tuning);
 /* SetTuning and Autotune return hipErrorNotSupported while a server holds the

@@ -240,3 +240,66 @@ def test_panoN_cfg2_n8_sparse_equals_dense_per_level_path(hm, torch_gpu):
     _, dense = run_panoN(hm, torch, rig, frames, 6, debug=True)
     diff = fast != dense
     assert not diff.any(), f"{diff.any(-1).sum()} pixels differ"
+
+
+def run_tuned(hm, torch, rig, frames, levels, fuse_up, minimize=True):
+    """One frame with fuse_up set and launch timing on: (canvas, labels of the launches)."""
+    b = frames[0].shape[0]
+    pano = hm.StitchPanoN(b, levels, rig, minimize_blend=minimize)
+    assert pano.status == 0, pano.status
+    pano.set_tuning(fuse_up=fuse_up)
+    pano.set_timing(True)
+    ins = [torch.from_numpy(np.ascontiguousarray(f)).cuda() for f in frames]
+    w, h = pano.canvas_size()
+    canvas = torch.full((b, h, w, 4), 77, dtype=torch.uint8, device="cuda")
+    assert pano.process(ins, canvas) == 0
+    torch.cuda.synchronize()
+    return canvas.cpu().numpy(), set(pano.timing_report())
+
+
+def spread_rig(n, w=150, step=110, h=72):
+    """n identity-mapped images side by side, each overlapping the next by w - step columns,
+    a wavy seam of bands: every level-0 tile (128 columns) meets at most three images."""
+    cw = step * (n - 1) + w
+    ys, xs = np.mgrid[0:h, 0:cw]
+    seam = np.clip((xs + 6 * np.sin(ys / 5.0) - (w - step) // 2) // step, 0, n - 1).astype(np.uint8)
+    return IdRig([(w, h)] * n, [(step * i, 0) for i in range(n)], seam)
+
+
+@pytest.mark.parametrize("n", [3, 5, 8])
+@pytest.mark.parametrize("levels", [3, 6])
+@pytest.mark.parametrize("minimize", [True, False])
+def test_panoN_fused_levels_1_and_0_bit_exact(hm, torch_gpu, oracle, n, levels, minimize):
+    """fuse_up: levels 1 and 0 of the up pass in one launch (R1 kept in LDS) against the oracle
+    and against the two launches, on rigs whose tiles meet at most four images (synthetic
+    cylindrical rigs for 3 and 5 images, side-by-side images for 8)."""
+    if n < 8:
+        rig = small_n_rig(n, 80 + n)
+        frames = frames_for(rig, 2, 51 + n)
+    else:
+        rig = spread_rig(n)
+        rng = np.random.default_rng(51 + n)
+        frames = []
+        for i in range(n):  # random pixels, opaque but for a transparent patch per image
+            f = rng.integers(0, 256, (2, 72, 150, 4)).astype(np.uint8)
+            f[..., 3] = 255
+            f[:, 10 + 5 * i: 30 + 5 * i, 40: 60, 3] = 0
+            frames.append(f)
+    fused, labels = run_tuned(hm, torch_gpu, rig, frames, levels, 1, minimize)
+    assert "up_l10" in labels and "up_level0" not in labels, labels
+    split, labels0 = run_tuned(hm, torch_gpu, rig, frames, levels, 0, minimize)
+    assert "up_l10" not in labels0 and "up_level0" in labels0, labels0
+    np.testing.assert_array_equal(fused, split)
+    np.testing.assert_array_equal(fused, oracle.panoN(rig, frames, levels, minimize=minimize))
+
+
+def test_panoN_fused_levels_fall_back_with_more_than_four_images_per_tile(hm, torch_gpu, oracle):
+    """Six images over the whole canvas: every tile meets six, so fuse_up keeps the two launches."""
+    n, w, h = 6, 160, 96
+    ys, xs = np.mgrid[0:h, 0:w]
+    seam = np.clip((xs * n) // w, 0, n - 1).astype(np.uint8)
+    rig = IdRig([(w, h)] * n, [(0, 0)] * n, seam)
+    imgs = [patterned_u8(w, h, i) for i in range(n)]
+    got, labels = run_tuned(hm, torch_gpu, rig, imgs, 4, 1)
+    assert "up_l10" not in labels and "up_level0" in labels, labels
+    np.testing.assert_array_equal(got, oracle.panoN(rig, imgs, 4))
