@@ -150,13 +150,17 @@ bool l10n_window_fits(int n, const hmdev::ImgRect* z0, const hmdev::ImgRect* z1,
 constexpr int kTailBase = 2;
 struct UpTailLaunch {
   int n = 0;  // tail levels; [0] = kTailBase
-  const float* g[hmdev::kMaxTail][2] = {};
+  const float* g[hmdev::kMaxTail][hmdev::kMaxImages] = {};
+  hmdev::ImgRect z[hmdev::kMaxTail][hmdev::kMaxImages] = {};  // valid regions (three or more images)
   const float* mask[hmdev::kMaxTail] = {};
   int w[hmdev::kMaxTail] = {}, h[hmdev::kMaxTail] = {};
   float* r_out = nullptr;
   int wx0 = 0, wy0 = 0, wx1 = -1, wy1 = -1;  // base-level pixels produced
 };
+// Two images (either kind), or three or four (the 3- and N-image blends, in
+// the burst form only: u.n - 1 <= hmdev::kTailLds).
 hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int batch, hipStream_t s);
+constexpr int kMaxTailImages = 4;
 
 // Where the level-0 reconstruction goes.
 struct Level0Out {

@@ -40,7 +40,9 @@ struct hmStitchPano {
   std::atomic<bool> served{false};
   // Which knobs matter for this object (the autotuner sweeps only those).
   virtual bool tunes_remap_px() const { return false; }
-  bool tunes_tail() const { return !hard_seam && n_images == 2 && num_levels >= 4; }
+  bool tunes_tail() const {
+    return !hard_seam && num_levels >= 4 && n_images <= hm::kMaxTailImages;
+  }
   bool tunes_overlap() const { return side != nullptr; }
   bool tunes_fuse_up() const {
     return !hard_seam && ((n_images == 2 && num_levels >= 2) ||

@@ -371,8 +371,12 @@ extern "C" int hmStitchPanoNCreate(int batch_size, int num_levels, const hmContr
     // holds more LDS per workgroup (R1 and every slot's G1 and G2 taps), which
     // pays where tiles meet few images (3 x 3840x2160 -> 6200x2300: 51 us
     // against 57 us for the two launches) and not with eight (14000x2600:
-    // 252 us against 213 us).  The autotuner tries both.
+    // 252 us against 213 us).  The small levels from level 3 on in one launch
+    // (three or four images): 0.152 ms per 6200x2300 frame against 0.156 ms
+    // with a launch per level, 0.167 ms from level 2.  The autotuner tries
+    // every setting.
     p->tuning.fuse_up = p->n_images <= 3 ? 1 : 0;
+    p->tuning.tail_base = 3;
     p->apply_tuning();
   }
   return p->status;

@@ -265,14 +265,57 @@ static hipError_t dispatch_cnk(int c, int n, int kind, F&& f) {
   return hipErrorInvalidValue;
 }
 
+// The burst tail of three or four images (3- and N-image blends).
+template <int C, int N, int KIND>
+static hipError_t up_tail_multi_t(const UpTailLaunch& u, int batch, hipStream_t s) {
+  UpTailArgs<N> a;
+  a.n = u.n;
+  for (int j = 0; j < kMaxTail; ++j) {
+    const int jj = j < u.n ? j : 0;
+    for (int i = 0; i < N; ++i) {
+      a.lv[j].g[i] = u.g[jj][i];
+      a.lv[j].z[i] = u.z[jj][i];
+    }
+    a.lv[j].mask = u.mask[jj];
+    a.lv[j].w = u.w[jj];
+    a.lv[j].h = u.h[jj];
+  }
+  a.r_out = u.r_out;
+  a.wx0 = u.wx0;
+  a.wy0 = u.wy0;
+  a.wx1 = u.wx1;
+  a.wy1 = u.wy1;
+  constexpr int T = kTailLdsTile;
+  a.tiles_x = (u.wx1 - u.wx0 + T) / T;
+  a.tiles_y = (u.wy1 - u.wy0 + T) / T;
+  const dim3 grid((unsigned)(a.tiles_x * a.tiles_y), 1u, (unsigned)batch);
+  if (g_recorder) return g_recorder->fail("up_tail_lds");  // no batched form
+  hipLaunchKernelGGL((up_tail_lds_kernel<C, N, KIND, T>), grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_up_tail(int c, int n, int kind, const UpTailLaunch& u, int batch, hipStream_t s) {
-  if (n != 2 || u.n < 2 || u.n > kMaxTail) return hipErrorInvalidValue;
+  if (n < 2 || n > kMaxTailImages || u.n < 2 || u.n > kMaxTail) return hipErrorInvalidValue;
   if (u.wx1 < u.wx0 || u.wy1 < u.wy0) return hipSuccess;
+  if (n > 2) {
+    if (u.n - 1 > kTailLds || (kind != kBlendN && kind != kBlend3) || (kind == kBlend3 && n != 3) ||
+        (c != 3 && c != 4))
+      return hipErrorInvalidValue;
+    auto by_c = [&](auto Cv) -> hipError_t {
+      constexpr int kC = decltype(Cv)::value;
+      if (kind == kBlend3) return up_tail_multi_t<kC, 3, kBlend3>(u, batch, s);
+      return n == 3 ? up_tail_multi_t<kC, 3, kBlendN>(u, batch, s) : up_tail_multi_t<kC, 4, kBlendN>(u, batch, s);
+    };
+    return c == 4 ? by_c(std::integral_constant<int, 4>{}) : by_c(std::integral_constant<int, 3>{});
+  }
   UpTailArgs<2> a;
   a.n = u.n;
   for (int j = 0; j < kMaxTail; ++j) {
     const int jj = j < u.n ? j : 0;
-    for (int i = 0; i < 2; ++i) a.lv[j].g[i] = u.g[jj][i];
+    for (int i = 0; i < 2; ++i) {
+      a.lv[j].g[i] = u.g[jj][i];
+      a.lv[j].z[i] = u.z[jj][i];
+    }
     a.lv[j].mask = u.mask[jj];
     a.lv[j].w = u.w[jj];
     a.lv[j].h = u.h[jj];
@@ -758,14 +801,22 @@ hipError_t Pyramid::run_up(const void* const* g0, bool g0_u8, const Level0Out& o
   }
   // Two images: levels tail_base .. L-1 in one tile-fused launch, giving R at
   // tail_base on the footprint the finer levels read.
+  // Three or four images (sparse, the 3- and N-image blends): the same
+  // tail in its burst form, loading each image inside its valid region only,
+  // when no batched pass is being recorded.
   int top = levels - 1;  // first level run in quads
   const int tb = tail_base;
-  if (n == 2 && tb > 0 && levels - 1 > tb && levels - tb <= kMaxTail) {
+  const bool tail_multi = n > 2 && n <= kMaxTailImages && (kind == kBlendN || kind == kBlend3) &&
+                          levels - 1 - tb <= kTailLds && !g_recorder;
+  if ((n == 2 || tail_multi) && tb > 0 && levels - 1 > tb && levels - tb <= kMaxTail) {
     UpTailLaunch t;
     t.n = levels - tb;
     for (int j = 0; j < t.n; ++j) {
       const int l = tb + j;
-      for (int i = 0; i < n; ++i) t.g[j][i] = g[i][l].as<float>();
+      for (int i = 0; i < n; ++i) {
+        t.g[j][i] = g[i][l].as<float>();
+        t.z[j][i] = Z[l][i];
+      }
       t.mask[j] = mask[l].as<float>();
       t.w[j] = W[l];
       t.h[j] = H[l];

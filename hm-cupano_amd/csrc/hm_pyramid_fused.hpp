@@ -516,6 +516,7 @@ struct TailLevel {
   const float* g[N];  // Gaussian level of each image
   const float* mask;  // mask level, M interleaved channels
   int w, h;
+  ImgRect z[N];       // valid region of each image (read by the 3- and N-image blends)
 };
 
 template <int N>
@@ -698,7 +699,12 @@ __device__ __forceinline__ void up_tail_lds_body(const UpTailArgs<N>& a, Blk blk
           const size_t pix = ((size_t)b * d.h + y) * d.w + x;
           const int slot = (j == 0 ? 0 : SB + (j - 1) * SU) + s;
 #pragma unroll
-          for (int i = 0; i < N; ++i) gv[slot][i] = ld_px<C>(d.g[i], pix);
+          for (int i = 0; i < N; ++i) {
+            // Three or more images are sparse: outside its valid region an
+            // image's level is zero (see LevelDesc) and its buffer is not read.
+            gv[slot][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (KIND == kBlend2 || in_rect(d.z[i], x, y)) gv[slot][i] = ld_px<C>(d.g[i], pix);
+          }
 #pragma unroll
           for (int c = 0; c < M; ++c) mv[slot][c] = d.mask[((size_t)y * d.w + x) * M + c];
         }

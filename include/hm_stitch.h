@@ -223,8 +223,9 @@ int hmStitchPanoDeviceBytes(const hmStitchPano* pano, size_t* bytes);
 /* Launch-shape knobs of a pano object.  No counterpart in the reference.
  * Every setting computes the same output; only the speed differs.
  *   remap_px   pixels per thread of the 2-image remap kernels: 2, 4 or 8;
- *   tail_base  2-image soft seam: levels tail_base..L-1 of the up pass run as
- *              one tile-fused launch (1..3), or 0: one launch per level;
+ *   tail_base  soft seam with 2 to 4 images: levels tail_base..L-1 of the up
+ *              pass run as one tile-fused launch (1..3; with 3 or 4 images
+ *              when at most 5 levels are in it), or 0: one launch per level;
  *   overlap    1: the canvas outside the blend region is remapped on a second
  *              stream overlapping the pyramid, forked after the blend inputs;
  *              2 (2-image class): the same, forked at the start of the frame;

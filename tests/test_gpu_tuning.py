@@ -100,3 +100,37 @@ def test_invalid_tuning_is_refused(hm, torch_gpu):
                                  "canvas_grid": 0, "fuse_down": 0, "fuse_up": 1}
     with pytest.raises(hm.HmError):
         pano.set_tuning(4, 2, 1, 0, 1, 0, 0, 2)
+
+
+@pytest.mark.parametrize("n", [3, 4, 5])
+def test_multi_image_tail_and_fused_levels_are_bit_exact(hm, torch_gpu, oracle, n):
+    """Three to five images at levels 6: the up pass's small levels in one burst launch from
+    tail_base (three and four images) or one launch each (five), with and without levels 1
+    and 0 fused, all equal to the oracle; the 3-image class's tail likewise."""
+    import hm_synth
+
+    torch = torch_gpu
+    rig = hm_synth.small_rig(n=n, seed=90 + n, src=(120, 56), canvas=(90 * n + 60, 64))
+    frames = frames_for(rig, 1, 11 + n)
+    dev = to_dev(torch, frames)
+    want = oracle.panoN(rig, frames, 6)
+    pano = hm.StitchPanoN(1, 6, rig)
+    for tb, fu in itertools.product((0, 1, 2, 3), (0, 1)):
+        pano.set_tuning(4, tb, 1, 0, 1, 0, 0, fu)
+        pano.set_timing(True)
+        c = pano.alloc_canvas()
+        assert pano.process(dev, c) == 0
+        torch.cuda.synchronize()
+        labels = set(pano.timing_report())
+        assert ("up_tail" in labels) == (n <= 4 and tb > 0), (tb, fu, labels)
+        np.testing.assert_array_equal(c.cpu().numpy(), want, err_msg=f"n={n} tail_base={tb} fuse_up={fu}")
+    if n == 3:
+        p3 = hm.StitchPano3(1, 6, rig)
+        w3, h3 = p3.canvas_size()
+        want3 = oracle.pano3(rig, frames, 6, fused=True, canvas=np.zeros((1, h3, w3, 4), np.uint8))
+        for tb in (0, 2, 3):
+            p3.set_tuning(4, tb, 1)
+            c = torch.zeros(want3.shape, dtype=torch.uint8, device="cuda")
+            assert p3.process(dev[0], dev[1], dev[2], c, fused=True) == 0
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(c.cpu().numpy(), want3, err_msg=f"pano3 tail_base={tb}")
