@@ -2,7 +2,8 @@
 
 Every comparison is bit-exact: the remap and hard seam are integer work, and
 the blend runs the reference's float arithmetic in the reference's order with
-no fused multiply-add on either side."""
+no multiply-add contraction on either side (the down pass's exact division by
+the tap count aside, whose bits are IEEE division's: tests/test_div_count.py)."""
 import json
 import os
 
